@@ -1,0 +1,122 @@
+/*
+ * shockx.h -- C ABI of libshockx.so, the MI355X (gfx950) implementation of SHOCK's
+ * right-hand-side and time-step hot path.
+ *
+ * Drop-in boundary.  SHOCK's driver calls the path through plain C functions that take
+ * its global structs (src/SHOCK.h:69-642); those structs embed MPI and CGNS types, so this
+ * library exposes the same operations over an opaque handle and plain pointers instead.
+ * Each entry point below names the reference function it replaces.  Field arrays use the
+ * reference's own layout: ghost-padded FP64 SoA, linear index i*ng1*ng2 + j*ng2 + k
+ * (k fastest), ng = n + S + 1 per direction, ng2 = 1 in 2D (src/Functions.c:56-103).
+ *
+ * Conventions: every function returns SHX_OK (0) or a negative SHX_ERR_* code and never
+ * aborts; shx_last_error() gives the message.  All work is enqueued on the handle's own
+ * HIP stream; functions that return host data synchronize that stream.  A handle is not
+ * thread-safe (the reference is single-threaded per rank as well).
+ */
+#ifndef SHOCKX_H
+#define SHOCKX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHX_OK               0
+#define SHX_ERR_ARG         -1  /* invalid argument or configuration */
+#define SHX_ERR_HIP         -2  /* HIP runtime error (message in shx_last_error) */
+#define SHX_ERR_UNSUPPORTED -3  /* configuration not implemented */
+#define SHX_ERR_STATE       -4  /* call out of order (e.g. stepping before a metric is set) */
+
+/* Reference behaviours that can be switched off for verification runs. */
+#define SHX_QUIRK_ZETA_VISC_J   1u /* src/ZD.c:547-548: zeta viscous stencil k offset taken from j */
+#define SHX_QUIRK_METRIC_DIAG3D 2u /* src/Functions.c:6479-6484: 3D metric keeps only x_xi, y_eta, z_zeta */
+#define SHX_QUIRKS_REFERENCE (SHX_QUIRK_ZETA_VISC_J | SHX_QUIRK_METRIC_DIAG3D)
+
+/* Boundary kinds per face (left, right, bottom, top, behind, in front). */
+#define SHX_BC_PERIODIC 0       /* self-interface, src/Functions.c:3580-3684 */
+
+/* Arithmetic forms. */
+#define SHX_FP64 0
+
+/* Case description: the subset of strct_configuration (src/SHOCK.h:69-544) the path reads. */
+typedef struct shx_desc {
+    int dims;          /* MESHDIMENSIONS: 2 or 3 */
+    int space_order;   /* SPACEORDER: 5 (WENO5) or 9 (WENO9) */
+    int time_order;    /* numerics:TimeOrder: 3 (SSP-RK3) or 4 (RK4) */
+    int n[3];          /* interior points (zonesize); n[2] ignored in 2D */
+    double gamma;      /* fluidproperties:Gamma */
+    double mach;       /* fluidproperties:Mach */
+    double reynolds;   /* fluidproperties:Reynolds */
+    double prandtl;    /* fluidproperties:Prandtl */
+    double T0;         /* fluidproperties:T, Sutherland constant = 100.4 / T0 */
+    int inviscid;      /* options:Inviscid */
+    int manufactured;  /* options:ManufacturedSolution (mu = 1, source added each stage) */
+    unsigned quirks;   /* SHX_QUIRK_* bits; SHX_QUIRKS_REFERENCE reproduces the reference */
+    int bc[6];         /* SHX_BC_* per face */
+    int precision;     /* SHX_FP64 */
+    double eps_override; /* 0: WENO epsilon = 10*sqrt(DBL_MIN) as src/Functions.c:1829 */
+} shx_desc;
+
+typedef struct shx_case shx_case;
+
+const char *shx_last_error(void);
+int shx_device_count(int *count);
+
+/* Allocates the device arrays of one block (replaces the path's part of AllocMemory,
+ * src/Functions.c:900-1290, and DefineParameters, src/Functions.c:1758-2032). */
+int shx_create(const shx_desc *desc, shx_case **out);
+int shx_destroy(shx_case *c);
+int shx_padded_extent(const shx_case *c, int ng[3]);
+
+/* Geometry (once per run).  shx_build_metric: CreateMetric (src/Functions.c:6388-6509) on the
+ * extended coordinate arrays (extent ng+2 per direction, 1 in 2D z; the reference's
+ * x_extrapolate), followed by the periodic ghost transfer of the metric
+ * (TransferMeshParameter, src/Functions.c:3481-3578).  shx_upload_metric takes metric arrays a
+ * caller already has (strct_mesh xi_x ... zeta_z, jacobian), order
+ * xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z,J. */
+int shx_build_metric(shx_case *c, const double *xe, const double *ye, const double *ze);
+int shx_upload_metric(shx_case *c, const double *const met[10]);
+int shx_download_metric(shx_case *c, double *const met[10]);
+
+/* State: primitive rho, u, v, w, e (strct_U, src/SHOCK.h:590-620).  Upload sets U_lastStep
+ * (and therefore U_RK at the next step start); download returns U_lastStep. */
+int shx_upload_state(shx_case *c, const double *const prim[5]);
+int shx_download_state(shx_case *c, double *const prim[5]);
+
+/* Manufactured-solution source per node (Q += J * src each stage,
+ * AddManufacturedSolutionSource src/ManufacturedSolution.c:25-175); NULL clears it. */
+int shx_upload_source(shx_case *c, const double *const src[5]);
+
+/* One time step with fixed tau: CalcRungeKutta (src/Functions.c:2034-2390) followed by the
+ * step-end copy U_lastStep <- U_RK (WriteValuesFromU1ToU2, src/SHOCK.c:663-666). */
+int shx_calc_runge_kutta(shx_case *c, double tau);
+/* nsteps of the above (the startSimulation loop body, src/SHOCK.c:557-666, without I/O). */
+int shx_run(shx_case *c, double tau, int nsteps);
+
+/* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
+ * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux
+ * assembly, src/Functions.c:2111-2293).  For parity tests. */
+int shx_rhs(shx_case *c, double *const q[5]);
+
+/* checkNAN (src/Functions.c:5419-5467) on U_lastStep: count of non-finite interior rho;
+ * those nodes get rho = u = v = w = 1e6 as in the reference. */
+int shx_check_nan(shx_case *c, int *count);
+
+/* CFL time step (new; the reference uses a fixed tau): dt = cfl / max over interior nodes of
+ * sum_d (|theta_d| + c |grad xi_d|), computed on U_lastStep. */
+int shx_cfl_dt(shx_case *c, double cfl, double *dt);
+
+int shx_synchronize(shx_case *c);
+
+/* Per-kernel timing with HIP events on the handle's stream.  When enabled, every launch is
+ * bracketed by events; shx_kernel_times synchronizes and returns, per kernel class, the total
+ * milliseconds and the launch count since the last reset.  names[k] points to static strings. */
+#define SHX_NUM_KERNELS 10
+int shx_set_timing(shx_case *c, int enable);
+int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
+                     int launches[SHX_NUM_KERNELS]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

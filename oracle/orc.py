@@ -1,0 +1,196 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the shock_amd package.
+
+Arrays are numpy float64 of shape (ng0, ng1, ng2), C-contiguous, the reference's
+ghost-padded k-fastest layout (src/Functions.c:56-103).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+QUIRK_ZETA_VISC_J = 1
+QUIRK_METRIC_DIAG3D = 2
+QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
+
+
+class OrcCfg(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int), ("S", ctypes.c_int), ("n", ctypes.c_int * 3),
+        ("time_order", ctypes.c_int), ("inviscid", ctypes.c_int), ("mms", ctypes.c_int),
+        ("quirks", ctypes.c_uint),
+        ("gamma", ctypes.c_double), ("mach", ctypes.c_double), ("reynolds", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("T0", ctypes.c_double), ("eps_override", ctypes.c_double),
+        ("g", ctypes.c_int), ("ng", ctypes.c_int * 3), ("ntot", ctypes.c_long),
+        ("upsilon", ctypes.c_double), ("psi", ctypes.c_double), ("gam_heat", ctypes.c_double),
+        ("suth", ctypes.c_double), ("eps", ctypes.c_double),
+        ("rk_a", ctypes.c_double * 4), ("rk_b", ctypes.c_double * 4), ("rk_q", ctypes.c_double * 4),
+        ("rk_f", ctypes.c_double * 4), ("rk_m", ctypes.c_double * 4),
+        ("zd_mid", ctypes.c_double * 10), ("zd_int", ctypes.c_double * 10), ("zd_der", ctypes.c_double * 11),
+    ]
+
+
+def build():
+    """Compile liboracle.so from the C restatement (gcc, no GPU)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        P = ctypes.c_void_p
+        _lib.orc_define.argtypes = [ctypes.POINTER(OrcCfg)]
+        _lib.orc_phi_w5.restype = ctypes.c_double
+        _lib.orc_phi_w5.argtypes = [ctypes.c_double, P, P]
+        _lib.orc_phi_w9.restype = ctypes.c_double
+        _lib.orc_phi_w9.argtypes = [ctypes.c_double, P, P]
+        _lib.orc_eigvec.argtypes = [P] + [ctypes.c_double] * 6 + [P, P]
+        _lib.orc_cfl_dt.restype = ctypes.c_double
+        _lib.orc_cfl_dt.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P]
+        _lib.orc_check_nan.restype = ctypes.c_int
+        _lib.orc_visc_fac.argtypes = [ctypes.c_int, P, P]
+        _lib.orc_step.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P]
+        _lib.orc_rk_update.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_int, ctypes.c_double, P, P, P, P, P]
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _bundle(arrs):
+    """C array of double* for a list of numpy arrays (kept alive by the caller)."""
+    t = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return t
+
+
+class Oracle:
+    """One block's configuration plus helpers that allocate padded fields."""
+
+    def __init__(self, dims=2, S=5, n=(16, 16, 1), time_order=4, gamma=1.4, mach=0.5,
+                 reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, mms=False,
+                 quirks=QUIRKS_REFERENCE, eps_override=0.0):
+        c = OrcCfg()
+        c.dims, c.S, c.time_order = dims, S, time_order
+        for d in range(3):
+            c.n[d] = n[d] if d < len(n) else 1
+        c.inviscid, c.mms, c.quirks = int(inviscid), int(mms), quirks
+        c.gamma, c.mach, c.reynolds, c.prandtl, c.T0 = gamma, mach, reynolds, prandtl, T0
+        c.eps_override = eps_override
+        if lib().orc_define(ctypes.byref(c)) != 0:
+            raise ValueError("invalid oracle configuration")
+        self.c = c
+        self.shape = (c.ng[0], c.ng[1], c.ng[2])
+        self.g = c.g
+
+    # ---- allocation
+    def zeros(self, nf):
+        return [np.zeros(self.shape) for _ in range(nf)]
+
+    # ---- kernels
+    def calc_values(self, met, prim):
+        der = self.zeros(8)
+        lib().orc_calc_values(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(der))
+        return der
+
+    def ghost_periodic(self, fields):
+        lib().orc_ghost_periodic(ctypes.byref(self.c), _bundle(fields), len(fields))
+
+    def weno_dir(self, d, met, prim, der, q):
+        lib().orc_weno_dir(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der), _bundle(q))
+
+    def weno_interface(self, d, met, prim, der, i, j, k):
+        out = np.zeros(5)
+        lib().orc_weno_interface(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
+                                 i, j, k, _ptr(out))
+        return out
+
+    def visc_derivs(self, prim, der):
+        grad = self.zeros(12)
+        lib().orc_visc_derivs(ctypes.byref(self.c), _bundle(prim), _bundle(der), _bundle(grad))
+        return grad
+
+    def visc_dir(self, d, met, prim, der, grad, q):
+        lib().orc_visc_dir(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
+                           _bundle(grad), _bundle(q))
+
+    def rhs(self, met, prim, src=None):
+        """Q (5 fields) of one stage for state prim; refills prim's ghosts periodically."""
+        scratch = self.zeros(30)
+        lib().orc_rhs(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(scratch),
+                      _bundle(src) if src is not None else None)
+        return scratch[20:25], scratch[:8], scratch[8:20]
+
+    def step(self, tau, met, un, urk, src=None, scratch=None):
+        """One CalcRungeKutta step in place on urk (un is the state at the start of the step)."""
+        if scratch is None:
+            scratch = self.zeros(30)
+        lib().orc_step(ctypes.byref(self.c), ctypes.c_double(tau), _bundle(met), _bundle(un),
+                       _bundle(urk), _bundle(scratch), _bundle(src) if src is not None else None)
+        return scratch
+
+    def run(self, tau, met, prim0, steps, src=None):
+        """Advance prim0 by `steps` full steps (startSimulation semantics: U_lastStep <- U_RK)."""
+        un = [a.copy() for a in prim0]
+        urk = [a.copy() for a in prim0]
+        scratch = self.zeros(30)
+        for _ in range(steps):
+            self.step(tau, met, un, urk, src, scratch)
+            for a, b in zip(un, urk):
+                a[...] = b
+        return urk
+
+    def rk_update(self, s, tau, met, un, urk, q, qsum):
+        lib().orc_rk_update(ctypes.byref(self.c), s, ctypes.c_double(tau), _bundle(met), _bundle(un),
+                            _bundle(urk), _bundle(q), _bundle(qsum))
+
+    def cfl_dt(self, cfl, met, prim):
+        return lib().orc_cfl_dt(ctypes.byref(self.c), cfl, _bundle(met), _bundle(prim))
+
+    def check_nan(self, prim):
+        return lib().orc_check_nan(ctypes.byref(self.c), _bundle(prim))
+
+    def metric_from_coords(self, xe, ye, ze):
+        met = self.zeros(10)
+        lib().orc_metric_from_coords(ctypes.byref(self.c), _ptr(xe), _ptr(ye), _ptr(ze), _bundle(met))
+        return met
+
+
+def phi_w5(eps, f, d):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    return lib().orc_phi_w5(eps, _ptr(f), _ptr(d))
+
+
+def phi_w9(eps, f, d):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    return lib().orc_phi_w9(eps, _ptr(f), _ptr(d))
+
+
+def eigvec(nh, u, v, w, rho, c, gamma):
+    nh = np.ascontiguousarray(nh, dtype=np.float64)
+    L = np.zeros(25)
+    R = np.zeros(25)
+    lib().orc_eigvec(_ptr(nh), u, v, w, rho, c, gamma, _ptr(L), _ptr(R))
+    return L.reshape(5, 5), R.reshape(5, 5)
+
+
+def visc_fac(d, m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    out = np.zeros(30)
+    lib().orc_visc_fac(d, _ptr(m), _ptr(out))
+    return out

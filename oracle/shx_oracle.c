@@ -1,0 +1,735 @@
+/*
+ * shx_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) for the SHOCK RHS / RK path.
+ *
+ * Plain-C FP64 restatement of the reference algorithm.  Every routine cites the
+ * reference file:line it follows.  Operation order inside each expression follows
+ * the reference so that the oracle reproduces the reference's rounding where the
+ * compiler does not contract (build with -ffp-contract=off).
+ *
+ * Not linked into the product.  See shx_oracle.h for the pinning status.
+ */
+#include "shx_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NEQ 5
+
+long orc_idx(const orc_cfg *c, int i, int j, int k)
+{
+    return ((long)i * c->ng[1] + j) * c->ng[2] + k;
+}
+
+/* DefineParameters, src/Functions.c:1758-2032; ghost/index setup src/Functions.c:56-103;
+ * Gamma (heat-flux factor) src/Functions.c:141-152. */
+int orc_define(orc_cfg *c)
+{
+    int d;
+    if (c->dims != 2 && c->dims != 3) return -1;
+    if (c->S != 5 && c->S != 9) return -1;
+    if (c->time_order != 3 && c->time_order != 4) return -1;
+    c->g = (c->S + 1) / 2;
+    if (c->dims == 2) c->n[2] = 1;
+    for (d = 0; d < 3; d++) c->ng[d] = c->n[d] + c->S + 1;
+    if (c->dims == 2) c->ng[2] = 1;
+    c->ntot = (long)c->ng[0] * c->ng[1] * c->ng[2];
+
+    c->upsilon = 1.0 / (c->gamma * (c->mach * c->mach));
+    c->psi = 1.0 / c->reynolds;
+    c->suth = 100.4 / c->T0;
+    c->gam_heat = 1.0 / ((c->gamma - 1.0) * pow(c->mach, 2.0) * c->reynolds * c->prandtl);
+    c->eps = (c->eps_override > 0.0) ? c->eps_override : 10. * pow(DBL_MIN, 0.5);
+
+    memset(c->rk_a, 0, sizeof c->rk_a); memset(c->rk_b, 0, sizeof c->rk_b);
+    memset(c->rk_q, 0, sizeof c->rk_q); memset(c->rk_f, 0, sizeof c->rk_f);
+    memset(c->rk_m, 0, sizeof c->rk_m);
+    if (c->time_order == 4) {
+        /* classic RK4, src/Functions.c:1961-1989 */
+        double a[4] = {1., 1., 1., 1.}, q[4] = {0.5, 0.5, 1., 1. / 6.};
+        double f[4] = {0., 0., 0., 1. / 6.}, m[4] = {1., 2., 2., 0.};
+        memcpy(c->rk_a, a, sizeof a); memcpy(c->rk_q, q, sizeof q);
+        memcpy(c->rk_f, f, sizeof f); memcpy(c->rk_m, m, sizeof m);
+    } else {
+        /* SSP-RK3, src/Functions.c:1990-2009 */
+        double a[3] = {1., 3. / 4., 1. / 3.}, b[3] = {0., 1. / 4., 2. / 3.};
+        double q[3] = {1., 1. / 4., 2. / 3.};
+        memcpy(c->rk_a, a, sizeof a); memcpy(c->rk_b, b, sizeof b); memcpy(c->rk_q, q, sizeof q);
+    }
+
+    memset(c->zd_mid, 0, sizeof c->zd_mid); memset(c->zd_int, 0, sizeof c->zd_int);
+    memset(c->zd_der, 0, sizeof c->zd_der);
+    if (c->S == 9) {
+        /* src/Functions.c:1858-1894 */
+        static const double in9[10] = {1. / 1260., -23. / 2520., 127. / 2520., -473. / 2520., 1627. / 2520.,
+                                       1627. / 2520., -473. / 2520., 127. / 2520., -23. / 2520., 1. / 1260.};
+        static const double mi9[10] = {-3. / 9450., 351. / 75600., -2649. / 75600., 15351. / 75600., -110649. / 75600.,
+                                       110649. / 75600., -15351. / 75600., 2649. / 75600., -351. / 75600., 3. / 9450.};
+        static const double de9[11] = {-1. / 1260., 25. / 2520., -150. / 2520., 600. / 2520., -2100. / 2520., 0.,
+                                       2100. / 2520., -600. / 2520., 150. / 2520., -25. / 2520., 1. / 1260.};
+        memcpy(c->zd_int, in9, sizeof in9); memcpy(c->zd_mid, mi9, sizeof mi9); memcpy(c->zd_der, de9, sizeof de9);
+    } else {
+        /* src/Functions.c:1915-1942 */
+        static const double in5[6] = {1. / 60., -8. / 60., 37. / 60., 37. / 60., -8. / 60., 1. / 60.};
+        static const double mi5[6] = {-1. / 90., 25. / 180., -245. / 180., 245. / 180., -25. / 180., 1. / 90.};
+        static const double de5[7] = {-1. / 60., 3. / 20., -3. / 4., 0., 3. / 4., -3. / 20., 1. / 60.};
+        memcpy(c->zd_int, in5, sizeof in5); memcpy(c->zd_mid, mi5, sizeof mi5); memcpy(c->zd_der, de5, sizeof de5);
+    }
+    return 0;
+}
+
+/* range helpers: interior [g, g+n-1]; all [0, ng-1]; 2D k range is [0,0] */
+static int rs(const orc_cfg *c, int d) { return (c->dims == 2 && d == 2) ? 0 : c->g; }
+static int re(const orc_cfg *c, int d) { return (c->dims == 2 && d == 2) ? 0 : c->g + c->n[d] - 1; }
+static int ge(const orc_cfg *c, int d) { return c->ng[d] - 1; }
+static long stride(const orc_cfg *c, int d)
+{
+    return d == 0 ? (long)c->ng[1] * c->ng[2] : (d == 1 ? (long)c->ng[2] : 1L);
+}
+
+/* CalcValues, src/Functions.c:2392-2481 (3D form; in 2D w = 0 and the z-metric terms are 0,
+ * which makes the 3D expressions reduce exactly to the 2D ones). */
+void orc_calc_values(const orc_cfg *c, double *const met[10], double *const prim[5], double *const der[8])
+{
+    long n;
+    for (n = 0; n < c->ntot; n++) {
+        double rho = prim[0][n], u = prim[1][n], v = prim[2][n], w = prim[3][n], e = prim[4][n];
+        double p;
+        der[4][n] = u * met[0][n] + v * met[1][n] + w * met[2][n];
+        der[5][n] = u * met[3][n] + v * met[4][n] + w * met[5][n];
+        der[6][n] = u * met[6][n] + v * met[7][n] + w * met[8][n];
+        p = fabs(rho * (c->gamma - 1) / c->upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
+        der[0][n] = p;
+        der[1][n] = fabs(p / rho);
+        der[2][n] = sqrt(c->upsilon * c->gamma * p / rho);
+        der[3][n] = ((1.0 + c->suth) * pow(p / rho, 1.5) / (p / rho + c->suth));
+        if (c->mms) der[3][n] = 1.0;
+    }
+}
+
+/* CalcEigenVectorsIn{Xi,Eta,Zeta}Direction, src/WENO.c:1085-1548 (identical formulas for the three
+ * directions with the averaged normalized metric row nh).  L and R row-major: L[c*5+m], R[m*5+c]. */
+void orc_eigvec(const double nh[3], double u, double v, double w, double rho, double cs,
+                double gamma, double L[25], double R[25])
+{
+    double nx = nh[0], ny = nh[1], nz = nh[2];
+    double s2 = sqrt(2.);
+    double M = sqrt(u * u + v * v + w * w) / cs;
+    double q2 = (u * u + v * v + w * w);
+    double gc = ((gamma - 1.0) / (cs * cs));
+    double ics = (1. / (cs * s2));
+    double un = (u * nx + v * ny + w * nz);
+
+    R[0 * 5 + 0] = nx; R[0 * 5 + 1] = ny; R[0 * 5 + 2] = nz; R[0 * 5 + 3] = ics; R[0 * 5 + 4] = ics;
+    R[1 * 5 + 0] = u * nx; R[1 * 5 + 1] = u * ny - rho * nz; R[1 * 5 + 2] = u * nz + rho * ny;
+    R[1 * 5 + 3] = ics * (u + nx * cs); R[1 * 5 + 4] = ics * (u - nx * cs);
+    R[2 * 5 + 0] = v * nx + rho * nz; R[2 * 5 + 1] = v * ny; R[2 * 5 + 2] = v * nz - rho * nx;
+    R[2 * 5 + 3] = ics * (v + ny * cs); R[2 * 5 + 4] = ics * (v - ny * cs);
+    R[3 * 5 + 0] = w * nx - rho * ny; R[3 * 5 + 1] = w * ny + rho * nx; R[3 * 5 + 2] = w * nz;
+    R[3 * 5 + 3] = ics * (w + nz * cs); R[3 * 5 + 4] = ics * (w - nz * cs);
+    R[4 * 5 + 0] = q2 / 2. * nx + rho * (v * nz - w * ny);
+    R[4 * 5 + 1] = q2 / 2. * ny + rho * (w * nx - u * nz);
+    R[4 * 5 + 2] = q2 / 2. * nz + rho * (u * ny - v * nx);
+    R[4 * 5 + 3] = ics * (q2 / 2. + cs * cs / (gamma - 1.) + cs * un);
+    R[4 * 5 + 4] = ics * (q2 / 2. + cs * cs / (gamma - 1.) - cs * un);
+
+    L[0 * 5 + 0] = nx * (1.0 - 0.5 * (gamma - 1.0) * (M * M)) - 1. / rho * (v * nz - w * ny);
+    L[0 * 5 + 1] = gc * u * nx;
+    L[0 * 5 + 2] = gc * v * nx + (nz) / rho;
+    L[0 * 5 + 3] = gc * w * nx - (ny) / rho;
+    L[0 * 5 + 4] = -gc * nx;
+    L[1 * 5 + 0] = ny * (1.0 - 0.5 * (gamma - 1.0) * (M * M)) - 1. / rho * (w * nx - u * nz);
+    L[1 * 5 + 1] = gc * u * ny - (nz) / rho;
+    L[1 * 5 + 2] = gc * v * ny;
+    L[1 * 5 + 3] = gc * w * ny + (nx) / rho;
+    L[1 * 5 + 4] = -gc * ny;
+    L[2 * 5 + 0] = nz * (1.0 - 0.5 * (gamma - 1.0) * (M * M)) - 1. / rho * (u * ny - v * nx);
+    L[2 * 5 + 1] = gc * u * nz + (ny) / rho;
+    L[2 * 5 + 2] = gc * v * nz - (nx) / rho;
+    L[2 * 5 + 3] = gc * w * nz;
+    L[2 * 5 + 4] = -gc * nz;
+    L[3 * 5 + 0] = 1. / (s2 * cs) * ((gamma - 1.0) * q2 / 2. - cs * un);
+    L[3 * 5 + 1] = (-(gamma - 1.0) * u / cs + nx) / s2;
+    L[3 * 5 + 2] = (-(gamma - 1.0) * v / cs + ny) / s2;
+    L[3 * 5 + 3] = (-(gamma - 1.0) * w / cs + nz) / s2;
+    L[3 * 5 + 4] = (gamma - 1.) / (s2 * cs);
+    L[4 * 5 + 0] = 1. / (s2 * cs) * ((gamma - 1.0) * q2 / 2. + cs * un);
+    L[4 * 5 + 1] = (-(gamma - 1.) * u / cs - nx) / s2;
+    L[4 * 5 + 2] = (-(gamma - 1.) * v / cs - ny) / s2;
+    L[4 * 5 + 3] = (-(gamma - 1.) * w / cs - nz) / s2;
+    L[4 * 5 + 4] = (gamma - 1.) / (s2 * cs);
+}
+
+/* Phi_Function_W5, src/WENO.c:118-204: Jiang-Shu WENO5 deviation from the central flux,
+ * smoothness indicators from the flux differences d[0..3] (scaled by 12), weights 1:6:3. */
+double orc_phi_w5(double eps, const double *f, const double *d)
+{
+    double is0, is1, is2, a0, a1, a2, w0, w2, q1, q2;
+    (void)f;
+    is0 = 13. * (d[0] - d[1]) * (d[0] - d[1]) + 3. * (d[0] - 3. * d[1]) * (d[0] - 3. * d[1]);
+    is1 = 13. * (d[1] - d[2]) * (d[1] - d[2]) + 3. * (d[1] + d[2]) * (d[1] + d[2]);
+    is2 = 13. * (d[2] - d[3]) * (d[2] - d[3]) + 3. * (3. * d[2] - d[3]) * (3. * d[2] - d[3]);
+    a0 = 1. / ((is0 + eps) * (is0 + eps));
+    a1 = 6. / ((is1 + eps) * (is1 + eps));
+    a2 = 3. / ((is2 + eps) * (is2 + eps));
+    w0 = a0 / (a0 + a1 + a2);
+    w2 = a2 / (a0 + a1 + a2);
+    q1 = d[0] - 2. * d[1] + d[2];
+    q2 = d[1] - 2. * d[2] + d[3];
+    return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * q2;
+}
+
+/* Phi_Function_W9, src/WENO.c:11-116: Balsara-Shu WENO9 smoothness indicators from the point
+ * fluxes f[0..8], optimal weights {1/126, 10/63, 10/21, 20/63, 5/126}, deviation from
+ * 4th differences of the flux differences d[0..7]. */
+double orc_phi_w9(double eps, const double *f, const double *d)
+{
+    static const double dk[5] = {1. / 126., 10. / 63., 10. / 21., 20. / 63., 5. / 126.};
+    double is0, is1, is2, is3, is4, a0, a1, a2, a3, a4, w0, w2, w3, w4, q1, q2, q3, q4;
+    is0 = f[0] * (22658. * f[0] - 208501. * f[1] + 364863. * f[2] - 288007. * f[3] + 86329. * f[4])
+        + f[1] * (482963. * f[1] - 1704396. * f[2] + 1358458. * f[3] - 411487. * f[4])
+        + f[2] * (1521393. * f[2] - 2462076. * f[3] + 758823. * f[4])
+        + f[3] * (1020563. * f[3] - 649501. * f[4])
+        + f[4] * (107918. * f[4]);
+    is1 = f[1] * (6908. * f[1] - 60871. * f[2] + 99213. * f[3] - 70237. * f[4] + 18079. * f[5])
+        + f[2] * (138563. * f[2] - 464976. * f[3] + 337018. * f[4] - 88297. * f[5])
+        + f[3] * (406293. * f[3] - 611976. * f[4] + 165153. * f[5])
+        + f[4] * (242723. * f[4] - 140251. * f[5])
+        + f[5] * (22658. * f[5]);
+    is2 = f[2] * (6908. * f[2] - 51001. * f[3] + 67923. * f[4] - 38947. * f[5] + 8209. * f[6])
+        + f[3] * (104963. * f[3] - 299076. * f[4] + 179098. * f[5] - 38947. * f[6])
+        + f[4] * (231153. * f[4] - 299076. * f[5] + 67923. * f[6])
+        + f[5] * (104963. * f[5] - 51001. * f[6])
+        + f[6] * (6908. * f[6]);
+    is3 = f[3] * (22658. * f[3] - 140251. * f[4] + 165153. * f[5] - 88297. * f[6] + 18079. * f[7])
+        + f[4] * (242723. * f[4] - 611976. * f[5] + 337018. * f[6] - 70237. * f[7])
+        + f[5] * (406293. * f[5] - 464976. * f[6] + 99213. * f[7])
+        + f[6] * (138563. * f[6] - 60871. * f[7])
+        + f[7] * (6908. * f[7]);
+    is4 = f[4] * (107918. * f[4] - 649501. * f[5] + 758823. * f[6] - 411487. * f[7] + 86329. * f[8])
+        + f[5] * (1020563. * f[5] - 2462076. * f[6] + 1358458. * f[7] - 288007. * f[8])
+        + f[6] * (1521393. * f[6] - 1704396. * f[7] + 364863. * f[8])
+        + f[7] * (482963. * f[7] - 208501. * f[8])
+        + f[8] * (22658. * f[8]);
+    a0 = dk[0] / ((is0 + eps) * (is0 + eps));
+    a1 = dk[1] / ((is1 + eps) * (is1 + eps));
+    a2 = dk[2] / ((is2 + eps) * (is2 + eps));
+    a3 = dk[3] / ((is3 + eps) * (is3 + eps));
+    a4 = dk[4] / ((is4 + eps) * (is4 + eps));
+    w0 = a0 / (a0 + a1 + a2 + a3 + a4);
+    w2 = a2 / (a0 + a1 + a2 + a3 + a4);
+    w3 = a3 / (a0 + a1 + a2 + a3 + a4);
+    w4 = a4 / (a0 + a1 + a2 + a3 + a4);
+    q1 = d[0] - 4. * d[1] + 6. * d[2] - 4. * d[3] + d[4];
+    q2 = d[1] - 4. * d[2] + 6. * d[3] - 4. * d[4] + d[5];
+    q3 = d[2] - 4. * d[3] + 6. * d[4] - 4. * d[5] + d[6];
+    q4 = d[3] - 4. * d[4] + 6. * d[5] - 4. * d[6] + d[7];
+    return 1. / 5. * w0 * q1 + 1. / 20. * w2 * q2 + 1. / 60. * w3 * (3. * q2 - 2. * q3)
+         + 1. / 60. * w4 * (3. * q2 - 2. * q3 + 3. * q4) - 1. / 840. * (39. * q2 - 14. * q3 + 3. * q4);
+}
+
+/* pointwise flux in direction dir (src/WENO.c:251-290, BC_Corrector = 1 away from walls) */
+static void point_flux(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                       double *const der[8], long n, double F[5])
+{
+    double J = met[9][n], rho = prim[0][n], th = der[4 + dir][n], p = der[0][n];
+    F[0] = J * (rho * th);
+    F[1] = J * (rho * th * prim[1][n] * 1.0 + met[3 * dir + 0][n] * p * c->upsilon);
+    F[2] = J * (rho * th * prim[2][n] * 1.0 + met[3 * dir + 1][n] * p * c->upsilon);
+    F[3] = J * (rho * th * prim[3][n] * 1.0 + met[3 * dir + 2][n] * p * c->upsilon);
+    F[4] = J * (rho * th * (prim[4][n] + p / rho * c->upsilon));
+}
+
+static double metric_norm(double *const met[10], int dir, long n)
+{
+    double a = met[3 * dir][n], b = met[3 * dir + 1][n], d = met[3 * dir + 2][n];
+    return sqrt(pow(a, 2) + pow(b, 2) + pow(d, 2.));
+}
+
+/* One interface i+1/2 along dir at node n (CalcFluxesIn*Direction inner body,
+ * src/WENO.c:296-464; GetLambdaMax* src/WENO.c:1550-1752; Theta_Function_W5/W9
+ * src/WENO.c:1754-1816).  F is the pointwise flux array set for this direction. */
+static void interface_flux(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                           double *const der[8], double *const F[5], long n, double out[5])
+{
+    long st = stride(c, dir);
+    long np1 = n + st;
+    int r = (c->S - 1) / 2, cc, m, s;
+    double Jh, lam = 0.0, nh[3], L[25], R[25];
+    double fp[9], fm[9], dp[8], dm[8], phis[5];
+    double u, v, w, rho, cs;
+
+    Jh = (met[9][n] + met[9][np1]) / 2.0;
+    u = 0.5 * (prim[1][n] + prim[1][np1]);
+    v = 0.5 * (prim[2][n] + prim[2][np1]);
+    w = 0.5 * (prim[3][n] + prim[3][np1]);
+    rho = 0.5 * (prim[0][n] + prim[0][np1]);
+    cs = 0.5 * (der[2][n] + der[2][np1]);
+    for (m = 0; m < 3; m++) {
+        nh[m] = 0.5 * (met[3 * dir + m][n] / metric_norm(met, dir, n) +
+                       met[3 * dir + m][np1] / metric_norm(met, dir, np1));
+    }
+    orc_eigvec(nh, u, v, w, rho, cs, c->gamma, L, R);
+
+    for (s = -r; s <= r + 1; s++) {
+        long ns = n + s * st;
+        double a = met[3 * dir][ns], b = met[3 * dir + 1][ns], d = met[3 * dir + 2][ns];
+        double l = fabs(der[4 + dir][ns]) + der[2][ns] * sqrt(a * a + b * b + d * d);
+        if (lam < l) lam = l;
+    }
+
+    for (cc = 0; cc < NEQ; cc++) {
+        for (s = 0; s < c->S; s++) {
+            long npl = n + (s - r) * st;          /* plus stencil i-r .. i+r      */
+            long nmi = n + (r + 1 - s) * st;      /* minus stencil i+r+1 .. i-r+1 */
+            fp[s] = 0.5 * (L[cc * 5 + 0] * (F[0][npl] + lam * prim[0][npl] * Jh)
+                         + L[cc * 5 + 1] * (F[1][npl] + lam * prim[0][npl] * prim[1][npl] * Jh)
+                         + L[cc * 5 + 2] * (F[2][npl] + lam * prim[0][npl] * prim[2][npl] * Jh)
+                         + L[cc * 5 + 3] * (F[3][npl] + lam * prim[0][npl] * prim[3][npl] * Jh)
+                         + L[cc * 5 + 4] * (F[4][npl] + lam * prim[0][npl] * prim[4][npl] * Jh));
+            fm[s] = 0.5 * (L[cc * 5 + 0] * (F[0][nmi] - lam * prim[0][nmi] * Jh)
+                         + L[cc * 5 + 1] * (F[1][nmi] - lam * prim[0][nmi] * prim[1][nmi] * Jh)
+                         + L[cc * 5 + 2] * (F[2][nmi] - lam * prim[0][nmi] * prim[2][nmi] * Jh)
+                         + L[cc * 5 + 3] * (F[3][nmi] - lam * prim[0][nmi] * prim[3][nmi] * Jh)
+                         + L[cc * 5 + 4] * (F[4][nmi] - lam * prim[0][nmi] * prim[4][nmi] * Jh));
+        }
+        for (s = 0; s <= c->S - 2; s++) {
+            dp[s] = fp[s + 1] - fp[s];
+            dm[s] = fm[s] - fm[s + 1];
+        }
+        if (c->S == 9)
+            phis[cc] = (-orc_phi_w9(c->eps, fp, dp) + orc_phi_w9(c->eps, fm, dm));
+        else
+            phis[cc] = (-orc_phi_w5(c->eps, fp, dp) + orc_phi_w5(c->eps, fm, dm));
+    }
+    for (m = 0; m < NEQ; m++) {
+        double pr = phis[0] * R[m * 5 + 0] + phis[1] * R[m * 5 + 1] + phis[2] * R[m * 5 + 2]
+                  + phis[3] * R[m * 5 + 3] + phis[4] * R[m * 5 + 4];
+        double th;
+        const double *Fm = F[m];
+        if (c->S == 9) {
+            th = 1. / 840. * (533. * (Fm[np1] + Fm[n]) - 139. * (Fm[n + 2 * st] + Fm[n - st])
+                              + 29. * (Fm[n + 3 * st] + Fm[n - 2 * st]) - 3. * (Fm[n + 4 * st] + Fm[n - 3 * st]));
+        } else {
+            th = 1. / 12. * (-1. * (Fm[n - st] + Fm[n + 2 * st]) + 7. * (Fm[n] + Fm[np1]));
+        }
+        out[m] = pr + th;
+    }
+}
+
+static void fill_point_flux(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                            double *const der[8], double *const F[5])
+{
+    int lo[3], hi[3], d, i, j, k;
+    for (d = 0; d < 3; d++) { lo[d] = rs(c, d); hi[d] = re(c, d); }
+    lo[dir] = 0; hi[dir] = ge(c, dir);
+    for (i = lo[0]; i <= hi[0]; i++)
+        for (j = lo[1]; j <= hi[1]; j++)
+            for (k = lo[2]; k <= hi[2]; k++) {
+                long n = orc_idx(c, i, j, k);
+                double Fv[5];
+                int m;
+                point_flux(c, dir, met, prim, der, n, Fv);
+                for (m = 0; m < 5; m++) F[m][n] = Fv[m];
+            }
+}
+
+static double **alloc_fields(const orc_cfg *c, int nf)
+{
+    double **f = (double **)malloc(sizeof(double *) * nf);
+    int m;
+    for (m = 0; m < nf; m++) f[m] = (double *)calloc((size_t)c->ntot, sizeof(double));
+    return f;
+}
+static void free_fields(double **f, int nf)
+{
+    int m;
+    for (m = 0; m < nf; m++) free(f[m]);
+    free(f);
+}
+
+/* CalcFluxesIn{Xi,Eta,Zeta}Direction, src/WENO.c:206-1083: interface fluxes for interfaces
+ * g-1 .. g+n-1 along dir (interior transverse), then Q -= F(i+1/2) - F(i-1/2). */
+void orc_weno_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                  double *const der[8], double *const q[5])
+{
+    double **F = alloc_fields(c, 5), **Fh = alloc_fields(c, 5);
+    int lo[3], hi[3], d, i, j, k, m;
+    long st = stride(c, dir);
+    fill_point_flux(c, dir, met, prim, der, F);
+    for (d = 0; d < 3; d++) { lo[d] = rs(c, d); hi[d] = re(c, d); }
+    lo[dir] = rs(c, dir) - 1;
+#pragma omp parallel for private(j, k, m) schedule(static)
+    for (i = lo[0]; i <= hi[0]; i++)
+        for (j = lo[1]; j <= hi[1]; j++)
+            for (k = lo[2]; k <= hi[2]; k++) {
+                long n = orc_idx(c, i, j, k);
+                double out[5];
+                interface_flux(c, dir, met, prim, der, F, n, out);
+                for (m = 0; m < 5; m++) Fh[m][n] = out[m];
+            }
+    lo[dir] = rs(c, dir);
+    for (i = lo[0]; i <= hi[0]; i++)
+        for (j = lo[1]; j <= hi[1]; j++)
+            for (k = lo[2]; k <= hi[2]; k++) {
+                long n = orc_idx(c, i, j, k);
+                for (m = 0; m < 5; m++) q[m][n] = q[m][n] - (Fh[m][n] - Fh[m][n - st]);
+            }
+    free_fields(F, 5);
+    free_fields(Fh, 5);
+}
+
+void orc_weno_interface(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                        double *const der[8], int i, int j, int k, double out[5])
+{
+    double **F = alloc_fields(c, 5);
+    fill_point_flux(c, dir, met, prim, der, F);
+    interface_flux(c, dir, met, prim, der, F, orc_idx(c, i, j, k), out);
+    free_fields(F, 5);
+}
+
+/* CreateViscidMetric, src/Functions.c:6245-6386: 30 factors of one node for direction dir.
+ * m = (xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z,J).  The table is the contraction
+ * of the Newtonian stress (4/3, 1, -2/3 weights) with metric rows a (= dir) and b in {xi,eta,zeta}. */
+void orc_visc_fac(int dir, const double m[10], double fac[30])
+{
+    const double J = m[9];
+    const double *a = &m[3 * dir];      /* own-direction row (ax, ay, az) */
+    int b, k;
+    /* For each component block (x: 0..8, y: 9..17, z: 18..26) and each gradient slot
+     * (u_b, v_b, w_b for b = xi, eta, zeta) the factor multiplies d(vel)/d(b). */
+    for (b = 0; b < 3; b++) {
+        const double *r = &m[3 * b];    /* row of direction b (rx, ry, rz) */
+        /* x-momentum block */
+        fac[0 + b]  = J * (4.0 / 3.0 * r[0] * a[0] + r[1] * a[1] + r[2] * a[2]);
+        fac[3 + b]  = J * (-2.0 / 3.0 * r[1] * a[0] + r[0] * a[1]);
+        fac[6 + b]  = J * (-2.0 / 3.0 * r[2] * a[0] + r[0] * a[2]);
+        /* y-momentum block */
+        fac[9 + b]  = J * (r[1] * a[0] - 2.0 / 3.0 * r[0] * a[1]);
+        fac[12 + b] = J * (r[0] * a[0] + 4.0 / 3.0 * r[1] * a[1] + r[2] * a[2]);
+        fac[15 + b] = J * (-2.0 / 3.0 * r[2] * a[1] + r[1] * a[2]);
+        /* z-momentum block */
+        fac[18 + b] = J * (r[2] * a[0] - 2.0 / 3.0 * r[0] * a[2]);
+        fac[21 + b] = J * (r[2] * a[1] - 2.0 / 3.0 * r[1] * a[2]);
+        fac[24 + b] = J * (r[0] * a[0] + r[1] * a[1] + 4.0 / 3.0 * r[2] * a[2]);
+        /* heat flux */
+        fac[27 + b] = J * (r[0] * a[0] + r[1] * a[1] + r[2] * a[2]);
+    }
+    (void)k;
+}
+
+/* CalcDeviationsForDirectViscidFluxes, src/ZD.c:697-810: nodal central derivatives of u,v,w,T
+ * along each direction; range: own direction interior, transverse directions all nodes. */
+void orc_visc_derivs(const orc_cfg *c, double *const prim[5], double *const der[8], double *const grad[12])
+{
+    int dir, lo[3], hi[3], d, i, j, k, l;
+    for (dir = 0; dir < c->dims; dir++) {
+        long st = stride(c, dir);
+        int half = (c->S + 1) / 2;
+        for (d = 0; d < 3; d++) { lo[d] = 0; hi[d] = ge(c, d); }
+        lo[dir] = rs(c, dir); hi[dir] = re(c, dir);
+        for (i = lo[0]; i <= hi[0]; i++)
+            for (j = lo[1]; j <= hi[1]; j++)
+                for (k = lo[2]; k <= hi[2]; k++) {
+                    long n = orc_idx(c, i, j, k);
+                    double du = 0.0, dv = 0.0, dw = 0.0, dT = 0.0;
+                    for (l = 0; l <= c->S + 1; l++) {
+                        long nl = n + (l - half) * st;
+                        du += c->zd_der[l] * prim[1][nl];
+                        dv += c->zd_der[l] * prim[2][nl];
+                        dT += c->zd_der[l] * der[1][nl];
+                        if (c->dims == 3) dw += c->zd_der[l] * prim[3][nl];
+                    }
+                    grad[0 + dir][n] = du;
+                    grad[3 + dir][n] = dv;
+                    if (c->dims == 3) grad[6 + dir][n] = dw;
+                    grad[9 + dir][n] = dT;
+                }
+    }
+}
+
+/* CalcViscidFluxesIn{Xi,Eta,Zeta}DirectionDirectly, src/ZD.c:47-694.  With
+ * ORC_QUIRK_ZETA_VISC_J the zeta stencil's k offset is taken from j (src/ZD.c:547-548). */
+void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                  double *const der[8], double *const grad[12], double *const q[5])
+{
+    int i, j, k, l, s, b;
+    int half = (c->S + 1) / 2;
+    long st = stride(c, dir);
+    int three = (c->dims == 3);
+#pragma omp parallel for private(j, k, l, s, b) schedule(static)
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                /* d[side][slot]: slot 0..8 = u_b,v_b,w_b blocks (u_xi,u_eta,u_zeta,v_xi,...), 9..11 = T_b */
+                double dm[12], dp[12], Km[30], Kp[30], fm[30], f0[30], fp[30], mv[10];
+                long nm = n - st, np = n + st;
+                for (s = 0; s < 12; s++) { dm[s] = 0.; dp[s] = 0.; }
+                for (l = 0; l <= c->S; l++) {
+                    long im, ip;
+                    if (dir == 2 && (c->quirks & ORC_QUIRK_ZETA_VISC_J)) {
+                        long base = ((long)i * c->ng[1] + j) * c->ng[2];
+                        im = base + (j + l - half);
+                        ip = base + (j + l - half) + 1;
+                    } else {
+                        im = n + (l - half) * st;
+                        ip = n + (l - half + 1) * st;
+                    }
+                    /* own-direction midpoint derivatives of u, v, (w), T */
+                    dm[0 * 3 + dir] += c->zd_mid[l] * prim[1][im]; dp[0 * 3 + dir] += c->zd_mid[l] * prim[1][ip];
+                    dm[1 * 3 + dir] += c->zd_mid[l] * prim[2][im]; dp[1 * 3 + dir] += c->zd_mid[l] * prim[2][ip];
+                    if (three) { dm[2 * 3 + dir] += c->zd_mid[l] * prim[3][im]; dp[2 * 3 + dir] += c->zd_mid[l] * prim[3][ip]; }
+                    dm[9 + dir] += c->zd_mid[l] * der[1][im]; dp[9 + dir] += c->zd_mid[l] * der[1][ip];
+                    /* cross derivatives interpolated to the midpoints */
+                    for (b = 0; b < 3; b++) {
+                        if (b == dir) continue;
+                        if (b == 2 && !three) continue;
+                        dm[0 * 3 + b] += c->zd_int[l] * grad[0 * 3 + b][im]; dp[0 * 3 + b] += c->zd_int[l] * grad[0 * 3 + b][ip];
+                        dm[1 * 3 + b] += c->zd_int[l] * grad[1 * 3 + b][im]; dp[1 * 3 + b] += c->zd_int[l] * grad[1 * 3 + b][ip];
+                        if (three) { dm[2 * 3 + b] += c->zd_int[l] * grad[2 * 3 + b][im]; dp[2 * 3 + b] += c->zd_int[l] * grad[2 * 3 + b][ip]; }
+                        dm[9 + b] += c->zd_int[l] * grad[9 + b][im]; dp[9 + b] += c->zd_int[l] * grad[9 + b][ip];
+                    }
+                }
+                for (s = 0; s < 10; s++) mv[s] = met[s][nm];
+                orc_visc_fac(dir, mv, fm);
+                for (s = 0; s < 10; s++) mv[s] = met[s][n];
+                orc_visc_fac(dir, mv, f0);
+                for (s = 0; s < 10; s++) mv[s] = met[s][np];
+                orc_visc_fac(dir, mv, fp);
+                for (l = 0; l < 30; l++) {
+                    Km[l] = 0.5 * (der[3][nm] * fm[l] + der[3][n] * f0[l]);
+                    Kp[l] = 0.5 * (der[3][n] * f0[l] + der[3][np] * fp[l]);
+                }
+                {
+                    double mom[3], heat, prod, work;
+                    int blk;
+                    for (blk = 0; blk < 3; blk++) {
+                        double acc = dp[0] * Kp[blk * 9 + 0] - dm[0] * Km[blk * 9 + 0];
+                        for (s = 1; s < 9; s++) acc = acc + dp[s] * Kp[blk * 9 + s] - dm[s] * Km[blk * 9 + s];
+                        mom[blk] = acc;
+                    }
+                    q[1][n] = q[1][n] + c->psi * mom[0];
+                    q[2][n] = q[2][n] + c->psi * mom[1];
+                    if (three) q[3][n] = q[3][n] + c->psi * mom[2];
+                    heat = dp[9] * Kp[27] - dm[9] * Km[27] + dp[10] * Kp[28] - dm[10] * Km[28]
+                         + dp[11] * Kp[29] - dm[11] * Km[29];
+                    /* u*(block x) + v*(block y) + w*(block z), each term separately as in src/ZD.c:200-229 */
+                    work = 0.0;
+                    {
+                        double vel[3] = {prim[1][n], prim[2][n], prim[3][n]};
+                        int first = 1;
+                        for (blk = 0; blk < 3; blk++)
+                            for (s = 0; s < 9; s++) {
+                                double t = vel[blk] * (dp[s] * Kp[blk * 9 + s] - dm[s] * Km[blk * 9 + s]);
+                                work = first ? t : work + t;
+                                first = 0;
+                            }
+                    }
+                    /* nodal product term: d(vel_blk)/d(dir) * grad_s * mu * fac */
+                    prod = 0.0;
+                    {
+                        int first = 1;
+                        for (blk = 0; blk < 3; blk++) {
+                            double gd = grad[blk * 3 + dir][n];
+                            for (s = 0; s < 9; s++) {
+                                double t = gd * grad[s][n] * der[3][n] * f0[blk * 9 + s];
+                                prod = first ? t : prod + t;
+                                first = 0;
+                            }
+                        }
+                    }
+                    q[4][n] = q[4][n] + c->gam_heat * heat + c->psi * (work + prod);
+                }
+            }
+}
+
+void orc_zero(const orc_cfg *c, double *const f[], int nf)
+{
+    int m;
+    for (m = 0; m < nf; m++) memset(f[m], 0, sizeof(double) * (size_t)c->ntot);
+}
+
+/* Periodic ghost fill: the fixed point of the reference's ghost-inclusive halo passes
+ * (TransferFlowParameterWithGhosts, src/Functions.c:3580-3684, repeated 1-3 times per stage,
+ * src/Functions.c:2117-2136) when every face is a periodic self-interface: each ghost node takes
+ * the value of the interior node at the wrapped index in every direction. */
+void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf)
+{
+    int i, j, k, m, dd;
+    for (i = 0; i < c->ng[0]; i++)
+        for (j = 0; j < c->ng[1]; j++)
+            for (k = 0; k < c->ng[2]; k++) {
+                int idx[3] = {i, j, k}, src[3], ghost = 0;
+                for (dd = 0; dd < 3; dd++) {
+                    src[dd] = idx[dd];
+                    if (dd == 2 && c->dims == 2) continue;
+                    if (idx[dd] < c->g) { src[dd] = idx[dd] + c->n[dd]; ghost = 1; }
+                    else if (idx[dd] > c->g + c->n[dd] - 1) { src[dd] = idx[dd] - c->n[dd]; ghost = 1; }
+                }
+                if (ghost) {
+                    long to = orc_idx(c, i, j, k), from = orc_idx(c, src[0], src[1], src[2]);
+                    for (m = 0; m < nf; m++) f[m][to] = f[m][from];
+                }
+            }
+}
+
+/* RK stage update over interior nodes, src/Functions.c:2300-2365 (in place on urk). */
+void orc_rk_update(const orc_cfg *c, int s, double tau, double *const met[10],
+                   double *const un[5], double *const urk[5], double *const q[5], double *const qsum[5])
+{
+    int i, j, k, m;
+    double a = c->rk_a[s], b = c->rk_b[s], qf = c->rk_q[s], ff = c->rk_f[s], mf = c->rk_m[s];
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                double rho_old, J = met[9][n];
+                for (m = 0; m < 5; m++) qsum[m][n] = qsum[m][n] + mf * q[m][n];
+                rho_old = urk[0][n];
+                urk[0][n] = a * un[0][n] + b * urk[0][n] + tau / J * (qf * q[0][n] + ff * qsum[0][n]);
+                for (m = 1; m < 5; m++) {
+                    if (m == 3 && c->dims == 2) { urk[3][n] = 0.0; continue; }
+                    urk[m][n] = (a * un[m][n] * un[0][n] + b * urk[m][n] * rho_old
+                                 + tau / J * (qf * q[m][n] + ff * qsum[m][n])) / urk[0][n];
+                }
+            }
+}
+
+/* AddManufacturedSolutionSource, src/ManufacturedSolution.c:25-175: Q += J * source. */
+void orc_add_source(const orc_cfg *c, double *const met[10], double *const src[5], double *const q[5])
+{
+    int i, j, k, m;
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                for (m = 0; m < 5; m++) {
+                    if (m == 3 && c->dims == 2) continue;
+                    q[m][n] += met[9][n] * src[m][n];
+                }
+            }
+}
+
+void orc_rhs(const orc_cfg *c, double *const met[10], double *const prim[5],
+             double *const scratch[30], double *const src[5])
+{
+    double *const *der = scratch, *const *grad = scratch + 8, *const *q = scratch + 20;
+    int d;
+    orc_zero(c, q, 5);
+    orc_zero(c, grad, 12);
+    orc_ghost_periodic(c, prim, 5);
+    orc_calc_values(c, met, prim, der);
+    for (d = 0; d < c->dims; d++) orc_weno_dir(c, d, met, prim, der, q);
+    if (!c->inviscid) {
+        orc_visc_derivs(c, prim, der, grad);
+        for (d = 0; d < c->dims; d++) orc_visc_dir(c, d, met, prim, der, grad, q);
+    }
+    if (src) orc_add_source(c, met, src, q);
+}
+
+/* CalcRungeKutta, src/Functions.c:2034-2390, for an all-periodic single block. */
+void orc_step(const orc_cfg *c, double tau, double *const met[10], double *const un[5],
+              double *const urk[5], double *const scratch[30], double *const src[5])
+{
+    double *const *q = scratch + 20, *const *qsum = scratch + 25;
+    int s;
+    orc_zero(c, qsum, 5);
+    for (s = 0; s < c->time_order; s++) {
+        orc_rhs(c, met, urk, scratch, src);
+        orc_rk_update(c, s, tau, met, un, urk, q, qsum);
+    }
+}
+
+/* checkNAN, src/Functions.c:5419-5467: count non-finite interior rho, overwrite with 1e6. */
+int orc_check_nan(const orc_cfg *c, double *const prim[5])
+{
+    int i, j, k, cnt = 0;
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                if (isinf(prim[0][n]) || isnan(prim[0][n])) {
+                    cnt++;
+                    prim[1][n] = prim[2][n] = prim[3][n] = prim[0][n] = 1000000;
+                }
+            }
+    return cnt;
+}
+
+/* CFL time step (absent in the reference, which runs with a fixed tau, src/Import.c:49):
+ * dt = CFL / max_interior sum_d (|theta_d| + c |grad xi_d|), the per-node spectral radius of
+ * GetLambdaMax* (src/WENO.c:1560-1615) summed over directions, unit index spacing. */
+double orc_cfl_dt(const orc_cfg *c, double cfl, double *const met[10], double *const prim[5])
+{
+    int i, j, k, d;
+    double mx = 0.0;
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                double rho = prim[0][n], u = prim[1][n], v = prim[2][n], w = prim[3][n], e = prim[4][n];
+                double p = fabs(rho * (c->gamma - 1) / c->upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
+                double cs = sqrt(c->upsilon * c->gamma * p / rho), sum = 0.0;
+                for (d = 0; d < c->dims; d++) {
+                    double a = met[3 * d][n], b = met[3 * d + 1][n], z = met[3 * d + 2][n];
+                    double th = u * a + v * b + w * z;
+                    sum += fabs(th) + cs * sqrt(a * a + b * b + z * z);
+                }
+                if (sum > mx) mx = sum;
+            }
+    return mx > 0.0 ? cfl / mx : 0.0;
+}
+
+/* CreateMetric, src/Functions.c:6388-6509 (2nd-order central differences in index space on the
+ * extended coordinate arrays; 3D off-diagonal terms zeroed under ORC_QUIRK_METRIC_DIAG3D). */
+void orc_metric_from_coords(const orc_cfg *c, const double *xe, const double *ye,
+                            const double *ze, double *const met[10])
+{
+    int ne1 = c->ng[1] + 2, ne2 = (c->dims == 3) ? c->ng[2] + 2 : 1;
+    int i, j, k;
+    for (i = 0; i < c->ng[0]; i++)
+        for (j = 0; j < c->ng[1]; j++)
+            for (k = 0; k < c->ng[2]; k++) {
+                int ie = i + 1, je = j + 1, ke = (c->dims == 3) ? k + 1 : 0;
+                long ce = ((long)ie * ne1 + je) * ne2 + ke;
+                long ip = ce + (long)ne1 * ne2, im = ce - (long)ne1 * ne2;
+                long jp = ce + ne2, jm = ce - ne2, kp = ce + 1, km = ce - 1;
+                double x_xi, y_xi, z_xi, x_eta, y_eta, z_eta, x_zeta, y_zeta, z_zeta, J;
+                long n = orc_idx(c, i, j, k);
+                if (c->dims == 2) {
+                    x_xi = (-0.5 * xe[im] + 0.5 * xe[ip]);
+                    y_xi = (-0.5 * ye[im] + 0.5 * ye[ip]);
+                    z_xi = 0.0;
+                    x_eta = (-0.5 * xe[jm] + 0.5 * xe[jp]);
+                    y_eta = (-0.5 * ye[jm] + 0.5 * ye[jp]);
+                    z_eta = 0.0;
+                    x_zeta = 0.0; y_zeta = 0.0; z_zeta = 1.0;
+                } else {
+                    x_xi = (-0.5 * xe[im] + 0.5 * xe[ip]);
+                    y_xi = (-0.5 * ye[im] + 0.5 * ye[ip]);
+                    z_xi = (-0.5 * ze[im] + 0.5 * ze[ip]);
+                    x_eta = (-0.5 * xe[jm] + 0.5 * xe[jp]);
+                    y_eta = (-0.5 * ye[jm] + 0.5 * ye[jp]);
+                    z_eta = (-0.5 * ze[jm] + 0.5 * ze[jp]);
+                    x_zeta = (-0.5 * xe[km] + 0.5 * xe[kp]);
+                    y_zeta = (-0.5 * ye[km] + 0.5 * ye[kp]);
+                    z_zeta = (-0.5 * ze[km] + 0.5 * ze[kp]);
+                    if (c->quirks & ORC_QUIRK_METRIC_DIAG3D) {
+                        y_xi = 0.0; z_xi = 0.0; x_eta = 0.0; z_eta = 0.0; x_zeta = 0.0; y_zeta = 0.0;
+                    }
+                }
+                J = x_xi * y_eta * z_zeta + x_eta * y_zeta * z_xi + x_zeta * y_xi * z_eta
+                  - x_zeta * y_eta * z_xi - x_eta * y_xi * z_zeta - x_xi * y_zeta * z_eta;
+                met[9][n] = J;
+                met[0][n] = (y_eta * z_zeta - y_zeta * z_eta) / J;
+                met[1][n] = (x_zeta * z_eta - x_eta * z_zeta) / J;
+                met[2][n] = (x_eta * y_zeta - x_zeta * y_eta) / J;
+                met[3][n] = (y_zeta * z_xi - y_xi * z_zeta) / J;
+                met[4][n] = (x_xi * z_zeta - x_zeta * z_xi) / J;
+                met[5][n] = (x_zeta * y_xi - x_xi * y_zeta) / J;
+                met[6][n] = (y_xi * z_eta - y_eta * z_xi) / J;
+                met[7][n] = (x_eta * z_xi - x_xi * z_eta) / J;
+                met[8][n] = (x_xi * y_eta - x_eta * y_xi) / J;
+            }
+}

@@ -1,0 +1,126 @@
+/*
+ * shx_oracle.h -- CPU oracle for the SHOCK RHS / RK-stage hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the
+ * reference algorithm (mag1000/SHOCK, src/WENO.c, src/ZD.c, src/Functions.c,
+ * src/BC.c) used as the parity checker for the HIP library.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product path (shock_amd/, libshockx.so) never links, loads or calls it.
+ *
+ * Layout follows the reference exactly: ghost-padded SoA arrays of doubles,
+ * linear index ijk = i*ng[1]*ng[2] + j*ng[2] + k (k fastest), g = (S+1)/2
+ * ghost layers, interior i in [g, g+n-1] (src/Functions.c:56-103).  In 2D
+ * ng[2] = 1 and k = 0.
+ *
+ * Pinning status per function (how each one is checked, tests/test_oracle_*.py):
+ *   orc_define            rationals vs exact Taylor-moment solutions; RK tableaux vs textbook
+ *   orc_calc_values       closed form (ideal gas, Sutherland)
+ *   orc_eigvec            L*R = I and R*diag(lambda)*L = flux Jacobian (exact algebra)
+ *   orc_phi_w5/w9         eps -> inf reduces to the linear 5th/9th-order upwind flux
+ *                         (coefficients derived by exact rational solve); smoothness
+ *                         indicators vs Jiang-Shu / Balsara-Shu integrals (sympy)
+ *   orc_weno_dir          free-stream preservation, telescoping conservation,
+ *                         mirror symmetry, order of accuracy on smooth data
+ *   orc_visc_*            polynomial exactness vs analytic Navier-Stokes viscous
+ *                         terms, order of accuracy
+ *   orc_rk_*              closed form (linear ODE per node reproduces RK3/RK4 polynomials)
+ *   orc_metric_*          exact on affine grids; 2nd-order central differences
+ *   orc_ghost_periodic    permutation identity
+ *   orc_cfl_dt, orc_check_nan   brute force
+ * The reference itself cannot be compiled here (it needs mpi.h and cgnslib.h,
+ * which the image lacks), so no compiled-reference fixture exists.
+ */
+#ifndef SHX_ORACLE_H
+#define SHX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* quirk bits (reference behaviours that a "fixed" mode may switch off) */
+#define ORC_QUIRK_ZETA_VISC_J   1u  /* src/ZD.c:547-548: zeta viscous stencil offset uses j, not k */
+#define ORC_QUIRK_METRIC_DIAG3D 2u  /* src/Functions.c:6479-6484: 3D metric drops off-diagonal terms */
+#define ORC_QUIRKS_REFERENCE (ORC_QUIRK_ZETA_VISC_J | ORC_QUIRK_METRIC_DIAG3D)
+
+/* boundary kinds per face */
+#define ORC_BC_PERIODIC 0
+
+typedef struct orc_cfg {
+    /* inputs */
+    int dims;          /* 2 or 3 (MESHDIMENSIONS) */
+    int S;             /* 5 or 9 (SPACEORDER) */
+    int n[3];          /* interior points per direction (n[2] = 1 in 2D) */
+    int time_order;    /* 3 (SSP-RK3) or 4 (classic RK4) */
+    int inviscid;      /* options:Inviscid */
+    int mms;           /* options:ManufacturedSolution: mu = 1, sources added */
+    unsigned quirks;   /* ORC_QUIRK_* */
+    double gamma, mach, reynolds, prandtl, T0;
+    double eps_override; /* 0 -> reference epsilon 10*sqrt(DBL_MIN) */
+    /* derived by orc_define */
+    int g;
+    int ng[3];
+    long ntot;
+    double upsilon, psi, gam_heat, suth, eps;
+    double rk_a[4], rk_b[4], rk_q[4], rk_f[4], rk_m[4];
+    double zd_mid[10];   /* midpoint-derivative coefficients, S+1 used */
+    double zd_int[10];   /* midpoint-interpolation coefficients, S+1 used */
+    double zd_der[11];   /* nodal first-derivative coefficients, S+2 used */
+} orc_cfg;
+
+/* Field bundles are arrays of padded FP64 arrays:
+ *   prim[5]    = rho, u, v, w, e          (stored state, strct_U)
+ *   der[8]     = p, T, c, mu, theta1, theta2, theta3, (unused)
+ *   met[10]    = xi_x, xi_y, xi_z, eta_x, eta_y, eta_z, zeta_x, zeta_y, zeta_z, J
+ *   q[5]       = Mass, xiMomentum, etaMomentum, zetaMomentum, Energy (strct_Flux)
+ *   grad[12]   = u_xi,u_eta,u_zeta, v_xi,v_eta,v_zeta, w_xi,w_eta,w_zeta, T_xi,T_eta,T_zeta
+ */
+
+int  orc_define(orc_cfg *c);
+long orc_idx(const orc_cfg *c, int i, int j, int k);
+
+void orc_calc_values(const orc_cfg *c, double *const met[10], double *const prim[5], double *const der[8]);
+
+void orc_eigvec(const double nh[3], double u, double v, double w, double rho, double cs,
+                double gamma, double L[25], double R[25]);
+double orc_phi_w5(double eps, const double *f, const double *d);
+double orc_phi_w9(double eps, const double *f, const double *d);
+
+void orc_weno_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                  double *const der[8], double *const q[5]);
+/* single-interface probe used by the tests: interface between node (i,j,k) and its +1 neighbour */
+void orc_weno_interface(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                        double *const der[8], int i, int j, int k, double out[5]);
+
+void orc_visc_fac(int dir, const double m[10], double fac[30]);
+void orc_visc_derivs(const orc_cfg *c, double *const prim[5], double *const der[8], double *const grad[12]);
+void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                  double *const der[8], double *const grad[12], double *const q[5]);
+
+void orc_zero(const orc_cfg *c, double *const f[], int nf);
+void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);
+void orc_rk_update(const orc_cfg *c, int s, double tau, double *const met[10],
+                   double *const un[5], double *const urk[5], double *const q[5], double *const qsum[5]);
+void orc_add_source(const orc_cfg *c, double *const met[10], double *const src[5], double *const q[5]);
+
+/* One full time step (CalcRungeKutta, src/Functions.c:2034-2390) for an all-periodic block.
+ * scratch: der[8] + grad[12] + q[5] + qsum[5] = 30 padded arrays supplied by the caller.
+ * src may be NULL (no manufactured source). */
+void orc_step(const orc_cfg *c, double tau, double *const met[10], double *const un[5],
+              double *const urk[5], double *const scratch[30], double *const src[5]);
+
+/* Full RHS of one stage without the update: Q for state prim (ghosts refilled periodically). */
+void orc_rhs(const orc_cfg *c, double *const met[10], double *const prim[5],
+             double *const scratch[30], double *const src[5]);
+
+int    orc_check_nan(const orc_cfg *c, double *const prim[5]);
+double orc_cfl_dt(const orc_cfg *c, double cfl, double *const met[10], double *const prim[5]);
+
+/* Metric from the extended coordinate arrays (extent ng+2 per direction, 1 in 2D z),
+ * CreateMetric src/Functions.c:6388-6509.  Writes all padded nodes. */
+void orc_metric_from_coords(const orc_cfg *c, const double *xe, const double *ye,
+                            const double *ze, double *const met[10]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

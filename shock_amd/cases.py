@@ -1,0 +1,145 @@
+"""Case builders: grids and initial states for the SHOCK RHS / RK path.
+
+Host-side setup only (numpy); the time-stepping path runs on the GPU through
+libshockx.so.  Arrays use the reference's ghost-padded k-fastest layout
+(src/Functions.c:56-103): shape (ng0, ng1, ng2), ng = n + S + 1, g = (S+1)/2
+ghost layers, interior indices [g, g+n-1].  In 2D ng2 = 1.
+
+Non-dimensionalisation follows SHOCK (docs/goveq.tex:143-163): Upsilon =
+1/(gamma Ma^2), e = |u|^2/2 + Upsilon p / ((gamma-1) rho), c^2 = Upsilon gamma p / rho.
+Inputs quoted in acoustic units (c = 1 at p = 1/gamma, rho = 1) map to SHOCK units
+by u_S = u / Ma and p_S = gamma p.
+"""
+import numpy as np
+
+
+def padded_shape(dims, S, n):
+    g = (S + 1) // 2
+    ng = [n[0] + S + 1, n[1] + S + 1, (n[2] + S + 1) if dims == 3 else 1]
+    return tuple(ng), g
+
+
+def _wrap(ip, g, n):
+    """interior source index and period shift for padded index ip (periodic)."""
+    rel = ip - g
+    shift = np.floor_divide(rel, n)
+    return g + rel - shift * n, shift
+
+
+def periodic_ext_coords(dims, S, n, mapping, period=(1.0, 1.0, 1.0)):
+    """Extended coordinate arrays (one extra layer beyond the ghosts, as the reference's
+    x_extrapolate, src/Functions.c:6388-6420) for a block periodic in every direction.
+
+    mapping(xi, eta, zeta) -> (x, y, z) gives the interior coordinates with xi = i/n.
+    Ghost and extra-layer nodes copy the wrapped interior node plus the period times the
+    wrap count, which is what the periodic mesh transfer produces
+    (WriteValuesFromBufferToMesh, src/Functions.c:2657-2661).
+    """
+    ng, g = padded_shape(dims, S, n)
+    ext = [ng[0] + 2, ng[1] + 2, (ng[2] + 2) if dims == 3 else 1]
+    nn = [n[0], n[1], n[2] if dims == 3 else 1]
+    # interior coordinates
+    ii = [np.arange(nn[d]) / nn[d] for d in range(3)]
+    if dims == 2:
+        ii[2] = np.zeros(1)
+    XI, ETA, ZETA = np.meshgrid(ii[0], ii[1], ii[2], indexing="ij")
+    X, Y, Z = mapping(XI, ETA, ZETA)
+    X, Y, Z = [np.broadcast_to(a, XI.shape).astype(np.float64) for a in (X, Y, Z)]
+    # index maps for the extended arrays: extended index e <-> padded index e-1
+    src = []
+    shifts = []
+    for d in range(3):
+        if d == 2 and dims == 2:
+            src.append(np.zeros(1, dtype=int))
+            shifts.append(np.zeros(1))
+            continue
+        pe = np.arange(ext[d]) - 1
+        s_idx, sh = _wrap(pe, g, nn[d])
+        src.append(s_idx - g)
+        shifts.append(sh.astype(np.float64))
+    I, J, K = np.meshgrid(src[0], src[1], src[2], indexing="ij")
+    SI, SJ, SK = np.meshgrid(shifts[0], shifts[1], shifts[2], indexing="ij")
+    xe = np.ascontiguousarray(X[I, J, K] + SI * period[0])
+    ye = np.ascontiguousarray(Y[I, J, K] + SJ * period[1])
+    ze = np.ascontiguousarray(Z[I, J, K] + SK * period[2]) if dims == 3 else np.zeros(xe.shape)
+    return xe, ye, ze
+
+
+def interior_coords(dims, S, n, xe, ye, ze):
+    """Padded node coordinates (without the extra layer) from the extended arrays."""
+    if dims == 3:
+        return xe[1:-1, 1:-1, 1:-1], ye[1:-1, 1:-1, 1:-1], ze[1:-1, 1:-1, 1:-1]
+    return xe[1:-1, 1:-1, :], ye[1:-1, 1:-1, :], ze[1:-1, 1:-1, :]
+
+
+def cartesian_unit(dims):
+    if dims == 2:
+        return lambda xi, eta, zeta: (xi, eta, np.zeros_like(xi))
+    return lambda xi, eta, zeta: (xi, eta, zeta)
+
+
+def sine_bump_3d(amp=0.05):
+    """x = xi + amp sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta), y = eta, z = zeta."""
+    tp = 2.0 * np.pi
+    return lambda xi, eta, zeta: (xi + amp * np.sin(tp * xi) * np.sin(tp * eta) * np.sin(tp * zeta), eta, zeta)
+
+
+def energy(u, v, w, p, rho, gamma, mach):
+    ups = 1.0 / (gamma * (mach * mach))
+    return 0.5 * (u * u + v * v + w * w) + ups * p / ((gamma - 1.0) * rho)
+
+
+def mms_periodic_2d_state(x, y, gamma=1.4, mach=0.5):
+    """Manufactured field on the unit square (acoustic units), mapped to SHOCK units:
+    rho = 1 + 0.1 sin(2pi x) sin(2pi y), u = 0.5 + 0.1 cos(2pi x) sin(2pi y),
+    v = 0.1 sin(2pi x) cos(2pi y), p = 1/gamma + 0.05 cos(2pi (x + y))."""
+    tp = 2.0 * np.pi
+    rho = 1.0 + 0.1 * np.sin(tp * x) * np.sin(tp * y)
+    u = (0.5 + 0.1 * np.cos(tp * x) * np.sin(tp * y)) / mach
+    v = (0.1 * np.sin(tp * x) * np.cos(tp * y)) / mach
+    w = np.zeros_like(x)
+    p = gamma * (1.0 / gamma + 0.05 * np.cos(tp * (x + y)))
+    e = energy(u, v, w, p, rho, gamma, mach)
+    return [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w, e)]
+
+
+def random_phase_velocity(n, seed, kpeak=4.0, mt=0.5, c=1.0):
+    """Solenoidal random-phase velocity field on an n^3 periodic box with energy spectrum
+    E(k) ~ k^4 exp(-2 (k/kpeak)^2), scaled so that sqrt(<|u|^2>) = mt * c."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz1 = np.fft.rfftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(k1, k1, kz1, indexing="ij")
+    kk = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    kk_safe = np.where(kk == 0.0, 1.0, kk)
+    spec = kk_safe ** 4 * np.exp(-2.0 * (kk_safe / kpeak) ** 2)
+    amp = np.sqrt(spec / (4.0 * np.pi * kk_safe * kk_safe))
+    amp[kk == 0.0] = 0.0
+    uh = []
+    for _ in range(3):
+        ph = rng.uniform(0.0, 2.0 * np.pi, size=kk.shape)
+        uh.append(amp * np.exp(1j * ph))
+    # solenoidal projection: u_hat -= k (k . u_hat) / |k|^2
+    kdotu = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / (kk_safe * kk_safe)
+    uh = [uh[0] - KX * kdotu, uh[1] - KY * kdotu, uh[2] - KZ * kdotu]
+    vel = [np.fft.irfftn(a, s=(n, n, n), axes=(0, 1, 2)) for a in uh]
+    rms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    scale = mt * c / rms
+    return [v * scale for v in vel]
+
+
+def turbulence_3d_state(dims, S, n, seed, gamma=1.4, mach=0.5, mt=0.5):
+    """Decaying compressible turbulence: uniform rho = 1, p = 1 (SHOCK units), random-phase
+    solenoidal velocity with turbulent Mach number mt (c = 1/mach in SHOCK units)."""
+    ng, g = padded_shape(dims, S, n)
+    c = 1.0 / mach
+    u, v, w = random_phase_velocity(n[0], seed, mt=mt, c=c)
+    prim = [np.zeros(ng) for _ in range(5)]
+    sl = (slice(g, g + n[0]), slice(g, g + n[1]), slice(g, g + n[2]))
+    prim[0][...] = 1.0
+    prim[1][sl] = u
+    prim[2][sl] = v
+    prim[3][sl] = w
+    p = 1.0
+    prim[4][...] = energy(prim[1], prim[2], prim[3], p, prim[0], gamma, mach)
+    return prim

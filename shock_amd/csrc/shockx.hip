@@ -1,0 +1,1128 @@
+// shockx.hip -- libshockx.so: gfx950 kernels and the C ABI of the SHOCK RHS / RK-stage path.
+//
+// Stage pipeline (one RK stage, FP64 form; src/Functions.c:2107-2365):
+//   k_ghost_periodic   on-device ghost fill of rho,u,v,w,e (replaces the MPI self-interface
+//                      halo, src/Functions.c:3580-3684)
+//   k_calc_values      T and mu on every padded node (CalcValues, src/Functions.c:2392-2481;
+//                      p, c and theta are recomputed inside the flux kernels instead of stored)
+//   k_weno<S>          one launch per direction: LDS-staged pencil, characteristic WENO
+//                      interface fluxes and their divergence fused, interface fluxes never
+//                      reach HBM (CalcFluxesIn*Direction, src/WENO.c:206-1083)
+//   k_visc_grad        nodal derivatives of u,v,w,T (src/ZD.c:697-810)
+//   k_visc_rk          viscous fluxes of all directions + source + RK update
+//                      (src/ZD.c:47-694, src/Functions.c:2287-2365), writes the next stage
+//                      state into a second buffer (the reference updates in place after all
+//                      fluxes are done, which double buffering reproduces exactly)
+//   k_rk               source + RK update for inviscid runs
+#include "shockx_kernels.h"
+#include "../../include/shockx.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+using namespace shx;
+
+// ----------------------------------------------------------------------------------------
+// kernels
+// ----------------------------------------------------------------------------------------
+
+// Periodic ghost fill of nf fields: every ghost node copies the interior node at the wrapped
+// index in each direction (fixed point of the ghost-inclusive halo passes,
+// src/Functions.c:2117-2136).
+__global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gm.ntot) return;
+    int idx[3];
+    idx[2] = (int)(n % gm.ng[2]);
+    const long r = n / gm.ng[2];
+    idx[1] = (int)(r % gm.ng[1]);
+    idx[0] = (int)(r / gm.ng[1]);
+    bool ghost = false;
+    long src = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int s = idx[d];
+        if (gm.ng[d] > 1) {
+            if (s < gm.lo[d]) { s += gm.n[d]; ghost = true; }
+            else if (s >= gm.lo[d] + gm.n[d]) { s -= gm.n[d]; ghost = true; }
+        }
+        src += (long)s * gm.st[d];
+    }
+    if (!ghost) return;
+#pragma unroll
+    for (int m = 0; m < 10; m++)
+        if (m < nf) f.f[m][n] = f.f[m][src];
+}
+
+// T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
+__global__ void k_calc_values(CFields5 pr, double *T, double *mu, long ntot, Consts k)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= ntot) return;
+    const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
+    const double p = pressure(k, rho, u, v, w, e);
+    T[n] = fabs(p / rho);
+    mu[n] = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+}
+
+// ---- WENO sweep ---------------------------------------------------------------------------
+struct WenoArgs {
+    CFields5 pr;          // rho, u, v, w, e of the stage input (ghosts filled)
+    const double *m0, *m1, *m2, *J;   // metric row of the sweep direction and the Jacobian
+    Fields5 q;            // stage residual, interior nodes
+    long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
+    int nD, nA, nB;       // interior counts
+    int oD, oA, oB;       // padded index of the first interior node per axis
+    int ngD;              // padded extent along the sweep
+    int first;            // 1: Q = -div (first direction of the stage), 0: Q -= div
+    int coalesce_d;       // load mapping: 1 when the sweep axis is contiguous in memory
+    Consts k;
+};
+
+// per-node values staged in LDS
+enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V_V, V_W, V_C,
+       V_NX, V_NY, V_NZ, V_J, NV };
+
+template <int S, int TA, int TDI>
+__global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
+{
+    constexpr int G = (S + 1) / 2;
+    constexpr int R = G - 1;
+    constexpr int WD = TDI - 1 + 2 * G;   // window nodes along the sweep
+    constexpr int NW = WD * TA;
+    constexpr int NT = TA * TDI;
+    __shared__ double sm[NV * NW];
+
+    const int t = threadIdx.x;
+    const int a0 = blockIdx.x * TA;
+    const int b = blockIdx.y;
+    const int d0 = blockIdx.z * (TDI - 1);       // first node of this tile (interior-relative)
+    const long bbase = (long)(b + a.oB) * a.sB;
+    const Consts &k = a.k;
+
+    // phase 1: per-node flux, conserved vector, spectral radius and eigenvector inputs
+    for (int idx = t; idx < NW; idx += NT) {
+        int aa, dd;
+        if (a.coalesce_d) { aa = idx / WD; dd = idx - aa * WD; }
+        else { dd = idx / TA; aa = idx - dd * TA; }
+        const int ai = min(a0 + aa, a.nA - 1);
+        int dp = d0 - G + dd + a.oD;
+        dp = max(0, min(dp, a.ngD - 1));
+        const long n = bbase + (long)(ai + a.oA) * a.sA + (long)dp * a.sD;
+        const double rho = a.pr.f[0][n], u = a.pr.f[1][n], v = a.pr.f[2][n], w = a.pr.f[3][n],
+                     e = a.pr.f[4][n];
+        const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n], J = a.J[n];
+        const double th = u * mx + v * my + w * mz;
+        const double p = pressure(k, rho, u, v, w, e);
+        const double cs = sqrt(k.upsilon * k.gamma * p / rho);
+        const double pu = p * k.upsilon;
+        const double rt = rho * th;
+        const int o = aa * WD + dd;
+        sm[V_F0 * NW + o] = J * rt;
+        sm[V_F1 * NW + o] = J * (rt * u + mx * pu);
+        sm[V_F2 * NW + o] = J * (rt * v + my * pu);
+        sm[V_F3 * NW + o] = J * (rt * w + mz * pu);
+        sm[V_F4 * NW + o] = J * (rt * (e + pu / rho));
+        sm[V_U0 * NW + o] = rho;
+        sm[V_U1 * NW + o] = rho * u;
+        sm[V_U2 * NW + o] = rho * v;
+        sm[V_U3 * NW + o] = rho * w;
+        sm[V_U4 * NW + o] = rho * e;
+        const double mn = sqrt(mx * mx + my * my + mz * mz);
+        sm[V_LAM * NW + o] = fabs(th) + cs * mn;
+        sm[V_U * NW + o] = u;
+        sm[V_V * NW + o] = v;
+        sm[V_W * NW + o] = w;
+        sm[V_C * NW + o] = cs;
+        sm[V_NX * NW + o] = mx / mn;
+        sm[V_NY * NW + o] = my / mn;
+        sm[V_NZ * NW + o] = mz / mn;
+        sm[V_J * NW + o] = J;
+    }
+    __syncthreads();
+
+    // phase 2: interface flux between nodes (d0-1+di) and (d0+di)
+    const int ta = t / TDI, di = t - ta * TDI;
+    const int ob = ta * WD + di;                 // window position of the stencil's first node
+    const int oc = ob + R;                       // left node of the interface
+    double out[5];
+    {
+        const double Jh = (sm[V_J * NW + oc] + sm[V_J * NW + oc + 1]) / 2.0;
+        const double u = 0.5 * (sm[V_U * NW + oc] + sm[V_U * NW + oc + 1]);
+        const double v = 0.5 * (sm[V_V * NW + oc] + sm[V_V * NW + oc + 1]);
+        const double w = 0.5 * (sm[V_W * NW + oc] + sm[V_W * NW + oc + 1]);
+        const double rho = 0.5 * (sm[V_U0 * NW + oc] + sm[V_U0 * NW + oc + 1]);
+        const double cs = 0.5 * (sm[V_C * NW + oc] + sm[V_C * NW + oc + 1]);
+        const double nx = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
+        const double ny = 0.5 * (sm[V_NY * NW + oc] + sm[V_NY * NW + oc + 1]);
+        const double nz = 0.5 * (sm[V_NZ * NW + oc] + sm[V_NZ * NW + oc + 1]);
+        double L[5][5], Rm[5][5];
+        eigvec(nx, ny, nz, u, v, w, rho, cs, k.gamma, L, Rm);
+        double lam = 0.0;
+#pragma unroll
+        for (int s = 0; s < 2 * G; s++) lam = fmax(lam, sm[V_LAM * NW + ob + s]);
+        const double lj = lam * Jh;
+
+        double phis[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            double fp[S], fm[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const int op = ob + s;                 // plus stencil: nodes m-r .. m+r
+                const int om = ob + 2 * R + 1 - s;     // minus stencil: nodes m+r+1 .. m-r+1
+                double accp = 0.0, accm = 0.0;
+#pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    accp += L[c][q] * (sm[(V_F0 + q) * NW + op] + lj * sm[(V_U0 + q) * NW + op]);
+                    accm += L[c][q] * (sm[(V_F0 + q) * NW + om] - lj * sm[(V_U0 + q) * NW + om]);
+                }
+                fp[s] = 0.5 * accp;
+                fm[s] = 0.5 * accm;
+            }
+            if constexpr (S == 5) {
+                phis[c] = -phi_w5(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
+                          + phi_w5(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]);
+            } else {
+                double dp_[8], dm_[8];
+#pragma unroll
+                for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
+                phis[c] = -phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            const double *F = &sm[(V_F0 + m) * NW];
+            double th;
+            if constexpr (S == 5) {
+                th = 1. / 12. * (-1. * (F[oc - 1] + F[oc + 2]) + 7. * (F[oc] + F[oc + 1]));
+            } else {
+                th = 1. / 840. * (533. * (F[oc + 1] + F[oc]) - 139. * (F[oc + 2] + F[oc - 1])
+                                  + 29. * (F[oc + 3] + F[oc - 2]) - 3. * (F[oc + 4] + F[oc - 3]));
+            }
+            out[m] = phis[0] * Rm[m][0] + phis[1] * Rm[m][1] + phis[2] * Rm[m][2] + phis[3] * Rm[m][3]
+                     + phis[4] * Rm[m][4] + th;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 5; m++) sm[m * NT + t] = out[m];
+    __syncthreads();
+
+    // phase 3: divergence into Q for the tile's TDI-1 nodes
+    if (di < TDI - 1) {
+        const int dn = d0 + di;
+        const int an = a0 + ta;
+        if (dn < a.nD && an < a.nA) {
+            const long n = bbase + (long)(an + a.oA) * a.sA + (long)(dn + a.oD) * a.sD;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const double dv = sm[m * NT + t + 1] - sm[m * NT + t];
+                a.q.f[m][n] = (a.first ? 0.0 : a.q.f[m][n]) - dv;
+            }
+        }
+    }
+}
+
+// ---- viscous derivatives ------------------------------------------------------------------
+// grad order: u_xi,u_eta,u_zeta, v_xi,v_eta,v_zeta, w_xi,w_eta,w_zeta, T_xi,T_eta,T_zeta.
+// Direction d is evaluated where the node's d index is interior (all transverse nodes).
+__global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, Consts k)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gm.ntot) return;
+    int idx[3];
+    idx[2] = (int)(n % gm.ng[2]);
+    const long r = n / gm.ng[2];
+    idx[1] = (int)(r % gm.ng[1]);
+    idx[0] = (int)(r / gm.ng[1]);
+    const int half = gm.g;
+    const int nl = 2 * gm.g + 1;   // S + 2 points
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= gm.dims) continue;
+        if (idx[d] < gm.lo[d] || idx[d] >= gm.lo[d] + gm.n[d]) continue;
+        const long st = gm.st[d];
+        double du = 0.0, dv = 0.0, dw = 0.0, dT = 0.0;
+        for (int l = 0; l < nl; l++) {
+            const long m = n + (long)(l - half) * st;
+            const double c = k.zd_der[l];
+            du += c * pr.f[1][m];
+            dv += c * pr.f[2][m];
+            dT += c * T[m];
+            if (gm.dims == 3) dw += c * pr.f[3][m];
+        }
+        gr.f[0 + d][n] = du;
+        gr.f[3 + d][n] = dv;
+        if (gm.dims == 3) gr.f[6 + d][n] = dw;
+        gr.f[9 + d][n] = dT;
+    }
+}
+
+// ---- viscous fluxes + source + RK update --------------------------------------------------
+struct RKArgs {
+    CFields5 un;        // U_lastStep
+    CFields5 us;        // stage input U_RK
+    Fields5 out;        // stage output U_RK(s+1)
+    Fields5 q;          // stage residual (inviscid part already accumulated)
+    Fields5 qsum;       // RK4 accumulator
+    CFields5 src;       // manufactured source (may be null)
+    CFields10 met;
+    CFields12 grad;
+    const double *T, *mu;
+    Geom gm;
+    Consts k;
+    double a, b, qf, ff, mf, tau;
+    int first_stage;
+    int viscous;
+    int zeta_bug;
+    int has_src;
+};
+
+template <int dir, int S>
+__device__ __forceinline__ void visc_dir(const RKArgs &A, const long n, const int i,
+                                         const int j, double &qx, double &qy, double &qz, double &qe)
+{
+    const Geom &gm = A.gm;
+    const Consts &k = A.k;
+    const long st = gm.st[dir];
+    constexpr int half = (S + 1) / 2;
+    const int three = (gm.dims == 3);
+    constexpr int np = S + 1;  // points per midpoint stencil
+    double dm[12], dpl[12];
+#pragma unroll
+    for (int s = 0; s < 12; s++) { dm[s] = 0.0; dpl[s] = 0.0; }
+    long bug_base = 0;
+    const bool bug = (dir == 2) && A.zeta_bug;
+    if (bug) bug_base = ((long)i * gm.ng[1] + j) * gm.ng[2];
+#pragma unroll
+    for (int l = 0; l < np; l++) {
+        long im, ip;
+        if (bug) { im = bug_base + (j + l - half); ip = im + 1; }
+        else { im = n + (long)(l - half) * st; ip = im + st; }
+        const double cm = k.zd_mid[l], ci = k.zd_int[l];
+        dm[0 * 3 + dir] += cm * A.us.f[1][im]; dpl[0 * 3 + dir] += cm * A.us.f[1][ip];
+        dm[1 * 3 + dir] += cm * A.us.f[2][im]; dpl[1 * 3 + dir] += cm * A.us.f[2][ip];
+        if (three) { dm[2 * 3 + dir] += cm * A.us.f[3][im]; dpl[2 * 3 + dir] += cm * A.us.f[3][ip]; }
+        dm[9 + dir] += cm * A.T[im]; dpl[9 + dir] += cm * A.T[ip];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            if (b == dir || (b == 2 && !three)) continue;
+            dm[b] += ci * A.grad.f[b][im]; dpl[b] += ci * A.grad.f[b][ip];
+            dm[3 + b] += ci * A.grad.f[3 + b][im]; dpl[3 + b] += ci * A.grad.f[3 + b][ip];
+            if (three) { dm[6 + b] += ci * A.grad.f[6 + b][im]; dpl[6 + b] += ci * A.grad.f[6 + b][ip]; }
+            dm[9 + b] += ci * A.grad.f[9 + b][im]; dpl[9 + b] += ci * A.grad.f[9 + b][ip];
+        }
+    }
+    double mv[10], fm[30], f0[30], fp[30];
+    const long nm = n - st, npp = n + st;
+#pragma unroll
+    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][nm];
+    visc_fac(dir, mv, fm);
+#pragma unroll
+    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][n];
+    visc_fac(dir, mv, f0);
+#pragma unroll
+    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][npp];
+    visc_fac(dir, mv, fp);
+    const double mum = A.mu[nm], mu0 = A.mu[n], mup = A.mu[npp];
+    double mom[3];
+    double work = 0.0;
+    const double vel[3] = {A.us.f[1][n], A.us.f[2][n], A.us.f[3][n]};
+#pragma unroll
+    for (int blk = 0; blk < 3; blk++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            const double Km = 0.5 * (mum * fm[blk * 9 + s] + mu0 * f0[blk * 9 + s]);
+            const double Kp = 0.5 * (mu0 * f0[blk * 9 + s] + mup * fp[blk * 9 + s]);
+            const double tdiff = dpl[s] * Kp - dm[s] * Km;
+            acc += tdiff;
+            work += vel[blk] * tdiff;
+        }
+        mom[blk] = acc;
+    }
+    double heat = 0.0;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const double Km = 0.5 * (mum * fm[27 + s] + mu0 * f0[27 + s]);
+        const double Kp = 0.5 * (mu0 * f0[27 + s] + mup * fp[27 + s]);
+        heat += dpl[9 + s] * Kp - dm[9 + s] * Km;
+    }
+    double prod = 0.0;
+#pragma unroll
+    for (int blk = 0; blk < 3; blk++) {
+        const double gd = A.grad.f[blk * 3 + dir][n];
+#pragma unroll
+        for (int s = 0; s < 9; s++) prod += gd * A.grad.f[s][n] * mu0 * f0[blk * 9 + s];
+    }
+    qx += k.psi * mom[0];
+    qy += k.psi * mom[1];
+    if (three) qz += k.psi * mom[2];
+    qe += k.gam_heat * heat + k.psi * (work + prod);
+}
+
+template <int S>
+__device__ __forceinline__ void node_rhs(const RKArgs &A, const long n, const int i, const int j, double qv[5])
+{
+    const Geom &gm = A.gm;
+    double q0 = A.q.f[0][n], qx = A.q.f[1][n], qy = A.q.f[2][n], qz = A.q.f[3][n], qe = A.q.f[4][n];
+    if (A.viscous) {
+        visc_dir<0, S>(A, n, i, j, qx, qy, qz, qe);
+        visc_dir<1, S>(A, n, i, j, qx, qy, qz, qe);
+        if (gm.dims == 3) visc_dir<2, S>(A, n, i, j, qx, qy, qz, qe);
+    }
+    const double J = A.met.f[9][n];
+    if (A.has_src) {
+        q0 += J * A.src.f[0][n];
+        qx += J * A.src.f[1][n];
+        qy += J * A.src.f[2][n];
+        if (gm.dims == 3) qz += J * A.src.f[3][n];
+        qe += J * A.src.f[4][n];
+    }
+    qv[0] = q0; qv[1] = qx; qv[2] = qy; qv[3] = qz; qv[4] = qe;
+}
+
+__device__ __forceinline__ long interior_node(const Geom &gm, const long t, int &i, int &j)
+{
+    const int kk = (int)(t % gm.n[2]);
+    const long r = t / gm.n[2];
+    const int jj = (int)(r % gm.n[1]);
+    const int ii = (int)(r / gm.n[1]);
+    i = ii + gm.lo[0];
+    j = jj + gm.lo[1];
+    return (long)i * gm.st[0] + (long)j * gm.st[1] + kk + gm.lo[2];
+}
+
+// Viscous fluxes + source + RK update (src/Functions.c:2300-2365) of one interior node.
+template <int S>
+__global__ __launch_bounds__(256) void k_visc_rk(RKArgs A)
+{
+    const Geom &gm = A.gm;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if (t >= nint) return;
+    int i, j;
+    const long n = interior_node(gm, t, i, j);
+    double qv[5];
+    node_rhs<S>(A, n, i, j, qv);
+    const double J = A.met.f[9][n];
+    double qs[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
+        qs[m] = old + A.mf * qv[m];
+        if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
+    }
+    const double tj = A.tau / J;
+    const double rho_n = A.un.f[0][n], rho_s = A.us.f[0][n];
+    const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
+    A.out.f[0][n] = rho;
+#pragma unroll
+    for (int m = 1; m < 5; m++) {
+        if (m == 3 && gm.dims == 2) { A.out.f[3][n] = 0.0; continue; }
+        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * A.us.f[m][n] * rho_s
+                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
+    }
+}
+
+// Completes the stage residual in place (viscous + source, no update); used by shx_rhs.
+template <int S>
+__global__ __launch_bounds__(256) void k_rhs_finish(RKArgs A)
+{
+    const Geom &gm = A.gm;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if (t >= nint) return;
+    int i, j;
+    const long n = interior_node(gm, t, i, j);
+    double qv[5];
+    node_rhs<S>(A, n, i, j, qv);
+#pragma unroll
+    for (int m = 0; m < 5; m++) A.q.f[m][n] = qv[m];
+}
+
+// ---- metric --------------------------------------------------------------------------------
+// CreateMetric (src/Functions.c:6388-6509) on the extended coordinates.
+__global__ void k_metric(const double *xe, const double *ye, const double *ze, Fields10 met, Geom gm,
+                         int diag3d)
+{
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gm.ntot) return;
+    const int k = (int)(n % gm.ng[2]);
+    const long r = n / gm.ng[2];
+    const int j = (int)(r % gm.ng[1]);
+    const int i = (int)(r / gm.ng[1]);
+    const int ne1 = gm.ng[1] + 2, ne2 = (gm.dims == 3) ? gm.ng[2] + 2 : 1;
+    const int ke = (gm.dims == 3) ? k + 1 : 0;
+    const long ce = ((long)(i + 1) * ne1 + (j + 1)) * ne2 + ke;
+    const long sI = (long)ne1 * ne2, sJ = ne2;
+    double x_xi, y_xi, z_xi, x_eta, y_eta, z_eta, x_zeta, y_zeta, z_zeta;
+    x_xi = (-0.5 * xe[ce - sI] + 0.5 * xe[ce + sI]);
+    y_xi = (-0.5 * ye[ce - sI] + 0.5 * ye[ce + sI]);
+    x_eta = (-0.5 * xe[ce - sJ] + 0.5 * xe[ce + sJ]);
+    y_eta = (-0.5 * ye[ce - sJ] + 0.5 * ye[ce + sJ]);
+    if (gm.dims == 2) {
+        z_xi = 0.0; z_eta = 0.0; x_zeta = 0.0; y_zeta = 0.0; z_zeta = 1.0;
+    } else {
+        z_xi = (-0.5 * ze[ce - sI] + 0.5 * ze[ce + sI]);
+        z_eta = (-0.5 * ze[ce - sJ] + 0.5 * ze[ce + sJ]);
+        x_zeta = (-0.5 * xe[ce - 1] + 0.5 * xe[ce + 1]);
+        y_zeta = (-0.5 * ye[ce - 1] + 0.5 * ye[ce + 1]);
+        z_zeta = (-0.5 * ze[ce - 1] + 0.5 * ze[ce + 1]);
+        if (diag3d) { y_xi = 0.0; z_xi = 0.0; x_eta = 0.0; z_eta = 0.0; x_zeta = 0.0; y_zeta = 0.0; }
+    }
+    const double J = x_xi * y_eta * z_zeta + x_eta * y_zeta * z_xi + x_zeta * y_xi * z_eta
+                   - x_zeta * y_eta * z_xi - x_eta * y_xi * z_zeta - x_xi * y_zeta * z_eta;
+    met.f[9][n] = J;
+    met.f[0][n] = (y_eta * z_zeta - y_zeta * z_eta) / J;
+    met.f[1][n] = (x_zeta * z_eta - x_eta * z_zeta) / J;
+    met.f[2][n] = (x_eta * y_zeta - x_zeta * y_eta) / J;
+    met.f[3][n] = (y_zeta * z_xi - y_xi * z_zeta) / J;
+    met.f[4][n] = (x_xi * z_zeta - x_zeta * z_xi) / J;
+    met.f[5][n] = (x_zeta * y_xi - x_xi * y_zeta) / J;
+    met.f[6][n] = (y_xi * z_eta - y_eta * z_xi) / J;
+    met.f[7][n] = (x_eta * z_xi - x_xi * z_eta) / J;
+    met.f[8][n] = (x_xi * y_eta - x_eta * y_xi) / J;
+}
+
+// ---- reductions ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// CFL spectral-radius maximum over interior nodes; positive doubles order like their bit
+// patterns, so the block maxima combine with an unsigned 64-bit atomic max.
+__global__ void k_cfl_max(CFields5 pr, CFields10 met, Geom gm, Consts k, unsigned long long *out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    double v = 0.0;
+    if (t < nint) {
+        const int kk = (int)(t % gm.n[2]);
+        const long r = t / gm.n[2];
+        const int jj = (int)(r % gm.n[1]);
+        const int ii = (int)(r / gm.n[1]);
+        const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+        const double rho = pr.f[0][n], u = pr.f[1][n], vv = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
+        const double p = pressure(k, rho, u, vv, w, e);
+        const double cs = sqrt(k.upsilon * k.gamma * p / rho);
+        for (int d = 0; d < gm.dims; d++) {
+            const double a = met.f[3 * d][n], b = met.f[3 * d + 1][n], z = met.f[3 * d + 2][n];
+            v += fabs(u * a + vv * b + w * z) + cs * sqrt(a * a + b * b + z * z);
+        }
+    }
+    v = wave_max(v);
+    __shared__ double red[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int s = 0; s < (int)(blockDim.x >> 6); s++) m = fmax(m, red[s]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// checkNAN (src/Functions.c:5419-5467): count and overwrite non-finite interior rho.
+__global__ void k_check_nan(Fields5 pr, Geom gm, int *count)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    int bad = 0;
+    if (t < nint) {
+        const int kk = (int)(t % gm.n[2]);
+        const long r = t / gm.n[2];
+        const int jj = (int)(r % gm.n[1]);
+        const int ii = (int)(r / gm.n[1]);
+        const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+        const double rho = pr.f[0][n];
+        if (!isfinite(rho)) {
+            bad = 1;
+            pr.f[0][n] = 1000000; pr.f[1][n] = 1000000; pr.f[2][n] = 1000000; pr.f[3][n] = 1000000;
+        }
+    }
+    bad = wave_sum(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
+// ----------------------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------------------
+
+static char g_err[512];
+
+static int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(SHX_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                               \
+    } while (0)
+
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC_RK, KT_RK, KT_REDUCE, KT_METRIC };
+static const char *g_knames[SHX_NUM_KERNELS] = {
+    "ghost_periodic", "calc_values", "weno_xi", "weno_eta", "weno_zeta",
+    "visc_grad", "visc_rk", "rk_update", "reduce", "metric"};
+
+struct shx_case {
+    shx_desc d;
+    Geom gm;
+    Consts k;
+    double rk_a[4], rk_b[4], rk_q[4], rk_f[4], rk_m[4];
+    hipStream_t stream;
+    double *prim[3][5];    // three state buffers; cur = U_lastStep
+    int cur;
+    double *q[5], *qsum[5], *met[10], *grad[12], *src[5], *T, *mu;
+    int has_src, has_metric, has_state;
+    unsigned long long *d_red;
+    int *d_cnt;
+    int timing;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;
+    size_t ev_used;
+    double kt_ms[SHX_NUM_KERNELS];
+    int kt_n[SHX_NUM_KERNELS];
+};
+
+extern "C" const char *shx_last_error(void) { return g_err; }
+
+extern "C" int shx_device_count(int *count)
+{
+    if (!count) return set_err(SHX_ERR_ARG, "null count");
+    HIPCHK(hipGetDeviceCount(count));
+    return SHX_OK;
+}
+
+// event bracketing for per-kernel timing
+static int tbeg(shx_case *c, int kind)
+{
+    if (!c->timing) return SHX_OK;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+        for (int s = 0; s < 256; s++) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+            c->ev_kind.push_back(0);
+        }
+    }
+    c->ev_kind[c->ev_used] = kind;
+    HIPCHK(hipEventRecord(c->ev_pool[c->ev_used], c->stream));
+    c->ev_used++;
+    return SHX_OK;
+}
+static int tend(shx_case *c)
+{
+    if (!c->timing) return SHX_OK;
+    c->ev_kind[c->ev_used] = -1;
+    HIPCHK(hipEventRecord(c->ev_pool[c->ev_used], c->stream));
+    c->ev_used++;
+    return SHX_OK;
+}
+static int drain_events(shx_case *c)
+{
+    if (c->ev_used == 0) return SHX_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t s = 0; s + 1 < c->ev_used; s += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[s], c->ev_pool[s + 1]));
+        c->kt_ms[c->ev_kind[s]] += ms;
+        c->kt_n[c->ev_kind[s]] += 1;
+    }
+    c->ev_used = 0;
+    return SHX_OK;
+}
+#define LAUNCH_CHECK() HIPCHK(hipGetLastError())
+
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+extern "C" int shx_create(const shx_desc *desc, shx_case **out)
+{
+    if (!desc || !out) return set_err(SHX_ERR_ARG, "null argument");
+    *out = nullptr;
+    const shx_desc &d = *desc;
+    if (d.dims != 2 && d.dims != 3) return set_err(SHX_ERR_ARG, "dims must be 2 or 3 (got %d)", d.dims);
+    if (d.space_order != 5 && d.space_order != 9)
+        return set_err(SHX_ERR_ARG, "space_order must be 5 or 9 (got %d)", d.space_order);
+    if (d.time_order != 3 && d.time_order != 4)
+        return set_err(SHX_ERR_ARG, "time_order must be 3 or 4 (got %d)", d.time_order);
+    if (d.precision != SHX_FP64) return set_err(SHX_ERR_UNSUPPORTED, "only the FP64 form is built");
+    for (int f = 0; f < 2 * d.dims; f++)
+        if (d.bc[f] != SHX_BC_PERIODIC)
+            return set_err(SHX_ERR_UNSUPPORTED, "face %d: only periodic boundaries are implemented", f);
+    const int S = d.space_order, g = (S + 1) / 2;
+    for (int a = 0; a < d.dims; a++)
+        if (d.n[a] < S + 1) return set_err(SHX_ERR_ARG, "n[%d]=%d below the stencil width %d", a, d.n[a], S + 1);
+    if (!(d.gamma > 1.0) || !(d.mach > 0.0) || !(d.reynolds > 0.0) || !(d.prandtl > 0.0) || !(d.T0 > 0.0))
+        return set_err(SHX_ERR_ARG, "non-physical fluid properties");
+
+    shx_case *c = new shx_case();
+    c->d = d;
+    Geom &gm = c->gm;
+    gm.dims = d.dims;
+    gm.g = g;
+    for (int a = 0; a < 3; a++) {
+        const bool live = (a < d.dims);
+        gm.n[a] = live ? d.n[a] : 1;
+        gm.ng[a] = live ? d.n[a] + S + 1 : 1;
+        gm.lo[a] = live ? g : 0;
+    }
+    gm.st[2] = 1;
+    gm.st[1] = gm.ng[2];
+    gm.st[0] = (long)gm.ng[1] * gm.ng[2];
+    gm.ntot = (long)gm.ng[0] * gm.ng[1] * gm.ng[2];
+    if (gm.ntot > (1L << 31) - 1) { delete c; return set_err(SHX_ERR_ARG, "block too large for 32-bit tiles"); }
+    if ((d.quirks & SHX_QUIRK_ZETA_VISC_J) && d.dims == 3 && gm.ng[1] > gm.ng[2]) {
+        // the reference's zeta stencil indexes k with j: for ng_j > ng_k it leaves the row
+        delete c;
+        return set_err(SHX_ERR_UNSUPPORTED, "zeta-index quirk needs n_j <= n_k");
+    }
+
+    Consts &k = c->k;
+    memset(&k, 0, sizeof k);
+    k.gamma = d.gamma;
+    k.upsilon = 1.0 / (d.gamma * (d.mach * d.mach));
+    k.psi = 1.0 / d.reynolds;
+    k.gam_heat = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * d.prandtl);
+    k.suth = 100.4 / d.T0;
+    k.eps = d.eps_override > 0.0 ? d.eps_override : 10. * pow(DBL_MIN, 0.5);
+    k.mms = d.manufactured ? 1 : 0;
+    if (S == 9) {
+        const double in9[10] = {1. / 1260., -23. / 2520., 127. / 2520., -473. / 2520., 1627. / 2520.,
+                                1627. / 2520., -473. / 2520., 127. / 2520., -23. / 2520., 1. / 1260.};
+        const double mi9[10] = {-3. / 9450., 351. / 75600., -2649. / 75600., 15351. / 75600., -110649. / 75600.,
+                                110649. / 75600., -15351. / 75600., 2649. / 75600., -351. / 75600., 3. / 9450.};
+        const double de9[11] = {-1. / 1260., 25. / 2520., -150. / 2520., 600. / 2520., -2100. / 2520., 0.,
+                                2100. / 2520., -600. / 2520., 150. / 2520., -25. / 2520., 1. / 1260.};
+        memcpy(k.zd_int, in9, sizeof in9); memcpy(k.zd_mid, mi9, sizeof mi9); memcpy(k.zd_der, de9, sizeof de9);
+    } else {
+        const double in5[6] = {1. / 60., -8. / 60., 37. / 60., 37. / 60., -8. / 60., 1. / 60.};
+        const double mi5[6] = {-1. / 90., 25. / 180., -245. / 180., 245. / 180., -25. / 180., 1. / 90.};
+        const double de5[7] = {-1. / 60., 3. / 20., -3. / 4., 0., 3. / 4., -3. / 20., 1. / 60.};
+        memcpy(k.zd_int, in5, sizeof in5); memcpy(k.zd_mid, mi5, sizeof mi5); memcpy(k.zd_der, de5, sizeof de5);
+    }
+    if (d.time_order == 4) {
+        const double a[4] = {1., 1., 1., 1.}, b[4] = {0., 0., 0., 0.}, q[4] = {0.5, 0.5, 1., 1. / 6.};
+        const double f[4] = {0., 0., 0., 1. / 6.}, m[4] = {1., 2., 2., 0.};
+        memcpy(c->rk_a, a, sizeof a); memcpy(c->rk_b, b, sizeof b); memcpy(c->rk_q, q, sizeof q);
+        memcpy(c->rk_f, f, sizeof f); memcpy(c->rk_m, m, sizeof m);
+    } else {
+        const double a[4] = {1., 3. / 4., 1. / 3., 0.}, b[4] = {0., 1. / 4., 2. / 3., 0.};
+        const double q[4] = {1., 1. / 4., 2. / 3., 0.}, z[4] = {0., 0., 0., 0.};
+        memcpy(c->rk_a, a, sizeof a); memcpy(c->rk_b, b, sizeof b); memcpy(c->rk_q, q, sizeof q);
+        memcpy(c->rk_f, z, sizeof z); memcpy(c->rk_m, z, sizeof z);
+    }
+
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) { delete c; return set_err(SHX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    const size_t bytes = sizeof(double) * (size_t)gm.ntot;
+    std::vector<double **> all;
+    for (int b = 0; b < 3; b++) for (int m = 0; m < 5; m++) all.push_back(&c->prim[b][m]);
+    for (int m = 0; m < 5; m++) { all.push_back(&c->q[m]); all.push_back(&c->qsum[m]); }
+    for (int m = 0; m < 10; m++) all.push_back(&c->met[m]);
+    for (int m = 0; m < 12; m++) all.push_back(&c->grad[m]);
+    all.push_back(&c->T);
+    all.push_back(&c->mu);
+    for (double **p : all) {
+        e = hipMalloc((void **)p, bytes);
+        if (e == hipSuccess) e = hipMemsetAsync(*p, 0, bytes, c->stream);
+        if (e != hipSuccess) {
+            shx_destroy(c);
+            return set_err(SHX_ERR_HIP, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        }
+    }
+    e = hipMalloc((void **)&c->d_red, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_cnt, sizeof(int));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { shx_destroy(c); return set_err(SHX_ERR_HIP, "setup: %s", hipGetErrorString(e)); }
+    c->cur = 0;
+    *out = c;
+    return SHX_OK;
+}
+
+extern "C" int shx_destroy(shx_case *c)
+{
+    if (!c) return SHX_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int b = 0; b < 3; b++) for (int m = 0; m < 5; m++) hipFree(c->prim[b][m]);
+    for (int m = 0; m < 5; m++) { hipFree(c->q[m]); hipFree(c->qsum[m]); hipFree(c->src[m]); }
+    for (int m = 0; m < 10; m++) hipFree(c->met[m]);
+    for (int m = 0; m < 12; m++) hipFree(c->grad[m]);
+    hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return SHX_OK;
+}
+
+extern "C" int shx_padded_extent(const shx_case *c, int ng[3])
+{
+    if (!c || !ng) return set_err(SHX_ERR_ARG, "null argument");
+    for (int a = 0; a < 3; a++) ng[a] = c->gm.ng[a];
+    return SHX_OK;
+}
+
+static int copy_in(shx_case *c, double *dst, const double *src)
+{
+    if (!src) return set_err(SHX_ERR_ARG, "null field pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)c->gm.ntot, hipMemcpyHostToDevice, c->stream));
+    return SHX_OK;
+}
+static int copy_out(shx_case *c, double *dst, const double *src)
+{
+    if (!dst) return set_err(SHX_ERR_ARG, "null field pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)c->gm.ntot, hipMemcpyDeviceToHost, c->stream));
+    return SHX_OK;
+}
+
+static int ghost_fill(shx_case *c, double *const *f, int nf)
+{
+    Fields10 ff;
+    memset(&ff, 0, sizeof ff);
+    for (int m = 0; m < nf; m++) ff.f[m] = f[m];
+    int rc = tbeg(c, KT_GHOST);
+    if (rc) return rc;
+    k_ghost_periodic<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(ff, nf, c->gm);
+    LAUNCH_CHECK();
+    return tend(c);
+}
+
+extern "C" int shx_build_metric(shx_case *c, const double *xe, const double *ye, const double *ze)
+{
+    if (!c || !xe || !ye || (c->gm.dims == 3 && !ze)) return set_err(SHX_ERR_ARG, "null argument");
+    const Geom &gm = c->gm;
+    const long next = (long)(gm.ng[0] + 2) * (gm.ng[1] + 2) * (gm.dims == 3 ? gm.ng[2] + 2 : 1);
+    const size_t bytes = sizeof(double) * (size_t)next;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    HIPCHK(hipMalloc((void **)&dx, bytes));
+    HIPCHK(hipMalloc((void **)&dy, bytes));
+    HIPCHK(hipMalloc((void **)&dz, bytes));
+    HIPCHK(hipMemcpyAsync(dx, xe, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dy, ye, bytes, hipMemcpyHostToDevice, c->stream));
+    if (gm.dims == 3) HIPCHK(hipMemcpyAsync(dz, ze, bytes, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemsetAsync(dz, 0, bytes, c->stream));
+    Fields10 mf;
+    for (int m = 0; m < 10; m++) mf.f[m] = c->met[m];
+    int rc = tbeg(c, KT_METRIC);
+    if (rc) return rc;
+    k_metric<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(dx, dy, dz, mf, gm,
+                                                       (c->d.quirks & SHX_QUIRK_METRIC_DIAG3D) ? 1 : 0);
+    LAUNCH_CHECK();
+    if ((rc = tend(c))) return rc;
+    if ((rc = ghost_fill(c, c->met, 10))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(dx); hipFree(dy); hipFree(dz);
+    c->has_metric = 1;
+    return SHX_OK;
+}
+
+extern "C" int shx_upload_metric(shx_case *c, const double *const met[10])
+{
+    if (!c || !met) return set_err(SHX_ERR_ARG, "null argument");
+    for (int m = 0; m < 10; m++) { int rc = copy_in(c, c->met[m], met[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_metric = 1;
+    return SHX_OK;
+}
+
+extern "C" int shx_download_metric(shx_case *c, double *const met[10])
+{
+    if (!c || !met) return set_err(SHX_ERR_ARG, "null argument");
+    for (int m = 0; m < 10; m++) { int rc = copy_out(c, met[m], c->met[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_upload_state(shx_case *c, const double *const prim[5])
+{
+    if (!c || !prim) return set_err(SHX_ERR_ARG, "null argument");
+    for (int m = 0; m < 5; m++) { int rc = copy_in(c, c->prim[c->cur][m], prim[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_state = 1;
+    return SHX_OK;
+}
+
+extern "C" int shx_download_state(shx_case *c, double *const prim[5])
+{
+    if (!c || !prim) return set_err(SHX_ERR_ARG, "null argument");
+    for (int m = 0; m < 5; m++) { int rc = copy_out(c, prim[m], c->prim[c->cur][m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_upload_source(shx_case *c, const double *const src[5])
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null argument");
+    if (!src) { c->has_src = 0; return SHX_OK; }
+    const size_t bytes = sizeof(double) * (size_t)c->gm.ntot;
+    for (int m = 0; m < 5; m++) {
+        if (!c->src[m]) HIPCHK(hipMalloc((void **)&c->src[m], bytes));
+        int rc = copy_in(c, c->src[m], src[m]);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_src = 1;
+    return SHX_OK;
+}
+
+// one WENO sweep along axis dir of the state `pr`
+static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
+{
+    const Geom &gm = c->gm;
+    WenoArgs a;
+    memset(&a, 0, sizeof a);
+    for (int m = 0; m < 5; m++) { a.pr.f[m] = pr[m]; a.q.f[m] = c->q[m]; }
+    a.m0 = c->met[3 * dir]; a.m1 = c->met[3 * dir + 1]; a.m2 = c->met[3 * dir + 2]; a.J = c->met[9];
+    // transverse axes: fast = the contiguous non-sweep axis with extent > 1
+    int A, B;
+    if (gm.dims == 3) {
+        if (dir == 2) { A = 1; B = 0; } else { A = 2; B = (dir == 0) ? 1 : 0; }
+    } else {
+        A = (dir == 0) ? 1 : 0; B = 2;
+    }
+    a.sD = gm.st[dir]; a.sA = gm.st[A]; a.sB = gm.st[B];
+    a.nD = gm.n[dir]; a.nA = gm.n[A]; a.nB = gm.n[B];
+    a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
+    a.ngD = gm.ng[dir];
+    a.first = first;
+    a.coalesce_d = (a.sD == 1);
+    a.k = c->k;
+    int rc = tbeg(c, KT_WENO0 + dir);
+    if (rc) return rc;
+    if (a.coalesce_d) {
+        constexpr int TA = 4, TDI = 64;
+        dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+        if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+        else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+    } else {
+        constexpr int TA = 16, TDI = 16;
+        dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+        if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+        else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+    }
+    LAUNCH_CHECK();
+    return tend(c);
+}
+
+static void fill_rk_args(shx_case *c, RKArgs &A, double *const *us, double *const *out)
+{
+    memset(&A, 0, sizeof A);
+    for (int m = 0; m < 5; m++) {
+        A.un.f[m] = c->prim[c->cur][m];
+        A.us.f[m] = us[m];
+        A.out.f[m] = out ? out[m] : nullptr;
+        A.q.f[m] = c->q[m];
+        A.qsum.f[m] = c->qsum[m];
+        A.src.f[m] = c->src[m];
+    }
+    for (int m = 0; m < 10; m++) A.met.f[m] = c->met[m];
+    for (int m = 0; m < 12; m++) A.grad.f[m] = c->grad[m];
+    A.T = c->T; A.mu = c->mu;
+    A.gm = c->gm; A.k = c->k;
+    A.viscous = !c->d.inviscid;
+    A.zeta_bug = (c->d.quirks & SHX_QUIRK_ZETA_VISC_J) ? 1 : 0;
+    A.has_src = c->has_src;
+}
+
+// fluxes of one stage into c->q (inviscid part) and the viscous pre-pass
+static int stage_fluxes(shx_case *c, double *const *pr)
+{
+    const Geom &gm = c->gm;
+    int rc;
+    if ((rc = ghost_fill(c, pr, 5))) return rc;
+    if (!c->d.inviscid) {
+        CFields5 cp;
+        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+        if ((rc = tbeg(c, KT_VALUES))) return rc;
+        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm.ntot, c->k);
+        LAUNCH_CHECK();
+        if ((rc = tend(c))) return rc;
+    }
+    for (int dir = 0; dir < gm.dims; dir++)
+        if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
+    if (!c->d.inviscid) {
+        CFields5 cp;
+        Fields12 gf;
+        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+        for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
+        if ((rc = tbeg(c, KT_GRAD))) return rc;
+        k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
+        LAUNCH_CHECK();
+        if ((rc = tend(c))) return rc;
+    }
+    return SHX_OK;
+}
+
+static int launch_visc_rk(shx_case *c, const RKArgs &A, long nint, int finish_only)
+{
+    const unsigned nb = nblk(nint, 256);
+    if (c->d.space_order == 5) {
+        if (finish_only) k_rhs_finish<5><<<nb, 256, 0, c->stream>>>(A);
+        else k_visc_rk<5><<<nb, 256, 0, c->stream>>>(A);
+    } else {
+        if (finish_only) k_rhs_finish<9><<<nb, 256, 0, c->stream>>>(A);
+        else k_visc_rk<9><<<nb, 256, 0, c->stream>>>(A);
+    }
+    LAUNCH_CHECK();
+    return SHX_OK;
+}
+
+static int check_ready(shx_case *c)
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null handle");
+    if (!c->has_metric) return set_err(SHX_ERR_STATE, "no metric: call shx_build_metric or shx_upload_metric");
+    if (!c->has_state) return set_err(SHX_ERR_STATE, "no state: call shx_upload_state");
+    return SHX_OK;
+}
+
+extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    const int un = c->cur;
+    const int bx = (un + 1) % 3, by = (un + 2) % 3;
+    int in = un, outb = bx;
+    for (int s = 0; s < c->d.time_order; s++) {
+        if ((rc = stage_fluxes(c, c->prim[in]))) return rc;
+        RKArgs A;
+        fill_rk_args(c, A, c->prim[in], c->prim[outb]);
+        A.a = c->rk_a[s]; A.b = c->rk_b[s]; A.qf = c->rk_q[s]; A.ff = c->rk_f[s]; A.mf = c->rk_m[s];
+        A.tau = tau;
+        A.first_stage = (s == 0);
+        if ((rc = tbeg(c, A.viscous ? KT_VISC_RK : KT_RK))) return rc;
+        if ((rc = launch_visc_rk(c, A, nint, 0))) return rc;
+        if ((rc = tend(c))) return rc;
+        // stage s+1 reads what stage s wrote and writes the other scratch buffer
+        in = outb;
+        outb = (outb == bx) ? by : bx;
+    }
+    c->cur = in;   // U_lastStep <- U_RK (pointer swap instead of WriteValuesFromU1ToU2)
+    return SHX_OK;
+}
+
+extern "C" int shx_run(shx_case *c, double tau, int nsteps)
+{
+    if (nsteps < 0) return set_err(SHX_ERR_ARG, "negative step count");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = shx_calc_runge_kutta(c, tau);
+        if (rc) return rc;
+        if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
+    }
+    return SHX_OK;
+}
+
+extern "C" int shx_rhs(shx_case *c, double *const q[5])
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!q) return set_err(SHX_ERR_ARG, "null output");
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if ((rc = stage_fluxes(c, c->prim[c->cur]))) return rc;
+    RKArgs A;
+    fill_rk_args(c, A, c->prim[c->cur], nullptr);
+    if ((rc = launch_visc_rk(c, A, nint, 1))) return rc;
+    for (int m = 0; m < 5; m++) { rc = copy_out(c, q[m], c->q[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_check_nan(shx_case *c, int *count)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!count) return set_err(SHX_ERR_ARG, "null output");
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    Fields5 pf;
+    for (int m = 0; m < 5; m++) pf.f[m] = c->prim[c->cur][m];
+    HIPCHK(hipMemsetAsync(c->d_cnt, 0, sizeof(int), c->stream));
+    if ((rc = tbeg(c, KT_REDUCE))) return rc;
+    k_check_nan<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, gm, c->d_cnt);
+    LAUNCH_CHECK();
+    if ((rc = tend(c))) return rc;
+    HIPCHK(hipMemcpyAsync(count, c->d_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_cfl_dt(shx_case *c, double cfl, double *dt)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!dt || !(cfl > 0.0)) return set_err(SHX_ERR_ARG, "bad cfl or null output");
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    CFields5 pf;
+    CFields10 mf;
+    for (int m = 0; m < 5; m++) pf.f[m] = c->prim[c->cur][m];
+    for (int m = 0; m < 10; m++) mf.f[m] = c->met[m];
+    HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
+    if ((rc = tbeg(c, KT_REDUCE))) return rc;
+    k_cfl_max<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, mf, gm, c->k, c->d_red);
+    LAUNCH_CHECK();
+    if ((rc = tend(c))) return rc;
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, c->d_red, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double mx;
+    memcpy(&mx, &bits, sizeof mx);
+    *dt = mx > 0.0 ? cfl / mx : 0.0;
+    return SHX_OK;
+}
+
+extern "C" int shx_synchronize(shx_case *c)
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_set_timing(shx_case *c, int enable)
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null handle");
+    int rc = drain_events(c);
+    if (rc) return rc;
+    c->timing = enable ? 1 : 0;
+    for (int s = 0; s < SHX_NUM_KERNELS; s++) { c->kt_ms[s] = 0.0; c->kt_n[s] = 0; }
+    return SHX_OK;
+}
+
+extern "C" int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
+                                int launches[SHX_NUM_KERNELS])
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null handle");
+    int rc = drain_events(c);
+    if (rc) return rc;
+    for (int s = 0; s < SHX_NUM_KERNELS; s++) {
+        if (names) names[s] = g_knames[s];
+        if (ms) ms[s] = c->kt_ms[s];
+        if (launches) launches[s] = c->kt_n[s];
+    }
+    return SHX_OK;
+}

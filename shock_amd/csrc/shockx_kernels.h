@@ -1,0 +1,186 @@
+// shockx_kernels.h -- gfx950 device code for the SHOCK RHS / RK-stage path (FP64 form).
+//
+// Layout: ghost-padded SoA, linear index i*ng1*ng2 + j*ng2 + k (k fastest), the reference's
+// own layout (src/Functions.c:56-103), so host arrays cross the C ABI without a transpose.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shx {
+
+// Physical and scheme constants shared by every kernel (kernel-argument struct).
+struct Consts {
+    double gamma, upsilon, psi, gam_heat, suth, eps;
+    double zd_mid[10], zd_int[10], zd_der[11];
+    int mms;
+};
+
+struct Fields5 { double *f[5]; };
+struct CFields5 { const double *f[5]; };
+struct Fields10 { double *f[10]; };
+struct CFields10 { const double *f[10]; };
+struct CFields12 { const double *f[12]; };
+struct Fields12 { double *f[12]; };
+
+struct Geom {
+    int ng[3];        // padded extents
+    int n[3];         // interior counts (n[2] = 1 in 2D)
+    int lo[3];        // padded index of the first interior node (g, or 0 for the 2D z axis)
+    long st[3];       // strides
+    int dims, g;
+    long ntot;
+};
+
+// ---------------------------------------------------------------------------------------
+// Characteristic eigenvectors of the Euler flux Jacobian in direction nh at the arithmetic
+// interface average (CalcEigenVectorsIn*Direction, src/WENO.c:1085-1548).
+__device__ __forceinline__ void eigvec(const double nx, const double ny, const double nz,
+                                       const double u, const double v, const double w,
+                                       const double rho, const double cs, const double gamma,
+                                       double L[5][5], double R[5][5])
+{
+    const double s2 = 1.4142135623730951;  // sqrt(2.), correctly rounded
+    const double q2 = (u * u + v * v + w * w);
+    const double M = sqrt(q2) / cs;
+    const double gc = ((gamma - 1.0) / (cs * cs));
+    const double ics = (1. / (cs * s2));
+    const double un = (u * nx + v * ny + w * nz);
+    const double irho = 1. / rho;
+    const double hm = 1.0 - 0.5 * (gamma - 1.0) * (M * M);
+    const double h0 = q2 / 2. + cs * cs / (gamma - 1.);
+
+    R[0][0] = nx; R[0][1] = ny; R[0][2] = nz; R[0][3] = ics; R[0][4] = ics;
+    R[1][0] = u * nx; R[1][1] = u * ny - rho * nz; R[1][2] = u * nz + rho * ny;
+    R[1][3] = ics * (u + nx * cs); R[1][4] = ics * (u - nx * cs);
+    R[2][0] = v * nx + rho * nz; R[2][1] = v * ny; R[2][2] = v * nz - rho * nx;
+    R[2][3] = ics * (v + ny * cs); R[2][4] = ics * (v - ny * cs);
+    R[3][0] = w * nx - rho * ny; R[3][1] = w * ny + rho * nx; R[3][2] = w * nz;
+    R[3][3] = ics * (w + nz * cs); R[3][4] = ics * (w - nz * cs);
+    R[4][0] = q2 / 2. * nx + rho * (v * nz - w * ny);
+    R[4][1] = q2 / 2. * ny + rho * (w * nx - u * nz);
+    R[4][2] = q2 / 2. * nz + rho * (u * ny - v * nx);
+    R[4][3] = ics * (h0 + cs * un);
+    R[4][4] = ics * (h0 - cs * un);
+
+    L[0][0] = nx * hm - irho * (v * nz - w * ny);
+    L[0][1] = gc * u * nx;
+    L[0][2] = gc * v * nx + nz * irho;
+    L[0][3] = gc * w * nx - ny * irho;
+    L[0][4] = -gc * nx;
+    L[1][0] = ny * hm - irho * (w * nx - u * nz);
+    L[1][1] = gc * u * ny - nz * irho;
+    L[1][2] = gc * v * ny;
+    L[1][3] = gc * w * ny + nx * irho;
+    L[1][4] = -gc * ny;
+    L[2][0] = nz * hm - irho * (u * ny - v * nx);
+    L[2][1] = gc * u * nz + ny * irho;
+    L[2][2] = gc * v * nz - nx * irho;
+    L[2][3] = gc * w * nz;
+    L[2][4] = -gc * nz;
+    const double gs = (gamma - 1.0) / cs;
+    L[3][0] = ics * ((gamma - 1.0) * q2 / 2. - cs * un);
+    L[3][1] = (-gs * u + nx) / s2;
+    L[3][2] = (-gs * v + ny) / s2;
+    L[3][3] = (-gs * w + nz) / s2;
+    L[3][4] = (gamma - 1.) * ics;
+    L[4][0] = ics * ((gamma - 1.0) * q2 / 2. + cs * un);
+    L[4][1] = (-gs * u - nx) / s2;
+    L[4][2] = (-gs * v - ny) / s2;
+    L[4][3] = (-gs * w - nz) / s2;
+    L[4][4] = (gamma - 1.) * ics;
+}
+
+// Jiang-Shu WENO5 deviation from the central flux (Phi_Function_W5, src/WENO.c:118-204).
+__device__ __forceinline__ double phi_w5(const double eps, const double d0, const double d1,
+                                         const double d2, const double d3)
+{
+    const double t0 = d0 - d1, t1 = d0 - 3. * d1;
+    const double t2 = d1 - d2, t3 = d1 + d2;
+    const double t4 = d2 - d3, t5 = 3. * d2 - d3;
+    const double is0 = 13. * t0 * t0 + 3. * t1 * t1;
+    const double is1 = 13. * t2 * t2 + 3. * t3 * t3;
+    const double is2 = 13. * t4 * t4 + 3. * t5 * t5;
+    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps;
+    const double a0 = 1. / (s0 * s0);
+    const double a1 = 6. / (s1 * s1);
+    const double a2 = 3. / (s2 * s2);
+    const double ia = 1. / (a0 + a1 + a2);
+    const double w0 = a0 * ia, w2 = a2 * ia;
+    const double q1 = d0 - 2. * d1 + d2;
+    const double qq2 = d1 - 2. * d2 + d3;
+    return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * qq2;
+}
+
+// Balsara-Shu WENO9 (Phi_Function_W9, src/WENO.c:11-116); f: 9 point values, d: 8 differences.
+__device__ __forceinline__ double phi_w9(const double eps, const double *f, const double *d)
+{
+    const double is0 = f[0] * (22658. * f[0] - 208501. * f[1] + 364863. * f[2] - 288007. * f[3] + 86329. * f[4])
+        + f[1] * (482963. * f[1] - 1704396. * f[2] + 1358458. * f[3] - 411487. * f[4])
+        + f[2] * (1521393. * f[2] - 2462076. * f[3] + 758823. * f[4])
+        + f[3] * (1020563. * f[3] - 649501. * f[4])
+        + f[4] * (107918. * f[4]);
+    const double is1 = f[1] * (6908. * f[1] - 60871. * f[2] + 99213. * f[3] - 70237. * f[4] + 18079. * f[5])
+        + f[2] * (138563. * f[2] - 464976. * f[3] + 337018. * f[4] - 88297. * f[5])
+        + f[3] * (406293. * f[3] - 611976. * f[4] + 165153. * f[5])
+        + f[4] * (242723. * f[4] - 140251. * f[5])
+        + f[5] * (22658. * f[5]);
+    const double is2 = f[2] * (6908. * f[2] - 51001. * f[3] + 67923. * f[4] - 38947. * f[5] + 8209. * f[6])
+        + f[3] * (104963. * f[3] - 299076. * f[4] + 179098. * f[5] - 38947. * f[6])
+        + f[4] * (231153. * f[4] - 299076. * f[5] + 67923. * f[6])
+        + f[5] * (104963. * f[5] - 51001. * f[6])
+        + f[6] * (6908. * f[6]);
+    const double is3 = f[3] * (22658. * f[3] - 140251. * f[4] + 165153. * f[5] - 88297. * f[6] + 18079. * f[7])
+        + f[4] * (242723. * f[4] - 611976. * f[5] + 337018. * f[6] - 70237. * f[7])
+        + f[5] * (406293. * f[5] - 464976. * f[6] + 99213. * f[7])
+        + f[6] * (138563. * f[6] - 60871. * f[7])
+        + f[7] * (6908. * f[7]);
+    const double is4 = f[4] * (107918. * f[4] - 649501. * f[5] + 758823. * f[6] - 411487. * f[7] + 86329. * f[8])
+        + f[5] * (1020563. * f[5] - 2462076. * f[6] + 1358458. * f[7] - 288007. * f[8])
+        + f[6] * (1521393. * f[6] - 1704396. * f[7] + 364863. * f[8])
+        + f[7] * (482963. * f[7] - 208501. * f[8])
+        + f[8] * (22658. * f[8]);
+    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps, s3 = is3 + eps, s4 = is4 + eps;
+    const double a0 = (1. / 126.) / (s0 * s0);
+    const double a1 = (10. / 63.) / (s1 * s1);
+    const double a2 = (10. / 21.) / (s2 * s2);
+    const double a3 = (20. / 63.) / (s3 * s3);
+    const double a4 = (5. / 126.) / (s4 * s4);
+    const double ia = 1. / (a0 + a1 + a2 + a3 + a4);
+    const double w0 = a0 * ia, w2 = a2 * ia, w3 = a3 * ia, w4 = a4 * ia;
+    const double q1 = d[0] - 4. * d[1] + 6. * d[2] - 4. * d[3] + d[4];
+    const double q2 = d[1] - 4. * d[2] + 6. * d[3] - 4. * d[4] + d[5];
+    const double q3 = d[2] - 4. * d[3] + 6. * d[4] - 4. * d[5] + d[6];
+    const double q4 = d[3] - 4. * d[4] + 6. * d[5] - 4. * d[6] + d[7];
+    return 1. / 5. * w0 * q1 + 1. / 20. * w2 * q2 + 1. / 60. * w3 * (3. * q2 - 2. * q3)
+         + 1. / 60. * w4 * (3. * q2 - 2. * q3 + 3. * q4) - 1. / 840. * (39. * q2 - 14. * q3 + 3. * q4);
+}
+
+// 30 viscous metric factors of one node for direction `dir` (CreateViscidMetric,
+// src/Functions.c:6245-6386): the Newtonian stress contracted with metric rows a (= dir) and
+// r_b (b = xi, eta, zeta), times J.
+__device__ __forceinline__ void visc_fac(const int dir, const double m[10], double fac[30])
+{
+    const double J = m[9];
+    const double a0 = m[3 * dir], a1 = m[3 * dir + 1], a2 = m[3 * dir + 2];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const double r0 = m[3 * b], r1 = m[3 * b + 1], r2 = m[3 * b + 2];
+        fac[0 + b] = J * (4.0 / 3.0 * r0 * a0 + r1 * a1 + r2 * a2);
+        fac[3 + b] = J * (-2.0 / 3.0 * r1 * a0 + r0 * a1);
+        fac[6 + b] = J * (-2.0 / 3.0 * r2 * a0 + r0 * a2);
+        fac[9 + b] = J * (r1 * a0 - 2.0 / 3.0 * r0 * a1);
+        fac[12 + b] = J * (r0 * a0 + 4.0 / 3.0 * r1 * a1 + r2 * a2);
+        fac[15 + b] = J * (-2.0 / 3.0 * r2 * a1 + r1 * a2);
+        fac[18 + b] = J * (r2 * a0 - 2.0 / 3.0 * r0 * a2);
+        fac[21 + b] = J * (r2 * a1 - 2.0 / 3.0 * r1 * a2);
+        fac[24 + b] = J * (r0 * a0 + r1 * a1 + 4.0 / 3.0 * r2 * a2);
+        fac[27 + b] = J * (r0 * a0 + r1 * a1 + r2 * a2);
+    }
+}
+
+__device__ __forceinline__ double pressure(const Consts &k, double rho, double u, double v, double w, double e)
+{
+    return fabs(rho * (k.gamma - 1) / k.upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
+}
+
+}  // namespace shx

@@ -1,0 +1,207 @@
+"""Python mirror of the C ABI in include/shockx.h (ctypes over libshockx.so).
+
+Mirrors the reference's entry points for the RHS / RK path: CalcRungeKutta
+(src/Functions.c:2034), checkNAN (src/Functions.c:5419), CreateMetric
+(src/Functions.c:6388) and the state/metric arrays of strct_U / strct_mesh
+(src/SHOCK.h:547-620).  Errors raise ShxError with the library's message; there is
+no CPU fallback: without the built library every call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libshockx.so")
+
+QUIRK_ZETA_VISC_J = 1
+QUIRK_METRIC_DIAG3D = 2
+QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
+BC_PERIODIC = 0
+FP64 = 0
+NUM_KERNELS = 10
+
+
+class ShxError(RuntimeError):
+    pass
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int), ("space_order", ctypes.c_int), ("time_order", ctypes.c_int),
+        ("n", ctypes.c_int * 3),
+        ("gamma", ctypes.c_double), ("mach", ctypes.c_double), ("reynolds", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("T0", ctypes.c_double),
+        ("inviscid", ctypes.c_int), ("manufactured", ctypes.c_int), ("quirks", ctypes.c_uint),
+        ("bc", ctypes.c_int * 6), ("precision", ctypes.c_int), ("eps_override", ctypes.c_double),
+    ]
+
+
+EXPORTS = [
+    "shx_last_error", "shx_device_count", "shx_create", "shx_destroy", "shx_padded_extent",
+    "shx_build_metric", "shx_upload_metric", "shx_download_metric", "shx_upload_state",
+    "shx_download_state", "shx_upload_source", "shx_calc_runge_kutta", "shx_run", "shx_rhs",
+    "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libshockx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ShxError(f"{LIB_PATH} is missing: build it with `python -m shock_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.shx_last_error.restype = ctypes.c_char_p
+        L.shx_create.argtypes = [ctypes.POINTER(Desc), ctypes.POINTER(P)]
+        L.shx_destroy.argtypes = [P]
+        L.shx_padded_extent.argtypes = [P, P]
+        L.shx_build_metric.argtypes = [P, P, P, P]
+        for f in ("shx_upload_metric", "shx_download_metric", "shx_upload_state", "shx_download_state",
+                  "shx_upload_source", "shx_rhs"):
+            getattr(L, f).argtypes = [P, P]
+        L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
+        L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
+        L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.shx_synchronize.argtypes = [P]
+        L.shx_set_timing.argtypes = [P, ctypes.c_int]
+        L.shx_kernel_times.argtypes = [P, P, P, P]
+        L.shx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().shx_last_error()
+        raise ShxError(f"shockx error {rc}: {msg.decode() if msg else ''}")
+
+
+def _f64(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"field shape {a.shape} != padded shape {shape}")
+    return a
+
+
+class Case:
+    """One block on the GPU (a shx_case handle)."""
+
+    def __init__(self, dims=2, space_order=5, time_order=4, n=(64, 64, 1), gamma=1.4, mach=0.5,
+                 reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, manufactured=False,
+                 quirks=QUIRKS_REFERENCE, bc=(BC_PERIODIC,) * 6, precision=FP64, eps_override=0.0):
+        d = Desc()
+        d.dims, d.space_order, d.time_order = dims, space_order, time_order
+        for a in range(3):
+            d.n[a] = n[a] if a < len(n) else 1
+        d.gamma, d.mach, d.reynolds, d.prandtl, d.T0 = gamma, mach, reynolds, prandtl, T0
+        d.inviscid, d.manufactured, d.quirks = int(inviscid), int(manufactured), quirks
+        for f in range(6):
+            d.bc[f] = bc[f]
+        d.precision, d.eps_override = precision, eps_override
+        h = ctypes.c_void_p()
+        _check(lib().shx_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.desc = d
+        ng = (ctypes.c_int * 3)()
+        _check(lib().shx_padded_extent(h, ng))
+        self.shape = (ng[0], ng[1], ng[2])
+        self.g = (space_order + 1) // 2
+
+    def close(self):
+        if self.h:
+            lib().shx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- helpers
+    def _ptrs(self, arrs):
+        arrs = [_f64(a, self.shape) for a in arrs]
+        return arrs, (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+    def _outs(self, nf):
+        arrs = [np.zeros(self.shape) for _ in range(nf)]
+        return arrs, (ctypes.c_void_p * nf)(*[a.ctypes.data for a in arrs])
+
+    # ---- geometry
+    def build_metric(self, xe, ye, ze):
+        xe = np.ascontiguousarray(xe, dtype=np.float64)
+        ye = np.ascontiguousarray(ye, dtype=np.float64)
+        ze = np.ascontiguousarray(ze, dtype=np.float64)
+        _check(lib().shx_build_metric(self.h, xe.ctypes.data, ye.ctypes.data, ze.ctypes.data))
+
+    def upload_metric(self, met):
+        keep, p = self._ptrs(met)
+        _check(lib().shx_upload_metric(self.h, p))
+
+    def download_metric(self):
+        out, p = self._outs(10)
+        _check(lib().shx_download_metric(self.h, p))
+        return out
+
+    # ---- state
+    def upload_state(self, prim):
+        keep, p = self._ptrs(prim)
+        _check(lib().shx_upload_state(self.h, p))
+
+    def download_state(self):
+        out, p = self._outs(5)
+        _check(lib().shx_download_state(self.h, p))
+        return out
+
+    def upload_source(self, src):
+        if src is None:
+            _check(lib().shx_upload_source(self.h, None))
+            return
+        keep, p = self._ptrs(src)
+        _check(lib().shx_upload_source(self.h, p))
+
+    # ---- hot path
+    def calc_runge_kutta(self, tau):
+        _check(lib().shx_calc_runge_kutta(self.h, tau))
+
+    def run(self, tau, nsteps):
+        _check(lib().shx_run(self.h, tau, nsteps))
+
+    def rhs(self):
+        out, p = self._outs(5)
+        _check(lib().shx_rhs(self.h, p))
+        return out
+
+    def check_nan(self):
+        c = ctypes.c_int()
+        _check(lib().shx_check_nan(self.h, ctypes.byref(c)))
+        return c.value
+
+    def cfl_dt(self, cfl):
+        dt = ctypes.c_double()
+        _check(lib().shx_cfl_dt(self.h, cfl, ctypes.byref(dt)))
+        return dt.value
+
+    def synchronize(self):
+        _check(lib().shx_synchronize(self.h))
+
+    def set_timing(self, on):
+        _check(lib().shx_set_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        names = (ctypes.c_char_p * NUM_KERNELS)()
+        ms = (ctypes.c_double * NUM_KERNELS)()
+        cnt = (ctypes.c_int * NUM_KERNELS)()
+        _check(lib().shx_kernel_times(self.h, names, ms, cnt))
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(NUM_KERNELS)}
