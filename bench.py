@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Throughput of SHOCK's RHS / RK-stage path on one MI355X.
+
+Workload (the headline configuration): 3D single block 256^3 on a periodic curvilinear grid
+x = xi + 0.05 sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta), WENO5 + viscous fluxes, FP64,
+classic RK4, Re = 2000, Mach 0.5, decaying turbulence from a seeded random-phase velocity
+field (E(k) ~ k^4 exp(-2 (k/4)^2), turbulent Mach 0.5, uniform rho and p), fixed tau from
+CFL 0.5 on the initial state.  A "step" is one full RK4 time step (4 stages); the metric is
+interior cell-updates per second per RK stage with the state resident in HBM.
+
+Prints one JSON line.  Per-kernel durations come from HIP events recorded on the library's
+stream during the timed region; the CPU baseline is the C oracle (OpenMP) timed on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from shock_amd import cases, shockx  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64 = 8
+
+
+def algo_bytes(name, n, ng, time_order=4):
+    """Algorithmic HBM bytes of one launch of kernel class `name` for an n^3 block with padded
+    extent ng^3: each input array element read once, each output element written once."""
+    nint = n ** 3
+    npad = ng ** 3
+    ghost = npad - nint
+    if name == "weno_xi":            # rho,u,v,w,e + metric row (3) + J in, Q (5) out
+        return nint * (9 + 5) * FP64
+    if name in ("weno_eta", "weno_zeta"):   # same + Q read-modify-write
+        return nint * (9 + 5 + 5) * FP64
+    if name == "calc_values":        # rho,u,v,w,e in, T, mu out (all padded nodes)
+        return npad * (5 + 2) * FP64
+    if name == "visc_grad":          # u,v,w,T in (padded), 12 derivative fields out
+        return npad * 4 * FP64 + 12 * n * ng * ng * FP64
+    if name == "visc_rk":            # U_n, U_s, Q, metric(10), grad(12), T, mu in; U_s+1 out;
+        qsum = 2 * 5 * (time_order - 1) / time_order if time_order == 4 else 0.0   # Qsum r/w
+        return nint * (5 + 5 + 5 + 10 + 12 + 2 + 5 + qsum) * FP64
+    if name == "ghost_periodic":     # 5 fields read + written on ghost nodes
+        return ghost * 5 * 2 * FP64
+    return None
+
+
+def cpu_baseline(n_sample, steps, seed, threads):
+    """C oracle (OpenMP, `threads` threads) on an n_sample^3 block of the same workload."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # noqa: E402  (test infrastructure: CPU baseline leg only)
+    nn = (n_sample,) * 3
+    o = orc.Oracle(dims=3, S=5, n=nn, time_order=4, mach=0.5, reynolds=2000.0)
+    xe, ye, ze = cases.periodic_ext_coords(3, 5, nn, cases.sine_bump_3d(0.05))
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    prim = cases.turbulence_3d_state(3, 5, nn, seed, mach=0.5)
+    tau = o.cfl_dt(0.5, met, prim)
+    un = [a.copy() for a in prim]
+    urk = [a.copy() for a in prim]
+    scratch = o.zeros(30)
+    o.step(tau, met, un, urk, None, scratch)     # warm-up step
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(tau, met, un, urk, None, scratch)
+        for a, b in zip(un, urk):
+            a[...] = b
+    dt = time.perf_counter() - t0
+    return n_sample ** 3 * 4 * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="interior points per direction")
+    ap.add_argument("--seed", type=int, default=20251202)
+    ap.add_argument("--cpu-n", type=int, default=96, help="CPU baseline sample size per direction")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout is "
+              "a design (one block per GPU, halo exchange per stage) and is not built.", file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    S, time_order = 5, 4
+    nn = (n, n, n)
+    t_setup = time.perf_counter()
+    case = shockx.Case(dims=3, space_order=S, time_order=time_order, n=nn, gamma=1.4, mach=0.5,
+                       reynolds=2000.0, prandtl=0.72, T0=300.0, quirks=shockx.QUIRKS_REFERENCE)
+    xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
+    case.build_metric(xe, ye, ze)
+    del xe, ye, ze
+    prim = cases.turbulence_3d_state(3, S, nn, args.seed, mach=0.5)
+    case.upload_state(prim)
+    del prim
+    tau = case.cfl_dt(0.5)
+    t_setup = time.perf_counter() - t_setup
+    print(f"[bench] setup {t_setup:.1f}s tau={tau:.6g} n={n}", file=sys.stderr, flush=True)
+
+    case.run(tau, args.warmup)
+    case.synchronize()
+    if not args.no_timing:
+        case.set_timing(True)
+    case.synchronize()
+    t0 = time.perf_counter()
+    case.run(tau, args.steps)
+    case.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = case.kernel_times() if not args.no_timing else {}
+    nan = case.check_nan()
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; non-finite nodes {nan}", file=sys.stderr, flush=True)
+
+    stages = args.steps * time_order
+    value = n ** 3 * stages / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    roofline = None
+    kernels = {}
+    if kt:
+        ng = n + S + 1
+        for name, (ms, cnt) in kt.items():
+            if cnt == 0:
+                continue
+            avg = ms / cnt
+            b = algo_bytes(name, n, ng, time_order)
+            kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt,
+                             "share": round(ms / (1e3 * elapsed), 4),
+                             "algo_GBs": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+        dom = max(kernels, key=lambda k: kt[k][0])
+        avg = kt[dom][0] / kt[dom][1]
+        b = algo_bytes(dom, n, ng, time_order)
+        ach = b / (avg * 1e-3) / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "traffic_config1.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                traffic = json.load(f).get(dom)
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algo_bytes": int(b), "avg_ms": round(avg, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps, args.seed, threads)
+        cpu = {"value": round(rate, 1), "unit": "cell-updates/s/RK-stage", "cores": threads, "kind": "port",
+               "sample": f"C oracle, {args.cpu_n}^3 block of the same workload (WENO5+viscous, FP64, RK4), "
+                         f"{args.cpu_steps} steps = {4 * args.cpu_steps} stages, {secs:.1f}s"}
+
+    # stage-level fraction of the HBM roofline at the compulsory 260 B/node/stage (fused RK4 stage)
+    stage_frac = value * 260.0 / (HBM_PEAK_GBS * 1e9)
+    out = {
+        "metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded random-phase turbulence)",
+        "config": {"workload": f"{n}^3 periodic curvilinear block, WENO5 + viscous, FP64, RK4, Re=2000",
+                   "grid": [n, n, n], "scheme": "WENO5", "viscous": True, "time_order": time_order,
+                   "precision": "fp64", "quirks": "reference", "tau": tau, "seed": args.seed},
+        "roofline": roofline,
+        "stage_hbm_frac_260B": round(stage_frac, 4),
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    case.close()
+
+
+if __name__ == "__main__":
+    main()

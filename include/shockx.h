@@ -108,6 +108,11 @@ int shx_cfl_dt(shx_case *c, double cfl, double *dt);
 
 int shx_synchronize(shx_case *c);
 
+/* Measurement aid (no reference counterpart): `reps` launches of a plain device copy of one
+ * padded field (8 * ng0*ng1*ng2 bytes read and written per launch), used to calibrate
+ * hardware byte counters against a kernel of known traffic. */
+int shx_copy_probe(shx_case *c, int reps);
+
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled, every launch is
  * bracketed by events; shx_kernel_times synchronizes and returns, per kernel class, the total
  * milliseconds and the launch count since the last reset.  names[k] points to static strings. */

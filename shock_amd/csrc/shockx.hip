@@ -560,6 +560,13 @@ __global__ void k_check_nan(Fields5 pr, Geom gm, int *count)
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
 }
 
+// Plain streaming copy of n doubles (calibration of hardware byte counters).
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 // ----------------------------------------------------------------------------------------
 // host side
 // ----------------------------------------------------------------------------------------
@@ -1093,6 +1100,17 @@ extern "C" int shx_cfl_dt(shx_case *c, double cfl, double *dt)
     double mx;
     memcpy(&mx, &bits, sizeof mx);
     *dt = mx > 0.0 ? cfl / mx : 0.0;
+    return SHX_OK;
+}
+
+extern "C" int shx_copy_probe(shx_case *c, int reps)
+{
+    if (!c || reps < 0) return set_err(SHX_ERR_ARG, "bad argument");
+    for (int r = 0; r < reps; r++) {
+        k_copy<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(c->T, c->mu, c->gm.ntot);
+        LAUNCH_CHECK();
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
     return SHX_OK;
 }
 

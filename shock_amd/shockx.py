@@ -42,6 +42,7 @@ EXPORTS = [
     "shx_build_metric", "shx_upload_metric", "shx_download_metric", "shx_upload_state",
     "shx_download_state", "shx_upload_source", "shx_calc_runge_kutta", "shx_run", "shx_rhs",
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
+    "shx_copy_probe",
 ]
 
 _lib = None
@@ -68,6 +69,7 @@ def lib():
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.shx_synchronize.argtypes = [P]
+        L.shx_copy_probe.argtypes = [P, ctypes.c_int]
         L.shx_set_timing.argtypes = [P, ctypes.c_int]
         L.shx_kernel_times.argtypes = [P, P, P, P]
         L.shx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -192,6 +194,9 @@ class Case:
         dt = ctypes.c_double()
         _check(lib().shx_cfl_dt(self.h, cfl, ctypes.byref(dt)))
         return dt.value
+
+    def copy_probe(self, reps=1):
+        _check(lib().shx_copy_probe(self.h, reps))
 
     def synchronize(self):
         _check(lib().shx_synchronize(self.h))
