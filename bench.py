@@ -43,9 +43,11 @@ def algo_bytes(name, n, ng, time_order=4):
         return npad * (5 + 2) * FP64
     if name == "visc_grad":          # u,v,w,T in (padded), 12 derivative fields out
         return npad * 4 * FP64 + 12 * n * ng * ng * FP64
-    if name == "visc_rk":            # U_n, U_s, Q, metric(10), grad(12), T, mu in; U_s+1 out;
-        qsum = 2 * 5 * (time_order - 1) / time_order if time_order == 4 else 0.0   # Qsum r/w
-        return nint * (5 + 5 + 5 + 10 + 12 + 2 + 5 + qsum) * FP64
+    if name in ("visc_xi", "visc_eta", "visc_zeta"):   # u,v,w,T,mu + 4 cross derivatives + diagonal
+        return nint * (5 + 4 + 4 + 4 + 4) * FP64      # metric (3) + J in; Q (4 momentum/energy) r/w
+    if name == "rk_update":          # U_n, U_s, Q, J in (Qsum r/w on RK4 stages), U_s+1 out
+        qsum = 2 * 5 * (time_order - 1) / time_order if time_order == 4 else 0.0
+        return nint * (5 + 5 + 5 + 1 + 5 + qsum) * FP64
     if name == "ghost_periodic":     # 5 fields read + written on ghost nodes
         return ghost * 5 * 2 * FP64
     return None

@@ -229,6 +229,396 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     }
 }
 
+// ---- viscous fluxes: face-based LDS sweep ------------------------------------------------
+// CalcViscidFluxesIn{Xi,Eta,Zeta}DirectionDirectly (src/ZD.c:47-694) restructured by faces:
+// the reference evaluates both half-point fluxes of every node; here each face flux of
+// direction D is computed once from an LDS pencil and differenced.  Face values: midpoint
+// derivatives of u,v,w,T along D and midpoint interpolations of the nodal cross derivatives;
+// coefficients K = (mu fac)_face averages; energy work term u_i (F(i+1/2) - F(i-1/2)) and the
+// nodal product term are evaluated per node as in the reference.
+struct ViscArgs {
+    const double *u, *v, *w, *T, *mu;
+    const double *cr[8];          // cross derivatives (layout depends on DIAG, see host)
+    CFields10 met;
+    Fields5 q;
+    long sD, sA, sB;
+    int nD, nA, nB, oD, oA, oB, ngD;
+    int coalesce_d;
+    int three;
+    int bug;                      // zeta stencil k offset taken from j (DIR = 2 only)
+    Consts k;
+};
+
+// fac_{i,k,b} (CreateViscidMetric, src/Functions.c:6245-6386) from row r (= b) and row a (= D).
+template <int I, int K>
+__device__ __forceinline__ double fac_ikb(const double J, const double r0, const double r1, const double r2,
+                                          const double a0, const double a1, const double a2)
+{
+    if constexpr (I == 0 && K == 0) return J * (4.0 / 3.0 * r0 * a0 + r1 * a1 + r2 * a2);
+    if constexpr (I == 0 && K == 1) return J * (-2.0 / 3.0 * r1 * a0 + r0 * a1);
+    if constexpr (I == 0 && K == 2) return J * (-2.0 / 3.0 * r2 * a0 + r0 * a2);
+    if constexpr (I == 1 && K == 0) return J * (r1 * a0 - 2.0 / 3.0 * r0 * a1);
+    if constexpr (I == 1 && K == 1) return J * (r0 * a0 + 4.0 / 3.0 * r1 * a1 + r2 * a2);
+    if constexpr (I == 1 && K == 2) return J * (-2.0 / 3.0 * r2 * a1 + r1 * a2);
+    if constexpr (I == 2 && K == 0) return J * (r2 * a0 - 2.0 / 3.0 * r0 * a2);
+    if constexpr (I == 2 && K == 1) return J * (r2 * a1 - 2.0 / 3.0 * r1 * a2);
+    if constexpr (I == 2 && K == 2) return J * (r0 * a0 + r1 * a1 + 4.0 / 3.0 * r2 * a2);
+    return J * (r0 * a0 + r1 * a1 + r2 * a2);   // heat flux (I == 3)
+}
+
+template <int DIR> struct Others { static constexpr int b1 = (DIR == 0) ? 1 : 0; static constexpr int b2 = (DIR == 2) ? 1 : 2; };
+
+// face flux (momentum x,y,z and heat) from slot values d[12] (slot k*3+b, T slots 9+b) and the
+// metric/mu of the face's two nodes (generic metric)
+template <int DIR>
+__device__ __forceinline__ void face_flux_generic(const double d[12], const double m0[10], const double m1[10],
+                                                  const double mu0, const double mu1, double F[4])
+{
+    const double J0 = m0[9], J1 = m1[9];
+    const double a00 = m0[3 * DIR], a01 = m0[3 * DIR + 1], a02 = m0[3 * DIR + 2];
+    const double a10 = m1[3 * DIR], a11 = m1[3 * DIR + 1], a12 = m1[3 * DIR + 2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) F[i] = 0.0;
+    F[3] = 0.0;
+#define SHX_FACE_TERM(I, K, B)                                                                    \
+    {                                                                                             \
+        const double f0 = fac_ikb<I, K>(J0, m0[3 * B], m0[3 * B + 1], m0[3 * B + 2], a00, a01, a02); \
+        const double f1 = fac_ikb<I, K>(J1, m1[3 * B], m1[3 * B + 1], m1[3 * B + 2], a10, a11, a12); \
+        F[I] += d[K * 3 + B] * (0.5 * (mu0 * f0 + mu1 * f1));                                     \
+    }
+#define SHX_FACE_ROW(I) \
+    SHX_FACE_TERM(I, 0, 0) SHX_FACE_TERM(I, 0, 1) SHX_FACE_TERM(I, 0, 2) \
+    SHX_FACE_TERM(I, 1, 0) SHX_FACE_TERM(I, 1, 1) SHX_FACE_TERM(I, 1, 2) \
+    SHX_FACE_TERM(I, 2, 0) SHX_FACE_TERM(I, 2, 1) SHX_FACE_TERM(I, 2, 2)
+    SHX_FACE_ROW(0)
+    SHX_FACE_ROW(1)
+    SHX_FACE_ROW(2)
+#undef SHX_FACE_ROW
+#undef SHX_FACE_TERM
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const double f0 = fac_ikb<3, 0>(J0, m0[3 * b], m0[3 * b + 1], m0[3 * b + 2], a00, a01, a02);
+        const double f1 = fac_ikb<3, 0>(J1, m1[3 * b], m1[3 * b + 1], m1[3 * b + 2], a10, a11, a12);
+        F[3] += d[9 + b] * (0.5 * (mu0 * f0 + mu1 * f1));
+    }
+}
+
+// diagonal metric (the reference's 3D metric): node values mD (row D, component D), mb[3]
+// (diagonal entries of all rows), J.  Only the non-zero factors of the generic table remain.
+template <int DIR>
+__device__ __forceinline__ void face_flux_diag(const double d[12], const double g0[3], const double J0,
+                                               const double g1[3], const double J1, const double mu0,
+                                               const double mu1, double F[4])
+{
+    const double mD0 = g0[DIR], mD1 = g1[DIR];
+    const double KA = 0.5 * (mu0 * (J0 * (mD0 * mD0)) + mu1 * (J1 * (mD1 * mD1)));
+    const double K43 = 0.5 * (mu0 * (J0 * (4.0 / 3.0 * mD0 * mD0)) + mu1 * (J1 * (4.0 / 3.0 * mD1 * mD1)));
+    double FD = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (k == DIR) { FD += d[k * 3 + k] * K43; continue; }
+        const double KC = 0.5 * (mu0 * (J0 * (-2.0 / 3.0 * g0[k] * mD0)) + mu1 * (J1 * (-2.0 / 3.0 * g1[k] * mD1)));
+        FD += d[k * 3 + k] * KC;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (i == DIR) { F[i] = FD; continue; }
+        const double KB = 0.5 * (mu0 * (J0 * (g0[i] * mD0)) + mu1 * (J1 * (g1[i] * mD1)));
+        F[i] = d[DIR * 3 + i] * KB + d[i * 3 + DIR] * KA;
+    }
+    F[3] = d[9 + DIR] * KA;
+}
+
+// nodal product term sum_blk g_{blk,D} sum_s g_s mu fac_{blk,s} (src/ZD.c:230-266)
+template <int DIR>
+__device__ __forceinline__ double node_prod_generic(const double g[9], const double m[10], const double mu)
+{
+    const double J = m[9], a0 = m[3 * DIR], a1 = m[3 * DIR + 1], a2 = m[3 * DIR + 2];
+    double prod = 0.0;
+#define SHX_PROD_TERM(I, K, B) \
+    prod += g[I * 3 + DIR] * g[K * 3 + B] * mu * fac_ikb<I, K>(J, m[3 * B], m[3 * B + 1], m[3 * B + 2], a0, a1, a2);
+#define SHX_PROD_ROW(I) \
+    SHX_PROD_TERM(I, 0, 0) SHX_PROD_TERM(I, 0, 1) SHX_PROD_TERM(I, 0, 2) \
+    SHX_PROD_TERM(I, 1, 0) SHX_PROD_TERM(I, 1, 1) SHX_PROD_TERM(I, 1, 2) \
+    SHX_PROD_TERM(I, 2, 0) SHX_PROD_TERM(I, 2, 1) SHX_PROD_TERM(I, 2, 2)
+    SHX_PROD_ROW(0)
+    SHX_PROD_ROW(1)
+    SHX_PROD_ROW(2)
+#undef SHX_PROD_ROW
+#undef SHX_PROD_TERM
+    return prod;
+}
+
+template <int DIR>
+__device__ __forceinline__ double node_prod_diag(const double g[9], const double gm[3], const double J, const double mu)
+{
+    const double mD = gm[DIR];
+    double prod = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double gd = g[i * 3 + DIR];
+        if (i == DIR) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double f = (k == DIR) ? J * (4.0 / 3.0 * mD * mD) : J * (-2.0 / 3.0 * gm[k] * mD);
+                prod += gd * g[k * 3 + k] * mu * f;
+            }
+        } else {
+            prod += gd * g[DIR * 3 + i] * mu * (J * (gm[i] * mD));
+            prod += gd * g[i * 3 + DIR] * mu * (J * (mD * mD));
+        }
+    }
+    return prod;
+}
+
+// LDS slots
+enum { VS_U, VS_V, VS_W, VS_T, VS_MU, VS_CR };
+
+template <int S, int DIR, int TA, int TDI, bool DIAG>
+__global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
+{
+    constexpr int G = (S + 1) / 2;
+    constexpr int WD = TDI - 1 + 2 * G;
+    constexpr int NW = WD * TA;
+    constexpr int NT = TA * TDI;
+    constexpr int NC = DIAG ? 4 : 8;                 // cross-derivative fields
+    constexpr int NM = DIAG ? 4 : 10;                // metric values per node
+    constexpr int VS_M = VS_CR + NC;
+    constexpr int NVS = VS_M + NM;
+    constexpr int b1 = Others<DIR>::b1, b2 = Others<DIR>::b2;
+    constexpr int NP = 2 * G + 1;                    // line positions used by the zeta-index quirk
+    constexpr int NLV = 4 + NC;                      // values per position: u,v,w,T + cross
+    __shared__ double sm[NVS * NW];
+    __shared__ double line[(DIR == 2) ? TA * NP * NLV : 1];
+    __shared__ double lined[(DIR == 2) ? TA * 2 * NLV : 1];
+
+    const int t = threadIdx.x;
+    const int a0 = blockIdx.x * TA;
+    const int b = blockIdx.y;
+    const int d0 = blockIdx.z * (TDI - 1);
+    const long bbase = (long)(b + a.oB) * a.sB;
+    const Consts &k = a.k;
+
+    for (int idx = t; idx < NW; idx += NT) {
+        int aa, dd;
+        if (a.coalesce_d) { aa = idx / WD; dd = idx - aa * WD; }
+        else { dd = idx / TA; aa = idx - dd * TA; }
+        const int ai = min(a0 + aa, a.nA - 1);
+        int dp = d0 - G + dd + a.oD;
+        dp = max(0, min(dp, a.ngD - 1));
+        const long n = bbase + (long)(ai + a.oA) * a.sA + (long)dp * a.sD;
+        const int o = aa * WD + dd;
+        sm[VS_U * NW + o] = a.u[n];
+        sm[VS_V * NW + o] = a.v[n];
+        sm[VS_W * NW + o] = a.w[n];
+        sm[VS_T * NW + o] = a.T[n];
+        sm[VS_MU * NW + o] = a.mu[n];
+#pragma unroll
+        for (int c = 0; c < NC; c++) sm[(VS_CR + c) * NW + o] = a.cr[c][n];
+        if constexpr (DIAG) {
+            sm[(VS_M + 0) * NW + o] = a.met.f[0][n];
+            sm[(VS_M + 1) * NW + o] = a.met.f[4][n];
+            sm[(VS_M + 2) * NW + o] = a.met.f[8][n];
+            sm[(VS_M + 3) * NW + o] = a.met.f[9][n];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 10; c++) sm[(VS_M + c) * NW + o] = a.met.f[c][n];
+        }
+    }
+    const bool bug = (DIR == 2) && a.bug;
+    if (bug) {
+        // the reference's zeta stencil (src/ZD.c:547-548) reads positions k' = j + l - g on the
+        // (i, j) line for every k: the face values are constant along the line
+        for (int idx = t; idx < TA * NP * NLV; idx += NT) {
+            const int ln = idx / (NP * NLV);
+            const int rem = idx - ln * NP * NLV;
+            const int pos = rem / NLV, f = rem - pos * NLV;
+            const int ai = min(a0 + ln, a.nA - 1);
+            const int jpad = ai + a.oA;                      // A axis is j for DIR = 2
+            const long n = bbase + (long)jpad * a.sA + (jpad - G + pos);
+            double val;
+            if (f == 0) val = a.u[n];
+            else if (f == 1) val = a.v[n];
+            else if (f == 2) val = a.w[n];
+            else if (f == 3) val = a.T[n];
+            else val = a.cr[f - 4][n];
+            line[(ln * NP + pos) * NLV + f] = val;
+        }
+    }
+    __syncthreads();
+    const int ta = t / TDI, di = t - ta * TDI;
+    if (bug && di < 2) {
+        // di = 0: minus-side values (positions 0..S), di = 1: plus side (positions 1..S+1)
+        const double *Lp = &line[(ta * NP + di) * NLV];
+#pragma unroll
+        for (int f = 0; f < NLV; f++) {
+            double acc = 0.0;
+            const double *cf = (f < 4) ? k.zd_mid : k.zd_int;
+#pragma unroll
+            for (int l = 0; l <= S; l++) acc += cf[l] * Lp[l * NLV + f];
+            lined[(ta * 2 + di) * NLV + f] = acc;
+        }
+    }
+    if (bug) __syncthreads();
+
+    const int ob = ta * WD + di;
+    const int oc = ob + G - 1;
+    double out[8];
+    {
+        const double mu0 = sm[VS_MU * NW + oc], mu1 = sm[VS_MU * NW + oc + 1];
+        // slot layout d[k*3+b], T at 9+b
+        auto face = [&](const double *own4, const double *cr, double F[4]) {
+            double d[12];
+#pragma unroll
+            for (int s = 0; s < 12; s++) d[s] = 0.0;
+            d[0 * 3 + DIR] = own4[0]; d[1 * 3 + DIR] = own4[1]; d[2 * 3 + DIR] = own4[2]; d[9 + DIR] = own4[3];
+            if constexpr (DIAG) {
+                d[b1 * 3 + b1] = cr[0]; d[DIR * 3 + b1] = cr[1];
+                d[b2 * 3 + b2] = cr[2]; d[DIR * 3 + b2] = cr[3];
+                double g0[3], g1[3];
+                g0[0] = sm[(VS_M + 0) * NW + oc]; g0[1] = sm[(VS_M + 1) * NW + oc]; g0[2] = sm[(VS_M + 2) * NW + oc];
+                g1[0] = sm[(VS_M + 0) * NW + oc + 1]; g1[1] = sm[(VS_M + 1) * NW + oc + 1]; g1[2] = sm[(VS_M + 2) * NW + oc + 1];
+                face_flux_diag<DIR>(d, g0, sm[(VS_M + 3) * NW + oc], g1, sm[(VS_M + 3) * NW + oc + 1], mu0, mu1, F);
+            } else {
+                d[0 * 3 + b1] = cr[0]; d[1 * 3 + b1] = cr[1]; d[2 * 3 + b1] = cr[2]; d[9 + b1] = cr[3];
+                d[0 * 3 + b2] = cr[4]; d[1 * 3 + b2] = cr[5]; d[2 * 3 + b2] = cr[6]; d[9 + b2] = cr[7];
+                double m0[10], m1[10];
+#pragma unroll
+                for (int c = 0; c < 10; c++) { m0[c] = sm[(VS_M + c) * NW + oc]; m1[c] = sm[(VS_M + c) * NW + oc + 1]; }
+                face_flux_generic<DIR>(d, m0, m1, mu0, mu1, F);
+            }
+        };
+        if (!bug) {
+            double own4[4] = {0.0, 0.0, 0.0, 0.0}, cr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) cr[c] = 0.0;
+#pragma unroll
+            for (int l = 0; l <= S; l++) {
+                const double cm = k.zd_mid[l], ci = k.zd_int[l];
+                own4[0] += cm * sm[VS_U * NW + ob + l];
+                own4[1] += cm * sm[VS_V * NW + ob + l];
+                own4[2] += cm * sm[VS_W * NW + ob + l];
+                own4[3] += cm * sm[VS_T * NW + ob + l];
+#pragma unroll
+                for (int c = 0; c < NC; c++) cr[c] += ci * sm[(VS_CR + c) * NW + ob + l];
+            }
+            double F[4];
+            face(own4, cr, F);
+#pragma unroll
+            for (int m = 0; m < 4; m++) { out[m] = F[m]; out[4 + m] = F[m]; }
+        } else {
+            double FA[4], FB[4];
+            face(&lined[(ta * 2 + 0) * NLV], &lined[(ta * 2 + 0) * NLV + 4], FA);
+            face(&lined[(ta * 2 + 1) * NLV], &lined[(ta * 2 + 1) * NLV + 4], FB);
+#pragma unroll
+            for (int m = 0; m < 4; m++) { out[m] = FA[m]; out[4 + m] = FB[m]; }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; m++) sm[m * NT + t] = out[m];
+    __syncthreads();
+
+    if (di < TDI - 1) {
+        const int dn = d0 + di, an = a0 + ta;
+        if (dn < a.nD && an < a.nA) {
+            // reload node data from global memory (the LDS window was overwritten)
+            const long n = bbase + (long)(an + a.oA) * a.sA + (long)(dn + a.oD) * a.sD;
+            // faces: minus = this thread's face (A values), plus = next thread's face (B values)
+            double dF[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) dF[m] = sm[(4 + m) * NT + t + 1] - sm[m * NT + t];
+            // nodal gradients: own-direction derivatives and the cross ones
+            double g[9];
+#pragma unroll
+            for (int s = 0; s < 9; s++) g[s] = 0.0;
+            {
+                double du = 0.0, dv = 0.0, dw = 0.0;
+#pragma unroll
+                for (int l = 0; l <= S + 1; l++) {
+                    const long nl = n + (long)(l - G) * a.sD;
+                    du += k.zd_der[l] * a.u[nl];
+                    dv += k.zd_der[l] * a.v[nl];
+                    dw += k.zd_der[l] * a.w[nl];
+                }
+                g[0 * 3 + DIR] = du; g[1 * 3 + DIR] = dv; g[2 * 3 + DIR] = a.three ? dw : 0.0;
+            }
+            const double mu = a.mu[n];
+            double prod;
+            if constexpr (DIAG) {
+                g[b1 * 3 + b1] = a.cr[0][n]; g[DIR * 3 + b1] = a.cr[1][n];
+                g[b2 * 3 + b2] = a.cr[2][n]; g[DIR * 3 + b2] = a.cr[3][n];
+                const double gmv[3] = {a.met.f[0][n], a.met.f[4][n], a.met.f[8][n]};
+                prod = node_prod_diag<DIR>(g, gmv, a.met.f[9][n], mu);
+            } else {
+                g[0 * 3 + b1] = a.cr[0][n]; g[1 * 3 + b1] = a.cr[1][n]; g[2 * 3 + b1] = a.cr[2][n];
+                g[0 * 3 + b2] = a.cr[4][n]; g[1 * 3 + b2] = a.cr[5][n]; g[2 * 3 + b2] = a.cr[6][n];
+                double m[10];
+#pragma unroll
+                for (int c = 0; c < 10; c++) m[c] = a.met.f[c][n];
+                prod = node_prod_generic<DIR>(g, m, mu);
+            }
+            const double work = a.u[n] * dF[0] + a.v[n] * dF[1] + a.w[n] * dF[2];
+            a.q.f[1][n] += k.psi * dF[0];
+            a.q.f[2][n] += k.psi * dF[1];
+            if (a.three) a.q.f[3][n] += k.psi * dF[2];
+            a.q.f[4][n] += k.gam_heat * dF[3] + k.psi * (work + prod);
+        }
+    }
+}
+
+// Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
+struct UpdArgs {
+    CFields5 un, us;
+    Fields5 out, q, qsum;
+    CFields5 src;
+    const double *J;
+    Geom gm;
+    double a, b, qf, ff, mf, tau;
+    int first_stage, has_src;
+};
+
+__global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
+{
+    const Geom &gm = A.gm;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if (t >= nint) return;
+    const int kk = (int)(t % gm.n[2]);
+    const long r = t / gm.n[2];
+    const int jj = (int)(r % gm.n[1]);
+    const int ii = (int)(r / gm.n[1]);
+    const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+    const double J = A.J[n];
+    double qv[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) qv[m] = A.q.f[m][n];
+    if (A.has_src) {
+        qv[0] += J * A.src.f[0][n];
+        qv[1] += J * A.src.f[1][n];
+        qv[2] += J * A.src.f[2][n];
+        if (gm.dims == 3) qv[3] += J * A.src.f[3][n];
+        qv[4] += J * A.src.f[4][n];
+    }
+    double qs[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
+        qs[m] = old + A.mf * qv[m];
+        if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
+    }
+    const double tj = A.tau / J;
+    const double rho_n = A.un.f[0][n], rho_s = A.us.f[0][n];
+    const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
+    A.out.f[0][n] = rho;
+#pragma unroll
+    for (int m = 1; m < 5; m++) {
+        if (m == 3 && gm.dims == 2) { A.out.f[3][n] = 0.0; continue; }
+        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * A.us.f[m][n] * rho_s
+                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
+    }
+}
+
 // ---- viscous derivatives ------------------------------------------------------------------
 // grad order: u_xi,u_eta,u_zeta, v_xi,v_eta,v_zeta, w_xi,w_eta,w_zeta, T_xi,T_eta,T_zeta.
 // Direction d is evaluated where the node's d index is interior (all transverse nodes).
@@ -264,187 +654,22 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
     }
 }
 
-// ---- viscous fluxes + source + RK update --------------------------------------------------
-struct RKArgs {
-    CFields5 un;        // U_lastStep
-    CFields5 us;        // stage input U_RK
-    Fields5 out;        // stage output U_RK(s+1)
-    Fields5 q;          // stage residual (inviscid part already accumulated)
-    Fields5 qsum;       // RK4 accumulator
-    CFields5 src;       // manufactured source (may be null)
-    CFields10 met;
-    CFields12 grad;
-    const double *T, *mu;
-    Geom gm;
-    Consts k;
-    double a, b, qf, ff, mf, tau;
-    int first_stage;
-    int viscous;
-    int zeta_bug;
-    int has_src;
-};
-
-template <int dir, int S>
-__device__ __forceinline__ void visc_dir(const RKArgs &A, const long n, const int i,
-                                         const int j, double &qx, double &qy, double &qz, double &qe)
+// Q += J * src (manufactured-solution source) for the stage residual returned by shx_rhs.
+__global__ void k_add_source(Fields5 q, CFields5 src, const double *J, Geom gm)
 {
-    const Geom &gm = A.gm;
-    const Consts &k = A.k;
-    const long st = gm.st[dir];
-    constexpr int half = (S + 1) / 2;
-    const int three = (gm.dims == 3);
-    constexpr int np = S + 1;  // points per midpoint stencil
-    double dm[12], dpl[12];
-#pragma unroll
-    for (int s = 0; s < 12; s++) { dm[s] = 0.0; dpl[s] = 0.0; }
-    long bug_base = 0;
-    const bool bug = (dir == 2) && A.zeta_bug;
-    if (bug) bug_base = ((long)i * gm.ng[1] + j) * gm.ng[2];
-#pragma unroll
-    for (int l = 0; l < np; l++) {
-        long im, ip;
-        if (bug) { im = bug_base + (j + l - half); ip = im + 1; }
-        else { im = n + (long)(l - half) * st; ip = im + st; }
-        const double cm = k.zd_mid[l], ci = k.zd_int[l];
-        dm[0 * 3 + dir] += cm * A.us.f[1][im]; dpl[0 * 3 + dir] += cm * A.us.f[1][ip];
-        dm[1 * 3 + dir] += cm * A.us.f[2][im]; dpl[1 * 3 + dir] += cm * A.us.f[2][ip];
-        if (three) { dm[2 * 3 + dir] += cm * A.us.f[3][im]; dpl[2 * 3 + dir] += cm * A.us.f[3][ip]; }
-        dm[9 + dir] += cm * A.T[im]; dpl[9 + dir] += cm * A.T[ip];
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            if (b == dir || (b == 2 && !three)) continue;
-            dm[b] += ci * A.grad.f[b][im]; dpl[b] += ci * A.grad.f[b][ip];
-            dm[3 + b] += ci * A.grad.f[3 + b][im]; dpl[3 + b] += ci * A.grad.f[3 + b][ip];
-            if (three) { dm[6 + b] += ci * A.grad.f[6 + b][im]; dpl[6 + b] += ci * A.grad.f[6 + b][ip]; }
-            dm[9 + b] += ci * A.grad.f[9 + b][im]; dpl[9 + b] += ci * A.grad.f[9 + b][ip];
-        }
-    }
-    double mv[10], fm[30], f0[30], fp[30];
-    const long nm = n - st, npp = n + st;
-#pragma unroll
-    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][nm];
-    visc_fac(dir, mv, fm);
-#pragma unroll
-    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][n];
-    visc_fac(dir, mv, f0);
-#pragma unroll
-    for (int s = 0; s < 10; s++) mv[s] = A.met.f[s][npp];
-    visc_fac(dir, mv, fp);
-    const double mum = A.mu[nm], mu0 = A.mu[n], mup = A.mu[npp];
-    double mom[3];
-    double work = 0.0;
-    const double vel[3] = {A.us.f[1][n], A.us.f[2][n], A.us.f[3][n]};
-#pragma unroll
-    for (int blk = 0; blk < 3; blk++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int s = 0; s < 9; s++) {
-            const double Km = 0.5 * (mum * fm[blk * 9 + s] + mu0 * f0[blk * 9 + s]);
-            const double Kp = 0.5 * (mu0 * f0[blk * 9 + s] + mup * fp[blk * 9 + s]);
-            const double tdiff = dpl[s] * Kp - dm[s] * Km;
-            acc += tdiff;
-            work += vel[blk] * tdiff;
-        }
-        mom[blk] = acc;
-    }
-    double heat = 0.0;
-#pragma unroll
-    for (int s = 0; s < 3; s++) {
-        const double Km = 0.5 * (mum * fm[27 + s] + mu0 * f0[27 + s]);
-        const double Kp = 0.5 * (mu0 * f0[27 + s] + mup * fp[27 + s]);
-        heat += dpl[9 + s] * Kp - dm[9 + s] * Km;
-    }
-    double prod = 0.0;
-#pragma unroll
-    for (int blk = 0; blk < 3; blk++) {
-        const double gd = A.grad.f[blk * 3 + dir][n];
-#pragma unroll
-        for (int s = 0; s < 9; s++) prod += gd * A.grad.f[s][n] * mu0 * f0[blk * 9 + s];
-    }
-    qx += k.psi * mom[0];
-    qy += k.psi * mom[1];
-    if (three) qz += k.psi * mom[2];
-    qe += k.gam_heat * heat + k.psi * (work + prod);
-}
-
-template <int S>
-__device__ __forceinline__ void node_rhs(const RKArgs &A, const long n, const int i, const int j, double qv[5])
-{
-    const Geom &gm = A.gm;
-    double q0 = A.q.f[0][n], qx = A.q.f[1][n], qy = A.q.f[2][n], qz = A.q.f[3][n], qe = A.q.f[4][n];
-    if (A.viscous) {
-        visc_dir<0, S>(A, n, i, j, qx, qy, qz, qe);
-        visc_dir<1, S>(A, n, i, j, qx, qy, qz, qe);
-        if (gm.dims == 3) visc_dir<2, S>(A, n, i, j, qx, qy, qz, qe);
-    }
-    const double J = A.met.f[9][n];
-    if (A.has_src) {
-        q0 += J * A.src.f[0][n];
-        qx += J * A.src.f[1][n];
-        qy += J * A.src.f[2][n];
-        if (gm.dims == 3) qz += J * A.src.f[3][n];
-        qe += J * A.src.f[4][n];
-    }
-    qv[0] = q0; qv[1] = qx; qv[2] = qy; qv[3] = qz; qv[4] = qe;
-}
-
-__device__ __forceinline__ long interior_node(const Geom &gm, const long t, int &i, int &j)
-{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if (t >= nint) return;
     const int kk = (int)(t % gm.n[2]);
     const long r = t / gm.n[2];
     const int jj = (int)(r % gm.n[1]);
     const int ii = (int)(r / gm.n[1]);
-    i = ii + gm.lo[0];
-    j = jj + gm.lo[1];
-    return (long)i * gm.st[0] + (long)j * gm.st[1] + kk + gm.lo[2];
-}
-
-// Viscous fluxes + source + RK update (src/Functions.c:2300-2365) of one interior node.
-template <int S>
-__global__ __launch_bounds__(256) void k_visc_rk(RKArgs A)
-{
-    const Geom &gm = A.gm;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    if (t >= nint) return;
-    int i, j;
-    const long n = interior_node(gm, t, i, j);
-    double qv[5];
-    node_rhs<S>(A, n, i, j, qv);
-    const double J = A.met.f[9][n];
-    double qs[5];
+    const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
-        qs[m] = old + A.mf * qv[m];
-        if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
+        if (m == 3 && gm.dims == 2) continue;
+        q.f[m][n] += J[n] * src.f[m][n];
     }
-    const double tj = A.tau / J;
-    const double rho_n = A.un.f[0][n], rho_s = A.us.f[0][n];
-    const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
-    A.out.f[0][n] = rho;
-#pragma unroll
-    for (int m = 1; m < 5; m++) {
-        if (m == 3 && gm.dims == 2) { A.out.f[3][n] = 0.0; continue; }
-        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * A.us.f[m][n] * rho_s
-                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
-    }
-}
-
-// Completes the stage residual in place (viscous + source, no update); used by shx_rhs.
-template <int S>
-__global__ __launch_bounds__(256) void k_rhs_finish(RKArgs A)
-{
-    const Geom &gm = A.gm;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    if (t >= nint) return;
-    int i, j;
-    const long n = interior_node(gm, t, i, j);
-    double qv[5];
-    node_rhs<S>(A, n, i, j, qv);
-#pragma unroll
-    for (int m = 0; m < 5; m++) A.q.f[m][n] = qv[m];
 }
 
 // ---- metric --------------------------------------------------------------------------------
@@ -590,10 +815,10 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                               \
     } while (0)
 
-enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC_RK, KT_RK, KT_REDUCE, KT_METRIC };
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC };
 static const char *g_knames[SHX_NUM_KERNELS] = {
     "ghost_periodic", "calc_values", "weno_xi", "weno_eta", "weno_zeta",
-    "visc_grad", "visc_rk", "rk_update", "reduce", "metric"};
+    "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric"};
 
 struct shx_case {
     shx_desc d;
@@ -605,6 +830,7 @@ struct shx_case {
     int cur;
     double *q[5], *qsum[5], *met[10], *grad[12], *src[5], *T, *mu;
     int has_src, has_metric, has_state;
+    int diag;              // 3D metric is diagonal (only xi_x, eta_y, zeta_z and J non-zero)
     unsigned long long *d_red;
     int *d_cnt;
     int timing;
@@ -843,6 +1069,7 @@ extern "C" int shx_build_metric(shx_case *c, const double *xe, const double *ye,
     HIPCHK(hipStreamSynchronize(c->stream));
     hipFree(dx); hipFree(dy); hipFree(dz);
     c->has_metric = 1;
+    c->diag = (gm.dims == 3 && (c->d.quirks & SHX_QUIRK_METRIC_DIAG3D)) ? 1 : 0;
     return SHX_OK;
 }
 
@@ -852,6 +1079,13 @@ extern "C" int shx_upload_metric(shx_case *c, const double *const met[10])
     for (int m = 0; m < 10; m++) { int rc = copy_in(c, c->met[m], met[m]); if (rc) return rc; }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_metric = 1;
+    // the diagonal-metric kernels apply when every off-diagonal entry is exactly zero
+    int diag = (c->gm.dims == 3);
+    static const int offd[6] = {1, 2, 3, 5, 6, 7};
+    for (int s = 0; s < 6 && diag; s++)
+        for (long n = 0; n < c->gm.ntot; n++)
+            if (met[offd[s]][n] != 0.0) { diag = 0; break; }
+    c->diag = diag;
     return SHX_OK;
 }
 
@@ -934,27 +1168,10 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     return tend(c);
 }
 
-static void fill_rk_args(shx_case *c, RKArgs &A, double *const *us, double *const *out)
-{
-    memset(&A, 0, sizeof A);
-    for (int m = 0; m < 5; m++) {
-        A.un.f[m] = c->prim[c->cur][m];
-        A.us.f[m] = us[m];
-        A.out.f[m] = out ? out[m] : nullptr;
-        A.q.f[m] = c->q[m];
-        A.qsum.f[m] = c->qsum[m];
-        A.src.f[m] = c->src[m];
-    }
-    for (int m = 0; m < 10; m++) A.met.f[m] = c->met[m];
-    for (int m = 0; m < 12; m++) A.grad.f[m] = c->grad[m];
-    A.T = c->T; A.mu = c->mu;
-    A.gm = c->gm; A.k = c->k;
-    A.viscous = !c->d.inviscid;
-    A.zeta_bug = (c->d.quirks & SHX_QUIRK_ZETA_VISC_J) ? 1 : 0;
-    A.has_src = c->has_src;
-}
-
-// fluxes of one stage into c->q (inviscid part) and the viscous pre-pass
+// residual of one stage into c->q: ghost fill, derived quantities, viscous derivatives, WENO
+// sweeps of every direction, then the viscous sweeps (the reference's accumulation order,
+// src/Functions.c:2161-2272)
+static int visc_sweep(shx_case *c, double *const *pr, int dir);
 static int stage_fluxes(shx_case *c, double *const *pr)
 {
     const Geom &gm = c->gm;
@@ -962,39 +1179,108 @@ static int stage_fluxes(shx_case *c, double *const *pr)
     if ((rc = ghost_fill(c, pr, 5))) return rc;
     if (!c->d.inviscid) {
         CFields5 cp;
+        Fields12 gf;
         for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+        for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         if ((rc = tbeg(c, KT_VALUES))) return rc;
         k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm.ntot, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
-    }
-    for (int dir = 0; dir < gm.dims; dir++)
-        if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
-    if (!c->d.inviscid) {
-        CFields5 cp;
-        Fields12 gf;
-        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
-        for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         if ((rc = tbeg(c, KT_GRAD))) return rc;
         k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
     }
+    for (int dir = 0; dir < gm.dims; dir++)
+        if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
+    if (!c->d.inviscid)
+        for (int dir = 0; dir < gm.dims; dir++)
+            if ((rc = visc_sweep(c, pr, dir))) return rc;
     return SHX_OK;
 }
 
-static int launch_visc_rk(shx_case *c, const RKArgs &A, long nint, int finish_only)
+template <int S, int DIR, int TA, int TDI>
+static void launch_visc_sweep_t(shx_case *c, const ViscArgs &a, bool diag)
 {
-    const unsigned nb = nblk(nint, 256);
-    if (c->d.space_order == 5) {
-        if (finish_only) k_rhs_finish<5><<<nb, 256, 0, c->stream>>>(A);
-        else k_visc_rk<5><<<nb, 256, 0, c->stream>>>(A);
+    dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+    if (diag) k_visc_sweep<S, DIR, TA, TDI, true><<<grid, TA * TDI, 0, c->stream>>>(a);
+    else k_visc_sweep<S, DIR, TA, TDI, false><<<grid, TA * TDI, 0, c->stream>>>(a);
+}
+
+template <int S>
+static void launch_visc_sweep_s(shx_case *c, const ViscArgs &a, int dir, bool diag)
+{
+    constexpr int TDN = (S == 5) ? 16 : 12;   // non-contiguous sweep: 16 x TDN tile
+    if (a.coalesce_d) {
+        if (dir == 1) launch_visc_sweep_t<S, 1, 4, 64>(c, a, false);       // 2D eta sweep
+        else launch_visc_sweep_t<S, 2, 4, 64>(c, a, diag);
     } else {
-        if (finish_only) k_rhs_finish<9><<<nb, 256, 0, c->stream>>>(A);
-        else k_visc_rk<9><<<nb, 256, 0, c->stream>>>(A);
+        if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDN>(c, a, diag);
+        else launch_visc_sweep_t<S, 1, 16, TDN>(c, a, diag);
     }
+}
+
+// viscous face sweep along axis dir: Q += viscous flux divergence (after the WENO sweeps)
+static int visc_sweep(shx_case *c, double *const *pr, int dir)
+{
+    const Geom &gm = c->gm;
+    ViscArgs a;
+    memset(&a, 0, sizeof a);
+    a.u = pr[1]; a.v = pr[2]; a.w = pr[3]; a.T = c->T; a.mu = c->mu;
+    const int b1 = (dir == 0) ? 1 : 0, b2 = (dir == 2) ? 1 : 2;
+    const bool diag = c->diag && gm.dims == 3;
+    if (diag) {
+        a.cr[0] = c->grad[b1 * 3 + b1]; a.cr[1] = c->grad[dir * 3 + b1];
+        a.cr[2] = c->grad[b2 * 3 + b2]; a.cr[3] = c->grad[dir * 3 + b2];
+    } else {
+        for (int q = 0; q < 3; q++) { a.cr[q] = c->grad[q * 3 + b1]; a.cr[4 + q] = c->grad[q * 3 + b2]; }
+        a.cr[3] = c->grad[9 + b1]; a.cr[7] = c->grad[9 + b2];
+    }
+    for (int m = 0; m < 10; m++) a.met.f[m] = c->met[m];
+    for (int m = 0; m < 5; m++) a.q.f[m] = c->q[m];
+    int A, B;
+    if (gm.dims == 3) {
+        if (dir == 2) { A = 1; B = 0; } else { A = 2; B = (dir == 0) ? 1 : 0; }
+    } else {
+        A = (dir == 0) ? 1 : 0; B = 2;
+    }
+    a.sD = gm.st[dir]; a.sA = gm.st[A]; a.sB = gm.st[B];
+    a.nD = gm.n[dir]; a.nA = gm.n[A]; a.nB = gm.n[B];
+    a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
+    a.ngD = gm.ng[dir];
+    a.coalesce_d = (a.sD == 1);
+    a.three = (gm.dims == 3);
+    a.bug = (dir == 2 && (c->d.quirks & SHX_QUIRK_ZETA_VISC_J)) ? 1 : 0;
+    a.k = c->k;
+    int rc = tbeg(c, KT_VISC0 + dir);
+    if (rc) return rc;
+    if (c->d.space_order == 5) launch_visc_sweep_s<5>(c, a, dir, diag);
+    else launch_visc_sweep_s<9>(c, a, dir, diag);
     LAUNCH_CHECK();
-    return SHX_OK;
+    return tend(c);
+}
+
+static int rk_update(shx_case *c, double *const *us, double *const *out, int s, double tau)
+{
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    UpdArgs A;
+    memset(&A, 0, sizeof A);
+    for (int m = 0; m < 5; m++) {
+        A.un.f[m] = c->prim[c->cur][m]; A.us.f[m] = us[m]; A.out.f[m] = out[m];
+        A.q.f[m] = c->q[m]; A.qsum.f[m] = c->qsum[m]; A.src.f[m] = c->src[m];
+    }
+    A.J = c->met[9];
+    A.gm = gm;
+    A.a = c->rk_a[s]; A.b = c->rk_b[s]; A.qf = c->rk_q[s]; A.ff = c->rk_f[s]; A.mf = c->rk_m[s];
+    A.tau = tau;
+    A.first_stage = (s == 0);
+    A.has_src = c->has_src;
+    int rc = tbeg(c, KT_RK);
+    if (rc) return rc;
+    k_rk_update<<<nblk(nint, 256), 256, 0, c->stream>>>(A);
+    LAUNCH_CHECK();
+    return tend(c);
 }
 
 static int check_ready(shx_case *c)
@@ -1016,14 +1302,7 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
     int in = un, outb = bx;
     for (int s = 0; s < c->d.time_order; s++) {
         if ((rc = stage_fluxes(c, c->prim[in]))) return rc;
-        RKArgs A;
-        fill_rk_args(c, A, c->prim[in], c->prim[outb]);
-        A.a = c->rk_a[s]; A.b = c->rk_b[s]; A.qf = c->rk_q[s]; A.ff = c->rk_f[s]; A.mf = c->rk_m[s];
-        A.tau = tau;
-        A.first_stage = (s == 0);
-        if ((rc = tbeg(c, A.viscous ? KT_VISC_RK : KT_RK))) return rc;
-        if ((rc = launch_visc_rk(c, A, nint, 0))) return rc;
-        if ((rc = tend(c))) return rc;
+        if ((rc = rk_update(c, c->prim[in], c->prim[outb], s, tau))) return rc;
         // stage s+1 reads what stage s wrote and writes the other scratch buffer
         in = outb;
         outb = (outb == bx) ? by : bx;
@@ -1051,9 +1330,13 @@ extern "C" int shx_rhs(shx_case *c, double *const q[5])
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
     if ((rc = stage_fluxes(c, c->prim[c->cur]))) return rc;
-    RKArgs A;
-    fill_rk_args(c, A, c->prim[c->cur], nullptr);
-    if ((rc = launch_visc_rk(c, A, nint, 1))) return rc;
+    if (c->has_src) {
+        Fields5 qf;
+        CFields5 sf;
+        for (int m = 0; m < 5; m++) { qf.f[m] = c->q[m]; sf.f[m] = c->src[m]; }
+        k_add_source<<<nblk(nint, 256), 256, 0, c->stream>>>(qf, sf, c->met[9], gm);
+        LAUNCH_CHECK();
+    }
     for (int m = 0; m < 5; m++) { rc = copy_out(c, q[m], c->q[m]); if (rc) return rc; }
     HIPCHK(hipStreamSynchronize(c->stream));
     return SHX_OK;

@@ -19,7 +19,7 @@ QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC = 0
 FP64 = 0
-NUM_KERNELS = 10
+NUM_KERNELS = 12
 
 
 class ShxError(RuntimeError):
