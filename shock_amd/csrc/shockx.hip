@@ -22,6 +22,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 
 using namespace shx;
@@ -118,8 +119,9 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
                      e = a.pr.f[4][n];
         const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n], J = a.J[n];
         const double th = u * mx + v * my + w * mz;
-        const double p = pressure(k, rho, u, v, w, e);
-        const double cs = sqrt(k.upsilon * k.gamma * p / rho);
+        const double p = pressure_fast(k, rho, u, v, w, e);
+        const double ir = frcp(rho);
+        const double cs = sqrt(k.upsilon * k.gamma * p * ir);
         const double pu = p * k.upsilon;
         const double rt = rho * th;
         const int o = aa * WD + dd;
@@ -127,21 +129,22 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         sm[V_F1 * NW + o] = J * (rt * u + mx * pu);
         sm[V_F2 * NW + o] = J * (rt * v + my * pu);
         sm[V_F3 * NW + o] = J * (rt * w + mz * pu);
-        sm[V_F4 * NW + o] = J * (rt * (e + pu / rho));
+        sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
         sm[V_U0 * NW + o] = rho;
         sm[V_U1 * NW + o] = rho * u;
         sm[V_U2 * NW + o] = rho * v;
         sm[V_U3 * NW + o] = rho * w;
         sm[V_U4 * NW + o] = rho * e;
         const double mn = sqrt(mx * mx + my * my + mz * mz);
+        const double imn = frcp(mn);
         sm[V_LAM * NW + o] = fabs(th) + cs * mn;
         sm[V_U * NW + o] = u;
         sm[V_V * NW + o] = v;
         sm[V_W * NW + o] = w;
         sm[V_C * NW + o] = cs;
-        sm[V_NX * NW + o] = mx / mn;
-        sm[V_NY * NW + o] = my / mn;
-        sm[V_NZ * NW + o] = mz / mn;
+        sm[V_NX * NW + o] = mx * imn;
+        sm[V_NY * NW + o] = my * imn;
+        sm[V_NZ * NW + o] = mz * imn;
         sm[V_J * NW + o] = J;
     }
     __syncthreads();
@@ -161,42 +164,56 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         const double nx = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
         const double ny = 0.5 * (sm[V_NY * NW + oc] + sm[V_NY * NW + oc + 1]);
         const double nz = 0.5 * (sm[V_NZ * NW + oc] + sm[V_NZ * NW + oc + 1]);
-        double L[5][5], Rm[5][5];
-        eigvec(nx, ny, nz, u, v, w, rho, cs, k.gamma, L, Rm);
+        const Eig e = make_eig(nx, ny, nz, u, v, w, rho, cs, k.gamma);
         double lam = 0.0;
 #pragma unroll
         for (int s = 0; s < 2 * G; s++) lam = fmax(lam, sm[V_LAM * NW + ob + s]);
         const double lj = lam * Jh;
 
+        // Characteristic projection per field c: a_s = L_c . F_s and b_s = L_c . U_s on the 2G
+        // stencil nodes, f+_s = (a_s + lam J b_s)/2 on nodes m-r..m+r and f-_s = (a - lam J b)/2
+        // on the mirrored nodes m+r+1..m-r+1 (src/WENO.c:334-385).
         double phis[5];
+        auto project = [&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            double Lc[5];
+            left_row<c>(e, Lc);
+            double av[2 * G], bv[2 * G];
 #pragma unroll
-        for (int c = 0; c < 5; c++) {
+            for (int s = 0; s < 2 * G; s++) {
+                double aa = 0.0, bb = 0.0;
+#pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    aa += Lc[q] * sm[(V_F0 + q) * NW + ob + s];
+                    bb += Lc[q] * sm[(V_U0 + q) * NW + ob + s];
+                }
+                av[s] = aa;
+                bv[s] = bb;
+            }
             double fp[S], fm[S];
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                const int op = ob + s;                 // plus stencil: nodes m-r .. m+r
-                const int om = ob + 2 * R + 1 - s;     // minus stencil: nodes m+r+1 .. m-r+1
-                double accp = 0.0, accm = 0.0;
-#pragma unroll
-                for (int q = 0; q < 5; q++) {
-                    accp += L[c][q] * (sm[(V_F0 + q) * NW + op] + lj * sm[(V_U0 + q) * NW + op]);
-                    accm += L[c][q] * (sm[(V_F0 + q) * NW + om] - lj * sm[(V_U0 + q) * NW + om]);
-                }
-                fp[s] = 0.5 * accp;
-                fm[s] = 0.5 * accm;
+                fp[s] = 0.5 * (av[s] + lj * bv[s]);
+                fm[s] = 0.5 * (av[2 * G - 1 - s] - lj * bv[2 * G - 1 - s]);
             }
             if constexpr (S == 5) {
-                phis[c] = -phi_w5(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
-                          + phi_w5(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]);
+                phis[c] = -phi_w5_fast(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
+                          + phi_w5_fast(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]);
             } else {
                 double dp_[8], dm_[8];
 #pragma unroll
                 for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
                 phis[c] = -phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_);
             }
-        }
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
+        };
+        project(std::integral_constant<int, 0>{});
+        project(std::integral_constant<int, 1>{});
+        project(std::integral_constant<int, 2>{});
+        project(std::integral_constant<int, 3>{});
+        project(std::integral_constant<int, 4>{});
+
+        auto assemble = [&](auto mm) {
+            constexpr int m = decltype(mm)::value;
             const double *F = &sm[(V_F0 + m) * NW];
             double th;
             if constexpr (S == 5) {
@@ -205,9 +222,15 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
                 th = 1. / 840. * (533. * (F[oc + 1] + F[oc]) - 139. * (F[oc + 2] + F[oc - 1])
                                   + 29. * (F[oc + 3] + F[oc - 2]) - 3. * (F[oc + 4] + F[oc - 3]));
             }
-            out[m] = phis[0] * Rm[m][0] + phis[1] * Rm[m][1] + phis[2] * Rm[m][2] + phis[3] * Rm[m][3]
-                     + phis[4] * Rm[m][4] + th;
-        }
+            double Rr[5];
+            right_row<m>(e, Rr);
+            out[m] = phis[0] * Rr[0] + phis[1] * Rr[1] + phis[2] * Rr[2] + phis[3] * Rr[3] + phis[4] * Rr[4] + th;
+        };
+        assemble(std::integral_constant<int, 0>{});
+        assemble(std::integral_constant<int, 1>{});
+        assemble(std::integral_constant<int, 2>{});
+        assemble(std::integral_constant<int, 3>{});
+        assemble(std::integral_constant<int, 4>{});
     }
     __syncthreads();
 #pragma unroll
@@ -942,6 +965,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     k.gam_heat = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * d.prandtl);
     k.suth = 100.4 / d.T0;
     k.eps = d.eps_override > 0.0 ? d.eps_override : 10. * pow(DBL_MIN, 0.5);
+    k.gm1_ups = (d.gamma - 1.0) / k.upsilon;
     k.mms = d.manufactured ? 1 : 0;
     if (S == 9) {
         const double in9[10] = {1. / 1260., -23. / 2520., 127. / 2520., -473. / 2520., 1627. / 2520.,

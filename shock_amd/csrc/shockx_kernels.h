@@ -11,6 +11,7 @@ namespace shx {
 // Physical and scheme constants shared by every kernel (kernel-argument struct).
 struct Consts {
     double gamma, upsilon, psi, gam_heat, suth, eps;
+    double gm1_ups;   // (gamma - 1) / Upsilon
     double zd_mid[10], zd_int[10], zd_der[11];
     int mms;
 };
@@ -88,6 +89,115 @@ __device__ __forceinline__ void eigvec(const double nx, const double ny, const d
     L[4][2] = (-gs * v - ny) / s2;
     L[4][3] = (-gs * w - nz) / s2;
     L[4][4] = (gamma - 1.) * ics;
+}
+
+// Reciprocal from the hardware estimate (v_rcp_f64) refined by two Newton steps: within an ulp
+// of 1/x for the normal, finite arguments the flux kernels pass, at a fraction of the cost of
+// the IEEE-exact division sequence.
+__device__ __forceinline__ double frcp(const double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Averaged interface state and the scalars the eigenvector entries share.
+struct Eig {
+    double nx, ny, nz, u, v, w, rho, cs;
+    double gm1, q2, hm, gc, ics, irho, un, gs, h0;
+};
+
+__device__ __forceinline__ Eig make_eig(double nx, double ny, double nz, double u, double v, double w,
+                                        double rho, double cs, double gamma)
+{
+    Eig e;
+    e.nx = nx; e.ny = ny; e.nz = nz; e.u = u; e.v = v; e.w = w; e.rho = rho; e.cs = cs;
+    e.gm1 = gamma - 1.0;
+    e.q2 = u * u + v * v + w * w;
+    const double ic = frcp(cs);
+    const double M = sqrt(e.q2) * ic;
+    e.hm = 1.0 - 0.5 * e.gm1 * (M * M);
+    e.gc = e.gm1 * (ic * ic);
+    e.ics = ic * 0.70710678118654752440;
+    e.irho = frcp(rho);
+    e.un = u * nx + v * ny + w * nz;
+    e.gs = e.gm1 * ic;
+    e.h0 = e.q2 / 2. + cs * cs * frcp(e.gm1);
+    return e;
+}
+
+// Row C of the left eigenvector matrix (src/WENO.c:1208-1236).
+template <int C>
+__device__ __forceinline__ void left_row(const Eig &e, double L[5])
+{
+    constexpr double is2 = 0.70710678118654752440;
+    if constexpr (C == 0) {
+        L[0] = e.nx * e.hm - e.irho * (e.v * e.nz - e.w * e.ny);
+        L[1] = e.gc * e.u * e.nx; L[2] = e.gc * e.v * e.nx + e.nz * e.irho;
+        L[3] = e.gc * e.w * e.nx - e.ny * e.irho; L[4] = -e.gc * e.nx;
+    } else if constexpr (C == 1) {
+        L[0] = e.ny * e.hm - e.irho * (e.w * e.nx - e.u * e.nz);
+        L[1] = e.gc * e.u * e.ny - e.nz * e.irho; L[2] = e.gc * e.v * e.ny;
+        L[3] = e.gc * e.w * e.ny + e.nx * e.irho; L[4] = -e.gc * e.ny;
+    } else if constexpr (C == 2) {
+        L[0] = e.nz * e.hm - e.irho * (e.u * e.ny - e.v * e.nx);
+        L[1] = e.gc * e.u * e.nz + e.ny * e.irho; L[2] = e.gc * e.v * e.nz - e.nx * e.irho;
+        L[3] = e.gc * e.w * e.nz; L[4] = -e.gc * e.nz;
+    } else if constexpr (C == 3) {
+        L[0] = e.ics * (e.gm1 * e.q2 / 2. - e.cs * e.un);
+        L[1] = (-e.gs * e.u + e.nx) * is2; L[2] = (-e.gs * e.v + e.ny) * is2;
+        L[3] = (-e.gs * e.w + e.nz) * is2; L[4] = e.gm1 * e.ics;
+    } else {
+        L[0] = e.ics * (e.gm1 * e.q2 / 2. + e.cs * e.un);
+        L[1] = (-e.gs * e.u - e.nx) * is2; L[2] = (-e.gs * e.v - e.ny) * is2;
+        L[3] = (-e.gs * e.w - e.nz) * is2; L[4] = e.gm1 * e.ics;
+    }
+}
+
+// Row M of the right eigenvector matrix (src/WENO.c:1176-1204): R[M][0..4].
+template <int M>
+__device__ __forceinline__ void right_row(const Eig &e, double R[5])
+{
+    if constexpr (M == 0) {
+        R[0] = e.nx; R[1] = e.ny; R[2] = e.nz; R[3] = e.ics; R[4] = e.ics;
+    } else if constexpr (M == 1) {
+        R[0] = e.u * e.nx; R[1] = e.u * e.ny - e.rho * e.nz; R[2] = e.u * e.nz + e.rho * e.ny;
+        R[3] = e.ics * (e.u + e.nx * e.cs); R[4] = e.ics * (e.u - e.nx * e.cs);
+    } else if constexpr (M == 2) {
+        R[0] = e.v * e.nx + e.rho * e.nz; R[1] = e.v * e.ny; R[2] = e.v * e.nz - e.rho * e.nx;
+        R[3] = e.ics * (e.v + e.ny * e.cs); R[4] = e.ics * (e.v - e.ny * e.cs);
+    } else if constexpr (M == 3) {
+        R[0] = e.w * e.nx - e.rho * e.ny; R[1] = e.w * e.ny + e.rho * e.nx; R[2] = e.w * e.nz;
+        R[3] = e.ics * (e.w + e.nz * e.cs); R[4] = e.ics * (e.w - e.nz * e.cs);
+    } else {
+        R[0] = e.q2 / 2. * e.nx + e.rho * (e.v * e.nz - e.w * e.ny);
+        R[1] = e.q2 / 2. * e.ny + e.rho * (e.w * e.nx - e.u * e.nz);
+        R[2] = e.q2 / 2. * e.nz + e.rho * (e.u * e.ny - e.v * e.nx);
+        R[3] = e.ics * (e.h0 + e.cs * e.un); R[4] = e.ics * (e.h0 - e.cs * e.un);
+    }
+}
+
+// WENO5 deviation with the reciprocal estimate (same formula as phi_w5 below).
+__device__ __forceinline__ double phi_w5_fast(const double eps, const double d0, const double d1,
+                                              const double d2, const double d3)
+{
+    const double t0 = d0 - d1, t1 = d0 - 3. * d1;
+    const double t2 = d1 - d2, t3 = d1 + d2;
+    const double t4 = d2 - d3, t5 = 3. * d2 - d3;
+    const double is0 = 13. * t0 * t0 + 3. * t1 * t1;
+    const double is1 = 13. * t2 * t2 + 3. * t3 * t3;
+    const double is2 = 13. * t4 * t4 + 3. * t5 * t5;
+    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps;
+    const double a0 = frcp(s0 * s0);
+    const double a1 = 6. * frcp(s1 * s1);
+    const double a2 = 3. * frcp(s2 * s2);
+    const double ia = frcp(a0 + a1 + a2);
+    const double w0 = a0 * ia, w2 = a2 * ia;
+    const double q1 = d0 - 2. * d1 + d2;
+    const double qq2 = d1 - 2. * d2 + d3;
+    return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * qq2;
 }
 
 // Jiang-Shu WENO5 deviation from the central flux (Phi_Function_W5, src/WENO.c:118-204).
@@ -181,6 +291,13 @@ __device__ __forceinline__ void visc_fac(const int dir, const double m[10], doub
 __device__ __forceinline__ double pressure(const Consts &k, double rho, double u, double v, double w, double e)
 {
     return fabs(rho * (k.gamma - 1) / k.upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
+}
+
+// p = |rho (gamma-1)/Upsilon (e - |u|^2/2)| with the constant ratio folded (CalcValues,
+// src/Functions.c:2445-2459)
+__device__ __forceinline__ double pressure_fast(const Consts &k, double rho, double u, double v, double w, double e)
+{
+    return fabs(rho * k.gm1_ups * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
 }
 
 }  // namespace shx
