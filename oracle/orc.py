@@ -181,6 +181,20 @@ def phi_w9(eps, f, d):
     return lib().orc_phi_w9(eps, _ptr(f), _ptr(d))
 
 
+def is_w5(d):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_is_w5(_ptr(d), _ptr(out))
+    return out
+
+
+def is_w9(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(5)
+    lib().orc_is_w9(_ptr(f), _ptr(out))
+    return out
+
+
 def eigvec(nh, u, v, w, rho, c, gamma):
     nh = np.ascontiguousarray(nh, dtype=np.float64)
     L = np.zeros(25)

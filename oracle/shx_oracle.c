@@ -162,13 +162,19 @@ void orc_eigvec(const double nh[3], double u, double v, double w, double rho, do
 
 /* Phi_Function_W5, src/WENO.c:118-204: Jiang-Shu WENO5 deviation from the central flux,
  * smoothness indicators from the flux differences d[0..3] (scaled by 12), weights 1:6:3. */
+void orc_is_w5(const double *d, double is[3])
+{
+    is[0] = 13. * (d[0] - d[1]) * (d[0] - d[1]) + 3. * (d[0] - 3. * d[1]) * (d[0] - 3. * d[1]);
+    is[1] = 13. * (d[1] - d[2]) * (d[1] - d[2]) + 3. * (d[1] + d[2]) * (d[1] + d[2]);
+    is[2] = 13. * (d[2] - d[3]) * (d[2] - d[3]) + 3. * (3. * d[2] - d[3]) * (3. * d[2] - d[3]);
+}
+
 double orc_phi_w5(double eps, const double *f, const double *d)
 {
-    double is0, is1, is2, a0, a1, a2, w0, w2, q1, q2;
+    double is0, is1, is2, a0, a1, a2, w0, w2, q1, q2, isv[3];
     (void)f;
-    is0 = 13. * (d[0] - d[1]) * (d[0] - d[1]) + 3. * (d[0] - 3. * d[1]) * (d[0] - 3. * d[1]);
-    is1 = 13. * (d[1] - d[2]) * (d[1] - d[2]) + 3. * (d[1] + d[2]) * (d[1] + d[2]);
-    is2 = 13. * (d[2] - d[3]) * (d[2] - d[3]) + 3. * (3. * d[2] - d[3]) * (3. * d[2] - d[3]);
+    orc_is_w5(d, isv);
+    is0 = isv[0]; is1 = isv[1]; is2 = isv[2];
     a0 = 1. / ((is0 + eps) * (is0 + eps));
     a1 = 6. / ((is1 + eps) * (is1 + eps));
     a2 = 3. / ((is2 + eps) * (is2 + eps));
@@ -182,10 +188,9 @@ double orc_phi_w5(double eps, const double *f, const double *d)
 /* Phi_Function_W9, src/WENO.c:11-116: Balsara-Shu WENO9 smoothness indicators from the point
  * fluxes f[0..8], optimal weights {1/126, 10/63, 10/21, 20/63, 5/126}, deviation from
  * 4th differences of the flux differences d[0..7]. */
-double orc_phi_w9(double eps, const double *f, const double *d)
+void orc_is_w9(const double *f, double is[5])
 {
-    static const double dk[5] = {1. / 126., 10. / 63., 10. / 21., 20. / 63., 5. / 126.};
-    double is0, is1, is2, is3, is4, a0, a1, a2, a3, a4, w0, w2, w3, w4, q1, q2, q3, q4;
+    double is0, is1, is2, is3, is4;
     is0 = f[0] * (22658. * f[0] - 208501. * f[1] + 364863. * f[2] - 288007. * f[3] + 86329. * f[4])
         + f[1] * (482963. * f[1] - 1704396. * f[2] + 1358458. * f[3] - 411487. * f[4])
         + f[2] * (1521393. * f[2] - 2462076. * f[3] + 758823. * f[4])
@@ -211,6 +216,15 @@ double orc_phi_w9(double eps, const double *f, const double *d)
         + f[6] * (1521393. * f[6] - 1704396. * f[7] + 364863. * f[8])
         + f[7] * (482963. * f[7] - 208501. * f[8])
         + f[8] * (22658. * f[8]);
+    is[0] = is0; is[1] = is1; is[2] = is2; is[3] = is3; is[4] = is4;
+}
+
+double orc_phi_w9(double eps, const double *f, const double *d)
+{
+    static const double dk[5] = {1. / 126., 10. / 63., 10. / 21., 20. / 63., 5. / 126.};
+    double is0, is1, is2, is3, is4, a0, a1, a2, a3, a4, w0, w2, w3, w4, q1, q2, q3, q4, isv[5];
+    orc_is_w9(f, isv);
+    is0 = isv[0]; is1 = isv[1]; is2 = isv[2]; is3 = isv[3]; is4 = isv[4];
     a0 = dk[0] / ((is0 + eps) * (is0 + eps));
     a1 = dk[1] / ((is1 + eps) * (is1 + eps));
     a2 = dk[2] / ((is2 + eps) * (is2 + eps));

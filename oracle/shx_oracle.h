@@ -84,6 +84,9 @@ void orc_eigvec(const double nh[3], double u, double v, double w, double rho, do
                 double gamma, double L[25], double R[25]);
 double orc_phi_w5(double eps, const double *f, const double *d);
 double orc_phi_w9(double eps, const double *f, const double *d);
+/* smoothness indicators as used inside orc_phi_w5 (from 4 differences) and orc_phi_w9 (9 values) */
+void orc_is_w5(const double *d, double is[3]);
+void orc_is_w9(const double *f, double is[5]);
 
 void orc_weno_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
                   double *const der[8], double *const q[5]);

@@ -115,6 +115,9 @@ def random_phase_velocity(n, seed, kpeak=4.0, mt=0.5, c=1.0):
     spec = kk_safe ** 4 * np.exp(-2.0 * (kk_safe / kpeak) ** 2)
     amp = np.sqrt(spec / (4.0 * np.pi * kk_safe * kk_safe))
     amp[kk == 0.0] = 0.0
+    # Nyquist modes have no representable -k partner: drop them so the field stays solenoidal
+    nyq = (np.abs(KX) == n // 2) | (np.abs(KY) == n // 2) | (np.abs(KZ) == n // 2)
+    amp[nyq] = 0.0
     uh = []
     for _ in range(3):
         ph = rng.uniform(0.0, 2.0 * np.pi, size=kk.shape)

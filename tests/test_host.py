@@ -1,0 +1,84 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every symbol the
+header declares; the case builders produce what the workloads specify."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from shock_amd import cases, shockx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "shockx.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(shx_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_and_python_mirror_agree():
+    assert declared_symbols() == sorted(shockx.EXPORTS)
+
+
+@pytest.mark.skipif(not os.path.exists(shockx.LIB_PATH), reason="libshockx.so not built")
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", shockx.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (shx_[a-z_0-9]+)", out))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, missing
+    lib = shockx.lib()          # loads without a GPU; no compute call is made here
+    for s in declared_symbols():
+        assert hasattr(lib, s)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(shockx, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(shockx, "_lib", None)
+    with pytest.raises(shockx.ShxError):
+        shockx.lib()
+
+
+def test_padded_shape_and_ghost_width():
+    assert cases.padded_shape(3, 5, (256, 256, 256)) == ((262, 262, 262), 3)
+    assert cases.padded_shape(3, 9, (384, 384, 384)) == ((394, 394, 394), 5)
+    assert cases.padded_shape(2, 5, (64, 64, 1)) == ((70, 70, 1), 3)
+
+
+def test_periodic_coordinates_wrap_with_translation():
+    n = (8, 6, 5)
+    xe, ye, ze = cases.periodic_ext_coords(3, 5, n, cases.sine_bump_3d(0.05))
+    g = 3
+    # extended index e <-> padded e-1; interior padded g..g+n-1; ghost padded g-1 = interior last - 1.0
+    assert np.allclose(xe[g, 2:-2, 2:-2], xe[g + n[0], 2:-2, 2:-2] - 1.0, atol=0)
+    assert np.array_equal(ye[:, g + 1, :], ye[:, g + 1 + n[1], :] - 1.0)
+    assert np.array_equal(ze[:, :, 1], ze[:, :, 1 + n[2]] - 1.0)
+    # interior nodes follow the mapping exactly
+    xi = np.arange(n[0]) / n[0]
+    assert np.allclose(xe[g + 1:g + 1 + n[0], g + 1, g + 1], xi + 0.0)   # eta = zeta = 0 row: bump vanishes
+
+
+def test_turbulence_field_is_solenoidal_with_requested_mach():
+    n = 32
+    u, v, w = cases.random_phase_velocity(n, 11, mt=0.5, c=2.0)
+    rms = np.sqrt(np.mean(u * u + v * v + w * w))
+    assert abs(rms - 1.0) < 1e-12
+    k = np.fft.fftfreq(n, d=1.0 / n)
+    KX, KY, KZ = np.meshgrid(k, k, k, indexing="ij")
+    div = KX * np.fft.fftn(u) + KY * np.fft.fftn(v) + KZ * np.fft.fftn(w)
+    assert np.max(np.abs(div)) < 1e-9 * np.max(np.abs(np.fft.fftn(u)))
+    # same seed, same field
+    u2, _, _ = cases.random_phase_velocity(n, 11, mt=0.5, c=2.0)
+    assert np.array_equal(u, u2)
+
+
+def test_manufactured_state_in_shock_units():
+    x = np.linspace(0, 1, 9)[:, None, None] * np.ones((1, 9, 1))
+    y = np.transpose(x, (1, 0, 2))
+    rho, u, v, w, e = cases.mms_periodic_2d_state(x, y, gamma=1.4, mach=0.5)
+    ups = 1.0 / (1.4 * 0.25)
+    p = (e - 0.5 * (u * u + v * v)) * (0.4 * rho) / ups
+    # p_S = gamma * (1/gamma + 0.05 cos(2 pi (x+y)))
+    assert np.allclose(p, 1.0 + 1.4 * 0.05 * np.cos(2 * np.pi * (x + y)))
+    assert np.allclose(u * 0.5, 0.5 + 0.1 * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * y))
