@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="launch a plain copy kernel of one padded field this many times after setup "
+                         "(reference traffic for hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -113,6 +116,8 @@ def main():
     t_setup = time.perf_counter() - t_setup
     print(f"[bench] setup {t_setup:.1f}s tau={tau:.6g} n={n}", file=sys.stderr, flush=True)
 
+    if args.copy_probe:
+        case.copy_probe(args.copy_probe)
     case.run(tau, args.warmup)
     case.synchronize()
     if not args.no_timing:

@@ -412,6 +412,7 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
     constexpr int NP = 2 * G + 1;                    // line positions used by the zeta-index quirk
     constexpr int NLV = 4 + NC;                      // values per position: u,v,w,T + cross
     __shared__ double sm[NVS * NW];
+    __shared__ double fo[8 * NT];                    // face outputs (minus/plus sides)
     __shared__ double line[(DIR == 2) ? TA * NP * NLV : 1];
     __shared__ double lined[(DIR == 2) ? TA * 2 * NLV : 1];
 
@@ -537,21 +538,20 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
             for (int m = 0; m < 4; m++) { out[m] = FA[m]; out[4 + m] = FB[m]; }
         }
     }
-    __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 8; m++) sm[m * NT + t] = out[m];
+    for (int m = 0; m < 8; m++) fo[m * NT + t] = out[m];
     __syncthreads();
 
     if (di < TDI - 1) {
         const int dn = d0 + di, an = a0 + ta;
         if (dn < a.nD && an < a.nA) {
-            // reload node data from global memory (the LDS window was overwritten)
             const long n = bbase + (long)(an + a.oA) * a.sA + (long)(dn + a.oD) * a.sD;
+            const int p = ob + G;                    // window position of this node
             // faces: minus = this thread's face (A values), plus = next thread's face (B values)
             double dF[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) dF[m] = sm[(4 + m) * NT + t + 1] - sm[m * NT + t];
-            // nodal gradients: own-direction derivatives and the cross ones
+            for (int m = 0; m < 4; m++) dF[m] = fo[(4 + m) * NT + t + 1] - fo[m * NT + t];
+            // nodal gradients: own-direction derivatives from the window, cross ones stored
             double g[9];
 #pragma unroll
             for (int s = 0; s < 9; s++) g[s] = 0.0;
@@ -559,29 +559,30 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
                 double du = 0.0, dv = 0.0, dw = 0.0;
 #pragma unroll
                 for (int l = 0; l <= S + 1; l++) {
-                    const long nl = n + (long)(l - G) * a.sD;
-                    du += k.zd_der[l] * a.u[nl];
-                    dv += k.zd_der[l] * a.v[nl];
-                    dw += k.zd_der[l] * a.w[nl];
+                    du += k.zd_der[l] * sm[VS_U * NW + ob + l];
+                    dv += k.zd_der[l] * sm[VS_V * NW + ob + l];
+                    dw += k.zd_der[l] * sm[VS_W * NW + ob + l];
                 }
                 g[0 * 3 + DIR] = du; g[1 * 3 + DIR] = dv; g[2 * 3 + DIR] = a.three ? dw : 0.0;
             }
-            const double mu = a.mu[n];
+            const double mu = sm[VS_MU * NW + p];
             double prod;
             if constexpr (DIAG) {
-                g[b1 * 3 + b1] = a.cr[0][n]; g[DIR * 3 + b1] = a.cr[1][n];
-                g[b2 * 3 + b2] = a.cr[2][n]; g[DIR * 3 + b2] = a.cr[3][n];
-                const double gmv[3] = {a.met.f[0][n], a.met.f[4][n], a.met.f[8][n]};
-                prod = node_prod_diag<DIR>(g, gmv, a.met.f[9][n], mu);
+                g[b1 * 3 + b1] = sm[(VS_CR + 0) * NW + p]; g[DIR * 3 + b1] = sm[(VS_CR + 1) * NW + p];
+                g[b2 * 3 + b2] = sm[(VS_CR + 2) * NW + p]; g[DIR * 3 + b2] = sm[(VS_CR + 3) * NW + p];
+                const double gmv[3] = {sm[(VS_M + 0) * NW + p], sm[(VS_M + 1) * NW + p], sm[(VS_M + 2) * NW + p]};
+                prod = node_prod_diag<DIR>(g, gmv, sm[(VS_M + 3) * NW + p], mu);
             } else {
-                g[0 * 3 + b1] = a.cr[0][n]; g[1 * 3 + b1] = a.cr[1][n]; g[2 * 3 + b1] = a.cr[2][n];
-                g[0 * 3 + b2] = a.cr[4][n]; g[1 * 3 + b2] = a.cr[5][n]; g[2 * 3 + b2] = a.cr[6][n];
+                g[0 * 3 + b1] = sm[(VS_CR + 0) * NW + p]; g[1 * 3 + b1] = sm[(VS_CR + 1) * NW + p];
+                g[2 * 3 + b1] = sm[(VS_CR + 2) * NW + p];
+                g[0 * 3 + b2] = sm[(VS_CR + 4) * NW + p]; g[1 * 3 + b2] = sm[(VS_CR + 5) * NW + p];
+                g[2 * 3 + b2] = sm[(VS_CR + 6) * NW + p];
                 double m[10];
 #pragma unroll
-                for (int c = 0; c < 10; c++) m[c] = a.met.f[c][n];
+                for (int c = 0; c < 10; c++) m[c] = sm[(VS_M + c) * NW + p];
                 prod = node_prod_generic<DIR>(g, m, mu);
             }
-            const double work = a.u[n] * dF[0] + a.v[n] * dF[1] + a.w[n] * dF[2];
+            const double work = sm[VS_U * NW + p] * dF[0] + sm[VS_V * NW + p] * dF[1] + sm[VS_W * NW + p] * dF[2];
             a.q.f[1][n] += k.psi * dF[0];
             a.q.f[2][n] += k.psi * dF[1];
             if (a.three) a.q.f[3][n] += k.psi * dF[2];
@@ -1223,24 +1224,32 @@ static int stage_fluxes(shx_case *c, double *const *pr)
     return SHX_OK;
 }
 
-template <int S, int DIR, int TA, int TDI>
-static void launch_visc_sweep_t(shx_case *c, const ViscArgs &a, bool diag)
+template <int S, int DIR, int TA, int TDI, bool DIAG>
+static void launch_visc_sweep_t(shx_case *c, const ViscArgs &a)
 {
     dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
-    if (diag) k_visc_sweep<S, DIR, TA, TDI, true><<<grid, TA * TDI, 0, c->stream>>>(a);
-    else k_visc_sweep<S, DIR, TA, TDI, false><<<grid, TA * TDI, 0, c->stream>>>(a);
+    k_visc_sweep<S, DIR, TA, TDI, DIAG><<<grid, TA * TDI, 0, c->stream>>>(a);
 }
 
 template <int S>
 static void launch_visc_sweep_s(shx_case *c, const ViscArgs &a, int dir, bool diag)
 {
-    constexpr int TDN = (S == 5) ? 16 : 12;   // non-contiguous sweep: 16 x TDN tile
+    // tiles sized so that the LDS window (13 or 23 values per node) plus the face buffer fit in
+    // 64 KB: diagonal metric 16 x 16 (4 x 64 contiguous), generic 16 x 12 (4 x 48)
+    constexpr int TDN = (S == 5) ? 16 : 12;
     if (a.coalesce_d) {
-        if (dir == 1) launch_visc_sweep_t<S, 1, 4, 64>(c, a, false);       // 2D eta sweep
-        else launch_visc_sweep_t<S, 2, 4, 64>(c, a, diag);
+        if (dir == 1) launch_visc_sweep_t<S, 1, 4, 48, false>(c, a);       // 2D eta sweep
+        else if (diag) launch_visc_sweep_t<S, 2, 4, 64, true>(c, a);
+        else launch_visc_sweep_t<S, 2, 4, 48, false>(c, a);
     } else {
-        if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDN>(c, a, diag);
-        else launch_visc_sweep_t<S, 1, 16, TDN>(c, a, diag);
+        if (diag) {
+            if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDN, true>(c, a);
+            else launch_visc_sweep_t<S, 1, 16, TDN, true>(c, a);
+        } else {
+            constexpr int TDG = (S == 5) ? 12 : 8;
+            if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDG, false>(c, a);
+            else launch_visc_sweep_t<S, 1, 16, TDG, false>(c, a);
+        }
     }
 }
 
