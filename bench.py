@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="interior points per direction")
     ap.add_argument("--seed", type=int, default=20251202)
-    ap.add_argument("--cpu-n", type=int, default=96, help="CPU baseline sample size per direction")
+    ap.add_argument("--cpu-n", type=int, default=128, help="CPU baseline sample size per direction")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

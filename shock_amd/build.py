@@ -15,6 +15,21 @@ HIPCC_FLAGS = [
 ]
 
 
+DRIVER_SRC = os.path.join(HERE, "host", "shockx_drive.c")
+DRIVER = os.path.join(HERE, "host", "shockx_drive")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_driver(verbose=False):
+    """The C host loop (plain gcc against include/shockx.h and libshockx.so)."""
+    cmd = ["gcc", "-O2", "-Wall", "-I", INCLUDE, DRIVER_SRC, "-L", OUT_DIR, "-lshockx",
+           "-Wl,-rpath,$ORIGIN/../lib", "-lm", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return DRIVER
+
+
 def build(verbose=False, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -26,6 +41,7 @@ def build(verbose=False, extra=()):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libshockx.so")
     os.replace(OUT + ".tmp", OUT)
+    build_driver(verbose)
     return OUT
 
 

@@ -82,3 +82,11 @@ def test_manufactured_state_in_shock_units():
     # p_S = gamma * (1/gamma + 0.05 cos(2 pi (x+y)))
     assert np.allclose(p, 1.0 + 1.4 * 0.05 * np.cos(2 * np.pi * (x + y)))
     assert np.allclose(u * 0.5, 0.5 + 0.1 * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * y))
+
+
+def test_c_driver_compiles_against_the_header(tmp_path):
+    from shock_amd import build
+    out = tmp_path / "drive.o"
+    subprocess.run(["gcc", "-c", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    build.DRIVER_SRC, "-o", str(out)], check=True)
+    assert out.exists()
