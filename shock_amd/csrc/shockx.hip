@@ -36,13 +36,10 @@ using namespace shx;
 // src/Functions.c:2117-2136).
 __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= gm.ntot) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
     int idx[3];
-    idx[2] = (int)(n % gm.ng[2]);
-    const long r = n / gm.ng[2];
-    idx[1] = (int)(r % gm.ng[1]);
-    idx[0] = (int)(r / gm.ng[1]);
+    const long n = padded_node(gm, t, idx);
     bool ghost = false;
     long src = 0;
 #pragma unroll
@@ -61,10 +58,12 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 }
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
-__global__ void k_calc_values(CFields5 pr, double *T, double *mu, long ntot, Consts k)
+__global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Consts k)
 {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= ntot) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
     const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
     const double p = pressure(k, rho, u, v, w, e);
     T[n] = fabs(p / rho);
@@ -82,6 +81,7 @@ struct WenoArgs {
     int ngD;              // padded extent along the sweep
     int first;            // 1: Q = -div (first direction of the stage), 0: Q -= div
     int coalesce_d;       // load mapping: 1 when the sweep axis is contiguous in memory
+    int alignA;           // tiles start at padded A index 0 (A axis contiguous, pitch padded)
     Consts k;
 };
 
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     __shared__ double sm[NV * NW];
 
     const int t = threadIdx.x;
-    const int a0 = blockIdx.x * TA;
+    // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
+    const int pa0 = a.alignA ? (int)blockIdx.x * TA : (int)blockIdx.x * TA + a.oA;
     const int b = blockIdx.y;
     const int d0 = blockIdx.z * (TDI - 1);       // first node of this tile (interior-relative)
     const long bbase = (long)(b + a.oB) * a.sB;
@@ -111,10 +112,10 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         int aa, dd;
         if (a.coalesce_d) { aa = idx / WD; dd = idx - aa * WD; }
         else { dd = idx / TA; aa = idx - dd * TA; }
-        const int ai = min(a0 + aa, a.nA - 1);
+        const int pa = max(a.oA, min(pa0 + aa, a.oA + a.nA - 1));
         int dp = d0 - G + dd + a.oD;
         dp = max(0, min(dp, a.ngD - 1));
-        const long n = bbase + (long)(ai + a.oA) * a.sA + (long)dp * a.sD;
+        const long n = bbase + (long)pa * a.sA + (long)dp * a.sD;
         const double rho = a.pr.f[0][n], u = a.pr.f[1][n], v = a.pr.f[2][n], w = a.pr.f[3][n],
                      e = a.pr.f[4][n];
         const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n], J = a.J[n];
@@ -240,9 +241,9 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     // phase 3: divergence into Q for the tile's TDI-1 nodes
     if (di < TDI - 1) {
         const int dn = d0 + di;
-        const int an = a0 + ta;
-        if (dn < a.nD && an < a.nA) {
-            const long n = bbase + (long)(an + a.oA) * a.sA + (long)(dn + a.oD) * a.sD;
+        const int pa = pa0 + ta;
+        if (dn < a.nD && pa >= a.oA && pa < a.oA + a.nA) {
+            const long n = bbase + (long)pa * a.sA + (long)(dn + a.oD) * a.sD;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 const double dv = sm[m * NT + t + 1] - sm[m * NT + t];
@@ -267,6 +268,7 @@ struct ViscArgs {
     long sD, sA, sB;
     int nD, nA, nB, oD, oA, oB, ngD;
     int coalesce_d;
+    int alignA;
     int three;
     int bug;                      // zeta stencil k offset taken from j (DIR = 2 only)
     Consts k;
@@ -417,7 +419,8 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
     __shared__ double lined[(DIR == 2) ? TA * 2 * NLV : 1];
 
     const int t = threadIdx.x;
-    const int a0 = blockIdx.x * TA;
+    // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
+    const int pa0 = a.alignA ? (int)blockIdx.x * TA : (int)blockIdx.x * TA + a.oA;
     const int b = blockIdx.y;
     const int d0 = blockIdx.z * (TDI - 1);
     const long bbase = (long)(b + a.oB) * a.sB;
@@ -427,10 +430,10 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
         int aa, dd;
         if (a.coalesce_d) { aa = idx / WD; dd = idx - aa * WD; }
         else { dd = idx / TA; aa = idx - dd * TA; }
-        const int ai = min(a0 + aa, a.nA - 1);
+        const int pa = max(a.oA, min(pa0 + aa, a.oA + a.nA - 1));
         int dp = d0 - G + dd + a.oD;
         dp = max(0, min(dp, a.ngD - 1));
-        const long n = bbase + (long)(ai + a.oA) * a.sA + (long)dp * a.sD;
+        const long n = bbase + (long)pa * a.sA + (long)dp * a.sD;
         const int o = aa * WD + dd;
         sm[VS_U * NW + o] = a.u[n];
         sm[VS_V * NW + o] = a.v[n];
@@ -457,8 +460,7 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
             const int ln = idx / (NP * NLV);
             const int rem = idx - ln * NP * NLV;
             const int pos = rem / NLV, f = rem - pos * NLV;
-            const int ai = min(a0 + ln, a.nA - 1);
-            const int jpad = ai + a.oA;                      // A axis is j for DIR = 2
+            const int jpad = max(a.oA, min(pa0 + ln, a.oA + a.nA - 1));   // A axis is j for DIR = 2
             const long n = bbase + (long)jpad * a.sA + (jpad - G + pos);
             double val;
             if (f == 0) val = a.u[n];
@@ -543,9 +545,9 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
     __syncthreads();
 
     if (di < TDI - 1) {
-        const int dn = d0 + di, an = a0 + ta;
-        if (dn < a.nD && an < a.nA) {
-            const long n = bbase + (long)(an + a.oA) * a.sA + (long)(dn + a.oD) * a.sD;
+        const int dn = d0 + di, pa = pa0 + ta;
+        if (dn < a.nD && pa >= a.oA && pa < a.oA + a.nA) {
+            const long n = bbase + (long)pa * a.sA + (long)(dn + a.oD) * a.sD;
             const int p = ob + G;                    // window position of this node
             // faces: minus = this thread's face (A values), plus = next thread's face (B values)
             double dF[4];
@@ -648,13 +650,10 @@ __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
 // Direction d is evaluated where the node's d index is interior (all transverse nodes).
 __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, Consts k)
 {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= gm.ntot) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
     int idx[3];
-    idx[2] = (int)(n % gm.ng[2]);
-    const long r = n / gm.ng[2];
-    idx[1] = (int)(r % gm.ng[1]);
-    idx[0] = (int)(r / gm.ng[1]);
+    const long n = padded_node(gm, t, idx);
     const int half = gm.g;
     const int nl = 2 * gm.g + 1;   // S + 2 points
 #pragma unroll
@@ -701,12 +700,11 @@ __global__ void k_add_source(Fields5 q, CFields5 src, const double *J, Geom gm)
 __global__ void k_metric(const double *xe, const double *ye, const double *ze, Fields10 met, Geom gm,
                          int diag3d)
 {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= gm.ntot) return;
-    const int k = (int)(n % gm.ng[2]);
-    const long r = n / gm.ng[2];
-    const int j = (int)(r % gm.ng[1]);
-    const int i = (int)(r / gm.ng[1]);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    const int i = idx[0], j = idx[1], k = idx[2];
     const int ne1 = gm.ng[1] + 2, ne2 = (gm.dims == 3) ? gm.ng[2] + 2 : 1;
     const int ke = (gm.dims == 3) ? k + 1 : 0;
     const long ce = ((long)(i + 1) * ne1 + (j + 1)) * ne2 + ke;
@@ -947,11 +945,18 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         gm.ng[a] = live ? d.n[a] + S + 1 : 1;
         gm.lo[a] = live ? g : 0;
     }
-    gm.st[2] = 1;
-    gm.st[1] = gm.ng[2];
-    gm.st[0] = (long)gm.ng[1] * gm.ng[2];
+    // rows of the contiguous axis are padded to 16 doubles (128 B) so tiles start on cache lines
+    if (d.dims == 3) {
+        const long pitch = ((long)gm.ng[2] + 15) / 16 * 16;
+        gm.st[2] = 1; gm.st[1] = pitch; gm.st[0] = (long)gm.ng[1] * pitch;
+        gm.nalloc = (long)gm.ng[0] * gm.ng[1] * pitch;
+    } else {
+        const long pitch = ((long)gm.ng[1] + 15) / 16 * 16;
+        gm.st[2] = 1; gm.st[1] = 1; gm.st[0] = pitch;
+        gm.nalloc = (long)gm.ng[0] * pitch;
+    }
     gm.ntot = (long)gm.ng[0] * gm.ng[1] * gm.ng[2];
-    if (gm.ntot > (1L << 31) - 1) { delete c; return set_err(SHX_ERR_ARG, "block too large for 32-bit tiles"); }
+    if (gm.nalloc > (1L << 31) - 1) { delete c; return set_err(SHX_ERR_ARG, "block too large for 32-bit tiles"); }
     if ((d.quirks & SHX_QUIRK_ZETA_VISC_J) && d.dims == 3 && gm.ng[1] > gm.ng[2]) {
         // the reference's zeta stencil indexes k with j: for ng_j > ng_k it leaves the row
         delete c;
@@ -996,7 +1001,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
 
     hipError_t e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) { delete c; return set_err(SHX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
-    const size_t bytes = sizeof(double) * (size_t)gm.ntot;
+    const size_t bytes = sizeof(double) * (size_t)gm.nalloc;
     std::vector<double **> all;
     for (int b = 0; b < 3; b++) for (int m = 0; m < 5; m++) all.push_back(&c->prim[b][m]);
     for (int m = 0; m < 5; m++) { all.push_back(&c->q[m]); all.push_back(&c->qsum[m]); }
@@ -1043,16 +1048,27 @@ extern "C" int shx_padded_extent(const shx_case *c, int ng[3])
     return SHX_OK;
 }
 
+// host arrays use the reference layout (dense rows); device rows are padded to the pitch
+static void row_geometry(const shx_case *c, size_t &width, size_t &height, size_t &dpitch)
+{
+    const Geom &gm = c->gm;
+    if (gm.dims == 3) { width = sizeof(double) * gm.ng[2]; height = (size_t)gm.ng[0] * gm.ng[1]; dpitch = sizeof(double) * gm.st[1]; }
+    else { width = sizeof(double) * gm.ng[1]; height = (size_t)gm.ng[0]; dpitch = sizeof(double) * gm.st[0]; }
+}
 static int copy_in(shx_case *c, double *dst, const double *src)
 {
     if (!src) return set_err(SHX_ERR_ARG, "null field pointer");
-    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)c->gm.ntot, hipMemcpyHostToDevice, c->stream));
+    size_t w, h, dp;
+    row_geometry(c, w, h, dp);
+    HIPCHK(hipMemcpy2DAsync(dst, dp, src, w, w, h, hipMemcpyHostToDevice, c->stream));
     return SHX_OK;
 }
 static int copy_out(shx_case *c, double *dst, const double *src)
 {
     if (!dst) return set_err(SHX_ERR_ARG, "null field pointer");
-    HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * (size_t)c->gm.ntot, hipMemcpyDeviceToHost, c->stream));
+    size_t w, h, dp;
+    row_geometry(c, w, h, dp);
+    HIPCHK(hipMemcpy2DAsync(dst, w, src, dp, w, h, hipMemcpyDeviceToHost, c->stream));
     return SHX_OK;
 }
 
@@ -1143,7 +1159,7 @@ extern "C" int shx_upload_source(shx_case *c, const double *const src[5])
 {
     if (!c) return set_err(SHX_ERR_ARG, "null argument");
     if (!src) { c->has_src = 0; return SHX_OK; }
-    const size_t bytes = sizeof(double) * (size_t)c->gm.ntot;
+    const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
     for (int m = 0; m < 5; m++) {
         if (!c->src[m]) HIPCHK(hipMalloc((void **)&c->src[m], bytes));
         int rc = copy_in(c, c->src[m], src[m]);
@@ -1175,17 +1191,19 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     a.ngD = gm.ng[dir];
     a.first = first;
     a.coalesce_d = (a.sD == 1);
+    a.alignA = (a.sA == 1);
+    const int spanA = a.alignA ? a.oA + a.nA : a.nA;   // tiles along A cover this many padded indices
     a.k = c->k;
     int rc = tbeg(c, KT_WENO0 + dir);
     if (rc) return rc;
     if (a.coalesce_d) {
         constexpr int TA = 4, TDI = 64;
-        dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+        dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
         if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
         else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
     } else {
         constexpr int TA = 16, TDI = 16;
-        dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+        dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
         if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
         else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
     }
@@ -1208,7 +1226,7 @@ static int stage_fluxes(shx_case *c, double *const *pr)
         for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
         for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         if ((rc = tbeg(c, KT_VALUES))) return rc;
-        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm.ntot, c->k);
+        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
         if ((rc = tbeg(c, KT_GRAD))) return rc;
@@ -1227,7 +1245,7 @@ static int stage_fluxes(shx_case *c, double *const *pr)
 template <int S, int DIR, int TA, int TDI, bool DIAG>
 static void launch_visc_sweep_t(shx_case *c, const ViscArgs &a)
 {
-    dim3 grid(nblk(a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
+    dim3 grid(nblk(a.alignA ? a.oA + a.nA : a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
     k_visc_sweep<S, DIR, TA, TDI, DIAG><<<grid, TA * TDI, 0, c->stream>>>(a);
 }
 
@@ -1282,6 +1300,7 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir)
     a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
     a.ngD = gm.ng[dir];
     a.coalesce_d = (a.sD == 1);
+    a.alignA = (a.sA == 1);
     a.three = (gm.dims == 3);
     a.bug = (dir == 2 && (c->d.quirks & SHX_QUIRK_ZETA_VISC_J)) ? 1 : 0;
     a.k = c->k;
@@ -1423,7 +1442,7 @@ extern "C" int shx_copy_probe(shx_case *c, int reps)
 {
     if (!c || reps < 0) return set_err(SHX_ERR_ARG, "bad argument");
     for (int r = 0; r < reps; r++) {
-        k_copy<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(c->T, c->mu, c->gm.ntot);
+        k_copy<<<nblk(c->gm.nalloc, 256), 256, 0, c->stream>>>(c->T, c->mu, c->gm.nalloc);
         LAUNCH_CHECK();
     }
     HIPCHK(hipStreamSynchronize(c->stream));

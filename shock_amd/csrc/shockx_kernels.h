@@ -27,10 +27,21 @@ struct Geom {
     int ng[3];        // padded extents
     int n[3];         // interior counts (n[2] = 1 in 2D)
     int lo[3];        // padded index of the first interior node (g, or 0 for the 2D z axis)
-    long st[3];       // strides
+    long st[3];       // strides; the contiguous axis is padded to a multiple of 16 doubles
     int dims, g;
-    long ntot;
+    long ntot;        // logical padded nodes ng0*ng1*ng2
+    long nalloc;      // doubles allocated per field (with the row padding)
 };
+
+// linear index of the t-th logical padded node (t in [0, ntot)); ijk receives its indices
+__device__ __forceinline__ long padded_node(const Geom &gm, const long t, int ijk[3])
+{
+    ijk[2] = (int)(t % gm.ng[2]);
+    const long r = t / gm.ng[2];
+    ijk[1] = (int)(r % gm.ng[1]);
+    ijk[0] = (int)(r / gm.ng[1]);
+    return (long)ijk[0] * gm.st[0] + (long)ijk[1] * gm.st[1] + (long)ijk[2] * gm.st[2];
+}
 
 // ---------------------------------------------------------------------------------------
 // Characteristic eigenvectors of the Euler flux Jacobian in direction nh at the arithmetic
