@@ -32,8 +32,16 @@ extern "C" {
 #define SHX_QUIRK_METRIC_DIAG3D 2u /* src/Functions.c:6479-6484: 3D metric keeps only x_xi, y_eta, z_zeta */
 #define SHX_QUIRKS_REFERENCE (SHX_QUIRK_ZETA_VISC_J | SHX_QUIRK_METRIC_DIAG3D)
 
-/* Boundary kinds per face (left, right, bottom, top, behind, in front). */
-#define SHX_BC_PERIODIC 0       /* self-interface, src/Functions.c:3580-3684 */
+/* Boundary kinds per face (left, right, bottom, top, behind, in front = i-, i+, j-, j+, k-, k+).
+ * The two faces of a direction are both periodic or both not.  Face writers run after the
+ * periodic transfer and CalcValues, in face order (SetAllBoundaryConditions, src/BC.c:13-74). */
+#define SHX_BC_PERIODIC           0 /* self-interface, src/Functions.c:3580-3684 */
+#define SHX_BC_MANUFACTURED       1 /* exact state on the face's ghost layers, interior tangential
+                                       range (WriteBCManufacturedSolution*, src/ManufacturedSolution.c:202-338) */
+#define SHX_BC_INFLOW_SUPERSONIC  2 /* inflow state on every ghost node of the face slab
+                                       (WriteInflowSupersonicNormalBoundary, src/BC.c:2505-2542) */
+#define SHX_BC_OUTFLOW_RIEMANN    3 /* characteristic outflow from the mirror node
+                                       (WriteOutflowSubsonicRiemannBoundary, src/BC.c:2572-2622) */
 
 /* Arithmetic forms. */
 #define SHX_FP64 0
@@ -55,6 +63,9 @@ typedef struct shx_desc {
     int bc[6];         /* SHX_BC_* per face */
     int precision;     /* SHX_FP64 */
     double eps_override; /* 0: WENO epsilon = 10*sqrt(DBL_MIN) as src/Functions.c:1829 */
+    /* inflow state (boundaryconditions:rho_inflow, p_inflow; u,v,w_inflow from the angle of
+     * attack, src/Import.c:90-91,233-235); the Riemann outflow uses u_inflow as its far field */
+    double rho_inflow, u_inflow, v_inflow, w_inflow, p_inflow;
 } shx_desc;
 
 typedef struct shx_case shx_case;
@@ -82,6 +93,11 @@ int shx_download_metric(shx_case *c, double *const met[10]);
  * (and therefore U_RK at the next step start); download returns U_lastStep. */
 int shx_upload_state(shx_case *c, const double *const prim[5]);
 int shx_download_state(shx_case *c, double *const prim[5]);
+
+/* Exact state for SHX_BC_MANUFACTURED faces: rho, u, v, w, e, p on the padded grid (the
+ * values WriteManufacturedSolution evaluates, src/ManufacturedSolution.c:566-648; only ghost
+ * nodes of manufactured faces are read).  Required before stepping when such a face exists. */
+int shx_upload_boundary_state(shx_case *c, const double *const bs[6]);
 
 /* Manufactured-solution source per node (Q += J * src each stage,
  * AddManufacturedSolutionSource src/ManufacturedSolution.c:25-175); NULL clears it. */
@@ -116,7 +132,7 @@ int shx_copy_probe(shx_case *c, int reps);
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled, every launch is
  * bracketed by events; shx_kernel_times synchronizes and returns, per kernel class, the total
  * milliseconds and the launch count since the last reset.  names[k] points to static strings. */
-#define SHX_NUM_KERNELS 12
+#define SHX_NUM_KERNELS 13
 int shx_set_timing(shx_case *c, int enable);
 int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
                      int launches[SHX_NUM_KERNELS]);

@@ -18,6 +18,7 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 QUIRK_ZETA_VISC_J = 1
 QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
+BC_PERIODIC, BC_MMS, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 
 
 class OrcCfg(ctypes.Structure):
@@ -27,6 +28,8 @@ class OrcCfg(ctypes.Structure):
         ("quirks", ctypes.c_uint),
         ("gamma", ctypes.c_double), ("mach", ctypes.c_double), ("reynolds", ctypes.c_double),
         ("prandtl", ctypes.c_double), ("T0", ctypes.c_double), ("eps_override", ctypes.c_double),
+        ("bc", ctypes.c_int * 6), ("rho_in", ctypes.c_double), ("u_in", ctypes.c_double),
+        ("v_in", ctypes.c_double), ("w_in", ctypes.c_double), ("p_in", ctypes.c_double),
         ("g", ctypes.c_int), ("ng", ctypes.c_int * 3), ("ntot", ctypes.c_long),
         ("upsilon", ctypes.c_double), ("psi", ctypes.c_double), ("gam_heat", ctypes.c_double),
         ("suth", ctypes.c_double), ("eps", ctypes.c_double),
@@ -62,6 +65,7 @@ def lib():
         _lib.orc_check_nan.restype = ctypes.c_int
         _lib.orc_visc_fac.argtypes = [ctypes.c_int, P, P]
         _lib.orc_step.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P]
+        _lib.orc_step_b.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P, P]
         _lib.orc_rk_update.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_int, ctypes.c_double, P, P, P, P, P]
     return _lib
 
@@ -82,8 +86,11 @@ class Oracle:
 
     def __init__(self, dims=2, S=5, n=(16, 16, 1), time_order=4, gamma=1.4, mach=0.5,
                  reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, mms=False,
-                 quirks=QUIRKS_REFERENCE, eps_override=0.0):
+                 quirks=QUIRKS_REFERENCE, eps_override=0.0, bc=(0, 0, 0, 0, 0, 0), inflow=(1.0, 1.0, 0.0, 0.0, 1.0)):
         c = OrcCfg()
+        for f in range(6):
+            c.bc[f] = bc[f]
+        c.rho_in, c.u_in, c.v_in, c.w_in, c.p_in = inflow
         c.dims, c.S, c.time_order = dims, S, time_order
         for d in range(3):
             c.n[d] = n[d] if d < len(n) else 1
@@ -106,6 +113,11 @@ class Oracle:
         lib().orc_calc_values(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(der))
         return der
 
+    def apply_bc(self, met, prim, der, bstate=None):
+        """Face boundary writers (in place on prim and der)."""
+        lib().orc_apply_bc(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(der),
+                           _bundle(bstate) if bstate is not None else None)
+
     def ghost_periodic(self, fields):
         lib().orc_ghost_periodic(ctypes.byref(self.c), _bundle(fields), len(fields))
 
@@ -127,28 +139,30 @@ class Oracle:
         lib().orc_visc_dir(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
                            _bundle(grad), _bundle(q))
 
-    def rhs(self, met, prim, src=None):
-        """Q (5 fields) of one stage for state prim; refills prim's ghosts periodically."""
+    def rhs(self, met, prim, src=None, bstate=None):
+        """Q (5 fields) of one stage for state prim; refills prim's ghosts (periodic + face BCs)."""
         scratch = self.zeros(30)
-        lib().orc_rhs(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(scratch),
-                      _bundle(src) if src is not None else None)
+        lib().orc_rhs_b(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(scratch),
+                        _bundle(src) if src is not None else None,
+                        _bundle(bstate) if bstate is not None else None)
         return scratch[20:25], scratch[:8], scratch[8:20]
 
-    def step(self, tau, met, un, urk, src=None, scratch=None):
+    def step(self, tau, met, un, urk, src=None, scratch=None, bstate=None):
         """One CalcRungeKutta step in place on urk (un is the state at the start of the step)."""
         if scratch is None:
             scratch = self.zeros(30)
-        lib().orc_step(ctypes.byref(self.c), ctypes.c_double(tau), _bundle(met), _bundle(un),
-                       _bundle(urk), _bundle(scratch), _bundle(src) if src is not None else None)
+        lib().orc_step_b(ctypes.byref(self.c), ctypes.c_double(tau), _bundle(met), _bundle(un),
+                         _bundle(urk), _bundle(scratch), _bundle(src) if src is not None else None,
+                         _bundle(bstate) if bstate is not None else None)
         return scratch
 
-    def run(self, tau, met, prim0, steps, src=None):
+    def run(self, tau, met, prim0, steps, src=None, bstate=None):
         """Advance prim0 by `steps` full steps (startSimulation semantics: U_lastStep <- U_RK)."""
         un = [a.copy() for a in prim0]
         urk = [a.copy() for a in prim0]
         scratch = self.zeros(30)
         for _ in range(steps):
-            self.step(tau, met, un, urk, src, scratch)
+            self.step(tau, met, un, urk, src, scratch, bstate)
             for a, b in zip(un, urk):
                 a[...] = b
         return urk

@@ -29,6 +29,10 @@ int orc_define(orc_cfg *c)
     if (c->dims != 2 && c->dims != 3) return -1;
     if (c->S != 5 && c->S != 9) return -1;
     if (c->time_order != 3 && c->time_order != 4) return -1;
+    for (d = 0; d < 6; d++) {
+        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_OUTFLOW_RIEMANN) return -1;
+        if ((c->bc[d] == ORC_BC_PERIODIC) != (c->bc[d ^ 1] == ORC_BC_PERIODIC)) return -1;
+    }
     c->g = (c->S + 1) / 2;
     if (c->dims == 2) c->n[2] = 1;
     for (d = 0; d < 3; d++) c->ng[d] = c->n[d] + c->S + 1;
@@ -578,6 +582,7 @@ void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf)
                 for (dd = 0; dd < 3; dd++) {
                     src[dd] = idx[dd];
                     if (dd == 2 && c->dims == 2) continue;
+                    if (c->bc[2 * dd] != ORC_BC_PERIODIC) continue;
                     if (idx[dd] < c->g) { src[dd] = idx[dd] + c->n[dd]; ghost = 1; }
                     else if (idx[dd] > c->g + c->n[dd] - 1) { src[dd] = idx[dd] - c->n[dd]; ghost = 1; }
                 }
@@ -586,6 +591,100 @@ void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf)
                     for (m = 0; m < nf; m++) f[m][to] = f[m][from];
                 }
             }
+}
+
+/* ---- boundary writers ---------------------------------------------------------------- */
+/* WriteManufacturedSolution at a ghost node (src/ManufacturedSolution.c:566-648): the exact
+ * rho,u,v,w,e,p (bs[0..5], evaluated by the caller) plus T = |p/rho|, c and mu = 1.  theta is not
+ * written there, so it keeps the CalcValues value. */
+static void bc_mms(const orc_cfg *c, double *const prim[5], double *const der[8], double *const bs[6], long n)
+{
+    int m;
+    for (m = 0; m < 5; m++) prim[m][n] = bs[m][n];
+    der[0][n] = bs[5][n];
+    der[1][n] = fabs(der[0][n] / prim[0][n]);
+    der[2][n] = (double)sqrtl((long double)(c->upsilon * c->gamma * der[0][n] / prim[0][n]));
+    der[3][n] = 1.0;
+}
+
+/* WriteInflowSupersonicNormalBoundary (src/BC.c:2505-2542) */
+static void bc_inflow_super(const orc_cfg *c, double *const met[10], double *const prim[5],
+                            double *const der[8], long n)
+{
+    double p = c->p_in, rho = c->rho_in, u = c->u_in, v = c->v_in, w = c->w_in;
+    prim[0][n] = rho; prim[1][n] = u; prim[2][n] = v; prim[3][n] = w;
+    der[0][n] = p;
+    der[4][n] = u * met[0][n] + v * met[1][n] + w * met[2][n];
+    der[5][n] = u * met[3][n] + v * met[4][n] + w * met[5][n];
+    der[6][n] = u * met[6][n] + v * met[7][n] + w * met[8][n];
+    prim[4][n] = (0.5 * ((u * u) + (v * v) + (w * w)) + p / rho / (c->gamma - 1.0) * c->upsilon);
+    der[2][n] = sqrt(c->upsilon * c->gamma * p / rho);
+    der[1][n] = c->p_in / c->rho_in;
+    der[3][n] = ((1.0 + c->suth) * pow(p / rho, 1.5) / (p / rho + c->suth));
+}
+
+/* WriteOutflowSubsonicRiemannBoundary (src/BC.c:2572-2622), mirror node s */
+static void bc_outflow_riemann(const orc_cfg *c, double *const met[10], double *const prim[5],
+                               double *const der[8], long n, long s)
+{
+    double cinf = 1.0 / c->mach, cii = der[2][s], d1, d2, rrr, u, v, w, rho, p;
+    u = 0.5 * (prim[1][s] + c->u_in) - (cinf - cii) / (c->gamma - 1.);
+    v = prim[2][s];
+    w = prim[3][s];
+    d1 = (0.5 * (cii + cinf) - 0.25 * (c->gamma - 1.) * (c->u_in - prim[1][s])) / cii;
+    d2 = 2. / (c->gamma - 1.);
+    rrr = pow(d1, d2);
+    rho = prim[0][s] * rrr;
+    d2 = 2. * c->gamma / (c->gamma - 1.);
+    rrr = pow(d1, d2);
+    p = der[0][s] * rrr;
+    prim[0][n] = rho; prim[1][n] = u; prim[2][n] = v; prim[3][n] = w;
+    der[0][n] = p;
+    der[4][n] = u * met[0][n] + v * met[1][n] + w * met[2][n];
+    der[5][n] = u * met[3][n] + v * met[4][n] + w * met[5][n];
+    der[6][n] = u * met[6][n] + v * met[7][n] + w * met[8][n];
+    der[2][n] = sqrt(c->upsilon * c->gamma * p / rho);
+    prim[4][n] = (0.5 * ((u * u) + (v * v) + (w * w)) + p / rho / (c->gamma - 1.0) * c->upsilon);
+    der[3][n] = der[3][s];
+    der[1][n] = der[1][s];
+}
+
+/* One face: the ghost layers outside the face; tangential range all nodes for inflow/outflow
+ * (src/BC.c:1344-1370, 2111-2140), interior for the manufactured solution
+ * (src/ManufacturedSolution.c:202-338).  Mirror index: ghost StartReal-1-l <-> StartReal+l. */
+static void apply_face(const orc_cfg *c, int face, double *const met[10], double *const prim[5],
+                       double *const der[8], double *const bs[6])
+{
+    int d = face / 2, upper = face % 2, kind = c->bc[face];
+    int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2, l, a, b;
+    int lo1, hi1, lo2, hi2;
+    if (kind == ORC_BC_PERIODIC) return;
+    if (kind == ORC_BC_MMS) { lo1 = rs(c, t1); hi1 = re(c, t1); lo2 = rs(c, t2); hi2 = re(c, t2); }
+    else { lo1 = 0; hi1 = ge(c, t1); lo2 = 0; hi2 = ge(c, t2); }
+    for (l = 0; l < c->g; l++) {
+        int gi = upper ? re(c, d) + 1 + l : rs(c, d) - 1 - l;
+        int si = upper ? re(c, d) - l : rs(c, d) + l;
+        for (a = lo1; a <= hi1; a++)
+            for (b = lo2; b <= hi2; b++) {
+                int idx[3], sdx[3];
+                long n, s;
+                idx[d] = gi; sdx[d] = si;
+                idx[t1] = a; sdx[t1] = a;
+                idx[t2] = b; sdx[t2] = b;
+                n = orc_idx(c, idx[0], idx[1], idx[2]);
+                s = orc_idx(c, sdx[0], sdx[1], sdx[2]);
+                if (kind == ORC_BC_MMS) bc_mms(c, prim, der, bs, n);
+                else if (kind == ORC_BC_INFLOW_SUPERSONIC) bc_inflow_super(c, met, prim, der, n);
+                else if (kind == ORC_BC_OUTFLOW_RIEMANN) bc_outflow_riemann(c, met, prim, der, n, s);
+            }
+    }
+}
+
+void orc_apply_bc(const orc_cfg *c, double *const met[10], double *const prim[5], double *const der[8],
+                  double *const bstate[6])
+{
+    int face;
+    for (face = 0; face < 2 * c->dims; face++) apply_face(c, face, met, prim, der, bstate);
 }
 
 /* RK stage update over interior nodes, src/Functions.c:2300-2365 (in place on urk). */
@@ -628,12 +727,19 @@ void orc_add_source(const orc_cfg *c, double *const met[10], double *const src[5
 void orc_rhs(const orc_cfg *c, double *const met[10], double *const prim[5],
              double *const scratch[30], double *const src[5])
 {
+    orc_rhs_b(c, met, prim, scratch, src, NULL);
+}
+
+void orc_rhs_b(const orc_cfg *c, double *const met[10], double *const prim[5],
+               double *const scratch[30], double *const src[5], double *const bstate[6])
+{
     double *const *der = scratch, *const *grad = scratch + 8, *const *q = scratch + 20;
     int d;
     orc_zero(c, q, 5);
     orc_zero(c, grad, 12);
     orc_ghost_periodic(c, prim, 5);
     orc_calc_values(c, met, prim, der);
+    orc_apply_bc(c, met, prim, der, bstate);
     for (d = 0; d < c->dims; d++) orc_weno_dir(c, d, met, prim, der, q);
     if (!c->inviscid) {
         orc_visc_derivs(c, prim, der, grad);
@@ -646,11 +752,18 @@ void orc_rhs(const orc_cfg *c, double *const met[10], double *const prim[5],
 void orc_step(const orc_cfg *c, double tau, double *const met[10], double *const un[5],
               double *const urk[5], double *const scratch[30], double *const src[5])
 {
+    orc_step_b(c, tau, met, un, urk, scratch, src, NULL);
+}
+
+void orc_step_b(const orc_cfg *c, double tau, double *const met[10], double *const un[5],
+                double *const urk[5], double *const scratch[30], double *const src[5],
+                double *const bstate[6])
+{
     double *const *q = scratch + 20, *const *qsum = scratch + 25;
     int s;
     orc_zero(c, qsum, 5);
     for (s = 0; s < c->time_order; s++) {
-        orc_rhs(c, met, urk, scratch, src);
+        orc_rhs_b(c, met, urk, scratch, src, bstate);
         orc_rk_update(c, s, tau, met, un, urk, q, qsum);
     }
 }

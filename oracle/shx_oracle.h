@@ -42,8 +42,11 @@ extern "C" {
 #define ORC_QUIRK_METRIC_DIAG3D 2u  /* src/Functions.c:6479-6484: 3D metric drops off-diagonal terms */
 #define ORC_QUIRKS_REFERENCE (ORC_QUIRK_ZETA_VISC_J | ORC_QUIRK_METRIC_DIAG3D)
 
-/* boundary kinds per face */
-#define ORC_BC_PERIODIC 0
+/* boundary kinds per face (order: left, right, bottom, top, behind, in front = i-, i+, j-, j+, k-, k+) */
+#define ORC_BC_PERIODIC 0          /* self-interface (src/Functions.c:3580-3684) */
+#define ORC_BC_MMS 1               /* manufactured-solution Dirichlet (src/ManufacturedSolution.c:202-338) */
+#define ORC_BC_INFLOW_SUPERSONIC 2 /* src/BC.c:2505-2542 */
+#define ORC_BC_OUTFLOW_RIEMANN 3   /* src/BC.c:2572-2622 */
 
 typedef struct orc_cfg {
     /* inputs */
@@ -56,6 +59,8 @@ typedef struct orc_cfg {
     unsigned quirks;   /* ORC_QUIRK_* */
     double gamma, mach, reynolds, prandtl, T0;
     double eps_override; /* 0 -> reference epsilon 10*sqrt(DBL_MIN) */
+    int bc[6];           /* ORC_BC_* per face; both faces of a direction periodic or neither */
+    double rho_in, u_in, v_in, w_in, p_in;   /* boundaryconditions:{rho,p}_inflow, InitializeValues u0 (AoA) */
     /* derived by orc_define */
     int g;
     int ng[3];
@@ -101,6 +106,12 @@ void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *cons
 
 void orc_zero(const orc_cfg *c, double *const f[], int nf);
 void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);
+/* Face boundary writers in the reference's order (SetAllBoundaryConditions, src/BC.c:13-74),
+ * applied after CalcValues: they overwrite primitive and derived values of ghost nodes.
+ * bstate: boundary state (rho,u,v,w,e,p on the padded grid) read by ORC_BC_MMS faces (may be NULL
+ * otherwise). */
+void orc_apply_bc(const orc_cfg *c, double *const met[10], double *const prim[5], double *const der[8],
+                  double *const bstate[6]);
 void orc_rk_update(const orc_cfg *c, int s, double tau, double *const met[10],
                    double *const un[5], double *const urk[5], double *const q[5], double *const qsum[5]);
 void orc_add_source(const orc_cfg *c, double *const met[10], double *const src[5], double *const q[5]);
@@ -114,6 +125,12 @@ void orc_step(const orc_cfg *c, double tau, double *const met[10], double *const
 /* Full RHS of one stage without the update: Q for state prim (ghosts refilled periodically). */
 void orc_rhs(const orc_cfg *c, double *const met[10], double *const prim[5],
              double *const scratch[30], double *const src[5]);
+/* variants with a boundary state for ORC_BC_MMS faces */
+void orc_rhs_b(const orc_cfg *c, double *const met[10], double *const prim[5],
+               double *const scratch[30], double *const src[5], double *const bstate[6]);
+void orc_step_b(const orc_cfg *c, double tau, double *const met[10], double *const un[5],
+                double *const urk[5], double *const scratch[30], double *const src[5],
+                double *const bstate[6]);
 
 int    orc_check_nan(const orc_cfg *c, double *const prim[5]);
 double orc_cfl_dt(const orc_cfg *c, double cfl, double *const met[10], double *const prim[5]);

@@ -19,50 +19,60 @@ def padded_shape(dims, S, n):
     return tuple(ng), g
 
 
-def _wrap(ip, g, n):
-    """interior source index and period shift for padded index ip (periodic)."""
-    rel = ip - g
-    shift = np.floor_divide(rel, n)
-    return g + rel - shift * n, shift
+def _extend_axis(a, axis, gx, periodic, shift):
+    """Add gx layers on both sides of `axis`.  Periodic: wrapped copy plus `shift` times the wrap
+    count (the periodic mesh transfer, src/Functions.c:2657-2661).  Otherwise each layer mirrors
+    the spacing of the interior node at the same distance from the face
+    (ExtrapolateGhostCells, src/Functions.c:6526-6670): x[-1-l] = x[-l] + x[l] - x[l+1]."""
+    n = a.shape[axis]
+    if periodic:
+        e = np.arange(-gx, n + gx)
+        wraps = np.floor_divide(e, n)
+        out = np.take(a, e - wraps * n, axis=axis)
+        sh = [1, 1, 1]
+        sh[axis] = e.size
+        return out + (wraps.astype(np.float64) * shift).reshape(sh)
+    out = np.moveaxis(np.zeros(a.shape[:axis] + (n + 2 * gx,) + a.shape[axis + 1:]), axis, 0)
+    src = np.moveaxis(a, axis, 0)
+    out[gx:gx + n] = src
+    for l in range(gx):
+        lo = gx - 1 - l
+        out[lo] = out[lo + 1] + out[gx + l] - out[gx + l + 1]
+        hi = gx + n + l
+        out[hi] = out[hi - 1] + out[gx + n - 1 - l] - out[gx + n - 2 - l]
+    return np.ascontiguousarray(np.moveaxis(out, 0, axis))
+
+
+def ext_coords(dims, S, n, mapping, periodic=(True, True, True), period=(1.0, 1.0, 1.0)):
+    """Extended coordinate arrays (g+1 layers beyond the interior on each side, the reference's
+    x_extrapolate, src/Functions.c:6388-6420 and 6672-6790) of a block.
+
+    mapping(xi, eta, zeta) -> (x, y, z) gives the interior coordinates; xi = i/n along periodic
+    directions (the end point is the next period's first node) and xi = i/(n-1) along
+    non-periodic ones (both faces are grid nodes).  Directions are extended in the order
+    i, j, k, each over the already extended earlier directions, as the reference does.
+    """
+    ng, g = padded_shape(dims, S, n)
+    ii = []
+    for d in range(3):
+        if d >= dims:
+            ii.append(np.zeros(1))
+        else:
+            ii.append(np.arange(n[d]) / (n[d] if periodic[d] else (n[d] - 1)))
+    XI, ETA, ZETA = np.meshgrid(ii[0], ii[1], ii[2], indexing="ij")
+    arrs = [np.broadcast_to(a, XI.shape).astype(np.float64) for a in mapping(XI, ETA, ZETA)]
+    for d in range(dims):
+        arrs = [_extend_axis(a, d, g + 1, periodic[d], period[d] if comp == d else 0.0)
+                for comp, a in enumerate(arrs)]
+    xe, ye, ze = [np.ascontiguousarray(a) for a in arrs]
+    if dims == 2:
+        ze = np.zeros(xe.shape)
+    return xe, ye, ze
 
 
 def periodic_ext_coords(dims, S, n, mapping, period=(1.0, 1.0, 1.0)):
-    """Extended coordinate arrays (one extra layer beyond the ghosts, as the reference's
-    x_extrapolate, src/Functions.c:6388-6420) for a block periodic in every direction.
-
-    mapping(xi, eta, zeta) -> (x, y, z) gives the interior coordinates with xi = i/n.
-    Ghost and extra-layer nodes copy the wrapped interior node plus the period times the
-    wrap count, which is what the periodic mesh transfer produces
-    (WriteValuesFromBufferToMesh, src/Functions.c:2657-2661).
-    """
-    ng, g = padded_shape(dims, S, n)
-    ext = [ng[0] + 2, ng[1] + 2, (ng[2] + 2) if dims == 3 else 1]
-    nn = [n[0], n[1], n[2] if dims == 3 else 1]
-    # interior coordinates
-    ii = [np.arange(nn[d]) / nn[d] for d in range(3)]
-    if dims == 2:
-        ii[2] = np.zeros(1)
-    XI, ETA, ZETA = np.meshgrid(ii[0], ii[1], ii[2], indexing="ij")
-    X, Y, Z = mapping(XI, ETA, ZETA)
-    X, Y, Z = [np.broadcast_to(a, XI.shape).astype(np.float64) for a in (X, Y, Z)]
-    # index maps for the extended arrays: extended index e <-> padded index e-1
-    src = []
-    shifts = []
-    for d in range(3):
-        if d == 2 and dims == 2:
-            src.append(np.zeros(1, dtype=int))
-            shifts.append(np.zeros(1))
-            continue
-        pe = np.arange(ext[d]) - 1
-        s_idx, sh = _wrap(pe, g, nn[d])
-        src.append(s_idx - g)
-        shifts.append(sh.astype(np.float64))
-    I, J, K = np.meshgrid(src[0], src[1], src[2], indexing="ij")
-    SI, SJ, SK = np.meshgrid(shifts[0], shifts[1], shifts[2], indexing="ij")
-    xe = np.ascontiguousarray(X[I, J, K] + SI * period[0])
-    ye = np.ascontiguousarray(Y[I, J, K] + SJ * period[1])
-    ze = np.ascontiguousarray(Z[I, J, K] + SK * period[2]) if dims == 3 else np.zeros(xe.shape)
-    return xe, ye, ze
+    """Extended coordinates of a block periodic in every direction (see ext_coords)."""
+    return ext_coords(dims, S, n, mapping, (True, True, True), period)
 
 
 def interior_coords(dims, S, n, xe, ye, ze):

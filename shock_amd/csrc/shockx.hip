@@ -31,9 +31,10 @@ using namespace shx;
 // kernels
 // ----------------------------------------------------------------------------------------
 
-// Periodic ghost fill of nf fields: every ghost node copies the interior node at the wrapped
-// index in each direction (fixed point of the ghost-inclusive halo passes,
-// src/Functions.c:2117-2136).
+// Periodic ghost fill of nf fields: every node that is a ghost in a periodic direction copies
+// the node at the wrapped index of each periodic direction (fixed point of the ghost-inclusive
+// halo passes, src/Functions.c:2117-2136; indices along non-periodic directions are kept, so a
+// face slab's ghosts wrap along the periodic tangential directions only).
 __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,7 +46,7 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         int s = idx[d];
-        if (gm.ng[d] > 1) {
+        if (gm.ng[d] > 1 && gm.per[d]) {
             if (s < gm.lo[d]) { s += gm.n[d]; ghost = true; }
             else if (s >= gm.lo[d] + gm.n[d]) { s -= gm.n[d]; ghost = true; }
         }
@@ -55,6 +56,83 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 #pragma unroll
     for (int m = 0; m < 10; m++)
         if (m < nf) f.f[m][n] = f.f[m][src];
+}
+
+// ---- face boundary writers ---------------------------------------------------------------
+// One launch per non-periodic face, in the reference's face order, after CalcValues.  A thread
+// owns one ghost node of the face slab: layer l outside the face, tangential indices (i1, i2).
+// Its mirror node is layer l inside the face (StartReal + l, EndReal - l).
+struct BcArgs {
+    Fields5 pr;              // stage input state; ghost rho,u,v,w,e are written
+    double *T, *mu;          // derived T and mu (viscous runs; null when inviscid)
+    const double *bs[6];     // exact rho,u,v,w,e,p for manufactured faces
+    int kind, dir, upper, g;
+    int t1, t2;              // tangential axes
+    int lo1, n1, lo2, n2;    // tangential index ranges
+    int first, last;         // first / last interior index along dir
+    long st[3];
+    double rho_in, u_in, v_in, w_in, p_in, cinf;
+    Consts k;
+};
+
+__global__ void k_face_bc(BcArgs a)
+{
+    const long per_layer = (long)a.n1 * a.n2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per_layer * a.g) return;
+    const int l = (int)(t / per_layer);
+    const long r = t - (long)l * per_layer;
+    const int i1 = a.lo1 + (int)(r / a.n2), i2 = a.lo2 + (int)(r % a.n2);
+    const int gi = a.upper ? a.last + 1 + l : a.first - 1 - l;
+    const int si = a.upper ? a.last - l : a.first + l;
+    const long base = (long)i1 * a.st[a.t1] + (long)i2 * a.st[a.t2];
+    const long n = base + (long)gi * a.st[a.dir], s = base + (long)si * a.st[a.dir];
+    const Consts &k = a.k;
+    if (a.kind == SHX_BC_MANUFACTURED) {
+        // WriteManufacturedSolution: exact state, T = |p/rho|, mu = 1 (c, p and theta are
+        // recomputed from rho,u,v,w,e where the flux kernels need them)
+        const double rho = a.bs[0][n];
+        for (int m = 0; m < 5; m++) a.pr.f[m][n] = a.bs[m][n];
+        if (a.T) { a.T[n] = fabs(a.bs[5][n] / rho); a.mu[n] = 1.0; }
+    } else if (a.kind == SHX_BC_INFLOW_SUPERSONIC) {
+        const double rho = a.rho_in, u = a.u_in, v = a.v_in, w = a.w_in, p = a.p_in;
+        a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
+        a.pr.f[4][n] = (0.5 * ((u * u) + (v * v) + (w * w)) + p / rho / (k.gamma - 1.0) * k.upsilon);
+        if (a.T) {
+            a.T[n] = a.p_in / a.rho_in;
+            a.mu[n] = ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+        }
+    } else {
+        // Riemann invariants: outgoing from the mirror node, incoming from the inflow state
+        const double rs = a.pr.f[0][s], us = a.pr.f[1][s], vs = a.pr.f[2][s], ws = a.pr.f[3][s];
+        const double ps = pressure(k, rs, us, vs, ws, a.pr.f[4][s]);
+        const double cii = sqrt(k.upsilon * k.gamma * ps / rs);
+        const double u = 0.5 * (us + a.u_in) - (a.cinf - cii) / (k.gamma - 1.);
+        const double d1 = (0.5 * (cii + a.cinf) - 0.25 * (k.gamma - 1.) * (a.u_in - us)) / cii;
+        const double rho = rs * pow(d1, 2. / (k.gamma - 1.));
+        const double p = ps * pow(d1, 2. * k.gamma / (k.gamma - 1.));
+        a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = vs; a.pr.f[3][n] = ws;
+        a.pr.f[4][n] = (0.5 * ((u * u) + (vs * vs) + (ws * ws)) + p / rho / (k.gamma - 1.0) * k.upsilon);
+        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+    }
+}
+
+// Ghost nodes of non-periodic directions carried from one stage buffer to the next: the
+// reference keeps one U_RK array whose ghost layers persist between stages, which matters where
+// a writer reads a ghost another face wrote in the previous stage.
+__global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    bool ghost = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (gm.ng[d] > 1 && !gm.per[d] && (idx[d] < gm.lo[d] || idx[d] >= gm.lo[d] + gm.n[d])) ghost = true;
+    if (!ghost) return;
+#pragma unroll
+    for (int m = 0; m < 5; m++) to.f[m][n] = from.f[m][n];
 }
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
@@ -837,10 +915,10 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                               \
     } while (0)
 
-enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC };
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES };
 static const char *g_knames[SHX_NUM_KERNELS] = {
     "ghost_periodic", "calc_values", "weno_xi", "weno_eta", "weno_zeta",
-    "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric"};
+    "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces"};
 
 struct shx_case {
     shx_desc d;
@@ -851,7 +929,10 @@ struct shx_case {
     double *prim[3][5];    // three state buffers; cur = U_lastStep
     int cur;
     double *q[5], *qsum[5], *met[10], *grad[12], *src[5], *T, *mu;
-    int has_src, has_metric, has_state;
+    double *bs[6];         // exact state for manufactured faces
+    int has_src, has_metric, has_state, has_bs;
+    int faces;             // number of non-periodic faces
+    int needs_bs;          // some face is SHX_BC_MANUFACTURED
     int diag;              // 3D metric is diagonal (only xi_x, eta_y, zeta_z and J non-zero)
     unsigned long long *d_red;
     int *d_cnt;
@@ -925,9 +1006,15 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.time_order != 3 && d.time_order != 4)
         return set_err(SHX_ERR_ARG, "time_order must be 3 or 4 (got %d)", d.time_order);
     if (d.precision != SHX_FP64) return set_err(SHX_ERR_UNSUPPORTED, "only the FP64 form is built");
-    for (int f = 0; f < 2 * d.dims; f++)
-        if (d.bc[f] != SHX_BC_PERIODIC)
-            return set_err(SHX_ERR_UNSUPPORTED, "face %d: only periodic boundaries are implemented", f);
+    for (int f = 0; f < 2 * d.dims; f++) {
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_OUTFLOW_RIEMANN)
+            return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
+        if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
+            return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
+        if ((d.bc[f] == SHX_BC_INFLOW_SUPERSONIC || d.bc[f] == SHX_BC_OUTFLOW_RIEMANN) &&
+            !(d.rho_inflow > 0.0 && d.p_inflow > 0.0))
+            return set_err(SHX_ERR_ARG, "face %d: inflow/outflow faces need rho_inflow > 0 and p_inflow > 0", f);
+    }
     const int S = d.space_order, g = (S + 1) / 2;
     for (int a = 0; a < d.dims; a++)
         if (d.n[a] < S + 1) return set_err(SHX_ERR_ARG, "n[%d]=%d below the stencil width %d", a, d.n[a], S + 1);
@@ -944,6 +1031,11 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         gm.n[a] = live ? d.n[a] : 1;
         gm.ng[a] = live ? d.n[a] + S + 1 : 1;
         gm.lo[a] = live ? g : 0;
+        gm.per[a] = live ? (d.bc[2 * a] == SHX_BC_PERIODIC) : 1;
+        if (live && !gm.per[a]) {
+            c->faces += 2;
+            if (d.bc[2 * a] == SHX_BC_MANUFACTURED || d.bc[2 * a + 1] == SHX_BC_MANUFACTURED) c->needs_bs = 1;
+        }
     }
     // rows of the contiguous axis are padded to 16 doubles (128 B) so tiles start on cache lines
     if (d.dims == 3) {
@@ -1034,6 +1126,7 @@ extern "C" int shx_destroy(shx_case *c)
     for (int m = 0; m < 5; m++) { hipFree(c->q[m]); hipFree(c->qsum[m]); hipFree(c->src[m]); }
     for (int m = 0; m < 10; m++) hipFree(c->met[m]);
     for (int m = 0; m < 12; m++) hipFree(c->grad[m]);
+    for (int m = 0; m < 6; m++) hipFree(c->bs[m]);
     hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1080,6 +1173,60 @@ static int ghost_fill(shx_case *c, double *const *f, int nf)
     int rc = tbeg(c, KT_GHOST);
     if (rc) return rc;
     k_ghost_periodic<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(ff, nf, c->gm);
+    LAUNCH_CHECK();
+    return tend(c);
+}
+
+// face writers of every non-periodic face, in face order (i-, i+, j-, j+, k-, k+)
+static int face_fill(shx_case *c, double *const *pr)
+{
+    if (!c->faces) return SHX_OK;
+    const Geom &gm = c->gm;
+    int rc = tbeg(c, KT_FACES);
+    if (rc) return rc;
+    for (int f = 0; f < 2 * gm.dims; f++) {
+        const int kind = c->d.bc[f];
+        if (kind == SHX_BC_PERIODIC) continue;
+        BcArgs a;
+        memset(&a, 0, sizeof a);
+        for (int m = 0; m < 5; m++) a.pr.f[m] = pr[m];
+        if (!c->d.inviscid) { a.T = c->T; a.mu = c->mu; }
+        for (int m = 0; m < 6; m++) a.bs[m] = c->bs[m];
+        a.kind = kind;
+        a.dir = f / 2;
+        a.upper = f % 2;
+        a.g = gm.g;
+        a.t1 = (a.dir == 0) ? 1 : 0;
+        a.t2 = (a.dir == 2) ? 1 : 2;
+        if (kind == SHX_BC_MANUFACTURED) {
+            a.lo1 = gm.lo[a.t1]; a.n1 = gm.n[a.t1]; a.lo2 = gm.lo[a.t2]; a.n2 = gm.n[a.t2];
+        } else {
+            a.lo1 = 0; a.n1 = gm.ng[a.t1]; a.lo2 = 0; a.n2 = gm.ng[a.t2];
+        }
+        a.first = gm.lo[a.dir];
+        a.last = gm.lo[a.dir] + gm.n[a.dir] - 1;
+        for (int d = 0; d < 3; d++) a.st[d] = gm.st[d];
+        a.rho_in = c->d.rho_inflow; a.u_in = c->d.u_inflow; a.v_in = c->d.v_inflow;
+        a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
+        a.cinf = 1.0 / c->d.mach;
+        a.k = c->k;
+        const long nodes = (long)a.n1 * a.n2 * a.g;
+        k_face_bc<<<nblk(nodes, 256), 256, 0, c->stream>>>(a);
+        LAUNCH_CHECK();
+    }
+    return tend(c);
+}
+
+// ghost layers of non-periodic directions from the stage input to the stage output buffer
+static int carry_ghosts(shx_case *c, double *const *from, double *const *to)
+{
+    if (!c->faces) return SHX_OK;
+    CFields5 a;
+    Fields5 b;
+    for (int m = 0; m < 5; m++) { a.f[m] = from[m]; b.f[m] = to[m]; }
+    int rc = tbeg(c, KT_FACES);
+    if (rc) return rc;
+    k_carry_ghosts<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(a, b, c->gm);
     LAUNCH_CHECK();
     return tend(c);
 }
@@ -1141,9 +1288,35 @@ extern "C" int shx_download_metric(shx_case *c, double *const met[10])
 extern "C" int shx_upload_state(shx_case *c, const double *const prim[5])
 {
     if (!c || !prim) return set_err(SHX_ERR_ARG, "null argument");
-    for (int m = 0; m < 5; m++) { int rc = copy_in(c, c->prim[c->cur][m], prim[m]); if (rc) return rc; }
+    // every stage buffer gets the full array: ghost nodes that no fill writes keep these values,
+    // as they do in the reference's single U_RK array
+    const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+    for (int m = 0; m < 5; m++) {
+        int rc = copy_in(c, c->prim[c->cur][m], prim[m]);
+        if (rc) return rc;
+        for (int b = 0; b < 3; b++)
+            if (b != c->cur)
+                HIPCHK(hipMemcpyAsync(c->prim[b][m], c->prim[c->cur][m], bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_state = 1;
+    return SHX_OK;
+}
+
+extern "C" int shx_upload_boundary_state(shx_case *c, const double *const bs[6])
+{
+    if (!c || !bs) return set_err(SHX_ERR_ARG, "null argument");
+    const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+    for (int m = 0; m < 6; m++) {
+        if (!c->bs[m]) {
+            HIPCHK(hipMalloc((void **)&c->bs[m], bytes));
+            HIPCHK(hipMemsetAsync(c->bs[m], 0, bytes, c->stream));
+        }
+        int rc = copy_in(c, c->bs[m], bs[m]);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_bs = 1;
     return SHX_OK;
 }
 
@@ -1229,10 +1402,13 @@ static int stage_fluxes(shx_case *c, double *const *pr)
         k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
+        if ((rc = face_fill(c, pr))) return rc;
         if ((rc = tbeg(c, KT_GRAD))) return rc;
         k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
+    } else if ((rc = face_fill(c, pr))) {
+        return rc;
     }
     for (int dir = 0; dir < gm.dims; dir++)
         if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
@@ -1340,6 +1516,8 @@ static int check_ready(shx_case *c)
     if (!c) return set_err(SHX_ERR_ARG, "null handle");
     if (!c->has_metric) return set_err(SHX_ERR_STATE, "no metric: call shx_build_metric or shx_upload_metric");
     if (!c->has_state) return set_err(SHX_ERR_STATE, "no state: call shx_upload_state");
+    if (c->needs_bs && !c->has_bs)
+        return set_err(SHX_ERR_STATE, "manufactured faces need shx_upload_boundary_state");
     return SHX_OK;
 }
 
@@ -1354,6 +1532,7 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
     int in = un, outb = bx;
     for (int s = 0; s < c->d.time_order; s++) {
         if ((rc = stage_fluxes(c, c->prim[in]))) return rc;
+        if ((rc = carry_ghosts(c, c->prim[in], c->prim[outb]))) return rc;
         if ((rc = rk_update(c, c->prim[in], c->prim[outb], s, tau))) return rc;
         // stage s+1 reads what stage s wrote and writes the other scratch buffer
         in = outb;

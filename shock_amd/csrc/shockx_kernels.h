@@ -28,6 +28,7 @@ struct Geom {
     int n[3];         // interior counts (n[2] = 1 in 2D)
     int lo[3];        // padded index of the first interior node (g, or 0 for the 2D z axis)
     long st[3];       // strides; the contiguous axis is padded to a multiple of 16 doubles
+    int per[3];       // 1 where the direction is periodic (ghosts wrap), 0 where face writers fill them
     int dims, g;
     long ntot;        // logical padded nodes ng0*ng1*ng2
     long nalloc;      // doubles allocated per field (with the row padding)

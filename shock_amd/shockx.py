@@ -17,9 +17,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libshockx.so")
 QUIRK_ZETA_VISC_J = 1
 QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
-BC_PERIODIC = 0
+BC_PERIODIC, BC_MANUFACTURED, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 FP64 = 0
-NUM_KERNELS = 12
+NUM_KERNELS = 13
 
 
 class ShxError(RuntimeError):
@@ -34,6 +34,8 @@ class Desc(ctypes.Structure):
         ("prandtl", ctypes.c_double), ("T0", ctypes.c_double),
         ("inviscid", ctypes.c_int), ("manufactured", ctypes.c_int), ("quirks", ctypes.c_uint),
         ("bc", ctypes.c_int * 6), ("precision", ctypes.c_int), ("eps_override", ctypes.c_double),
+        ("rho_inflow", ctypes.c_double), ("u_inflow", ctypes.c_double), ("v_inflow", ctypes.c_double),
+        ("w_inflow", ctypes.c_double), ("p_inflow", ctypes.c_double),
     ]
 
 
@@ -42,7 +44,7 @@ EXPORTS = [
     "shx_build_metric", "shx_upload_metric", "shx_download_metric", "shx_upload_state",
     "shx_download_state", "shx_upload_source", "shx_calc_runge_kutta", "shx_run", "shx_rhs",
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
-    "shx_copy_probe",
+    "shx_copy_probe", "shx_upload_boundary_state",
 ]
 
 _lib = None
@@ -62,7 +64,7 @@ def lib():
         L.shx_padded_extent.argtypes = [P, P]
         L.shx_build_metric.argtypes = [P, P, P, P]
         for f in ("shx_upload_metric", "shx_download_metric", "shx_upload_state", "shx_download_state",
-                  "shx_upload_source", "shx_rhs"):
+                  "shx_upload_source", "shx_rhs", "shx_upload_boundary_state"):
             getattr(L, f).argtypes = [P, P]
         L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
         L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
@@ -95,7 +97,9 @@ class Case:
 
     def __init__(self, dims=2, space_order=5, time_order=4, n=(64, 64, 1), gamma=1.4, mach=0.5,
                  reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, manufactured=False,
-                 quirks=QUIRKS_REFERENCE, bc=(BC_PERIODIC,) * 6, precision=FP64, eps_override=0.0):
+                 quirks=QUIRKS_REFERENCE, bc=(BC_PERIODIC,) * 6, precision=FP64, eps_override=0.0,
+                 inflow=(1.0, 1.0, 0.0, 0.0, 1.0)):
+        """inflow = (rho, u, v, w, p) of the inflow state (also the Riemann outflow's far field)."""
         d = Desc()
         d.dims, d.space_order, d.time_order = dims, space_order, time_order
         for a in range(3):
@@ -105,6 +109,7 @@ class Case:
         for f in range(6):
             d.bc[f] = bc[f]
         d.precision, d.eps_override = precision, eps_override
+        d.rho_inflow, d.u_inflow, d.v_inflow, d.w_inflow, d.p_inflow = inflow
         h = ctypes.c_void_p()
         _check(lib().shx_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
@@ -165,6 +170,11 @@ class Case:
         out, p = self._outs(5)
         _check(lib().shx_download_state(self.h, p))
         return out
+
+    def upload_boundary_state(self, bs):
+        """rho, u, v, w, e, p on the padded grid for manufactured-solution faces."""
+        keep, p = self._ptrs(bs)
+        _check(lib().shx_upload_boundary_state(self.h, p))
 
     def upload_source(self, src):
         if src is None:

@@ -1,0 +1,245 @@
+"""Pin the oracle's boundary writers and the manufactured-solution inputs.
+
+Checks (CPU only):
+  * the periodic transfer touches only periodic directions (src/Functions.c:3580-3684)
+  * supersonic inflow (src/BC.c:2505-2542): the inflow state on every ghost node of the slab
+  * subsonic Riemann outflow (src/BC.c:2572-2622): the ghost state carries the mirror node's
+    outgoing invariant u + 2c/(gamma-1) and entropy p/rho^gamma, the far field's incoming
+    invariant u_inflow - 2/(Ma (gamma-1)), the mirror's tangential velocity, T and mu
+  * manufactured-solution faces (src/ManufacturedSolution.c:202-338): exact state on the ghost
+    layers of the interior tangential range only
+  * face order (src/BC.c:13-74): a later face overwrites an earlier one at shared edges
+  * a uniform stream at the inflow state passes an inflow/outflow block unchanged
+  * the manufactured source equals a symbolic (sympy) residual of the conservation laws, and
+    the discrete residual of the exact field converges at the scheme's order
+"""
+import numpy as np
+import pytest
+
+import orc
+from helpers import interior
+from shock_amd import cases, mms
+
+GAMMA, MACH = 1.4, 0.5
+
+
+def _case(dims, S, n, bc, inflow=(1.0, 1.0, 0.0, 0.0, 1.0), mapping=None, **kw):
+    o = orc.Oracle(dims=dims, S=S, n=n, bc=bc, inflow=inflow, mach=MACH, **kw)
+    per = tuple(bc[2 * d] == orc.BC_PERIODIC for d in range(3))
+    xe, ye, ze = cases.ext_coords(dims, S, n, mapping or cases.cartesian_unit(dims), periodic=per)
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    return o, met, (xe, ye, ze)
+
+
+def _random_state(o, seed):
+    rng = np.random.default_rng(seed)
+    prim = o.zeros(5)
+    prim[0][...] = 1.0 + 0.2 * rng.random(o.shape)
+    prim[1][...] = 0.4 + 0.1 * rng.standard_normal(o.shape)
+    prim[2][...] = 0.1 * rng.standard_normal(o.shape)
+    prim[3][...] = 0.1 * rng.standard_normal(o.shape) if o.c.dims == 3 else 0.0
+    p = 1.0 + 0.2 * rng.random(o.shape)
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], p, prim[0], GAMMA, MACH)
+    return prim
+
+
+def test_periodic_fill_only_wraps_periodic_directions():
+    n = (8, 7, 9)
+    o, met, _ = _case(3, 5, n, bc=(orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0))
+    f = [np.random.default_rng(1).random(o.shape)]
+    before = f[0].copy()
+    o.ghost_periodic(f)
+    g = o.g
+    # i ghosts with interior j, k are left alone
+    assert np.array_equal(f[0][:g, g:g + n[1], g:g + n[2]], before[:g, g:g + n[1], g:g + n[2]])
+    # j ghosts wrap (for every i, including the i ghost slab)
+    assert np.array_equal(f[0][:, :g, g:g + n[2]], f[0][:, n[1]:n[1] + g, g:g + n[2]])
+    # an (i ghost, j ghost, k ghost) node copies (i ghost, wrapped j, wrapped k)
+    assert f[0][0, 0, 0] == before[0, n[1], n[2]]
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_supersonic_inflow_writes_the_inflow_state(dims):
+    n = (8, 7, 6) if dims == 3 else (8, 7, 1)
+    inflow = (1.3, 2.5, 0.2, -0.1 if dims == 3 else 0.0, 0.8)
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, inflow=inflow)
+    prim = _random_state(o, 3)
+    der = o.calc_values(met, prim)
+    o.apply_bc(met, prim, der)
+    g = o.g
+    sl = (slice(0, g), slice(None), slice(None))       # every tangential node, ghosts included
+    rho, u, v, w, p = inflow
+    ups = 1.0 / (GAMMA * MACH * MACH)
+    for m, val in enumerate((rho, u, v, w)):
+        assert np.all(prim[m][sl] == val)
+    e = 0.5 * (u * u + v * v + w * w) + p / rho / (GAMMA - 1.0) * ups
+    assert np.allclose(prim[4][sl], e, rtol=1e-15, atol=0)
+    assert np.all(der[0][sl] == p) and np.all(der[1][sl] == p / rho)
+    suth = 100.4 / 300.0
+    assert np.allclose(der[3][sl], (1 + suth) * (p / rho) ** 1.5 / (p / rho + suth), rtol=1e-15)
+    assert np.allclose(der[2][sl], np.sqrt(ups * GAMMA * p / rho), rtol=1e-15)
+    assert np.allclose(der[4][sl], u * met[0][sl] + v * met[1][sl] + w * met[2][sl], rtol=1e-15)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_riemann_outflow_carries_the_characteristic_invariants(dims):
+    n = (9, 6, 5) if dims == 3 else (9, 6, 1)
+    inflow = (1.0, 0.6, 0.0, 0.0, 1.0)
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, inflow=inflow)
+    prim = _random_state(o, 5)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    mirror_prim = [a.copy() for a in prim]
+    o.apply_bc(met, prim, der)
+    g, gm1, ups = o.g, GAMMA - 1.0, 1.0 / (GAMMA * MACH * MACH)
+    last = g + n[0] - 1
+    for l in range(g):
+        gh, mi = (last + 1 + l, slice(None), slice(None)), (last - l, slice(None), slice(None))
+        rg, ug = prim[0][gh], prim[1][gh]
+        pg = (prim[4][gh] - 0.5 * (ug * ug + prim[2][gh] ** 2 + prim[3][gh] ** 2)) * rg * gm1 / ups
+        cg = np.sqrt(ups * GAMMA * pg / rg)
+        rs, us = mirror_prim[0][mi], mirror_prim[1][mi]
+        ps, cs = der0[0][mi], der0[2][mi]
+        assert np.allclose(ug + 2 * cg / gm1, us + 2 * cs / gm1, rtol=1e-13)
+        assert np.allclose(ug - 2 * cg / gm1, inflow[1] - 2 * (1.0 / MACH) / gm1, rtol=1e-13)
+        assert np.allclose(pg / rg ** GAMMA, ps / rs ** GAMMA, rtol=1e-13)
+        assert np.array_equal(prim[2][gh], mirror_prim[2][mi])
+        assert np.array_equal(prim[3][gh], mirror_prim[3][mi])
+        assert np.array_equal(der[1][gh], der0[1][mi]) and np.array_equal(der[3][gh], der0[3][mi])
+        assert np.allclose(der[0][gh], pg, rtol=1e-13)
+
+
+def test_manufactured_faces_fill_interior_tangential_range_only():
+    n = (7, 8, 6)
+    o, met, _ = _case(3, 5, n, bc=(orc.BC_MMS,) * 6, mms=1)
+    prim = _random_state(o, 9)
+    der = o.calc_values(met, prim)
+    before_p, before_d = [a.copy() for a in prim], [a.copy() for a in der]
+    rng = np.random.default_rng(10)
+    bs = [1.0 + rng.random(o.shape) for _ in range(6)]
+    o.apply_bc(met, prim, der, bs)
+    g = o.g
+    face = (slice(0, g), slice(g, g + n[1]), slice(g, g + n[2]))
+    for m in range(5):
+        assert np.array_equal(prim[m][face], bs[m][face])
+    assert np.array_equal(der[0][face], bs[5][face])
+    assert np.array_equal(der[1][face], np.abs(bs[5][face] / bs[0][face]))
+    assert np.all(der[3][face] == 1.0)
+    assert np.array_equal(der[4][face], before_d[4][face])      # theta keeps the CalcValues value
+    edge = (slice(0, g), slice(0, g), slice(None))              # i-ghost x j-ghost edge untouched
+    for m in range(5):
+        assert np.array_equal(prim[m][edge], before_p[m][edge])
+    sl = interior(o, 3, n)
+    assert np.array_equal(prim[0][sl], before_p[0][sl])
+
+
+def test_later_faces_overwrite_shared_edges():
+    n = (8, 8, 1)
+    inflow = (1.1, 0.9, 0.1, 0.0, 0.95)
+    bc = (orc.BC_OUTFLOW_RIEMANN, orc.BC_OUTFLOW_RIEMANN, orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0)
+    o, met, _ = _case(2, 5, n, bc, inflow=inflow)
+    prim = _random_state(o, 11)
+    der = o.calc_values(met, prim)
+    o.apply_bc(met, prim, der)
+    g = o.g
+    corner = (slice(0, g), slice(0, g), slice(None))            # i- (first) and j- (later) overlap
+    for m, val in enumerate(inflow[:4]):
+        assert np.all(prim[m][corner] == val)
+
+
+@pytest.mark.parametrize("dims,S,inviscid", [(2, 5, False), (3, 5, False), (2, 9, True)])
+def test_uniform_stream_through_inflow_outflow_block(dims, S, inviscid):
+    # p = rho = 1 makes c = 1/Ma, the Riemann far-field sound speed: the outflow ghost equals
+    # its mirror and the stream is a steady solution of the discrete operator
+    n = (12, 10, 8) if dims == 3 else (12, 10, 1)
+    inflow = (1.0, 0.7, 0.0, 0.0, 1.0)
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    o, met, _ = _case(dims, S, n, bc, inflow=inflow, inviscid=int(inviscid))
+    prim = o.zeros(5)
+    prim[0][...] = 1.0
+    prim[1][...] = 0.7
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], 1.0, prim[0], GAMMA, MACH)
+    out = o.run(0.01, met, prim, 2)
+    sl = interior(o, dims, n)
+    for a, b in zip(out, prim):
+        assert np.max(np.abs(a[sl] - b[sl])) < 1e-13 * max(1.0, np.max(np.abs(b[sl])))
+
+
+@pytest.mark.parametrize("dims,supersonic,inviscid", [(2, False, False), (2, True, True), (3, False, False),
+                                                      (3, True, False)])
+def test_manufactured_source_matches_symbolic_residual(dims, supersonic, inviscid):
+    sp = pytest.importorskip("sympy")
+    P = mms.reference_params(dims, supersonic, inviscid, mach=MACH)
+    x, y, z = sp.symbols("x y z", real=True)
+    X = (x, y, z)
+    fn = {"s": sp.sin, "c": sp.cos}
+    ex = {}
+    for f in mms.FIELDS:
+        prm = [sp.Float(str(v), 30) for v in P[f]]
+        ex[f] = prm[0] + sum(prm[1 + d] * fn[mms.KINDS[f][d]](prm[4 + d] * sp.pi * X[d]) for d in range(3))
+    rho, p = ex["rho"], ex["p"]
+    U = [ex["u"], ex["v"], ex["w"]]
+    g = sp.Rational(7, 5)
+    ups = 1 / (g * sp.Rational(1, 4))
+    Re, Pr = 100, sp.Rational(72, 100)
+    psi, gh = sp.Rational(1, Re), 1 / ((g - 1) * sp.Rational(1, 4) * Re * Pr)
+    div = sum(sp.diff(U[i], X[i]) for i in range(3))
+    tau = [[sp.diff(U[j], X[i]) + sp.diff(U[i], X[j]) - (sp.Rational(2, 3) * div if i == j else 0)
+            for j in range(3)] for i in range(3)]
+    E = sp.Rational(1, 2) * rho * sum(u * u for u in U) + ups * p / (g - 1)
+    res = [sum(sp.diff(rho * U[i], X[i]) for i in range(3))]
+    for j in range(3):
+        r = sum(sp.diff(rho * U[i] * U[j] + (ups * p if i == j else 0), X[i]) for i in range(3))
+        if not inviscid:
+            r -= psi * sum(sp.diff(tau[i][j], X[i]) for i in range(3))
+        res.append(r)
+    r = sum(sp.diff((E + ups * p) * U[i], X[i]) for i in range(3))
+    if not inviscid:
+        r -= psi * sum(sp.diff(sum(tau[i][j] * U[j] for j in range(3)), X[i]) for i in range(3))
+        r -= gh * sum(sp.diff(sp.diff(p / rho, X[i]), X[i]) for i in range(3))
+    res.append(r)
+    pts = np.random.default_rng(4).random((6, 3))
+    if dims == 2:
+        pts[:, 2] = 0.0
+    got = mms.source(P, pts[:, 0], pts[:, 1], pts[:, 2], GAMMA, MACH, 100.0, 0.72, inviscid)
+    for m in range(5):
+        f = sp.lambdify(X, res[m], "mpmath")
+        want = np.array([float(f(*[sp.Float(float(c), 30) for c in pt])) for pt in pts])
+        scale = max(1.0, np.max(np.abs(want)))
+        assert np.max(np.abs(got[m] - want)) < 1e-12 * scale, (m, got[m], want)
+
+
+def _mms_residual(dims, S, nn, inviscid, eps=0.0, quirks=orc.QUIRKS_REFERENCE):
+    n = (nn, nn, nn if dims == 3 else 1)
+    o, met, ext = _case(dims, S, n, bc=(orc.BC_MMS,) * 6, mms=1, inviscid=int(inviscid), reynolds=100.0,
+                        eps_override=eps, quirks=quirks)
+    x, y, z = cases.interior_coords(dims, S, n, *ext)
+    P = mms.reference_params(dims, False, inviscid, mach=MACH)
+    ex = mms.exact_state(P, x, y, z, GAMMA, MACH)
+    src = mms.source(P, x, y, z, GAMMA, MACH, 100.0, 0.72, inviscid)
+    q, _, _ = o.rhs(met, [a.copy() for a in ex[:5]], src, ex)
+    sl = interior(o, dims, n)
+    return max(np.max(np.abs(q[m][sl] / met[9][sl])) for m in range(5))
+
+
+@pytest.mark.parametrize("dims,S,inviscid,eps,ns,order", [
+    (2, 5, False, 1e-6, (32, 64), 4.5),     # classical JS epsilon: the linear five-point order
+    (2, 5, True, 1e-6, (32, 64), 4.5),
+    (2, 9, False, 0.0, (16, 32), 6.5),      # the reference's epsilon
+    (3, 5, True, 1e-6, (12, 24), 4.5),
+])
+def test_manufactured_residual_converges(dims, S, inviscid, eps, ns, order):
+    e = [_mms_residual(dims, S, nn, inviscid, eps) for nn in ns]
+    assert np.log2(e[0] / e[1]) > order, e
+
+
+def test_zeta_quirk_leaves_a_3d_viscous_residual():
+    # the reference's zeta viscous stencil (k offset taken from j) is not a consistent
+    # discretisation: the manufactured residual stays O(1) under refinement unless it is off
+    on = [_mms_residual(3, 5, nn, False, 1e-6) for nn in (12, 24)]
+    off = [_mms_residual(3, 5, nn, False, 1e-6, quirks=orc.QUIRK_METRIC_DIAG3D) for nn in (12, 24)]
+    assert on[1] > 0.5 * on[0] and on[1] > 0.1
+    assert np.log2(off[0] / off[1]) > 4.5
