@@ -191,49 +191,47 @@ __device__ __forceinline__ void right_row(const Eig &e, double R[5])
     }
 }
 
-// WENO5 deviation with the reciprocal estimate (same formula as phi_w5 below).
+// Smoothness indicators s_k = IS_k + eps scaled by a common power of two so the largest lies in
+// [1/2, 1), each clamped from below at 2^lo.  The nonlinear weights d_k / s_k^2 normalised to sum
+// one are unchanged by the common scale; the clamp only touches stencils whose weight is below
+// 2^(2 lo) of the largest one's.
+template <int N>
+__device__ __forceinline__ void scaled_sq(double s[N], const int lo)
+{
+    double mx = s[0];
+#pragma unroll
+    for (int i = 1; i < N; i++) mx = fmax(mx, s[i]);
+    const int ex = __builtin_amdgcn_frexp_exp(mx);
+    const double floor_ = __builtin_amdgcn_ldexp(1.0, lo);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double t = fmax(__builtin_amdgcn_ldexp(s[i], -ex), floor_);
+        s[i] = t * t;
+    }
+}
+
+// Jiang-Shu WENO5 deviation from the central flux (Phi_Function_W5, src/WENO.c:118-204) with one
+// reciprocal for the weights: for q_k = s_k^2, w0 = q1 q2 / D and w2 = 3 q0 q1 / D with
+// D = q1 q2 + 6 q0 q2 + 3 q0 q1.
 __device__ __forceinline__ double phi_w5_fast(const double eps, const double d0, const double d1,
                                               const double d2, const double d3)
 {
     const double t0 = d0 - d1, t1 = d0 - 3. * d1;
     const double t2 = d1 - d2, t3 = d1 + d2;
     const double t4 = d2 - d3, t5 = 3. * d2 - d3;
-    const double is0 = 13. * t0 * t0 + 3. * t1 * t1;
-    const double is1 = 13. * t2 * t2 + 3. * t3 * t3;
-    const double is2 = 13. * t4 * t4 + 3. * t5 * t5;
-    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps;
-    const double a0 = frcp(s0 * s0);
-    const double a1 = 6. * frcp(s1 * s1);
-    const double a2 = 3. * frcp(s2 * s2);
-    const double ia = frcp(a0 + a1 + a2);
-    const double w0 = a0 * ia, w2 = a2 * ia;
-    const double q1 = d0 - 2. * d1 + d2;
-    const double qq2 = d1 - 2. * d2 + d3;
-    return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * qq2;
-}
-
-// Jiang-Shu WENO5 deviation from the central flux (Phi_Function_W5, src/WENO.c:118-204).
-__device__ __forceinline__ double phi_w5(const double eps, const double d0, const double d1,
-                                         const double d2, const double d3)
-{
-    const double t0 = d0 - d1, t1 = d0 - 3. * d1;
-    const double t2 = d1 - d2, t3 = d1 + d2;
-    const double t4 = d2 - d3, t5 = 3. * d2 - d3;
-    const double is0 = 13. * t0 * t0 + 3. * t1 * t1;
-    const double is1 = 13. * t2 * t2 + 3. * t3 * t3;
-    const double is2 = 13. * t4 * t4 + 3. * t5 * t5;
-    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps;
-    const double a0 = 1. / (s0 * s0);
-    const double a1 = 6. / (s1 * s1);
-    const double a2 = 3. / (s2 * s2);
-    const double ia = 1. / (a0 + a1 + a2);
-    const double w0 = a0 * ia, w2 = a2 * ia;
+    double q[3] = {13. * t0 * t0 + 3. * t1 * t1 + eps, 13. * t2 * t2 + 3. * t3 * t3 + eps,
+                   13. * t4 * t4 + 3. * t5 * t5 + eps};
+    scaled_sq<3>(q, -250);
+    const double P0 = q[1] * q[2], P2 = 3. * (q[0] * q[1]);
+    const double r = frcp(P0 + 6. * (q[0] * q[2]) + P2);
+    const double w0 = P0 * r, w2 = P2 * r;
     const double q1 = d0 - 2. * d1 + d2;
     const double qq2 = d1 - 2. * d2 + d3;
     return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * qq2;
 }
 
 // Balsara-Shu WENO9 (Phi_Function_W9, src/WENO.c:11-116); f: 9 point values, d: 8 differences.
+// The smoothness indicators keep the reference's quadratic forms in the point values.
 __device__ __forceinline__ double phi_w9(const double eps, const double *f, const double *d)
 {
     const double is0 = f[0] * (22658. * f[0] - 208501. * f[1] + 364863. * f[2] - 288007. * f[3] + 86329. * f[4])
@@ -261,14 +259,14 @@ __device__ __forceinline__ double phi_w9(const double eps, const double *f, cons
         + f[6] * (1521393. * f[6] - 1704396. * f[7] + 364863. * f[8])
         + f[7] * (482963. * f[7] - 208501. * f[8])
         + f[8] * (22658. * f[8]);
-    const double s0 = is0 + eps, s1 = is1 + eps, s2 = is2 + eps, s3 = is3 + eps, s4 = is4 + eps;
-    const double a0 = (1. / 126.) / (s0 * s0);
-    const double a1 = (10. / 63.) / (s1 * s1);
-    const double a2 = (10. / 21.) / (s2 * s2);
-    const double a3 = (20. / 63.) / (s3 * s3);
-    const double a4 = (5. / 126.) / (s4 * s4);
-    const double ia = 1. / (a0 + a1 + a2 + a3 + a4);
-    const double w0 = a0 * ia, w2 = a2 * ia, w3 = a3 * ia, w4 = a4 * ia;
+    // weights d_k / q_k normalised with one reciprocal: w_k = d_k prod_{j != k} q_j / D
+    double q[5] = {is0 + eps, is1 + eps, is2 + eps, is3 + eps, is4 + eps};
+    scaled_sq<5>(q, -120);
+    const double p01 = q[0] * q[1], p012 = p01 * q[2], s34 = q[3] * q[4], s234 = q[2] * s34;
+    const double P0 = q[1] * s234, P1 = q[0] * s234, P2 = p01 * s34, P3 = p012 * q[4], P4 = p012 * q[3];
+    const double A0 = (1. / 126.) * P0, A2 = (10. / 21.) * P2, A3 = (20. / 63.) * P3, A4 = (5. / 126.) * P4;
+    const double ia = frcp(A0 + (10. / 63.) * P1 + A2 + A3 + A4);
+    const double w0 = A0 * ia, w2 = A2 * ia, w3 = A3 * ia, w4 = A4 * ia;
     const double q1 = d[0] - 4. * d[1] + 6. * d[2] - 4. * d[3] + d[4];
     const double q2 = d[1] - 4. * d[2] + 6. * d[3] - 4. * d[4] + d[5];
     const double q3 = d[2] - 4. * d[3] + 6. * d[4] - 4. * d[5] + d[6];
