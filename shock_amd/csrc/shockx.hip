@@ -758,6 +758,89 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
     }
 }
 
+// 3D nodal derivatives, marching along i.  A block owns a GK x GJ column of (j, k) nodes over the
+// full padded extent and walks a chunk of i planes: the i derivative comes from a register window
+// of 2g+1 planes per thread (every plane is read from HBM once per chunk), the j and k derivatives
+// from the current plane staged in LDS with a g-wide halo.  Outputs as k_visc_grad; the T
+// derivatives only when `tgrad` (the diagonal-metric viscous sweeps never read them).
+constexpr int GK = 64, GJ = 4;
+template <int S>
+__global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(CFields5 pr, const double *T, Fields12 gr,
+                                                             Geom gm, Consts k, int ci, int tgrad)
+{
+    constexpr int G = (S + 1) / 2, NL = 2 * G + 1;
+    constexpr int LK = GK + 2 * G, LJ = GJ + 2 * G, LP = LK * LJ;
+    __shared__ double sm[4 * LP];
+    const int tx = threadIdx.x % GK, ty = threadIdx.x / GK;
+    const int k0 = blockIdx.x * GK, j0 = blockIdx.y * GJ;
+    const int kk = k0 + tx, jj = j0 + ty;
+    const bool mine = (kk < gm.ng[2]) && (jj < gm.ng[1]);
+    const int i0 = blockIdx.z * ci, i1 = min(i0 + ci, gm.ng[0]);
+    const double *src[4] = {pr.f[1], pr.f[2], pr.f[3], T};
+    const long col = (long)jj * gm.st[1] + kk;
+    double win[4][NL];
+    // prime the window with planes i0-G .. i0+G-1 (clamped; out-of-block planes are never used)
+#pragma unroll
+    for (int l = 0; l < NL - 1; l++) {
+        const int ip = max(0, min(i0 - G + l, gm.ng[0] - 1));
+#pragma unroll
+        for (int f = 0; f < 4; f++) win[f][l + 1] = mine ? src[f][(long)ip * gm.st[0] + col] : 0.0;
+    }
+    for (int i = i0; i < i1; i++) {
+        // shift the window and read plane i+G
+#pragma unroll
+        for (int l = 0; l < NL - 1; l++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) win[f][l] = win[f][l + 1];
+        const int ip = min(i + G, gm.ng[0] - 1);
+#pragma unroll
+        for (int f = 0; f < 4; f++) win[f][NL - 1] = mine ? src[f][(long)ip * gm.st[0] + col] : 0.0;
+        // stage plane i with its halo
+        const long pbase = (long)i * gm.st[0];
+        for (int e = threadIdx.x; e < LP; e += GK * GJ) {
+            const int lj = e / LK, lk = e - lj * LK;
+            const int j = j0 - G + lj, kq = k0 - G + lk;
+            const bool in = (j >= 0) && (j < gm.ng[1]) && (kq >= 0) && (kq < gm.ng[2]);
+            const long m = pbase + (long)j * gm.st[1] + kq;
+#pragma unroll
+            for (int f = 0; f < 4; f++) sm[f * LP + e] = in ? src[f][m] : 0.0;
+        }
+        __syncthreads();
+        if (mine) {
+            const long n = pbase + col;
+            const int c0 = (ty + G) * LK + (tx + G);
+            if (i >= gm.lo[0] && i < gm.lo[0] + gm.n[0]) {
+                double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+#pragma unroll
+                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * win[f][l];
+                gr.f[0][n] = d[0]; gr.f[3][n] = d[1]; gr.f[6][n] = d[2];
+                if (tgrad) gr.f[9][n] = d[3];
+            }
+            if (jj >= gm.lo[1] && jj < gm.lo[1] + gm.n[1]) {
+                double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+#pragma unroll
+                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G) * LK];
+                gr.f[1][n] = d[0]; gr.f[4][n] = d[1]; gr.f[7][n] = d[2];
+                if (tgrad) gr.f[10][n] = d[3];
+            }
+            if (kk >= gm.lo[2] && kk < gm.lo[2] + gm.n[2]) {
+                double d[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+#pragma unroll
+                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G)];
+                gr.f[2][n] = d[0]; gr.f[5][n] = d[1]; gr.f[8][n] = d[2];
+                if (tgrad) gr.f[11][n] = d[3];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Q += J * src (manufactured-solution source) for the stage residual returned by shx_rhs.
 __global__ void k_add_source(Fields5 q, CFields5 src, const double *J, Geom gm)
 {
@@ -1407,7 +1490,17 @@ static int stage_fluxes(shx_case *c, double *const *pr)
         if ((rc = tend(c))) return rc;
         if ((rc = face_fill(c, pr))) return rc;
         if ((rc = tbeg(c, KT_GRAD))) return rc;
-        k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
+        if (gm.dims == 3) {
+            constexpr int CI = 32;
+            dim3 grid(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI));
+            const int tgrad = c->diag ? 0 : 1;
+            if (c->d.space_order == 5)
+                k_visc_grad_march<5><<<grid, GK * GJ, 0, c->stream>>>(cp, c->T, gf, gm, c->k, CI, tgrad);
+            else
+                k_visc_grad_march<9><<<grid, GK * GJ, 0, c->stream>>>(cp, c->T, gf, gm, c->k, CI, tgrad);
+        } else {
+            k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
+        }
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
     } else if ((rc = face_fill(c, pr))) {
