@@ -152,6 +152,7 @@ __global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Const
 struct WenoArgs {
     CFields5 pr;          // rho, u, v, w, e of the stage input (ghosts filled)
     const double *m0, *m1, *m2, *J;   // metric row of the sweep direction and the Jacobian
+                                      // (diagonal metrics: m0 is the row's diagonal entry)
     Fields5 q;            // stage residual, interior nodes
     long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
     int nD, nA, nB;       // interior counts
@@ -167,7 +168,10 @@ struct WenoArgs {
 enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V_V, V_W, V_C,
        V_NX, V_NY, V_NZ, V_J, NV };
 
-template <int S, int TA, int TDI>
+// AX = -1: general metric rows; AX = 0, 1, 2: diagonal metric, the sweep's normal is the axis AX
+// (the reference's 3D metric, src/Functions.c:6479-6484), which drops two metric loads, two LDS
+// fields and the zero entries of the eigenvector rows.
+template <int S, int TA, int TDI, int AX>
 __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
 {
     constexpr int G = (S + 1) / 2;
@@ -175,7 +179,9 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     constexpr int WD = TDI - 1 + 2 * G;   // window nodes along the sweep
     constexpr int NW = WD * TA;
     constexpr int NT = TA * TDI;
-    __shared__ double sm[NV * NW];
+    constexpr int NVK = (AX < 0) ? NV : NV - 2;   // diagonal: V_NX holds the sign, J moves to V_NY
+    constexpr int VJ = (AX < 0) ? V_J : V_NY;
+    __shared__ double sm[NVK * NW];
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
@@ -196,35 +202,49 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         const long n = bbase + (long)pa * a.sA + (long)dp * a.sD;
         const double rho = a.pr.f[0][n], u = a.pr.f[1][n], v = a.pr.f[2][n], w = a.pr.f[3][n],
                      e = a.pr.f[4][n];
-        const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n], J = a.J[n];
-        const double th = u * mx + v * my + w * mz;
+        const double J = a.J[n];
         const double p = pressure_fast(k, rho, u, v, w, e);
         const double ir = frcp(rho);
         const double cs = sqrt(k.upsilon * k.gamma * p * ir);
         const double pu = p * k.upsilon;
-        const double rt = rho * th;
         const int o = aa * WD + dd;
-        sm[V_F0 * NW + o] = J * rt;
-        sm[V_F1 * NW + o] = J * (rt * u + mx * pu);
-        sm[V_F2 * NW + o] = J * (rt * v + my * pu);
-        sm[V_F3 * NW + o] = J * (rt * w + mz * pu);
-        sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
+        if constexpr (AX < 0) {
+            const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n];
+            const double th = u * mx + v * my + w * mz;
+            const double rt = rho * th;
+            sm[V_F0 * NW + o] = J * rt;
+            sm[V_F1 * NW + o] = J * (rt * u + mx * pu);
+            sm[V_F2 * NW + o] = J * (rt * v + my * pu);
+            sm[V_F3 * NW + o] = J * (rt * w + mz * pu);
+            sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
+            const double mn = sqrt(mx * mx + my * my + mz * mz);
+            const double imn = frcp(mn);
+            sm[V_LAM * NW + o] = fabs(th) + cs * mn;
+            sm[V_NX * NW + o] = mx * imn;
+            sm[V_NY * NW + o] = my * imn;
+            sm[V_NZ * NW + o] = mz * imn;
+        } else {
+            const double md = a.m0[n];
+            const double uu[3] = {u, v, w};
+            const double th = uu[AX] * md;
+            const double rt = rho * th;
+            sm[V_F0 * NW + o] = J * rt;
+#pragma unroll
+            for (int q = 0; q < 3; q++) sm[(V_F1 + q) * NW + o] = (q == AX) ? J * (rt * uu[q] + md * pu) : J * (rt * uu[q]);
+            sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
+            sm[V_LAM * NW + o] = fabs(th) + cs * fabs(md);
+            sm[V_NX * NW + o] = (md > 0.0) ? 1.0 : -1.0;
+        }
         sm[V_U0 * NW + o] = rho;
         sm[V_U1 * NW + o] = rho * u;
         sm[V_U2 * NW + o] = rho * v;
         sm[V_U3 * NW + o] = rho * w;
         sm[V_U4 * NW + o] = rho * e;
-        const double mn = sqrt(mx * mx + my * my + mz * mz);
-        const double imn = frcp(mn);
-        sm[V_LAM * NW + o] = fabs(th) + cs * mn;
         sm[V_U * NW + o] = u;
         sm[V_V * NW + o] = v;
         sm[V_W * NW + o] = w;
         sm[V_C * NW + o] = cs;
-        sm[V_NX * NW + o] = mx * imn;
-        sm[V_NY * NW + o] = my * imn;
-        sm[V_NZ * NW + o] = mz * imn;
-        sm[V_J * NW + o] = J;
+        sm[VJ * NW + o] = J;
     }
     __syncthreads();
 
@@ -234,15 +254,24 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     const int oc = ob + R;                       // left node of the interface
     double out[5];
     {
-        const double Jh = (sm[V_J * NW + oc] + sm[V_J * NW + oc + 1]) / 2.0;
+        const double Jh = (sm[VJ * NW + oc] + sm[VJ * NW + oc + 1]) / 2.0;
         const double u = 0.5 * (sm[V_U * NW + oc] + sm[V_U * NW + oc + 1]);
         const double v = 0.5 * (sm[V_V * NW + oc] + sm[V_V * NW + oc + 1]);
         const double w = 0.5 * (sm[V_W * NW + oc] + sm[V_W * NW + oc + 1]);
         const double rho = 0.5 * (sm[V_U0 * NW + oc] + sm[V_U0 * NW + oc + 1]);
         const double cs = 0.5 * (sm[V_C * NW + oc] + sm[V_C * NW + oc + 1]);
-        const double nx = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
-        const double ny = 0.5 * (sm[V_NY * NW + oc] + sm[V_NY * NW + oc + 1]);
-        const double nz = 0.5 * (sm[V_NZ * NW + oc] + sm[V_NZ * NW + oc + 1]);
+        double nx, ny, nz;
+        if constexpr (AX < 0) {
+            nx = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
+            ny = 0.5 * (sm[V_NY * NW + oc] + sm[V_NY * NW + oc + 1]);
+            nz = 0.5 * (sm[V_NZ * NW + oc] + sm[V_NZ * NW + oc + 1]);
+        } else {
+            const double ns = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
+            nx = (AX == 0) ? ns : 0.0;
+            ny = (AX == 1) ? ns : 0.0;
+            nz = (AX == 2) ? ns : 0.0;
+        }
+        const double nsig = (AX == 0) ? nx : ((AX == 1) ? ny : nz);
         const Eig e = make_eig(nx, ny, nz, u, v, w, rho, cs, k.gamma);
         double lam = 0.0;
 #pragma unroll
@@ -258,16 +287,19 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
             // re-read the stencil from LDS for every field instead of keeping 60 values live
             // across the five fields: 164 instead of 228 VGPRs, three waves per SIMD
             asm volatile("" ::: "memory");
-            double Lc[5];
-            left_row<c>(e, Lc);
+            double Lc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if constexpr (AX < 0) left_row<c>(e, Lc);
+            else left_row_ax<c, (AX < 0 ? 0 : AX)>(e, nsig, Lc);
             double av[2 * G], bv[2 * G];
 #pragma unroll
             for (int s = 0; s < 2 * G; s++) {
                 double aa = 0.0, bb = 0.0;
 #pragma unroll
                 for (int q = 0; q < 5; q++) {
-                    aa += Lc[q] * sm[(V_F0 + q) * NW + ob + s];
-                    bb += Lc[q] * sm[(V_U0 + q) * NW + ob + s];
+                    if (AX < 0 || lnz(c, AX, q)) {
+                        aa += Lc[q] * sm[(V_F0 + q) * NW + ob + s];
+                        bb += Lc[q] * sm[(V_U0 + q) * NW + ob + s];
+                    }
                 }
                 av[s] = aa;
                 bv[s] = bb;
@@ -304,9 +336,18 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
                 th = 1. / 840. * (533. * (F[oc + 1] + F[oc]) - 139. * (F[oc + 2] + F[oc - 1])
                                   + 29. * (F[oc + 3] + F[oc - 2]) - 3. * (F[oc + 4] + F[oc - 3]));
             }
-            double Rr[5];
-            right_row<m>(e, Rr);
-            out[m] = phis[0] * Rr[0] + phis[1] * Rr[1] + phis[2] * Rr[2] + phis[3] * Rr[3] + phis[4] * Rr[4] + th;
+            double Rr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if constexpr (AX < 0) {
+                right_row<m>(e, Rr);
+                out[m] = phis[0] * Rr[0] + phis[1] * Rr[1] + phis[2] * Rr[2] + phis[3] * Rr[3] + phis[4] * Rr[4] + th;
+            } else {
+                right_row_ax<m, (AX < 0 ? 0 : AX)>(e, nsig, Rr);
+                double acc = 0.0;
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++)
+                    if (rnz(m, AX, cc)) acc = acc + phis[cc] * Rr[cc];
+                out[m] = acc + th;
+            }
         };
         assemble(std::integral_constant<int, 0>{});
         assemble(std::integral_constant<int, 1>{});
@@ -1436,7 +1477,10 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     WenoArgs a;
     memset(&a, 0, sizeof a);
     for (int m = 0; m < 5; m++) { a.pr.f[m] = pr[m]; a.q.f[m] = c->q[m]; }
-    a.m0 = c->met[3 * dir]; a.m1 = c->met[3 * dir + 1]; a.m2 = c->met[3 * dir + 2]; a.J = c->met[9];
+    const bool diag = (c->diag && gm.dims == 3);
+    if (diag) { a.m0 = c->met[4 * dir]; a.m1 = nullptr; a.m2 = nullptr; }
+    else { a.m0 = c->met[3 * dir]; a.m1 = c->met[3 * dir + 1]; a.m2 = c->met[3 * dir + 2]; }
+    a.J = c->met[9];
     // transverse axes: fast = the contiguous non-sweep axis with extent > 1
     int A, B;
     if (gm.dims == 3) {
@@ -1458,13 +1502,25 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     if (a.coalesce_d) {
         constexpr int TA = 4, TDI = 64;
         dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
-        if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
-        else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+        if (c->d.space_order == 5) {
+            if (diag) k_weno<5, TA, TDI, 2><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else k_weno<5, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
+        } else {
+            if (diag) k_weno<9, TA, TDI, 2><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else k_weno<9, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
+        }
     } else {
         constexpr int TA = 16, TDI = 16;
         dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
-        if (c->d.space_order == 5) k_weno<5, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
-        else k_weno<9, TA, TDI><<<grid, TA * TDI, 0, c->stream>>>(a);
+        if (c->d.space_order == 5) {
+            if (diag && dir == 0) k_weno<5, TA, TDI, 0><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else if (diag) k_weno<5, TA, TDI, 1><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else k_weno<5, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
+        } else {
+            if (diag && dir == 0) k_weno<9, TA, TDI, 0><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else if (diag) k_weno<9, TA, TDI, 1><<<grid, TA * TDI, 0, c->stream>>>(a);
+            else k_weno<9, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
+        }
     }
     LAUNCH_CHECK();
     return tend(c);

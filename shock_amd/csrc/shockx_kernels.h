@@ -191,6 +191,74 @@ __device__ __forceinline__ void right_row(const Eig &e, double R[5])
     }
 }
 
+// ---- axis-aligned normals (diagonal metric) -----------------------------------------------------
+// With n = s e_AX (s = +-1) the eigenvector rows above lose most entries.  eps3(a, b, c) is the
+// Levi-Civita symbol; the rows below are the general ones with the zero components dropped, so
+// they produce the same values.
+__host__ __device__ constexpr int eps3(int a, int b, int c)
+{
+    return (a == b || b == c || a == c) ? 0 : (((a + 1) % 3 == b) ? 1 : -1);
+}
+// L[C][q] can be non-zero
+__host__ __device__ constexpr bool lnz(int C, int AX, int q)
+{
+    return (C == AX || C >= 3) ? true : (q == 0 || q == 1 + (3 - C - AX));
+}
+// R[M][C] can be non-zero
+__host__ __device__ constexpr bool rnz(int M, int AX, int C)
+{
+    return (C >= 3 || M == 4) ? true
+         : (M == 0 ? C == AX : (C == AX || eps3(M - 1, C, AX) != 0));
+}
+
+template <int C, int AX>
+__device__ __forceinline__ void left_row_ax(const Eig &e, const double s, double L[5])
+{
+    constexpr double is2 = 0.70710678118654752440;
+    const double uu[3] = {e.u, e.v, e.w};
+    if constexpr (C == AX) {
+        L[0] = s * e.hm;
+        L[1] = e.gc * e.u * s; L[2] = e.gc * e.v * s; L[3] = e.gc * e.w * s; L[4] = -e.gc * s;
+    } else if constexpr (C < 3) {
+        constexpr int T = 3 - C - AX;
+        constexpr double sg = (double)eps3(C, AX, T);
+        L[0] = sg * s * (e.irho * uu[T]);
+        L[1 + T] = -sg * s * e.irho;
+    } else {
+        constexpr double pm = (C == 3) ? 1.0 : -1.0;
+        L[0] = e.ics * (e.gm1 * e.q2 / 2. - pm * e.cs * e.un);
+#pragma unroll
+        for (int q = 0; q < 3; q++) L[1 + q] = (q == AX) ? (-e.gs * uu[q] + pm * s) * is2 : (-e.gs * uu[q]) * is2;
+        L[4] = e.gm1 * e.ics;
+    }
+}
+
+template <int M, int AX>
+__device__ __forceinline__ void right_row_ax(const Eig &e, const double s, double R[5])
+{
+    const double uu[3] = {e.u, e.v, e.w};
+    if constexpr (M == 0) {
+        R[AX] = s; R[3] = e.ics; R[4] = e.ics;
+    } else if constexpr (M < 4) {
+        constexpr int q = M - 1;
+#pragma unroll
+        for (int C = 0; C < 3; C++) {
+            if (C == AX) R[C] = uu[q] * s;
+            else if (eps3(q, C, AX) != 0) R[C] = -(double)eps3(q, C, AX) * (e.rho * s);
+        }
+        R[3] = (q == AX) ? e.ics * (uu[q] + s * e.cs) : e.ics * uu[q];
+        R[4] = (q == AX) ? e.ics * (uu[q] - s * e.cs) : e.ics * uu[q];
+    } else {
+#pragma unroll
+        for (int C = 0; C < 3; C++) {
+            if (C == AX) R[C] = e.q2 / 2. * s;
+            else R[C] = e.rho * ((double)eps3(C, 3 - C - AX, AX) * uu[3 - C - AX] * s);
+        }
+        R[3] = e.ics * (e.h0 + e.cs * e.un);
+        R[4] = e.ics * (e.h0 - e.cs * e.un);
+    }
+}
+
 // Smoothness indicators s_k = IS_k + eps scaled by a common power of two so the largest lies in
 // [1/2, 1), each clamped from below at 2^lo.  The nonlinear weights d_k / s_k^2 normalised to sum
 // one are unchanged by the common scale; the clamp only touches stencils whose weight is below
