@@ -29,25 +29,32 @@ HBM_PEAK_GBS = 8000.0
 FP64 = 8
 
 
-def algo_bytes(name, n, ng, time_order=4):
+def algo_bytes(name, n, ng, time_order=4, diag=True):
     """Algorithmic HBM bytes of one launch of kernel class `name` for an n^3 block with padded
-    extent ng^3: each input array element read once, each output element written once."""
+    extent ng^3 (row padding excluded): each input array element read once, each output element
+    written once.  diag: the metric is diagonal (the reference's 3D metric), so a sweep reads one
+    metric entry of its direction plus J.  The last viscous sweep of a stage also applies the
+    stage update (U_n, Q, Q_sum, U_s+1)."""
     nint = n ** 3
     npad = ng ** 3
     ghost = npad - nint
-    if name == "weno_xi":            # rho,u,v,w,e + metric row (3) + J in, Q (5) out
-        return nint * (9 + 5) * FP64
+    met = 2 if diag else 4                     # metric values read per node by a WENO sweep
+    qsum = 2 * 5 * (time_order - 1) / time_order if time_order == 4 else 0.0
+    if name == "weno_xi":            # rho,u,v,w,e + metric in, Q (5) out
+        return nint * (5 + met + 5) * FP64
     if name in ("weno_eta", "weno_zeta"):   # same + Q read-modify-write
-        return nint * (9 + 5 + 5) * FP64
+        return nint * (5 + met + 5 + 5) * FP64
     if name == "calc_values":        # rho,u,v,w,e in, T, mu out (all padded nodes)
         return npad * (5 + 2) * FP64
-    if name == "visc_grad":          # u,v,w,T in (padded), 12 derivative fields out
-        return npad * 4 * FP64 + 12 * n * ng * ng * FP64
-    if name in ("visc_xi", "visc_eta", "visc_zeta"):   # u,v,w,T,mu + 4 cross derivatives + diagonal
-        return nint * (5 + 4 + 4 + 4 + 4) * FP64      # metric (3) + J in; Q (4 momentum/energy) r/w
-    if name == "rk_update":          # U_n, U_s, Q, J in (Qsum r/w on RK4 stages), U_s+1 out
-        qsum = 2 * 5 * (time_order - 1) / time_order if time_order == 4 else 0.0
-        return nint * (5 + 5 + 5 + 1 + 5 + qsum) * FP64
+    if name == "visc_grad":          # u,v,w,T in (padded), 9 (diagonal) or 12 derivative fields out
+        return npad * 4 * FP64 + (9 if diag else 12) * n * ng * ng * FP64
+    vin = 5 + (4 + 4 if diag else 8 + 10)       # u,v,w,T,mu + cross derivatives + metric
+    if name in ("visc_xi", "visc_eta"):          # + Q (momentum, energy) read-modify-write
+        return nint * (vin + 8) * FP64
+    if name == "visc_zeta":                      # + Q (5) read, U_n (5) read, Q_sum, U_s+1 (5) written
+        return nint * (vin + 5 + 5 + qsum + 5) * FP64
+    if name == "rk_update":          # U_n, Q, J in (Qsum r/w on RK4 stages), U_s+1 out
+        return nint * (5 + 5 + 1 + 5 + qsum) * FP64
     if name == "ghost_periodic":     # 5 fields read + written on ghost nodes
         return ghost * 5 * 2 * FP64
     return None

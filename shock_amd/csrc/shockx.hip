@@ -382,6 +382,47 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
 // derivatives of u,v,w,T along D and midpoint interpolations of the nodal cross derivatives;
 // coefficients K = (mu fac)_face averages; energy work term u_i (F(i+1/2) - F(i-1/2)) and the
 // nodal product term are evaluated per node as in the reference.
+// Stage update of one interior node (src/Functions.c:2300-2365) with the manufactured source
+// (Q += J src): shared by k_rk_update and the last viscous sweep of a stage, which applies it
+// directly instead of storing Q.
+struct RkArgs {
+    CFields5 un, us, src;
+    Fields5 out, qsum;
+    double a, b, qf, ff, mf, tau;
+    int first_stage, has_src, three;
+};
+
+__device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const double J, double qv[5])
+{
+    if (A.has_src) {
+        qv[0] += J * A.src.f[0][n];
+        qv[1] += J * A.src.f[1][n];
+        qv[2] += J * A.src.f[2][n];
+        if (A.three) qv[3] += J * A.src.f[3][n];
+        qv[4] += J * A.src.f[4][n];
+    }
+    double qs[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
+        qs[m] = old + A.mf * qv[m];
+        if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
+    }
+    const double tj = A.tau / J;
+    const double rho_n = A.un.f[0][n];
+    // b = 0 on every RK4 stage and the first SSP-RK3 stage: U_s is not read there
+    const double rho_s = (A.b != 0.0) ? A.us.f[0][n] : 0.0;
+    const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
+    A.out.f[0][n] = rho;
+#pragma unroll
+    for (int m = 1; m < 5; m++) {
+        if (m == 3 && !A.three) { A.out.f[3][n] = 0.0; continue; }
+        const double us = (A.b != 0.0) ? A.us.f[m][n] : 0.0;
+        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * us * rho_s
+                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
+    }
+}
+
 struct ViscArgs {
     const double *u, *v, *w, *T, *mu;
     const double *cr[8];          // cross derivatives (layout depends on DIAG, see host)
@@ -393,6 +434,8 @@ struct ViscArgs {
     int alignA;
     int three;
     int bug;                      // zeta stencil k offset taken from j (DIR = 2 only)
+    int fuse_rk;                  // last sweep of the stage: apply the stage update (rk)
+    RkArgs rk;
     Consts k;
 };
 
@@ -707,23 +750,31 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
                 prod = node_prod_generic<DIR>(g, m, mu);
             }
             const double work = sm[VS_U * NW + p] * dF[0] + sm[VS_V * NW + p] * dF[1] + sm[VS_W * NW + p] * dF[2];
-            a.q.f[1][n] += k.psi * dF[0];
-            a.q.f[2][n] += k.psi * dF[1];
-            if (a.three) a.q.f[3][n] += k.psi * dF[2];
-            a.q.f[4][n] += k.gam_heat * dF[3] + k.psi * (work + prod);
+            if (a.fuse_rk) {
+                double qv[5];
+                qv[0] = a.q.f[0][n];
+                qv[1] = a.q.f[1][n] + k.psi * dF[0];
+                qv[2] = a.q.f[2][n] + k.psi * dF[1];
+                qv[3] = a.three ? a.q.f[3][n] + k.psi * dF[2] : a.q.f[3][n];
+                qv[4] = a.q.f[4][n] + (k.gam_heat * dF[3] + k.psi * (work + prod));
+                const double J = DIAG ? sm[(VS_M + 3) * NW + p] : sm[(VS_M + 9) * NW + p];
+                rk_point(a.rk, n, J, qv);
+            } else {
+                a.q.f[1][n] += k.psi * dF[0];
+                a.q.f[2][n] += k.psi * dF[1];
+                if (a.three) a.q.f[3][n] += k.psi * dF[2];
+                a.q.f[4][n] += k.gam_heat * dF[3] + k.psi * (work + prod);
+            }
         }
     }
 }
 
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
 struct UpdArgs {
-    CFields5 un, us;
-    Fields5 out, q, qsum;
-    CFields5 src;
+    Fields5 q;
     const double *J;
     Geom gm;
-    double a, b, qf, ff, mf, tau;
-    int first_stage, has_src;
+    RkArgs rk;
 };
 
 __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
@@ -737,34 +788,10 @@ __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
     const int jj = (int)(r % gm.n[1]);
     const int ii = (int)(r / gm.n[1]);
     const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
-    const double J = A.J[n];
     double qv[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) qv[m] = A.q.f[m][n];
-    if (A.has_src) {
-        qv[0] += J * A.src.f[0][n];
-        qv[1] += J * A.src.f[1][n];
-        qv[2] += J * A.src.f[2][n];
-        if (gm.dims == 3) qv[3] += J * A.src.f[3][n];
-        qv[4] += J * A.src.f[4][n];
-    }
-    double qs[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
-        qs[m] = old + A.mf * qv[m];
-        if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
-    }
-    const double tj = A.tau / J;
-    const double rho_n = A.un.f[0][n], rho_s = A.us.f[0][n];
-    const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
-    A.out.f[0][n] = rho;
-#pragma unroll
-    for (int m = 1; m < 5; m++) {
-        if (m == 3 && gm.dims == 2) { A.out.f[3][n] = 0.0; continue; }
-        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * A.us.f[m][n] * rho_s
-                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
-    }
+    rk_point(A.rk, n, A.J[n], qv);
 }
 
 // ---- viscous derivatives ------------------------------------------------------------------
@@ -1529,8 +1556,9 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
 // residual of one stage into c->q: ghost fill, derived quantities, viscous derivatives, WENO
 // sweeps of every direction, then the viscous sweeps (the reference's accumulation order,
 // src/Functions.c:2161-2272)
-static int visc_sweep(shx_case *c, double *const *pr, int dir);
-static int stage_fluxes(shx_case *c, double *const *pr)
+static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk);
+// rk: when given, the last viscous sweep applies this stage update instead of storing Q
+static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
 {
     const Geom &gm = c->gm;
     int rc;
@@ -1566,7 +1594,7 @@ static int stage_fluxes(shx_case *c, double *const *pr)
         if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
     if (!c->d.inviscid)
         for (int dir = 0; dir < gm.dims; dir++)
-            if ((rc = visc_sweep(c, pr, dir))) return rc;
+            if ((rc = visc_sweep(c, pr, dir, dir == gm.dims - 1 ? rk : nullptr))) return rc;
     return SHX_OK;
 }
 
@@ -1600,7 +1628,7 @@ static void launch_visc_sweep_s(shx_case *c, const ViscArgs &a, int dir, bool di
 }
 
 // viscous face sweep along axis dir: Q += viscous flux divergence (after the WENO sweeps)
-static int visc_sweep(shx_case *c, double *const *pr, int dir)
+static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
 {
     const Geom &gm = c->gm;
     ViscArgs a;
@@ -1632,6 +1660,8 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir)
     a.three = (gm.dims == 3);
     a.bug = (dir == 2 && (c->d.quirks & SHX_QUIRK_ZETA_VISC_J)) ? 1 : 0;
     a.k = c->k;
+    a.fuse_rk = rk ? 1 : 0;
+    if (rk) a.rk = *rk;
     int rc = tbeg(c, KT_VISC0 + dir);
     if (rc) return rc;
     if (c->d.space_order == 5) launch_visc_sweep_s<5>(c, a, dir, diag);
@@ -1640,22 +1670,32 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir)
     return tend(c);
 }
 
-static int rk_update(shx_case *c, double *const *us, double *const *out, int s, double tau)
+static RkArgs rk_args(shx_case *c, double *const *us, double *const *out, int s, double tau)
+{
+    RkArgs A;
+    memset(&A, 0, sizeof A);
+    for (int m = 0; m < 5; m++) {
+        A.un.f[m] = c->prim[c->cur][m]; A.us.f[m] = us[m]; A.out.f[m] = out[m];
+        A.qsum.f[m] = c->qsum[m]; A.src.f[m] = c->src[m];
+    }
+    A.a = c->rk_a[s]; A.b = c->rk_b[s]; A.qf = c->rk_q[s]; A.ff = c->rk_f[s]; A.mf = c->rk_m[s];
+    A.tau = tau;
+    A.first_stage = (s == 0);
+    A.has_src = c->has_src;
+    A.three = (c->gm.dims == 3);
+    return A;
+}
+
+static int rk_update(shx_case *c, const RkArgs &rk)
 {
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
     UpdArgs A;
     memset(&A, 0, sizeof A);
-    for (int m = 0; m < 5; m++) {
-        A.un.f[m] = c->prim[c->cur][m]; A.us.f[m] = us[m]; A.out.f[m] = out[m];
-        A.q.f[m] = c->q[m]; A.qsum.f[m] = c->qsum[m]; A.src.f[m] = c->src[m];
-    }
+    for (int m = 0; m < 5; m++) A.q.f[m] = c->q[m];
     A.J = c->met[9];
     A.gm = gm;
-    A.a = c->rk_a[s]; A.b = c->rk_b[s]; A.qf = c->rk_q[s]; A.ff = c->rk_f[s]; A.mf = c->rk_m[s];
-    A.tau = tau;
-    A.first_stage = (s == 0);
-    A.has_src = c->has_src;
+    A.rk = rk;
     int rc = tbeg(c, KT_RK);
     if (rc) return rc;
     k_rk_update<<<nblk(nint, 256), 256, 0, c->stream>>>(A);
@@ -1683,9 +1723,11 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
     const int bx = (un + 1) % 3, by = (un + 2) % 3;
     int in = un, outb = bx;
     for (int s = 0; s < c->d.time_order; s++) {
-        if ((rc = stage_fluxes(c, c->prim[in]))) return rc;
+        const RkArgs rk = rk_args(c, c->prim[in], c->prim[outb], s, tau);
+        // viscous stages apply the update in their last sweep; inviscid ones in k_rk_update
+        if ((rc = stage_fluxes(c, c->prim[in], c->d.inviscid ? nullptr : &rk))) return rc;
         if ((rc = carry_ghosts(c, c->prim[in], c->prim[outb]))) return rc;
-        if ((rc = rk_update(c, c->prim[in], c->prim[outb], s, tau))) return rc;
+        if (c->d.inviscid && (rc = rk_update(c, rk))) return rc;
         // stage s+1 reads what stage s wrote and writes the other scratch buffer
         in = outb;
         outb = (outb == bx) ? by : bx;
@@ -1712,7 +1754,7 @@ extern "C" int shx_rhs(shx_case *c, double *const q[5])
     if (!q) return set_err(SHX_ERR_ARG, "null output");
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    if ((rc = stage_fluxes(c, c->prim[c->cur]))) return rc;
+    if ((rc = stage_fluxes(c, c->prim[c->cur], nullptr))) return rc;
     if (c->has_src) {
         Fields5 qf;
         CFields5 sf;
