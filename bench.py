@@ -97,6 +97,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    ap.add_argument("--workload", choices=("turbulence", "mms-ladder"), default="turbulence",
+                    help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
+                         "manufactured-solution convergence ladder in both arithmetic forms")
+    ap.add_argument("--ladder", default="32,64,128", help="mms-ladder: rungs (nodes per direction)")
+    ap.add_argument("--scheme", type=int, choices=(5, 9), default=9, help="mms-ladder: WENO order")
+    ap.add_argument("--t-end", type=float, default=2.0, help="mms-ladder: marching time per rung")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="arithmetic form of the path (mixed: FP32 WENO nonlinear part)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="launch a plain copy kernel of one padded field this many times after setup "
                          "(reference traffic for hardware byte counters)")
@@ -107,12 +115,16 @@ def main():
               "a design (one block per GPU, halo exchange per stage) and is not built.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "mms-ladder":
+        return mms_ladder(args)
+
     n = args.n
     S, time_order = 5, 4
     nn = (n, n, n)
     t_setup = time.perf_counter()
     case = shockx.Case(dims=3, space_order=S, time_order=time_order, n=nn, gamma=1.4, mach=0.5,
-                       reynolds=2000.0, prandtl=0.72, T0=300.0, quirks=shockx.QUIRKS_REFERENCE)
+                       reynolds=2000.0, prandtl=0.72, T0=300.0, quirks=shockx.QUIRKS_REFERENCE,
+                       precision=shockx.MIXED if args.precision == "mixed" else shockx.FP64)
     xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
     case.build_metric(xe, ye, ze)
     del xe, ye, ze
@@ -181,11 +193,13 @@ def main():
     out = {
         "metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (seeded random-phase turbulence)",
-        "config": {"workload": f"{n}^3 periodic curvilinear block, WENO5 + viscous, FP64, RK4, Re=2000",
+        "config": {"workload": f"{n}^3 periodic curvilinear block, WENO5 + viscous, "
+                               f"{'FP64' if args.precision == 'fp64' else 'mixed FP32/FP64'}, RK4, Re=2000",
                    "grid": [n, n, n], "scheme": "WENO5", "viscous": True, "time_order": time_order,
-                   "precision": "fp64", "quirks": "reference", "tau": tau, "seed": args.seed},
+                   "precision": args.precision, "quirks": "reference", "tau": tau, "seed": args.seed},
         "roofline": roofline,
         "stage_hbm_frac_260B": round(stage_frac, 4),
         "kernels": kernels,
@@ -193,6 +207,37 @@ def main():
     }
     print(json.dumps(out), flush=True)
     case.close()
+
+
+def mms_ladder(args):
+    """Manufactured-solution ladder in FP64 and mixed forms: one JSON line with the errors, the
+    observed orders and the throughput of each rung."""
+    from shock_amd import ladder
+    rungs = [int(x) for x in args.ladder.split(",")]
+    res = {}
+    for prec, pv in (("fp64", shockx.FP64), ("mixed", shockx.MIXED)):
+        rows = []
+        for n in rungs:
+            r = ladder.rung(n, S=args.scheme, precision=pv, t_end=args.t_end)
+            print(f"[ladder] {prec} n={n} steps={r['steps']} L2_rho={r['full']['L2_rho']:.3e} "
+                  f"Linf_rho={r['full']['Linf_rho']:.3e} L2_p={r['full']['L2_p']:.3e} "
+                  f"drho_last={r['last_chunk_drho']:.2e} {r['cell_updates_per_s']:.3e} cu/s",
+                  file=sys.stderr, flush=True)
+            rows.append(r)
+        res[prec] = {"rungs": rows, "order_L2_rho": ladder.orders(rows, "L2_rho"),
+                     "order_Linf_rho": ladder.orders(rows, "Linf_rho"),
+                     "order_L2_p": ladder.orders(rows, "L2_p"),
+                     "order_L2_rho_last_plane": ladder.orders(rows, "L2_rho", "last_plane")}
+    ratio = [m["full"]["L2_rho"] / f["full"]["L2_rho"] for m, f in zip(res["mixed"]["rungs"], res["fp64"]["rungs"])]
+    speed = [m["cell_updates_per_s"] / f["cell_updates_per_s"] for m, f in zip(res["mixed"]["rungs"], res["fp64"]["rungs"])]
+    out = {"metric": "manufactured-solution error ladder", "unit": "L2/Linf of rho and p", "n_gpus": 1,
+           "dtype": "f64 and f64+f32", "data": "manufactured solution (reference subsonic NS set)",
+           "config": {"workload": f"3D MMS ladder {args.ladder}, WENO{args.scheme} + viscous, RK4, "
+                                  f"t_end={args.t_end}, stretched separable grid, Dirichlet faces",
+                      "scheme": f"WENO{args.scheme}", "quirks": "diag3d metric only"},
+           "fp64": res["fp64"], "mixed": res["mixed"], "mixed_over_fp64_L2_rho": ratio,
+           "mixed_speedup": speed}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -44,7 +44,10 @@ extern "C" {
                                        (WriteOutflowSubsonicRiemannBoundary, src/BC.c:2572-2622) */
 
 /* Arithmetic forms. */
-#define SHX_FP64 0
+#define SHX_FP64 0   /* every operation in FP64 (the reference's PRECISION=2) */
+#define SHX_MIXED 1  /* FP32 WENO nonlinear part (projected differences, smoothness indicators,
+                        weights, deviation); state, fluxes, central part, viscous terms, RK
+                        update, metric and CFL in FP64 */
 
 /* Case description: the subset of strct_configuration (src/SHOCK.h:69-544) the path reads. */
 typedef struct shx_desc {
@@ -61,7 +64,7 @@ typedef struct shx_desc {
     int manufactured;  /* options:ManufacturedSolution (mu = 1, source added each stage) */
     unsigned quirks;   /* SHX_QUIRK_* bits; SHX_QUIRKS_REFERENCE reproduces the reference */
     int bc[6];         /* SHX_BC_* per face */
-    int precision;     /* SHX_FP64 */
+    int precision;     /* SHX_FP64 or SHX_MIXED */
     double eps_override; /* 0: WENO epsilon = 10*sqrt(DBL_MIN) as src/Functions.c:1829 */
     /* inflow state (boundaryconditions:rho_inflow, p_inflow; u,v,w_inflow from the angle of
      * attack, src/Import.c:90-91,233-235); the Riemann outflow uses u_inflow as its far field */
@@ -136,6 +139,23 @@ int shx_copy_probe(shx_case *c, int reps);
 int shx_set_timing(shx_case *c, int enable);
 int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
                      int launches[SHX_NUM_KERNELS]);
+
+/* Batches (no reference counterpart: SHOCK runs one case per job).  A batch advances many
+ * independent periodic cases, each created, uploaded and owned as above, with one launch per
+ * kernel class and group of like cases (same dims, scheme, metric kind, precision, viscosity,
+ * RK order) per stage over a tile work list, replayed from a captured HIP graph.  tau holds one
+ * time step per case.  The cases keep their handles: their state can be read or replaced between
+ * shx_batch_run calls (work on a case's stream is ordered after the batch).  Destroy the batch
+ * before its cases. */
+typedef struct shx_batch shx_batch;
+int shx_batch_create(shx_case *const *cases, int ncases, const double *tau, shx_batch **out);
+int shx_batch_set_tau(shx_batch *b, const double *tau);
+int shx_batch_run(shx_batch *b, int nsteps);
+/* Picks the resident blocks per CU of each kernel class (ghost, values, grad, weno, visc, rk) by
+ * timing replayed steps; the case states are restored.  factors_out may be NULL. */
+int shx_batch_tune(shx_batch *b, int steps, int factors_out[6]);
+int shx_batch_synchronize(shx_batch *b);
+int shx_batch_destroy(shx_batch *b);
 
 #ifdef __cplusplus
 }

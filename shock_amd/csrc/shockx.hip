@@ -23,6 +23,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <type_traits>
+#include <algorithm>
 #include <vector>
 
 using namespace shx;
@@ -35,9 +36,8 @@ using namespace shx;
 // the node at the wrapped index of each periodic direction (fixed point of the ghost-inclusive
 // halo passes, src/Functions.c:2117-2136; indices along non-periodic directions are kept, so a
 // face slab's ghosts wrap along the periodic tangential directions only).
-__global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
+__device__ __forceinline__ void ghost_body(const Fields10 &f, const int nf, const Geom &gm, const long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= gm.ntot) return;
     int idx[3];
     const long n = padded_node(gm, t, idx);
@@ -56,6 +56,10 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 #pragma unroll
     for (int m = 0; m < 10; m++)
         if (m < nf) f.f[m][n] = f.f[m][src];
+}
+__global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
+{
+    ghost_body(f, nf, gm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- face boundary writers ---------------------------------------------------------------
@@ -136,9 +140,9 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
 }
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
-__global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Consts k)
+__device__ __forceinline__ void values_body(const CFields5 &pr, double *T, double *mu, const Geom &gm,
+                                            const Consts &k, const long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= gm.ntot) return;
     int idx[3];
     const long n = padded_node(gm, t, idx);
@@ -146,6 +150,10 @@ __global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Const
     const double p = pressure(k, rho, u, v, w, e);
     T[n] = fabs(p / rho);
     mu[n] = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+}
+__global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Consts k)
+{
+    values_body(pr, T, mu, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- WENO sweep ---------------------------------------------------------------------------
@@ -171,8 +179,13 @@ enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V
 // AX = -1: general metric rows; AX = 0, 1, 2: diagonal metric, the sweep's normal is the axis AX
 // (the reference's 3D metric, src/Functions.c:6479-6484), which drops two metric loads, two LDS
 // fields and the zero entries of the eigenvector rows.
-template <int S, int TA, int TDI, int AX>
-__global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
+// MIX: mixed-precision form.  Node fluxes, the central part Theta, the eigen-system and the
+// divergence stay FP64; the characteristic projection (of node-to-node differences of F and U,
+// formed in FP64 and rounded once), the smoothness indicators, weights and the deviation Phi, and
+// R . Phi are FP32.  Phi is a correction on top of Theta, so its rounding enters at the size of
+// the correction rather than of the flux.
+template <int S, int TA, int TDI, int AX, bool MIX>
+__device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const int by, const int bz)
 {
     constexpr int G = (S + 1) / 2;
     constexpr int R = G - 1;
@@ -182,12 +195,13 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
     constexpr int NVK = (AX < 0) ? NV : NV - 2;   // diagonal: V_NX holds the sign, J moves to V_NY
     constexpr int VJ = (AX < 0) ? V_J : V_NY;
     __shared__ double sm[NVK * NW];
+    __shared__ float dfu[MIX ? 10 * NW : 1];      // FP32 differences of F (0..4) and U (5..9)
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
-    const int pa0 = a.alignA ? (int)blockIdx.x * TA : (int)blockIdx.x * TA + a.oA;
-    const int b = blockIdx.y;
-    const int d0 = blockIdx.z * (TDI - 1);       // first node of this tile (interior-relative)
+    const int pa0 = a.alignA ? bx * TA : bx * TA + a.oA;
+    const int b = by;
+    const int d0 = bz * (TDI - 1);               // first node of this tile (interior-relative)
     const long bbase = (long)(b + a.oB) * a.sB;
     const Consts &k = a.k;
 
@@ -247,6 +261,18 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         sm[VJ * NW + o] = J;
     }
     __syncthreads();
+    if constexpr (MIX) {
+        // node-to-node differences of F and U, rounded once to FP32 (position o: node o+1 - node o)
+        for (int idx = t; idx < NW; idx += NT) {
+            if (idx % WD == WD - 1) continue;
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                dfu[q * NW + idx] = (float)(sm[(V_F0 + q) * NW + idx + 1] - sm[(V_F0 + q) * NW + idx]);
+                dfu[(5 + q) * NW + idx] = (float)(sm[(V_U0 + q) * NW + idx + 1] - sm[(V_U0 + q) * NW + idx]);
+            }
+        }
+        __syncthreads();
+    }
 
     // phase 2: interface flux between nodes (d0-1+di) and (d0+di)
     const int ta = t / TDI, di = t - ta * TDI;
@@ -282,8 +308,46 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
         // stencil nodes, f+_s = (a_s + lam J b_s)/2 on nodes m-r..m+r and f-_s = (a - lam J b)/2
         // on the mirrored nodes m+r+1..m-r+1 (src/WENO.c:334-385).
         double phis[5];
+        float phif[5];
         auto project = [&](auto cc) {
             constexpr int c = decltype(cc)::value;
+            if constexpr (MIX) {
+                double Ld[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if constexpr (AX < 0) left_row<c>(e, Ld);
+                else left_row_ax<c, (AX < 0 ? 0 : AX)>(e, nsig, Ld);
+                float Lf[5];
+#pragma unroll
+                for (int q = 0; q < 5; q++) Lf[q] = (float)Ld[q];
+                const float ljf = (float)lj;
+                // projected differences da_s = L_c . (F_{s+1} - F_s), db_s likewise for U
+                float dp[S - 1], dm[S - 1];
+                float da[2 * G - 1], db[2 * G - 1];
+#pragma unroll
+                for (int s = 0; s < 2 * G - 1; s++) {
+                    float x = 0.f, y = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        if (AX < 0 || lnz(c, AX, q)) {
+                            x += Lf[q] * dfu[q * NW + ob + s];
+                            y += Lf[q] * dfu[(5 + q) * NW + ob + s];
+                        }
+                    }
+                    da[s] = x;
+                    db[s] = y;
+                }
+                // differences of the split fluxes: f+ on nodes 0..S-1, f- mirrored
+#pragma unroll
+                for (int s = 0; s < S - 1; s++) {
+                    dp[s] = 0.5f * (da[s] + ljf * db[s]);
+                    dm[s] = 0.5f * (da[S - 1 - s] - ljf * db[S - 1 - s]);
+                }
+                if constexpr (S == 5) {
+                    phif[c] = -phi_w5_f(dp[0], dp[1], dp[2], dp[3]) + phi_w5_f(dm[0], dm[1], dm[2], dm[3]);
+                } else {
+                    phif[c] = -phi_w9_f(dp) + phi_w9_f(dm);
+                }
+                return;
+            } else {
             // re-read the stencil from LDS for every field instead of keeping 60 values live
             // across the five fields: 164 instead of 228 VGPRs, three waves per SIMD
             asm volatile("" ::: "memory");
@@ -319,6 +383,7 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
                 for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
                 phis[c] = -phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_);
             }
+            }
         };
         project(std::integral_constant<int, 0>{});
         project(std::integral_constant<int, 1>{});
@@ -337,7 +402,15 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
                                   + 29. * (F[oc + 3] + F[oc - 2]) - 3. * (F[oc + 4] + F[oc - 3]));
             }
             double Rr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if constexpr (AX < 0) {
+            if constexpr (MIX) {
+                if constexpr (AX < 0) right_row<m>(e, Rr);
+                else right_row_ax<m, (AX < 0 ? 0 : AX)>(e, nsig, Rr);
+                float acc = 0.f;
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++)
+                    if (AX < 0 || rnz(m, AX, cc)) acc += (float)Rr[cc] * phif[cc];
+                out[m] = (double)acc + th;
+            } else if constexpr (AX < 0) {
                 right_row<m>(e, Rr);
                 out[m] = phis[0] * Rr[0] + phis[1] * Rr[1] + phis[2] * Rr[2] + phis[3] * Rr[3] + phis[4] * Rr[4] + th;
             } else {
@@ -373,6 +446,11 @@ __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
             }
         }
     }
+}
+template <int S, int TA, int TDI, int AX, bool MIX>
+__global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
+{
+    weno_body<S, TA, TDI, AX, MIX>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---- viscous fluxes: face-based LDS sweep ------------------------------------------------
@@ -565,7 +643,7 @@ __device__ __forceinline__ double node_prod_diag(const double g[9], const double
 enum { VS_U, VS_V, VS_W, VS_T, VS_MU, VS_CR };
 
 template <int S, int DIR, int TA, int TDI, bool DIAG>
-__global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
+__device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const int by, const int bz)
 {
     constexpr int G = (S + 1) / 2;
     constexpr int WD = TDI - 1 + 2 * G;
@@ -585,9 +663,9 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
-    const int pa0 = a.alignA ? (int)blockIdx.x * TA : (int)blockIdx.x * TA + a.oA;
-    const int b = blockIdx.y;
-    const int d0 = blockIdx.z * (TDI - 1);
+    const int pa0 = a.alignA ? bx * TA : bx * TA + a.oA;
+    const int b = by;
+    const int d0 = bz * (TDI - 1);
     const long bbase = (long)(b + a.oB) * a.sB;
     const Consts &k = a.k;
 
@@ -768,6 +846,11 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
         }
     }
 }
+template <int S, int DIR, int TA, int TDI, bool DIAG>
+__global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
+{
+    visc_body<S, DIR, TA, TDI, DIAG>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
 struct UpdArgs {
@@ -777,10 +860,9 @@ struct UpdArgs {
     RkArgs rk;
 };
 
-__global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
+__device__ __forceinline__ void rk_body(const UpdArgs &A, const long t)
 {
     const Geom &gm = A.gm;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
     if (t >= nint) return;
     const int kk = (int)(t % gm.n[2]);
@@ -793,13 +875,17 @@ __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
     for (int m = 0; m < 5; m++) qv[m] = A.q.f[m][n];
     rk_point(A.rk, n, A.J[n], qv);
 }
+__global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
+{
+    rk_body(A, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // ---- viscous derivatives ------------------------------------------------------------------
 // grad order: u_xi,u_eta,u_zeta, v_xi,v_eta,v_zeta, w_xi,w_eta,w_zeta, T_xi,T_eta,T_zeta.
 // Direction d is evaluated where the node's d index is interior (all transverse nodes).
-__global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, Consts k)
+__device__ __forceinline__ void grad_body(const CFields5 &pr, const double *T, const Fields12 &gr,
+                                          const Geom &gm, const Consts &k, const long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= gm.ntot) return;
     int idx[3];
     const long n = padded_node(gm, t, idx);
@@ -825,6 +911,10 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
         gr.f[9 + d][n] = dT;
     }
 }
+__global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, Consts k)
+{
+    grad_body(pr, T, gr, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // 3D nodal derivatives, marching along i.  A block owns a GK x GJ column of (j, k) nodes over the
 // full padded extent and walks a chunk of i planes: the i derivative comes from a register window
@@ -832,18 +922,31 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
 // from the current plane staged in LDS with a g-wide halo.  Outputs as k_visc_grad; the T
 // derivatives only when `tgrad` (the diagonal-metric viscous sweeps never read them).
 constexpr int GK = 64, GJ = 4;
+struct GradArgs {
+    CFields5 pr;
+    const double *T;
+    Fields12 gr;
+    Geom gm;
+    Consts k;
+    int ci, tgrad;
+};
 template <int S>
-__global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(CFields5 pr, const double *T, Fields12 gr,
-                                                             Geom gm, Consts k, int ci, int tgrad)
+__device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
 {
+    const CFields5 &pr = a.pr;
+    const double *T = a.T;
+    const Fields12 &gr = a.gr;
+    const Geom &gm = a.gm;
+    const Consts &k = a.k;
+    const int ci = a.ci, tgrad = a.tgrad;
     constexpr int G = (S + 1) / 2, NL = 2 * G + 1;
     constexpr int LK = GK + 2 * G, LJ = GJ + 2 * G, LP = LK * LJ;
     __shared__ double sm[4 * LP];
     const int tx = threadIdx.x % GK, ty = threadIdx.x / GK;
-    const int k0 = blockIdx.x * GK, j0 = blockIdx.y * GJ;
+    const int k0 = bx * GK, j0 = by * GJ;
     const int kk = k0 + tx, jj = j0 + ty;
     const bool mine = (kk < gm.ng[2]) && (jj < gm.ng[1]);
-    const int i0 = blockIdx.z * ci, i1 = min(i0 + ci, gm.ng[0]);
+    const int i0 = bz * ci, i1 = min(i0 + ci, gm.ng[0]);
     const double *src[4] = {pr.f[1], pr.f[2], pr.f[3], T};
     const long col = (long)jj * gm.st[1] + kk;
     double win[4][NL];
@@ -905,6 +1008,109 @@ __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(CFields5 pr, const 
                 if (tgrad) gr.f[11][n] = d[3];
             }
         }
+        __syncthreads();
+    }
+}
+template <int S>
+__global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
+{
+    march_body<S>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// ---- batched launches ----------------------------------------------------------------------------
+// One launch runs a kernel class over many independent cases.  Case i owns the blocks
+// [prefix[i], prefix[i+1]) of a flat work list, with its own argument record and grid shape; the
+// blocks of the launch walk the list with a grid stride, so the launch can be sized to the device
+// (persistent blocks) independently of the amount of work.
+struct BatchTab {
+    const void *args;     // per-case argument records (device)
+    const int *prefix;    // ncase + 1 block offsets (device)
+    const int *gdim;      // grid x, y, z per case (device)
+    int ncase, total;
+};
+
+__device__ __forceinline__ int batch_case(const int *prefix, const int ncase, const int b)
+{
+    int lo = 0, hi = ncase - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct GhostArgs { Fields10 f; int nf; Geom gm; };
+struct ValuesArgs { CFields5 pr; double *T, *mu; Geom gm; Consts k; };
+
+__global__ __launch_bounds__(256) void k_ghost_batch(BatchTab tb)
+{
+    const GhostArgs *A = (const GhostArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        ghost_body(A[i].f, A[i].nf, A[i].gm, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_values_batch(BatchTab tb)
+{
+    const ValuesArgs *A = (const ValuesArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        values_body(A[i].pr, A[i].T, A[i].mu, A[i].gm, A[i].k, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rk_batch(BatchTab tb)
+{
+    const UpdArgs *A = (const UpdArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        rk_body(A[i], (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grad2d_batch(BatchTab tb)
+{
+    const GradArgs *A = (const GradArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        grad_body(A[i].pr, A[i].T, A[i].gr, A[i].gm, A[i].k, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(GK * GJ) void k_march_batch(BatchTab tb)
+{
+    const GradArgs *A = (const GradArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+        march_body<S>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
+        __syncthreads();
+    }
+}
+
+template <int S, int TA, int TDI, int AX, bool MIX>
+__global__ __launch_bounds__(TA * TDI) void k_weno_batch(BatchTab tb)
+{
+    const WenoArgs *A = (const WenoArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+        weno_body<S, TA, TDI, AX, MIX>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
+        __syncthreads();
+    }
+}
+
+template <int S, int DIR, int TA, int TDI, bool DIAG>
+__global__ __launch_bounds__(TA * TDI) void k_visc_batch(BatchTab tb)
+{
+    const ViscArgs *A = (const ViscArgs *)tb.args;
+    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
+        const int i = batch_case(tb.prefix, tb.ncase, b);
+        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+        visc_body<S, DIR, TA, TDI, DIAG>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
         __syncthreads();
     }
 }
@@ -1159,7 +1365,8 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         return set_err(SHX_ERR_ARG, "space_order must be 5 or 9 (got %d)", d.space_order);
     if (d.time_order != 3 && d.time_order != 4)
         return set_err(SHX_ERR_ARG, "time_order must be 3 or 4 (got %d)", d.time_order);
-    if (d.precision != SHX_FP64) return set_err(SHX_ERR_UNSUPPORTED, "only the FP64 form is built");
+    if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
+        return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
         if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_OUTFLOW_RIEMANN)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
@@ -1497,8 +1704,52 @@ extern "C" int shx_upload_source(shx_case *c, const double *const src[5])
     return SHX_OK;
 }
 
-// one WENO sweep along axis dir of the state `pr`
-static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
+// A kernel choice for one launch class: the single-case kernel, its batched twin and the tile.
+struct Pick {
+    const void *single, *batch;
+    int TA, TDI;
+};
+
+template <int S, int TA, int TDI, int AX, bool MIX>
+static Pick weno_pick_t()
+{
+    return {(const void *)&k_weno<S, TA, TDI, AX, MIX>, (const void *)&k_weno_batch<S, TA, TDI, AX, MIX>, TA, TDI};
+}
+template <int S, int TA, int TDI, bool MIX>
+static Pick weno_pick_m(int ax)
+{
+    if (ax == 0) return weno_pick_t<S, TA, TDI, 0, MIX>();
+    if (ax == 1) return weno_pick_t<S, TA, TDI, 1, MIX>();
+    if (ax == 2) return weno_pick_t<S, TA, TDI, 2, MIX>();
+    return weno_pick_t<S, TA, TDI, -1, MIX>();
+}
+template <int S, int TA, int TDI>
+static Pick weno_pick_s(int ax, bool mix)
+{
+    return mix ? weno_pick_m<S, TA, TDI, true>(ax) : weno_pick_m<S, TA, TDI, false>(ax);
+}
+// coalesced sweeps (the contiguous axis: k in 3D, j in 2D) use 4 x 64 tiles, the others 16 x 16
+static Pick weno_pick(const shx_case *c, int dir)
+{
+    const bool diag = (c->diag && c->gm.dims == 3);
+    const bool mix = (c->d.precision == SHX_MIXED);
+    const bool coalesce = (c->gm.st[dir] == 1);
+    const int ax = diag ? dir : -1;
+    if (coalesce) return (c->d.space_order == 5) ? weno_pick_s<5, 4, 64>(ax, mix) : weno_pick_s<9, 4, 64>(ax, mix);
+    return (c->d.space_order == 5) ? weno_pick_s<5, 16, 16>(ax, mix) : weno_pick_s<9, 16, 16>(ax, mix);
+}
+
+// transverse axes of a sweep: A = the contiguous non-sweep axis with extent > 1, B = the other
+static void sweep_axes(const Geom &gm, int dir, int &A, int &B)
+{
+    if (gm.dims == 3) {
+        if (dir == 2) { A = 1; B = 0; } else { A = 2; B = (dir == 0) ? 1 : 0; }
+    } else {
+        A = (dir == 0) ? 1 : 0; B = 2;
+    }
+}
+
+static WenoArgs weno_args(const shx_case *c, double *const *pr, int dir, int first)
 {
     const Geom &gm = c->gm;
     WenoArgs a;
@@ -1508,13 +1759,8 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     if (diag) { a.m0 = c->met[4 * dir]; a.m1 = nullptr; a.m2 = nullptr; }
     else { a.m0 = c->met[3 * dir]; a.m1 = c->met[3 * dir + 1]; a.m2 = c->met[3 * dir + 2]; }
     a.J = c->met[9];
-    // transverse axes: fast = the contiguous non-sweep axis with extent > 1
     int A, B;
-    if (gm.dims == 3) {
-        if (dir == 2) { A = 1; B = 0; } else { A = 2; B = (dir == 0) ? 1 : 0; }
-    } else {
-        A = (dir == 0) ? 1 : 0; B = 2;
-    }
+    sweep_axes(gm, dir, A, B);
     a.sD = gm.st[dir]; a.sA = gm.st[A]; a.sB = gm.st[B];
     a.nD = gm.n[dir]; a.nA = gm.n[A]; a.nB = gm.n[B];
     a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
@@ -1522,34 +1768,33 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
     a.first = first;
     a.coalesce_d = (a.sD == 1);
     a.alignA = (a.sA == 1);
-    const int spanA = a.alignA ? a.oA + a.nA : a.nA;   // tiles along A cover this many padded indices
     a.k = c->k;
+    return a;
+}
+
+// tile grid of a sweep: tiles along A cover the padded span when A is contiguous (aligned rows)
+static dim3 sweep_grid(int nA, int oA, int alignA, int nB, int nD, const Pick &p)
+{
+    const int spanA = alignA ? oA + nA : nA;
+    return dim3(nblk(spanA, p.TA), nB, nblk(nD, p.TDI - 1));
+}
+
+static int launch_pick(const void *fn, dim3 grid, int threads, void *arg, hipStream_t st)
+{
+    void *args[] = {arg};
+    HIPCHK(hipLaunchKernel(fn, grid, dim3(threads), args, 0, st));
+    return SHX_OK;
+}
+
+// one WENO sweep along axis dir of the state `pr`
+static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
+{
+    WenoArgs a = weno_args(c, pr, dir, first);
+    const Pick p = weno_pick(c, dir);
     int rc = tbeg(c, KT_WENO0 + dir);
     if (rc) return rc;
-    if (a.coalesce_d) {
-        constexpr int TA = 4, TDI = 64;
-        dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
-        if (c->d.space_order == 5) {
-            if (diag) k_weno<5, TA, TDI, 2><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else k_weno<5, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
-        } else {
-            if (diag) k_weno<9, TA, TDI, 2><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else k_weno<9, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
-        }
-    } else {
-        constexpr int TA = 16, TDI = 16;
-        dim3 grid(nblk(spanA, TA), a.nB, nblk(a.nD, TDI - 1));
-        if (c->d.space_order == 5) {
-            if (diag && dir == 0) k_weno<5, TA, TDI, 0><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else if (diag) k_weno<5, TA, TDI, 1><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else k_weno<5, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
-        } else {
-            if (diag && dir == 0) k_weno<9, TA, TDI, 0><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else if (diag) k_weno<9, TA, TDI, 1><<<grid, TA * TDI, 0, c->stream>>>(a);
-            else k_weno<9, TA, TDI, -1><<<grid, TA * TDI, 0, c->stream>>>(a);
-        }
-    }
-    LAUNCH_CHECK();
+    if ((rc = launch_pick(p.single, sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), p.TA * p.TDI, &a, c->stream)))
+        return rc;
     return tend(c);
 }
 
@@ -1578,10 +1823,9 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
             constexpr int CI = 32;
             dim3 grid(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI));
             const int tgrad = c->diag ? 0 : 1;
-            if (c->d.space_order == 5)
-                k_visc_grad_march<5><<<grid, GK * GJ, 0, c->stream>>>(cp, c->T, gf, gm, c->k, CI, tgrad);
-            else
-                k_visc_grad_march<9><<<grid, GK * GJ, 0, c->stream>>>(cp, c->T, gf, gm, c->k, CI, tgrad);
+            const GradArgs ga = {cp, c->T, gf, gm, c->k, CI, tgrad};
+            if (c->d.space_order == 5) k_visc_grad_march<5><<<grid, GK * GJ, 0, c->stream>>>(ga);
+            else k_visc_grad_march<9><<<grid, GK * GJ, 0, c->stream>>>(ga);
         } else {
             k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
         }
@@ -1599,36 +1843,35 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
 }
 
 template <int S, int DIR, int TA, int TDI, bool DIAG>
-static void launch_visc_sweep_t(shx_case *c, const ViscArgs &a)
+static Pick visc_pick_t()
 {
-    dim3 grid(nblk(a.alignA ? a.oA + a.nA : a.nA, TA), a.nB, nblk(a.nD, TDI - 1));
-    k_visc_sweep<S, DIR, TA, TDI, DIAG><<<grid, TA * TDI, 0, c->stream>>>(a);
+    return {(const void *)&k_visc_sweep<S, DIR, TA, TDI, DIAG>, (const void *)&k_visc_batch<S, DIR, TA, TDI, DIAG>, TA, TDI};
 }
 
+// tiles sized so that the LDS window (13 or 23 values per node) plus the face buffer fit:
+// diagonal metric 16 x 16 (4 x 64 contiguous), generic 16 x 12 (4 x 48)
 template <int S>
-static void launch_visc_sweep_s(shx_case *c, const ViscArgs &a, int dir, bool diag)
+static Pick visc_pick_s(int dir, bool coalesce, bool diag)
 {
-    // tiles sized so that the LDS window (13 or 23 values per node) plus the face buffer fit in
-    // 64 KB: diagonal metric 16 x 16 (4 x 64 contiguous), generic 16 x 12 (4 x 48)
     constexpr int TDN = (S == 5) ? 16 : 12;
-    if (a.coalesce_d) {
-        if (dir == 1) launch_visc_sweep_t<S, 1, 4, 48, false>(c, a);       // 2D eta sweep
-        else if (diag) launch_visc_sweep_t<S, 2, 4, 64, true>(c, a);
-        else launch_visc_sweep_t<S, 2, 4, 48, false>(c, a);
-    } else {
-        if (diag) {
-            if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDN, true>(c, a);
-            else launch_visc_sweep_t<S, 1, 16, TDN, true>(c, a);
-        } else {
-            constexpr int TDG = (S == 5) ? 12 : 8;
-            if (dir == 0) launch_visc_sweep_t<S, 0, 16, TDG, false>(c, a);
-            else launch_visc_sweep_t<S, 1, 16, TDG, false>(c, a);
-        }
+    constexpr int TDG = (S == 5) ? 12 : 8;
+    if (coalesce) {
+        if (dir == 1) return visc_pick_t<S, 1, 4, 48, false>();       // 2D eta sweep
+        if (diag) return visc_pick_t<S, 2, 4, 64, true>();
+        return visc_pick_t<S, 2, 4, 48, false>();
     }
+    if (diag) return (dir == 0) ? visc_pick_t<S, 0, 16, TDN, true>() : visc_pick_t<S, 1, 16, TDN, true>();
+    return (dir == 0) ? visc_pick_t<S, 0, 16, TDG, false>() : visc_pick_t<S, 1, 16, TDG, false>();
 }
 
-// viscous face sweep along axis dir: Q += viscous flux divergence (after the WENO sweeps)
-static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
+static Pick visc_pick(const shx_case *c, int dir)
+{
+    const bool diag = c->diag && c->gm.dims == 3;
+    const bool coalesce = (c->gm.st[dir] == 1);
+    return (c->d.space_order == 5) ? visc_pick_s<5>(dir, coalesce, diag) : visc_pick_s<9>(dir, coalesce, diag);
+}
+
+static ViscArgs visc_args(const shx_case *c, double *const *pr, int dir, const RkArgs *rk)
 {
     const Geom &gm = c->gm;
     ViscArgs a;
@@ -1646,11 +1889,7 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
     for (int m = 0; m < 10; m++) a.met.f[m] = c->met[m];
     for (int m = 0; m < 5; m++) a.q.f[m] = c->q[m];
     int A, B;
-    if (gm.dims == 3) {
-        if (dir == 2) { A = 1; B = 0; } else { A = 2; B = (dir == 0) ? 1 : 0; }
-    } else {
-        A = (dir == 0) ? 1 : 0; B = 2;
-    }
+    sweep_axes(gm, dir, A, B);
     a.sD = gm.st[dir]; a.sA = gm.st[A]; a.sB = gm.st[B];
     a.nD = gm.n[dir]; a.nA = gm.n[A]; a.nB = gm.n[B];
     a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
@@ -1662,11 +1901,18 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
     a.k = c->k;
     a.fuse_rk = rk ? 1 : 0;
     if (rk) a.rk = *rk;
+    return a;
+}
+
+// viscous face sweep along axis dir: Q += viscous flux divergence (after the WENO sweeps)
+static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
+{
+    ViscArgs a = visc_args(c, pr, dir, rk);
+    const Pick p = visc_pick(c, dir);
     int rc = tbeg(c, KT_VISC0 + dir);
     if (rc) return rc;
-    if (c->d.space_order == 5) launch_visc_sweep_s<5>(c, a, dir, diag);
-    else launch_visc_sweep_s<9>(c, a, dir, diag);
-    LAUNCH_CHECK();
+    if ((rc = launch_pick(p.single, sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), p.TA * p.TDI, &a, c->stream)))
+        return rc;
     return tend(c);
 }
 
@@ -1733,6 +1979,414 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
         outb = (outb == bx) ? by : bx;
     }
     c->cur = in;   // U_lastStep <- U_RK (pointer swap instead of WriteValuesFromU1ToU2)
+    return SHX_OK;
+}
+
+// ---- batches of independent cases --------------------------------------------------------------
+// A batch advances many cases of uneven size together.  Cases that share the kernel templates
+// (dimension, scheme, metric kind, arithmetic form, viscosity, RK order) form a group; per stage
+// every kernel class of a group is one launch over the concatenated tile lists of its cases
+// (k_*_batch), and the launches of one full RK step of every group are captured once into a HIP
+// graph that is replayed per step.  The step keeps each case's U_lastStep buffer in place: the
+// last stage writes the new state over U_n pointwise (each node reads U_n before it is
+// overwritten), so the captured pointers stay valid from step to step.  Blocks of a launch walk
+// its work list with a grid stride; the number of resident blocks per class is chosen by
+// shx_batch_tune.
+struct BatchLaunch {
+    const void *fn;
+    int threads, total, grid, klass;
+    size_t args_off, prefix_off, gdim_off;
+    BatchTab tab;
+};
+
+enum { BK_GHOST, BK_VALUES, BK_GRAD, BK_WENO, BK_VISC, BK_RK, BK_NUM };
+static const char *g_bknames[BK_NUM] = {"ghost", "values", "grad", "weno", "visc", "rk"};
+
+struct shx_batch {
+    std::vector<shx_case *> cases;
+    std::vector<double> tau;
+    hipStream_t stream = nullptr;
+    std::vector<BatchLaunch> seq;
+    std::vector<char> host;      // argument tables, laid out as on the device
+    char *dev = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    int factor[BK_NUM];          // resident blocks per CU per class (0: one block per work item)
+    int ncu = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// stage buffers with the state kept in place: (in, out) per stage
+static void batch_buffers(const shx_case *c, int s, int &in, int &out)
+{
+    const int cur = c->cur, bx = (cur + 1) % 3, by = (cur + 2) % 3;
+    if (c->d.time_order == 4) {
+        static const int io[4][2] = {{0, 1}, {1, 2}, {2, 1}, {1, 0}};
+        const int map[3] = {cur, bx, by};
+        in = map[io[s][0]]; out = map[io[s][1]];
+    } else {
+        static const int io[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+        const int map[3] = {cur, bx, by};
+        in = map[io[s][0]]; out = map[io[s][1]];
+    }
+}
+
+// append one launch over `cs` with per-case argument records `recs` (rsize bytes each) and grids
+static void batch_add(shx_batch *b, const void *fn, int threads, int klass, const std::vector<char> &recs,
+                      size_t rsize, const std::vector<dim3> &grids)
+{
+    BatchLaunch L;
+    memset(&L, 0, sizeof L);
+    L.fn = fn; L.threads = threads; L.klass = klass;
+    const int nc = (int)grids.size();
+    std::vector<int> prefix(nc + 1, 0), gdim(3 * nc);
+    for (int i = 0; i < nc; i++) {
+        prefix[i + 1] = prefix[i] + (int)(grids[i].x * grids[i].y * grids[i].z);
+        gdim[3 * i] = (int)grids[i].x; gdim[3 * i + 1] = (int)grids[i].y; gdim[3 * i + 2] = (int)grids[i].z;
+    }
+    L.total = prefix[nc];
+    L.args_off = align256(b->host.size());
+    b->host.resize(L.args_off + recs.size());
+    memcpy(b->host.data() + L.args_off, recs.data(), recs.size());
+    L.prefix_off = align256(b->host.size());
+    b->host.resize(L.prefix_off + prefix.size() * sizeof(int));
+    memcpy(b->host.data() + L.prefix_off, prefix.data(), prefix.size() * sizeof(int));
+    L.gdim_off = align256(b->host.size());
+    b->host.resize(L.gdim_off + gdim.size() * sizeof(int));
+    memcpy(b->host.data() + L.gdim_off, gdim.data(), gdim.size() * sizeof(int));
+    L.tab.ncase = nc;
+    L.tab.total = L.total;
+    (void)rsize;
+    b->seq.push_back(L);
+}
+
+template <class T>
+static void put(std::vector<char> &recs, const T &r)
+{
+    const size_t o = recs.size();
+    recs.resize(o + sizeof(T));
+    memcpy(recs.data() + o, &r, sizeof(T));
+}
+
+// the launches of one RK step of one group (cases with identical kernel templates)
+static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std::vector<double> &tau)
+{
+    shx_case *c0 = g[0];
+    const int dims = c0->gm.dims, S = c0->d.space_order;
+    const int nc = (int)g.size();
+    for (int s = 0; s < c0->d.time_order; s++) {
+        std::vector<RkArgs> rk(nc);
+        std::vector<double *const *> in(nc);
+        for (int i = 0; i < nc; i++) {
+            int ib, ob;
+            batch_buffers(g[i], s, ib, ob);
+            in[i] = g[i]->prim[ib];
+            rk[i] = rk_args(g[i], g[i]->prim[ib], g[i]->prim[ob], s, tau[i]);
+        }
+        {   // periodic ghost fill of the stage input
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            for (int i = 0; i < nc; i++) {
+                GhostArgs a;
+                memset(&a, 0, sizeof a);
+                for (int m = 0; m < 5; m++) a.f.f[m] = in[i][m];
+                a.nf = 5; a.gm = g[i]->gm;
+                put(recs, a);
+                grids.push_back(dim3(nblk(g[i]->gm.ntot, 256)));
+            }
+            batch_add(b, (const void *)&k_ghost_batch, 256, BK_GHOST, recs, sizeof(GhostArgs), grids);
+        }
+        if (!c0->d.inviscid) {
+            std::vector<char> recs, grecs;
+            std::vector<dim3> grids, ggrids;
+            for (int i = 0; i < nc; i++) {
+                shx_case *c = g[i];
+                ValuesArgs a;
+                memset(&a, 0, sizeof a);
+                for (int m = 0; m < 5; m++) a.pr.f[m] = in[i][m];
+                a.T = c->T; a.mu = c->mu; a.gm = c->gm; a.k = c->k;
+                put(recs, a);
+                grids.push_back(dim3(nblk(c->gm.ntot, 256)));
+                GradArgs ga;
+                memset(&ga, 0, sizeof ga);
+                ga.pr = a.pr; ga.T = c->T;
+                for (int m = 0; m < 12; m++) ga.gr.f[m] = c->grad[m];
+                ga.gm = c->gm; ga.k = c->k; ga.ci = 32; ga.tgrad = c->diag ? 0 : 1;
+                put(grecs, ga);
+                if (dims == 3) ggrids.push_back(dim3(nblk(c->gm.ng[2], GK), nblk(c->gm.ng[1], GJ), nblk(c->gm.ng[0], 32)));
+                else ggrids.push_back(dim3(nblk(c->gm.ntot, 256)));
+            }
+            batch_add(b, (const void *)&k_values_batch, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
+            if (dims == 3)
+                batch_add(b, S == 5 ? (const void *)&k_march_batch<5> : (const void *)&k_march_batch<9>, GK * GJ,
+                          BK_GRAD, grecs, sizeof(GradArgs), ggrids);
+            else
+                batch_add(b, (const void *)&k_grad2d_batch, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
+        }
+        for (int dir = 0; dir < dims; dir++) {
+            const Pick p = weno_pick(c0, dir);
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            for (int i = 0; i < nc; i++) {
+                const WenoArgs a = weno_args(g[i], in[i], dir, dir == 0);
+                put(recs, a);
+                grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
+            }
+            batch_add(b, p.batch, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids);
+        }
+        if (!c0->d.inviscid) {
+            for (int dir = 0; dir < dims; dir++) {
+                const Pick p = visc_pick(c0, dir);
+                std::vector<char> recs;
+                std::vector<dim3> grids;
+                for (int i = 0; i < nc; i++) {
+                    const ViscArgs a = visc_args(g[i], in[i], dir, dir == dims - 1 ? &rk[i] : nullptr);
+                    put(recs, a);
+                    grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
+                }
+                batch_add(b, p.batch, p.TA * p.TDI, BK_VISC, recs, sizeof(ViscArgs), grids);
+            }
+        } else {
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            for (int i = 0; i < nc; i++) {
+                shx_case *c = g[i];
+                UpdArgs A;
+                memset(&A, 0, sizeof A);
+                for (int m = 0; m < 5; m++) A.q.f[m] = c->q[m];
+                A.J = c->met[9]; A.gm = c->gm; A.rk = rk[i];
+                put(recs, A);
+                grids.push_back(dim3(nblk((long)c->gm.n[0] * c->gm.n[1] * c->gm.n[2], 256)));
+            }
+            batch_add(b, (const void *)&k_rk_batch, 256, BK_RK, recs, sizeof(UpdArgs), grids);
+        }
+    }
+    return SHX_OK;
+}
+
+static bool same_group(const shx_case *a, const shx_case *c)
+{
+    return a->gm.dims == c->gm.dims && a->d.space_order == c->d.space_order && a->diag == c->diag &&
+           a->d.precision == c->d.precision && a->d.inviscid == c->d.inviscid && a->d.time_order == c->d.time_order;
+}
+
+// (re)build the argument tables of every launch for the current time steps
+static int batch_build(shx_batch *b)
+{
+    b->seq.clear();
+    b->host.clear();
+    std::vector<int> done(b->cases.size(), 0);
+    for (size_t i = 0; i < b->cases.size(); i++) {
+        if (done[i]) continue;
+        std::vector<shx_case *> g;
+        std::vector<double> t;
+        for (size_t j = i; j < b->cases.size(); j++)
+            if (!done[j] && same_group(b->cases[i], b->cases[j])) {
+                g.push_back(b->cases[j]);
+                t.push_back(b->tau[j]);
+                done[j] = 1;
+            }
+        int rc = batch_group(b, g, t);
+        if (rc) return rc;
+    }
+    return SHX_OK;
+}
+
+static int batch_grids(shx_batch *b)
+{
+    for (BatchLaunch &L : b->seq) {
+        const int f = b->factor[L.klass];
+        if (f <= 0) { L.grid = L.total; continue; }
+        int per = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, L.fn, L.threads, 0));
+        L.grid = std::min(L.total, std::max(1, per) * b->ncu * f);
+    }
+    return SHX_OK;
+}
+
+static int batch_capture(shx_batch *b)
+{
+    if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (BatchLaunch &L : b->seq) {
+        void *args[] = {&L.tab};
+        hipError_t e = hipLaunchKernel(L.fn, dim3(L.grid), dim3(L.threads), args, 0, b->stream);
+        if (e != hipSuccess) {
+            hipGraph_t g;
+            hipStreamEndCapture(b->stream, &g);
+            if (g) hipGraphDestroy(g);
+            return set_err(SHX_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return SHX_OK;
+}
+
+// upload the tables (same layout on every rebuild, so captured pointers stay valid)
+static int batch_upload(shx_batch *b, bool fresh)
+{
+    if (fresh) {
+        if (b->dev) hipFree(b->dev);
+        b->dev = nullptr;
+        HIPCHK(hipMalloc((void **)&b->dev, std::max<size_t>(b->host.size(), 256)));
+    }
+    HIPCHK(hipMemcpyAsync(b->dev, b->host.data(), b->host.size(), hipMemcpyHostToDevice, b->stream));
+    for (BatchLaunch &L : b->seq) {
+        L.tab.args = b->dev + L.args_off;
+        L.tab.prefix = (const int *)(b->dev + L.prefix_off);
+        L.tab.gdim = (const int *)(b->dev + L.gdim_off);
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double *tau, shx_batch **out)
+{
+    if (!cases || !tau || !out || ncases <= 0) return set_err(SHX_ERR_ARG, "null argument or empty batch");
+    *out = nullptr;
+    for (int i = 0; i < ncases; i++) {
+        shx_case *c = cases[i];
+        int rc = check_ready(c);
+        if (rc) return rc;
+        if (c->faces) return set_err(SHX_ERR_UNSUPPORTED, "case %d: batches take periodic blocks only", i);
+        if (!(tau[i] > 0.0)) return set_err(SHX_ERR_ARG, "case %d: tau must be positive", i);
+        for (int j = 0; j < i; j++)
+            if (cases[j] == c) return set_err(SHX_ERR_ARG, "case %d appears twice", i);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    shx_batch *b = new shx_batch();
+    b->cases.assign(cases, cases + ncases);
+    b->tau.assign(tau, tau + ncases);
+    for (int k = 0; k < BK_NUM; k++) b->factor[k] = 2;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) { shx_batch_destroy(b); return set_err(SHX_ERR_HIP, "batch setup: %s", hipGetErrorString(e)); }
+    int rc = batch_build(b);
+    if (!rc) rc = batch_upload(b, true);
+    if (!rc) rc = batch_grids(b);
+    if (!rc) rc = batch_capture(b);
+    if (rc) { shx_batch_destroy(b); return rc; }
+    *out = b;
+    return SHX_OK;
+}
+
+extern "C" int shx_batch_set_tau(shx_batch *b, const double *tau)
+{
+    if (!b || !tau) return set_err(SHX_ERR_ARG, "null argument");
+    for (size_t i = 0; i < b->cases.size(); i++)
+        if (!(tau[i] > 0.0)) return set_err(SHX_ERR_ARG, "case %zu: tau must be positive", i);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->tau.assign(tau, tau + b->cases.size());
+    std::vector<BatchLaunch> old = b->seq;
+    int rc = batch_build(b);
+    if (rc) return rc;
+    for (size_t i = 0; i < b->seq.size(); i++) b->seq[i].grid = old[i].grid;
+    return batch_upload(b, false);
+}
+
+extern "C" int shx_batch_run(shx_batch *b, int nsteps)
+{
+    if (!b) return set_err(SHX_ERR_ARG, "null batch");
+    if (nsteps < 0) return set_err(SHX_ERR_ARG, "negative step count");
+    // order the batch after earlier work on every case's stream, and later work after the batch
+    for (shx_case *c : b->cases) {
+        HIPCHK(hipEventRecord(b->done, c->stream));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->done, 0));
+    }
+    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->done, b->stream));
+    for (shx_case *c : b->cases) HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));
+    return SHX_OK;
+}
+
+extern "C" int shx_batch_synchronize(shx_batch *b)
+{
+    if (!b) return set_err(SHX_ERR_ARG, "null batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return SHX_OK;
+}
+
+// Autotuning: for each kernel class in turn, time whole replayed steps with each candidate number
+// of resident blocks per CU (0 = one block per work item) and keep the fastest.  The case states
+// are restored afterwards, so tuning leaves the solution unchanged.
+extern "C" int shx_batch_tune(shx_batch *b, int steps, int factors_out[6])
+{
+    if (!b) return set_err(SHX_ERR_ARG, "null batch");
+    if (steps <= 0) steps = 1;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // snapshot U_lastStep of every case
+    std::vector<double *> snap;
+    for (shx_case *c : b->cases) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+        for (int m = 0; m < 5; m++) {
+            double *p = nullptr;
+            HIPCHK(hipMalloc((void **)&p, bytes));
+            HIPCHK(hipMemcpyAsync(p, c->prim[c->cur][m], bytes, hipMemcpyDeviceToDevice, b->stream));
+            snap.push_back(p);
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    static const int cand[4] = {1, 2, 4, 0};
+    int rc = SHX_OK;
+    for (int k = 0; k < BK_NUM && !rc; k++) {
+        bool used = false;
+        for (const BatchLaunch &L : b->seq) used |= (L.klass == k);
+        if (!used) continue;
+        float best = 1e30f;
+        int bestf = b->factor[k];
+        for (int ci = 0; ci < 4 && !rc; ci++) {
+            b->factor[k] = cand[ci];
+            if ((rc = batch_grids(b)) || (rc = batch_capture(b))) break;
+            HIPCHK(hipGraphLaunch(b->exec, b->stream));      // warm-up
+            HIPCHK(hipEventRecord(e0, b->stream));
+            for (int s = 0; s < steps; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+            HIPCHK(hipEventRecord(e1, b->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bestf = cand[ci]; }
+        }
+        b->factor[k] = bestf;
+    }
+    // restore the states
+    size_t si = 0;
+    for (shx_case *c : b->cases) {
+        const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+        for (int m = 0; m < 5; m++, si++) {
+            HIPCHK(hipMemcpyAsync(c->prim[c->cur][m], snap[si], bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (double *p : snap) hipFree(p);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = batch_grids(b)) || (rc = batch_capture(b))) return rc;
+    if (factors_out)
+        for (int k = 0; k < BK_NUM; k++) factors_out[k] = b->factor[k];
+    return SHX_OK;
+}
+
+extern "C" int shx_batch_destroy(shx_batch *b)
+{
+    if (!b) return SHX_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->dev) hipFree(b->dev);
+    if (b->done) hipEventDestroy(b->done);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
     return SHX_OK;
 }
 

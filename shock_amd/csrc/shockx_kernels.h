@@ -298,6 +298,89 @@ __device__ __forceinline__ double phi_w5_fast(const double eps, const double d0,
     return 1. / 3. * w0 * q1 + 1. / 6. * (w2 - 0.5) * qq2;
 }
 
+// ---- FP32 forms for the mixed-precision sweeps ------------------------------------------------
+// Same formulas as phi_w5_fast / phi_w9 on locally offset FP32 inputs (node-to-node differences).  epsilon is dropped: FP32 cannot hold the
+// reference's 10 sqrt(DBL_MIN), and the clamp in scaled_sq_f gives the ideal weights where all
+// indicators vanish, which is what an epsilon far below every indicator does.
+template <int N>
+__device__ __forceinline__ void scaled_sq_f(float s[N], const int lo)
+{
+    float mx = s[0];
+#pragma unroll
+    for (int i = 1; i < N; i++) mx = fmaxf(mx, s[i]);
+    const int ex = __builtin_amdgcn_frexp_expf(mx);
+    const float floor_ = __builtin_amdgcn_ldexpf(1.0f, lo);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const float t = fmaxf(__builtin_amdgcn_ldexpf(s[i], -ex), floor_);
+        s[i] = t * t;
+    }
+}
+
+__device__ __forceinline__ float phi_w5_f(const float d0, const float d1, const float d2, const float d3)
+{
+    const float t0 = d0 - d1, t1 = d0 - 3.f * d1;
+    const float t2 = d1 - d2, t3 = d1 + d2;
+    const float t4 = d2 - d3, t5 = 3.f * d2 - d3;
+    float q[3] = {13.f * t0 * t0 + 3.f * t1 * t1, 13.f * t2 * t2 + 3.f * t3 * t3, 13.f * t4 * t4 + 3.f * t5 * t5};
+    scaled_sq_f<3>(q, -30);
+    const float P0 = q[1] * q[2], P2 = 3.f * (q[0] * q[1]);
+    const float r = __builtin_amdgcn_rcpf(P0 + 6.f * (q[0] * q[2]) + P2);
+    const float w0 = P0 * r, w2 = P2 * r;
+    const float q1 = d0 - 2.f * d1 + d2;
+    const float qq2 = d1 - 2.f * d2 + d3;
+    return (1.f / 3.f) * w0 * q1 + (1.f / 6.f) * (w2 - 0.5f) * qq2;
+}
+
+// The Balsara-Shu indicators rewritten as sums of squares of difference combinations (the
+// LDL^T factorisation of each quadratic form in the four first differences of its sub-stencil,
+// exact rationals): every term is non-negative, so FP32 keeps them to ~1e-6 relative where the
+// point-value form loses 1e-4 to cancellation on smooth data.
+__device__ __forceinline__ float phi_w9_f(const float *d)
+{
+    const float y00 = d[0] + (-163185.f / 45316.f) * d[1] + (100839.f / 22658.f) * d[2] + (-86329.f / 45316.f) * d[3];
+    const float y01 = d[1] + (-41089198.f / 14249315.f) * d[2] + (1813059.f / 838195.f) * d[3];
+    const float y02 = d[2] + (-350813921.f / 232768856.f) * d[3];
+    const float y03 = d[3];
+    const float is0 = 22658.f * y00 * y00 + (299235615.f / 90632.f) * y01 * y01 + (39105167808.f / 14249315.f) * y02 * y02 + (116566909305.f / 29096107.f) * y03 * y03;
+    const float y10 = d[1] + (-47055.f / 13816.f) * d[2] + (26079.f / 6908.f) * d[3] + (-18079.f / 13816.f) * d[4];
+    const float y11 = d[2] + (-12448558.f / 5880675.f) * d[3] + (5686883.f / 5880675.f) * d[4];
+    const float y12 = d[3] + (-146761231.f / 232768856.f) * d[4];
+    const float y13 = d[4];
+    const float is1 = 6908.f * y10 * y10 + (123494175.f / 27632.f) * y11 * y11 + (13035055936.f / 1960225.f) * y12 * y12 + (116566909305.f / 29096107.f) * y13 * y13;
+    const float y20 = d[2] + (-37185.f / 13816.f) * d[3] + (15369.f / 6908.f) * d[4] + (-8209.f / 13816.f) * d[5];
+    const float y21 = d[3] + (-1032654.f / 838195.f) * d[4] + (5686883.f / 14249315.f) * d[5];
+    const float y22 = d[4] + (-20636113.f / 51089708.f) * d[5];
+    const float y23 = d[5];
+    const float is2 = 6908.f * y20 * y20 + (299235615.f / 27632.f) * y21 * y21 + (8583070944.f / 838195.f) * y22 * y22 + (233133818610.f / 217131259.f) * y23 * y23;
+    const float y30 = d[3] + (-94935.f / 45316.f) * d[4] + (35109.f / 22658.f) * d[5] + (-18079.f / 45316.f) * d[6];
+    const float y31 = d[4] + (-106728718.f / 110863235.f) * d[5] + (30822003.f / 110863235.f) * d[6];
+    const float y32 = d[5] + (-1745642471.f / 4217089556.f) * d[6];
+    const float y33 = d[6];
+    const float is3 = 22658.f * y30 * y30 + (2328127935.f / 90632.f) * y31 * y31 + (101210149344.f / 15837605.f) * y32 * y32 + (233133818610.f / 1054272389.f) * y33 * y33;
+    const float y40 = d[4] + (-433665.f / 215836.f) * d[5] + (162579.f / 107918.f) * d[6] + (-86329.f / 215836.f) * d[7];
+    const float y41 = d[5] + (-945305518.f / 890329635.f) * d[6] + (290071763.f / 890329635.f) * d[7];
+    const float y42 = d[6] + (-8384962321.f / 17843974736.f) * d[7];
+    const float y43 = d[7];
+    const float is4 = 107918.f * y40 * y40 + (18696922335.f / 431672.f) * y41 * y41 + (999262585216.f / 296776545.f) * y42 * y42 + (16652415615.f / 318642406.f) * y43 * y43;
+    // weights d_k / q_k with the indicators scaled by the largest and clamped at 2^-60 of it, so
+    // every q_k >= 2^-120 and the sum stays finite (products of four q, as in the FP64 form,
+    // would need a clamp near 2^-15 in FP32, which distorts weights that still matter)
+    float q[5] = {is0, is1, is2, is3, is4};
+    scaled_sq_f<5>(q, -60);
+    const float A0 = (1.f / 126.f) * __builtin_amdgcn_rcpf(q[0]), A1 = (10.f / 63.f) * __builtin_amdgcn_rcpf(q[1]);
+    const float A2 = (10.f / 21.f) * __builtin_amdgcn_rcpf(q[2]), A3 = (20.f / 63.f) * __builtin_amdgcn_rcpf(q[3]);
+    const float A4 = (5.f / 126.f) * __builtin_amdgcn_rcpf(q[4]);
+    const float ia = __builtin_amdgcn_rcpf(A0 + A1 + A2 + A3 + A4);
+    const float w0 = A0 * ia, w2 = A2 * ia, w3 = A3 * ia, w4 = A4 * ia;
+    const float q1 = d[0] - 4.f * d[1] + 6.f * d[2] - 4.f * d[3] + d[4];
+    const float q2 = d[1] - 4.f * d[2] + 6.f * d[3] - 4.f * d[4] + d[5];
+    const float q3 = d[2] - 4.f * d[3] + 6.f * d[4] - 4.f * d[5] + d[6];
+    const float q4 = d[3] - 4.f * d[4] + 6.f * d[5] - 4.f * d[6] + d[7];
+    return (1.f / 5.f) * w0 * q1 + (1.f / 20.f) * w2 * q2 + (1.f / 60.f) * w3 * (3.f * q2 - 2.f * q3)
+         + (1.f / 60.f) * w4 * (3.f * q2 - 2.f * q3 + 3.f * q4) - (1.f / 840.f) * (39.f * q2 - 14.f * q3 + 3.f * q4);
+}
+
 // Balsara-Shu WENO9 (Phi_Function_W9, src/WENO.c:11-116); f: 9 point values, d: 8 differences.
 // The smoothness indicators keep the reference's quadratic forms in the point values.
 __device__ __forceinline__ double phi_w9(const double eps, const double *f, const double *d)

@@ -18,7 +18,7 @@ QUIRK_ZETA_VISC_J = 1
 QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MANUFACTURED, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
-FP64 = 0
+FP64, MIXED = 0, 1
 NUM_KERNELS = 13
 
 
@@ -45,7 +45,10 @@ EXPORTS = [
     "shx_download_state", "shx_upload_source", "shx_calc_runge_kutta", "shx_run", "shx_rhs",
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
+    "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
+    "shx_batch_destroy",
 ]
+BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
 _lib = None
 
@@ -75,6 +78,12 @@ def lib():
         L.shx_set_timing.argtypes = [P, ctypes.c_int]
         L.shx_kernel_times.argtypes = [P, P, P, P]
         L.shx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.shx_batch_create.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(P)]
+        L.shx_batch_set_tau.argtypes = [P, P]
+        L.shx_batch_run.argtypes = [P, ctypes.c_int]
+        L.shx_batch_tune.argtypes = [P, ctypes.c_int, P]
+        L.shx_batch_synchronize.argtypes = [P]
+        L.shx_batch_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -220,3 +229,49 @@ class Case:
         cnt = (ctypes.c_int * NUM_KERNELS)()
         _check(lib().shx_kernel_times(self.h, names, ms, cnt))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(NUM_KERNELS)}
+
+
+class Batch:
+    """Many periodic cases advanced together (shx_batch): one launch per kernel class and group of
+    like cases per stage, replayed from a captured HIP graph.  The cases stay usable through their
+    own handles; close the batch before the cases."""
+
+    def __init__(self, cases, taus):
+        self.cases = list(cases)
+        hs = (ctypes.c_void_p * len(self.cases))(*[c.h.value for c in self.cases])
+        t = np.ascontiguousarray(taus, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().shx_batch_create(hs, len(self.cases), t.ctypes.data, ctypes.byref(h)))
+        self.h = h
+
+    def set_tau(self, taus):
+        t = np.ascontiguousarray(taus, dtype=np.float64)
+        _check(lib().shx_batch_set_tau(self.h, t.ctypes.data))
+
+    def run(self, nsteps):
+        _check(lib().shx_batch_run(self.h, nsteps))
+
+    def tune(self, steps=1):
+        f = (ctypes.c_int * 6)()
+        _check(lib().shx_batch_tune(self.h, steps, f))
+        return dict(zip(BATCH_CLASSES, list(f)))
+
+    def synchronize(self):
+        _check(lib().shx_batch_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().shx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

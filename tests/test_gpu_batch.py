@@ -1,0 +1,63 @@
+"""Batches: many independent cases advanced by shared launches from a captured graph must give
+each case exactly the state it reaches alone (same kernels, same arithmetic), and match the oracle
+to the single-case bar."""
+import numpy as np
+import pytest
+
+import orc
+from helpers import conserved, interior, rel_linf, setup_2d_mms, setup_3d_turb
+from shock_amd import cases, shockx
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(spec):
+    dims, S, n, prec, inviscid, seed = spec
+    if dims == 2:
+        o, (xe, ye, ze), met, prim, nn = setup_2d_mms(n=n, S=S, inviscid=inviscid)
+    else:
+        o, (xe, ye, ze), met, prim, nn = setup_3d_turb(n=n, S=S, inviscid=inviscid, seed=seed)
+    c = shockx.Case(dims=dims, space_order=S, time_order=4, n=nn, gamma=1.4, mach=o.c.mach,
+                    reynolds=o.c.reynolds, inviscid=inviscid, precision=prec)
+    c.build_metric(xe, ye, ze)
+    c.upload_state(prim)
+    return o, c, met, prim, nn
+
+
+SPECS = [(2, 5, 24, shockx.FP64, False, 1), (3, 5, 12, shockx.FP64, False, 2), (2, 9, 20, shockx.FP64, False, 3),
+         (3, 5, 16, shockx.FP64, False, 4), (3, 9, 12, shockx.FP64, False, 5), (3, 5, 14, shockx.MIXED, False, 6),
+         (2, 5, 28, shockx.FP64, True, 7), (3, 5, 12, shockx.FP64, False, 8)]
+
+
+def test_batch_equals_cases_alone_and_oracle():
+    made = [_make(s) for s in SPECS]
+    taus = [c.cfl_dt(0.4) for _, c, _, _, _ in made]
+    steps = 3
+    alone = []
+    for (o, c, met, prim, nn), tau in zip(made, taus):
+        c.run(tau, steps)
+        alone.append(c.download_state())
+        c.upload_state(prim)
+    with shockx.Batch([m[1] for m in made], taus) as b:
+        print("[batch] tuned factors", b.tune(1), flush=True)
+        b.run(steps)
+        b.synchronize()
+        for k, ((o, c, met, prim, nn), tau, ref) in enumerate(zip(made, taus, alone)):
+            got = c.download_state()
+            sl = interior(o, o.c.dims, nn)
+            diff = max(float(np.max(np.abs(a[sl] - r[sl]))) for a, r in zip(got, ref))
+            print(f"[batch] case {k} {SPECS[k]}: max |batch - alone| = {diff:.2e}", flush=True)
+            assert diff == 0.0
+            if SPECS[k][3] == shockx.FP64:
+                want = o.run(tau, met, prim, steps)
+                errs = rel_linf(conserved(got), conserved(want), sl)
+                if o.c.dims == 2:
+                    errs = errs[:3] + errs[4:]
+                print(f"[batch] case {k} vs oracle: " + " ".join(f"{e:.1e}" for e in errs), flush=True)
+                assert max(errs) <= 1e-12
+        # a second run continues from the batch's own state
+        b.set_tau([t * 0.5 for t in taus])
+        b.run(1)
+        b.synchronize()
+    for _, c, _, _, _ in made:
+        c.close()

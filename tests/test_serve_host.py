@@ -1,0 +1,58 @@
+"""Scheduler of the solve server, on the host with stand-in cases: admission by footprint and
+slots, every request takes exactly its own number of steps, retirement frees memory."""
+from shock_amd import serve
+
+
+class FakeCase:
+    def __init__(self, r, log):
+        self.r, self.log, self.steps = r, log, 0
+
+    def synchronize(self):
+        pass
+
+    def download_state(self):
+        return [self.steps]
+
+    def close(self):
+        self.log.append(("close", self.r.rid))
+
+
+class FakeBatch:
+    resident_peak = 0
+
+    def __init__(self, cs, taus):
+        self.cs = cs
+        FakeBatch.resident_peak = max(FakeBatch.resident_peak, len(cs))
+
+    def run(self, k):
+        for c in self.cs:
+            c.steps += k
+
+    def synchronize(self):
+        pass
+
+    def close(self):
+        pass
+
+
+def test_every_request_runs_its_steps_within_budget():
+    reqs = serve.make_requests(12, rate=200.0, seed=3, n_lo=16, n_hi=48)
+    log = []
+
+    def open_fn(r):
+        r.steps = 5 + r.rid * 3
+        r.tau = r.t_end / r.steps
+        return FakeCase(r, log)
+
+    fp = lambda n, S: n ** 3                                    # noqa: E731
+    budget = 3 * 48 ** 3
+    srv = serve.Server(budget_bytes=budget, max_slots=4, chunk=4, keep_results=True, open_fn=open_fn,
+                       batch_cls=FakeBatch, footprint_fn=fp)
+    stats, done = srv.run(reqs)
+    assert stats["requests"] == 12
+    for r in done:
+        assert r.result == [r.steps]                           # exactly its own step count
+        assert r.done >= r.admitted >= r.arrival
+    assert FakeBatch.resident_peak <= 4
+    assert sorted(rid for _, rid in log) == list(range(12))
+    assert stats["p99_s"] >= stats["p50_s"] >= 0.0
