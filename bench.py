@@ -97,9 +97,16 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
-    ap.add_argument("--workload", choices=("turbulence", "mms-ladder"), default="turbulence",
+    ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
+                    default="turbulence",
                     help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
-                         "manufactured-solution convergence ladder in both arithmetic forms")
+                         "manufactured-solution convergence ladder in both arithmetic forms; batch: 96 "
+                         "uneven cases per graph-replayed step; serve: Poisson solve requests; shock: "
+                         "the 384^3 WENO9 Mach-3 shock / vorticity block with inflow and outflow")
+    ap.add_argument("--cases", type=int, default=96, help="batch: number of cases; serve: requests")
+    ap.add_argument("--rate", type=float, default=0.5, help="serve: arrival rate (requests per second)")
+    ap.add_argument("--n-lo", type=int, default=64, help="serve: smallest request (nodes per direction)")
+    ap.add_argument("--n-hi", type=int, default=256, help="serve: largest request")
     ap.add_argument("--ladder", default="32,64,128", help="mms-ladder: rungs (nodes per direction)")
     ap.add_argument("--scheme", type=int, choices=(5, 9), default=9, help="mms-ladder: WENO order")
     ap.add_argument("--t-end", type=float, default=2.0, help="mms-ladder: marching time per rung")
@@ -117,6 +124,12 @@ def main():
 
     if args.workload == "mms-ladder":
         return mms_ladder(args)
+    if args.workload == "batch":
+        return batch_workload(args)
+    if args.workload == "serve":
+        return serve_workload(args)
+    if args.workload == "shock":
+        return shock_workload(args)
 
     n = args.n
     S, time_order = 5, 4
@@ -207,6 +220,152 @@ def main():
     }
     print(json.dumps(out), flush=True)
     case.close()
+
+
+def batch_cases(count, seed, precision):
+    """Batch workload: sizes log-uniform in node count from 32^2 (2D) to 160^3 (3D) -- counts up to
+    256^2 become 2D squares, larger ones 3D cubes; Mach U(0.2, 3), Reynolds logU(1e2, 1e4), WENO5 or
+    WENO9; 2D cases start from the manufactured field of the 2D parity case, 3D cases from the
+    random-phase turbulence on the curvilinear box."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        N = math.exp(rng.uniform(math.log(32 ** 2), math.log(160 ** 3)))
+        S = int(rng.choice([5, 9]))
+        mach = float(rng.uniform(0.2, 3.0))
+        re = float(math.exp(rng.uniform(math.log(1e2), math.log(1e4))))
+        if N <= 256 ** 2:
+            n = max(S + 1, int(round(math.sqrt(N))))
+            nn = (n, n, 1)
+            c = shockx.Case(dims=2, space_order=S, time_order=4, n=nn, mach=mach, reynolds=re, precision=precision)
+            xe, ye, ze = cases.periodic_ext_coords(2, S, nn, cases.cartesian_unit(2))
+            c.build_metric(xe, ye, ze)
+            x, y, z = cases.interior_coords(2, S, nn, xe, ye, ze)
+            c.upload_state(cases.mms_periodic_2d_state(np.ascontiguousarray(x), np.ascontiguousarray(y), mach=mach))
+            cells = n * n
+        else:
+            n = int(round(N ** (1.0 / 3.0)))
+            nn = (n, n, n)
+            c = shockx.Case(dims=3, space_order=S, time_order=4, n=nn, mach=mach, reynolds=re, precision=precision)
+            c.build_metric(*cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05)))
+            c.upload_state(cases.turbulence_3d_state(3, S, nn, int(rng.integers(1, 2 ** 31 - 1)), mach=mach))
+            cells = n ** 3
+        out.append((c, cells, c.cfl_dt(0.5), (2 if N <= 256 ** 2 else 3, n, S)))
+    return out
+
+
+def batch_workload(args):
+    """All cases advanced together from one graph replay per step, against the same cases stepped
+    one after another through the single-case path."""
+    prec = shockx.MIXED if args.precision == "mixed" else shockx.FP64
+    t0 = time.perf_counter()
+    cs = batch_cases(args.cases, args.seed, prec)
+    cells = sum(x[1] for x in cs)
+    print(f"[batch] {len(cs)} cases, {cells:.3e} nodes, setup {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    taus = [x[2] for x in cs]
+    # sequential single-case path
+    for c, _, tau, _ in cs:
+        c.run(tau, 1)
+    for c, _, _, _ in cs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for c, _, tau, _ in cs:
+        c.run(tau, args.steps)
+    for c, _, _, _ in cs:
+        c.synchronize()
+    seq = time.perf_counter() - t0
+    b = shockx.Batch([x[0] for x in cs], taus)
+    t0 = time.perf_counter()
+    factors = b.tune(1)
+    tune_s = time.perf_counter() - t0
+    b.run(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    b.close()
+    value = cells * 4 * args.steps / wall
+    mix = {"2d_w5": 0, "2d_w9": 0, "3d_w5": 0, "3d_w9": 0}
+    for _, _, _, (d, n, S) in cs:
+        mix[f"{d}d_w{S}"] += 1
+    out = {"metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if prec == shockx.FP64 else "f64+f32", "data": "synthetic (seeded cases)",
+           "config": {"workload": f"batch of {len(cs)} independent cases (32^2 2D to 160^3 3D, W5/W9, "
+                                  f"Mach 0.2-3, Re 1e2-1e4), one graph replay per RK step",
+                      "nodes": cells, "mix": mix, "seed": args.seed},
+           "sequential_value": round(cells * 4 * args.steps / seq, 1),
+           "batch_speedup": round(seq / wall, 3), "tuned_resident_blocks_per_cu": factors,
+           "tune_s": round(tune_s, 2)}
+    for c, _, _, _ in cs:
+        c.close()
+    print(json.dumps(out), flush=True)
+
+
+def serve_workload(args):
+    """Poisson solve requests through the continuous-batching server."""
+    from shock_amd import serve
+    prec = shockx.MIXED if args.precision == "mixed" else shockx.FP64
+    reqs = serve.make_requests(args.cases, args.rate, args.seed, n_lo=args.n_lo, n_hi=args.n_hi, precision=prec)
+    srv = serve.Server()
+    stats, done = srv.run(reqs)
+    out = {"metric": "time-to-solution p50/p99 and aggregate cell-updates/s/RK-stage",
+           "value": round(stats["cell_updates_per_s"], 1), "unit": "cell-updates/s/RK-stage", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64" if prec == shockx.FP64 else "f64+f32",
+           "data": "synthetic (seeded requests)",
+           "config": {"workload": f"{args.cases} Poisson requests at {args.rate}/s, 3D periodic turbulence "
+                                  f"{args.n_lo}^3-{args.n_hi}^3 (log-uniform), t_end U(0.5, 2), W5/W9",
+                      "seed": args.seed},
+           "p50_s": round(stats["p50_s"], 3), "p99_s": round(stats["p99_s"], 3), "mean_s": round(stats["mean_s"], 3),
+           "queue_wait_p50_s": round(stats["queue_wait_p50_s"], 3), "wall_s": round(stats["wall_s"], 2),
+           "requests": stats["requests"],
+           "sizes": sorted(r.n for r in done)}
+    print(json.dumps(out), flush=True)
+
+
+def shock_workload(args):
+    """Mach-3 normal shock meeting a vorticity field, 384^3 WENO9 + viscous (Re 5000), supersonic
+    inflow and Riemann outflow in x, periodic in y and z, perturbed curvilinear grid."""
+    prec = shockx.MIXED if args.precision == "mixed" else shockx.FP64
+    n = 384 if args.n == 256 else args.n
+    nn = (n, n, n)
+    S = 9
+    t0 = time.perf_counter()
+    coords, prim, inflow = cases.shock_3d(S, nn, args.seed)
+    bc = (shockx.BC_INFLOW_SUPERSONIC, shockx.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    case = shockx.Case(dims=3, space_order=S, time_order=4, n=nn, gamma=1.4, mach=3.0, reynolds=5000.0,
+                       quirks=shockx.QUIRKS_REFERENCE, bc=bc, inflow=inflow, precision=prec)
+    case.build_metric(*coords)
+    case.upload_state(prim)
+    del coords, prim
+    tau = case.cfl_dt(0.5)
+    print(f"[bench] shock setup {time.perf_counter() - t0:.1f}s tau={tau:.4g} n={n}", file=sys.stderr, flush=True)
+    case.run(tau, args.warmup)
+    case.synchronize()
+    if not args.no_timing:
+        case.set_timing(True)
+    t0 = time.perf_counter()
+    case.run(tau, args.steps)
+    case.synchronize()
+    wall = time.perf_counter() - t0
+    kt = case.kernel_times() if not args.no_timing else {}
+    nan = case.check_nan()
+    case.close()
+    value = n ** 3 * 4 * args.steps / wall
+    kernels = {k: {"avg_ms": round(v[0] / v[1], 4), "launches": v[1]} for k, v in kt.items() if v[1]}
+    out = {"metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if prec == shockx.FP64 else "f64+f32", "data": "synthetic (seeded shock/vorticity)",
+           "config": {"workload": f"{n}^3 Mach-3 normal shock + vorticity (rms 0.1 u1), WENO9 + viscous, Re=5000, "
+                                  f"inflow/Riemann outflow in x, periodic y/z, perturbed grid, RK4",
+                      "precision": args.precision, "seed": args.seed, "tau": tau},
+           "non_finite_nodes": nan, "kernels": kernels}
+    print(json.dumps(out), flush=True)
 
 
 def mms_ladder(args):

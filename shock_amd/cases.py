@@ -156,3 +156,70 @@ def turbulence_3d_state(dims, S, n, seed, gamma=1.4, mach=0.5, mt=0.5):
     p = 1.0
     prim[4][...] = energy(prim[1], prim[2], prim[3], p, prim[0], gamma, mach)
     return prim
+
+
+def perturbed_box(seed, amp_h, n, periodic=(False, True, True), modes=3):
+    """Mapping of the unit cube with a seeded smooth perturbation of each coordinate, at most
+    amp_h grid spacings: x_d = xi_d + amp_h h_d sum_m a_m sin(2 pi k_m . xi + phi_m) with the
+    amplitudes normalised so the sum stays within [-1, 1].  Wave numbers are integers along
+    periodic directions, so the perturbation is periodic there."""
+    rng = np.random.default_rng(seed)
+    tp = 2.0 * np.pi
+    terms = []
+    for d in range(3):
+        h = 1.0 / (n[d] if periodic[d] else (n[d] - 1))
+        ks = rng.integers(1, 3, size=(modes, 3)).astype(np.float64)
+        ph = rng.uniform(0.0, tp, size=(modes, 3))
+        a = rng.uniform(0.5, 1.0, size=modes)
+        a = a / a.sum()
+        terms.append((amp_h * h, ks, ph, a))
+
+    def mapping(xi, eta, zeta):
+        out = []
+        for d, base in enumerate((xi, eta, zeta)):
+            amp, ks, ph, a = terms[d]
+            pert = np.zeros_like(base, dtype=np.float64)
+            for m in range(len(a)):
+                pert = pert + a[m] * np.sin(tp * ks[m, 0] * xi + ph[m, 0]) * np.sin(tp * ks[m, 1] * eta + ph[m, 1]) \
+                    * np.sin(tp * ks[m, 2] * zeta + ph[m, 2])
+            out.append(base + amp * pert)
+        return tuple(out)
+    return mapping
+
+
+def normal_shock_states(mach1, gamma=1.4):
+    """Rankine-Hugoniot jump of a normal shock with upstream Mach mach1:
+    (rho2/rho1, u2/u1, p2/p1)."""
+    m2 = mach1 * mach1
+    r = (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    p = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0)
+    return r, 1.0 / r, p
+
+
+def shock_3d(S, n, seed, mach=3.0, gamma=1.4, x_shock=0.3, vort_rms=0.1, amp_h=0.05):
+    """Mach-`mach` planar normal shock at x = x_shock meeting a seeded random-phase vorticity field
+    (rms velocity vort_rms of the upstream speed) on a perturbed curvilinear box, inflow at x = 0,
+    outflow at x = 1, periodic in y and z.  SHOCK units with the reference Mach equal to the
+    upstream Mach: upstream rho = p = u = 1, c = 1/mach.  Returns the extended coordinates, the
+    primitive state (padded) and the inflow tuple (rho, u, v, w, p)."""
+    xe, ye, ze = ext_coords(3, S, n, perturbed_box(seed, amp_h, n), periodic=(False, True, True))
+    ng, g = padded_shape(3, S, n)
+    x = xe[1:-1, 1:-1, 1:-1]
+    rr, ur, pr_ = normal_shock_states(mach, gamma)
+    post = x > x_shock
+    rho = np.where(post, rr, 1.0)
+    u = np.where(post, ur, 1.0)
+    p = np.where(post, pr_, 1.0)
+    # vortical fluctuations on the index grid (a smooth periodic field), upstream speed 1
+    fu, fv, fw = random_phase_velocity(n[0], seed + 1, mt=vort_rms, c=1.0)
+    v = np.zeros(ng)
+    w = np.zeros(ng)
+    sl = (slice(g, g + n[0]), slice(g, g + n[1]), slice(g, g + n[2]))
+    uf = np.zeros(ng)
+    uf[sl] = fu
+    v[sl] = fv
+    w[sl] = fw
+    u = u + uf
+    e = energy(u, v, w, p, rho, gamma, mach)
+    prim = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, v, w, e)]
+    return (xe, ye, ze), prim, (1.0, 1.0, 0.0, 0.0, 1.0)

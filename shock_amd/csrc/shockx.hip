@@ -1,19 +1,23 @@
 // shockx.hip -- libshockx.so: gfx950 kernels and the C ABI of the SHOCK RHS / RK-stage path.
 //
-// Stage pipeline (one RK stage, FP64 form; src/Functions.c:2107-2365):
-//   k_ghost_periodic   on-device ghost fill of rho,u,v,w,e (replaces the MPI self-interface
-//                      halo, src/Functions.c:3580-3684)
+// Stage pipeline (one RK stage; src/Functions.c:2107-2365):
+//   k_ghost_periodic   on-device ghost fill of rho,u,v,w,e along periodic directions (replaces
+//                      the MPI self-interface halo, src/Functions.c:3580-3684)
 //   k_calc_values      T and mu on every padded node (CalcValues, src/Functions.c:2392-2481;
 //                      p, c and theta are recomputed inside the flux kernels instead of stored)
-//   k_weno<S>          one launch per direction: LDS-staged pencil, characteristic WENO
+//   k_face_bc          boundary writers of non-periodic faces, in face order (src/BC.c:13-74)
+//   k_visc_grad_march  nodal derivatives of u,v,w,T (src/ZD.c:697-810); k_visc_grad in 2D
+//   k_weno             one launch per direction: LDS-staged pencil, characteristic WENO
 //                      interface fluxes and their divergence fused, interface fluxes never
 //                      reach HBM (CalcFluxesIn*Direction, src/WENO.c:206-1083)
-//   k_visc_grad        nodal derivatives of u,v,w,T (src/ZD.c:697-810)
-//   k_visc_rk          viscous fluxes of all directions + source + RK update
-//                      (src/ZD.c:47-694, src/Functions.c:2287-2365), writes the next stage
-//                      state into a second buffer (the reference updates in place after all
-//                      fluxes are done, which double buffering reproduces exactly)
-//   k_rk               source + RK update for inviscid runs
+//   k_visc_sweep       one launch per direction: viscous face fluxes and their divergence
+//                      (src/ZD.c:47-694); the last one also applies the RK stage update and
+//                      the source (src/Functions.c:2287-2365) and writes the next stage state
+//                      into another buffer (the reference updates in place after all fluxes of
+//                      the stage are assembled, which the separate buffer reproduces exactly)
+//   k_rk_update        source + RK update for inviscid runs
+// Every hot kernel body is also reachable through a k_*_batch twin that runs many cases from
+// one launch over a tile work list (shx_batch_*).
 #include "shockx_kernels.h"
 #include "../../include/shockx.h"
 
