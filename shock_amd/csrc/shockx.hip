@@ -25,6 +25,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <type_traits>
 #include <algorithm>
@@ -1026,97 +1027,90 @@ __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
 // [prefix[i], prefix[i+1]) of a flat work list, with its own argument record and grid shape; the
 // blocks of the launch walk the list with a grid stride, so the launch can be sized to the device
 // (persistent blocks) independently of the amount of work.
-struct BatchTab {
-    const void *args;     // per-case argument records (device)
-    const int *prefix;    // ncase + 1 block offsets (device)
-    const int *gdim;      // grid x, y, z per case (device)
-    int ncase, total;
-};
+// The per-case records, the case of every block (cob), the first block of every case (prefix)
+// and the case grids (gdim) are read-only for the launch: __restrict__ lets the compiler read
+// the records with scalar loads, as it does kernel arguments.
+#define BATCH_PARAMS(T) const T *__restrict__ A, const int *__restrict__ cob, const int *__restrict__ prefix, \
+                        const int *__restrict__ gdim, const int total
 
-__device__ __forceinline__ int batch_case(const int *prefix, const int ncase, const int b)
+// P = false: one block per work item (the kernel then needs exactly the registers of its
+// single-case twin); P = true: persistent blocks walking the list with a grid stride.
+template <bool P, class F>
+__device__ __forceinline__ void batch_walk(const int total, F &&f)
 {
-    int lo = 0, hi = ncase - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    if constexpr (!P) {
+        if ((int)blockIdx.x < total) f((int)blockIdx.x);
+    } else {
+        for (int b = blockIdx.x; b < total; b += gridDim.x) {
+            f(b);
+            __syncthreads();
+        }
     }
-    return lo;
 }
 
 struct GhostArgs { Fields10 f; int nf; Geom gm; };
 struct ValuesArgs { CFields5 pr; double *T, *mu; Geom gm; Consts k; };
-
-__global__ __launch_bounds__(256) void k_ghost_batch(BatchTab tb)
+template <bool P>
+__global__ __launch_bounds__(256) void k_ghost_batch(BATCH_PARAMS(GhostArgs))
 {
-    const GhostArgs *A = (const GhostArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        ghost_body(A[i].f, A[i].nf, A[i].gm, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
-    }
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        ghost_body(A[i].f, A[i].nf, A[i].gm, (long)(b - prefix[i]) * 256 + threadIdx.x);
+    });
+}
+template <bool P>
+__global__ __launch_bounds__(256) void k_values_batch(BATCH_PARAMS(ValuesArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        values_body(A[i].pr, A[i].T, A[i].mu, A[i].gm, A[i].k, (long)(b - prefix[i]) * 256 + threadIdx.x);
+    });
+}
+template <bool P>
+__global__ __launch_bounds__(256) void k_rk_batch(BATCH_PARAMS(UpdArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        rk_body(A[i], (long)(b - prefix[i]) * 256 + threadIdx.x);
+    });
+}
+template <bool P>
+__global__ __launch_bounds__(256) void k_grad2d_batch(BATCH_PARAMS(GradArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        grad_body(A[i].pr, A[i].T, A[i].gr, A[i].gm, A[i].k, (long)(b - prefix[i]) * 256 + threadIdx.x);
+    });
 }
 
-__global__ __launch_bounds__(256) void k_values_batch(BatchTab tb)
+template <int S, bool P>
+__global__ __launch_bounds__(GK * GJ) void k_march_batch(BATCH_PARAMS(GradArgs))
 {
-    const ValuesArgs *A = (const ValuesArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        values_body(A[i].pr, A[i].T, A[i].mu, A[i].gm, A[i].k, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_rk_batch(BatchTab tb)
-{
-    const UpdArgs *A = (const UpdArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        rk_body(A[i], (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_grad2d_batch(BatchTab tb)
-{
-    const GradArgs *A = (const GradArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        grad_body(A[i].pr, A[i].T, A[i].gr, A[i].gm, A[i].k, (long)(b - tb.prefix[i]) * 256 + threadIdx.x);
-    }
-}
-
-template <int S>
-__global__ __launch_bounds__(GK * GJ) void k_march_batch(BatchTab tb)
-{
-    const GradArgs *A = (const GradArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
         march_body<S>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
-        __syncthreads();
-    }
+    });
 }
 
-template <int S, int TA, int TDI, int AX, bool MIX>
-__global__ __launch_bounds__(TA * TDI) void k_weno_batch(BatchTab tb)
+template <int S, int TA, int TDI, int AX, bool MIX, bool P>
+__global__ __launch_bounds__(TA * TDI) void k_weno_batch(BATCH_PARAMS(WenoArgs))
 {
-    const WenoArgs *A = (const WenoArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
         weno_body<S, TA, TDI, AX, MIX>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
-        __syncthreads();
-    }
+    });
 }
 
-template <int S, int DIR, int TA, int TDI, bool DIAG>
-__global__ __launch_bounds__(TA * TDI) void k_visc_batch(BatchTab tb)
+template <int S, int DIR, int TA, int TDI, bool DIAG, bool P>
+__global__ __launch_bounds__(TA * TDI) void k_visc_batch(BATCH_PARAMS(ViscArgs))
 {
-    const ViscArgs *A = (const ViscArgs *)tb.args;
-    for (int b = blockIdx.x; b < tb.total; b += gridDim.x) {
-        const int i = batch_case(tb.prefix, tb.ncase, b);
-        const int l = b - tb.prefix[i], gx = tb.gdim[3 * i], gy = tb.gdim[3 * i + 1];
+    batch_walk<P>(total, [&](const int b) {
+        const int i = cob[b];
+        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
         visc_body<S, DIR, TA, TDI, DIAG>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
-        __syncthreads();
-    }
+    });
 }
 
 // Q += J * src (manufactured-solution source) for the stage residual returned by shx_rhs.
@@ -1710,14 +1704,15 @@ extern "C" int shx_upload_source(shx_case *c, const double *const src[5])
 
 // A kernel choice for one launch class: the single-case kernel, its batched twin and the tile.
 struct Pick {
-    const void *single, *batch;
+    const void *single, *batch, *batchp;   // single case, batched flat, batched persistent
     int TA, TDI;
 };
 
 template <int S, int TA, int TDI, int AX, bool MIX>
 static Pick weno_pick_t()
 {
-    return {(const void *)&k_weno<S, TA, TDI, AX, MIX>, (const void *)&k_weno_batch<S, TA, TDI, AX, MIX>, TA, TDI};
+    return {(const void *)&k_weno<S, TA, TDI, AX, MIX>, (const void *)&k_weno_batch<S, TA, TDI, AX, MIX, false>,
+            (const void *)&k_weno_batch<S, TA, TDI, AX, MIX, true>, TA, TDI};
 }
 template <int S, int TA, int TDI, bool MIX>
 static Pick weno_pick_m(int ax)
@@ -1849,7 +1844,8 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
 template <int S, int DIR, int TA, int TDI, bool DIAG>
 static Pick visc_pick_t()
 {
-    return {(const void *)&k_visc_sweep<S, DIR, TA, TDI, DIAG>, (const void *)&k_visc_batch<S, DIR, TA, TDI, DIAG>, TA, TDI};
+    return {(const void *)&k_visc_sweep<S, DIR, TA, TDI, DIAG>, (const void *)&k_visc_batch<S, DIR, TA, TDI, DIAG, false>,
+            (const void *)&k_visc_batch<S, DIR, TA, TDI, DIAG, true>, TA, TDI};
 }
 
 // tiles sized so that the LDS window (13 or 23 values per node) plus the face buffer fit:
@@ -1997,10 +1993,12 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
 // its work list with a grid stride; the number of resident blocks per class is chosen by
 // shx_batch_tune.
 struct BatchLaunch {
-    const void *fn;
+    const void *fn, *fnp;          // flat and persistent kernels; launched: fn when grid == total
+    const void *run;
     int threads, total, grid, klass;
-    size_t args_off, prefix_off, gdim_off;
-    BatchTab tab;
+    size_t args_off, cob_off, prefix_off, gdim_off;
+    const void *A;
+    const int *cob, *prefix, *gdim;
 };
 
 enum { BK_GHOST, BK_VALUES, BK_GRAD, BK_WENO, BK_VISC, BK_RK, BK_NUM };
@@ -2018,6 +2016,7 @@ struct shx_batch {
     hipEvent_t done = nullptr;
     int factor[BK_NUM];          // resident blocks per CU per class (0: one block per work item)
     int ncu = 0;
+    int use_graph = 1;           // 0: launch the step's kernels directly (SHX_BATCH_GRAPH=0)
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2038,12 +2037,12 @@ static void batch_buffers(const shx_case *c, int s, int &in, int &out)
 }
 
 // append one launch over `cs` with per-case argument records `recs` (rsize bytes each) and grids
-static void batch_add(shx_batch *b, const void *fn, int threads, int klass, const std::vector<char> &recs,
-                      size_t rsize, const std::vector<dim3> &grids)
+static void batch_add(shx_batch *b, const void *fn, const void *fnp, int threads, int klass,
+                      const std::vector<char> &recs, size_t rsize, const std::vector<dim3> &grids)
 {
     BatchLaunch L;
     memset(&L, 0, sizeof L);
-    L.fn = fn; L.threads = threads; L.klass = klass;
+    L.fn = fn; L.fnp = fnp; L.threads = threads; L.klass = klass;
     const int nc = (int)grids.size();
     std::vector<int> prefix(nc + 1, 0), gdim(3 * nc);
     for (int i = 0; i < nc; i++) {
@@ -2051,17 +2050,21 @@ static void batch_add(shx_batch *b, const void *fn, int threads, int klass, cons
         gdim[3 * i] = (int)grids[i].x; gdim[3 * i + 1] = (int)grids[i].y; gdim[3 * i + 2] = (int)grids[i].z;
     }
     L.total = prefix[nc];
+    std::vector<int> cob(std::max(L.total, 1));
+    for (int i = 0; i < nc; i++)
+        for (int q = prefix[i]; q < prefix[i + 1]; q++) cob[q] = i;
     L.args_off = align256(b->host.size());
     b->host.resize(L.args_off + recs.size());
     memcpy(b->host.data() + L.args_off, recs.data(), recs.size());
+    L.cob_off = align256(b->host.size());
+    b->host.resize(L.cob_off + cob.size() * sizeof(int));
+    memcpy(b->host.data() + L.cob_off, cob.data(), cob.size() * sizeof(int));
     L.prefix_off = align256(b->host.size());
     b->host.resize(L.prefix_off + prefix.size() * sizeof(int));
     memcpy(b->host.data() + L.prefix_off, prefix.data(), prefix.size() * sizeof(int));
     L.gdim_off = align256(b->host.size());
     b->host.resize(L.gdim_off + gdim.size() * sizeof(int));
     memcpy(b->host.data() + L.gdim_off, gdim.data(), gdim.size() * sizeof(int));
-    L.tab.ncase = nc;
-    L.tab.total = L.total;
     (void)rsize;
     b->seq.push_back(L);
 }
@@ -2100,7 +2103,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 put(recs, a);
                 grids.push_back(dim3(nblk(g[i]->gm.ntot, 256)));
             }
-            batch_add(b, (const void *)&k_ghost_batch, 256, BK_GHOST, recs, sizeof(GhostArgs), grids);
+            batch_add(b, (const void *)&k_ghost_batch<false>, (const void *)&k_ghost_batch<true>, 256, BK_GHOST, recs, sizeof(GhostArgs), grids);
         }
         if (!c0->d.inviscid) {
             std::vector<char> recs, grecs;
@@ -2122,12 +2125,13 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 if (dims == 3) ggrids.push_back(dim3(nblk(c->gm.ng[2], GK), nblk(c->gm.ng[1], GJ), nblk(c->gm.ng[0], 32)));
                 else ggrids.push_back(dim3(nblk(c->gm.ntot, 256)));
             }
-            batch_add(b, (const void *)&k_values_batch, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
+            batch_add(b, (const void *)&k_values_batch<false>, (const void *)&k_values_batch<true>, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
             if (dims == 3)
-                batch_add(b, S == 5 ? (const void *)&k_march_batch<5> : (const void *)&k_march_batch<9>, GK * GJ,
+                batch_add(b, S == 5 ? (const void *)&k_march_batch<5, false> : (const void *)&k_march_batch<9, false>,
+                          S == 5 ? (const void *)&k_march_batch<5, true> : (const void *)&k_march_batch<9, true>, GK * GJ,
                           BK_GRAD, grecs, sizeof(GradArgs), ggrids);
             else
-                batch_add(b, (const void *)&k_grad2d_batch, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
+                batch_add(b, (const void *)&k_grad2d_batch<false>, (const void *)&k_grad2d_batch<true>, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
         }
         for (int dir = 0; dir < dims; dir++) {
             const Pick p = weno_pick(c0, dir);
@@ -2138,7 +2142,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 put(recs, a);
                 grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
             }
-            batch_add(b, p.batch, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids);
+            batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids);
         }
         if (!c0->d.inviscid) {
             for (int dir = 0; dir < dims; dir++) {
@@ -2150,7 +2154,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                     put(recs, a);
                     grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
                 }
-                batch_add(b, p.batch, p.TA * p.TDI, BK_VISC, recs, sizeof(ViscArgs), grids);
+                batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_VISC, recs, sizeof(ViscArgs), grids);
             }
         } else {
             std::vector<char> recs;
@@ -2164,7 +2168,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 put(recs, A);
                 grids.push_back(dim3(nblk((long)c->gm.n[0] * c->gm.n[1] * c->gm.n[2], 256)));
             }
-            batch_add(b, (const void *)&k_rk_batch, 256, BK_RK, recs, sizeof(UpdArgs), grids);
+            batch_add(b, (const void *)&k_rk_batch<false>, (const void *)&k_rk_batch<true>, 256, BK_RK, recs, sizeof(UpdArgs), grids);
         }
     }
     return SHX_OK;
@@ -2202,10 +2206,11 @@ static int batch_grids(shx_batch *b)
 {
     for (BatchLaunch &L : b->seq) {
         const int f = b->factor[L.klass];
-        if (f <= 0) { L.grid = L.total; continue; }
+        if (f <= 0) { L.grid = std::max(L.total, 1); L.run = L.fn; continue; }
         int per = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, L.fn, L.threads, 0));
-        L.grid = std::min(L.total, std::max(1, per) * b->ncu * f);
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, L.fnp, L.threads, 0));
+        L.grid = std::max(1, std::min(L.total, std::max(1, per) * b->ncu * f));
+        L.run = L.fnp;
     }
     return SHX_OK;
 }
@@ -2216,8 +2221,8 @@ static int batch_capture(shx_batch *b)
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (BatchLaunch &L : b->seq) {
-        void *args[] = {&L.tab};
-        hipError_t e = hipLaunchKernel(L.fn, dim3(L.grid), dim3(L.threads), args, 0, b->stream);
+        void *args[] = {&L.A, &L.cob, &L.prefix, &L.gdim, &L.total};
+        hipError_t e = hipLaunchKernel(L.run, dim3(L.grid), dim3(L.threads), args, 0, b->stream);
         if (e != hipSuccess) {
             hipGraph_t g;
             hipStreamEndCapture(b->stream, &g);
@@ -2240,9 +2245,10 @@ static int batch_upload(shx_batch *b, bool fresh)
     }
     HIPCHK(hipMemcpyAsync(b->dev, b->host.data(), b->host.size(), hipMemcpyHostToDevice, b->stream));
     for (BatchLaunch &L : b->seq) {
-        L.tab.args = b->dev + L.args_off;
-        L.tab.prefix = (const int *)(b->dev + L.prefix_off);
-        L.tab.gdim = (const int *)(b->dev + L.gdim_off);
+        L.A = b->dev + L.args_off;
+        L.cob = (const int *)(b->dev + L.cob_off);
+        L.prefix = (const int *)(b->dev + L.prefix_off);
+        L.gdim = (const int *)(b->dev + L.gdim_off);
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     return SHX_OK;
@@ -2265,7 +2271,11 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
     shx_batch *b = new shx_batch();
     b->cases.assign(cases, cases + ncases);
     b->tau.assign(tau, tau + ncases);
-    for (int k = 0; k < BK_NUM; k++) b->factor[k] = 2;
+    for (int k = 0; k < BK_NUM; k++) b->factor[k] = 0;
+    {
+        const char *env = getenv("SHX_BATCH_GRAPH");
+        if (env && env[0] == '0') b->use_graph = 0;
+    }
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -2291,7 +2301,7 @@ extern "C" int shx_batch_set_tau(shx_batch *b, const double *tau)
     std::vector<BatchLaunch> old = b->seq;
     int rc = batch_build(b);
     if (rc) return rc;
-    for (size_t i = 0; i < b->seq.size(); i++) b->seq[i].grid = old[i].grid;
+    for (size_t i = 0; i < b->seq.size(); i++) { b->seq[i].grid = old[i].grid; b->seq[i].run = old[i].run; }
     return batch_upload(b, false);
 }
 
@@ -2304,7 +2314,16 @@ extern "C" int shx_batch_run(shx_batch *b, int nsteps)
         HIPCHK(hipEventRecord(b->done, c->stream));
         HIPCHK(hipStreamWaitEvent(b->stream, b->done, 0));
     }
-    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    for (int s = 0; s < nsteps; s++) {
+        if (b->use_graph) {
+            HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        } else {
+            for (BatchLaunch &L : b->seq) {
+                void *args[] = {&L.A, &L.cob, &L.prefix, &L.gdim, &L.total};
+                HIPCHK(hipLaunchKernel(L.run, dim3(L.grid), dim3(L.threads), args, 0, b->stream));
+            }
+        }
+    }
     HIPCHK(hipEventRecord(b->done, b->stream));
     for (shx_case *c : b->cases) HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));
     return SHX_OK;
