@@ -146,8 +146,9 @@ def main():
     case.upload_state(prim)
     del prim
     tau = case.cfl_dt(0.5)
+    tiles = case.tune(3)
     t_setup = time.perf_counter() - t_setup
-    print(f"[bench] setup {t_setup:.1f}s tau={tau:.6g} n={n}", file=sys.stderr, flush=True)
+    print(f"[bench] setup {t_setup:.1f}s tau={tau:.6g} n={n} weno tiles {tiles}", file=sys.stderr, flush=True)
 
     if args.copy_probe:
         case.copy_probe(args.copy_probe)
@@ -216,6 +217,7 @@ def main():
                    "precision": args.precision, "quirks": "reference", "tau": tau, "seed": args.seed},
         "roofline": roofline,
         "stage_hbm_frac_260B": round(stage_frac, 4),
+        "weno_tile_variants": tiles,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
@@ -344,6 +346,8 @@ def shock_workload(args):
     case.upload_state(prim)
     del coords, prim
     tau = case.cfl_dt(0.5)
+    tiles = case.tune(2)
+    print(f"[bench] shock tiles {tiles}", file=sys.stderr, flush=True)
     print(f"[bench] shock setup {time.perf_counter() - t0:.1f}s tau={tau:.4g} n={n}", file=sys.stderr, flush=True)
     case.run(tau, args.warmup)
     case.synchronize()

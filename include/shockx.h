@@ -127,6 +127,11 @@ int shx_cfl_dt(shx_case *c, double cfl, double *dt);
 
 int shx_synchronize(shx_case *c);
 
+/* Autotuner (no reference counterpart): times the WENO sweep tile variants on the current state
+ * (4 x 64 or 2 x 128 along the contiguous axis, 16 x 16 or 8 x 32 otherwise) and keeps the fastest
+ * per direction; choice[3] (may be NULL) receives the variant indices.  The state is unchanged. */
+int shx_tune(shx_case *c, int reps, int choice[3]);
+
 /* Measurement aid (no reference counterpart): `reps` launches of a plain device copy of one
  * padded field (8 * ng0*ng1*ng2 bytes read and written per launch), used to calibrate
  * hardware byte counters against a kernel of known traffic. */

@@ -1292,6 +1292,7 @@ struct shx_case {
     int faces;             // number of non-periodic faces
     int needs_bs;          // some face is SHX_BC_MANUFACTURED
     int diag;              // 3D metric is diagonal (only xi_x, eta_y, zeta_z and J non-zero)
+    int wtile[3];          // WENO tile variant per direction (set by shx_tune)
     unsigned long long *d_red;
     int *d_cnt;
     int timing;
@@ -1714,28 +1715,41 @@ static Pick weno_pick_t()
     return {(const void *)&k_weno<S, TA, TDI, AX, MIX>, (const void *)&k_weno_batch<S, TA, TDI, AX, MIX, false>,
             (const void *)&k_weno_batch<S, TA, TDI, AX, MIX, true>, TA, TDI};
 }
-template <int S, int TA, int TDI, bool MIX>
+// tile variants of the WENO sweeps (TA transverse x TDI along the sweep), chosen per case and
+// direction by shx_tune: sweeps along the contiguous axis 4 x 64 or 2 x 128, the others 16 x 16
+// or 8 x 32.  Diagonal metrics only ever use AX = 2 on the contiguous axis and 0 or 1 otherwise.
+constexpr int NWTILE = 2;
+template <int S, int TA, int TDI, bool MIX, bool COAL>
 static Pick weno_pick_m(int ax)
 {
-    if (ax == 0) return weno_pick_t<S, TA, TDI, 0, MIX>();
-    if (ax == 1) return weno_pick_t<S, TA, TDI, 1, MIX>();
-    if (ax == 2) return weno_pick_t<S, TA, TDI, 2, MIX>();
-    return weno_pick_t<S, TA, TDI, -1, MIX>();
+    if constexpr (COAL) {
+        if (ax == 2) return weno_pick_t<S, TA, TDI, 2, MIX>();
+        return weno_pick_t<S, TA, TDI, -1, MIX>();
+    } else {
+        if (ax == 0) return weno_pick_t<S, TA, TDI, 0, MIX>();
+        if (ax == 1) return weno_pick_t<S, TA, TDI, 1, MIX>();
+        return weno_pick_t<S, TA, TDI, -1, MIX>();
+    }
 }
-template <int S, int TA, int TDI>
+template <int S, int TA, int TDI, bool COAL>
 static Pick weno_pick_s(int ax, bool mix)
 {
-    return mix ? weno_pick_m<S, TA, TDI, true>(ax) : weno_pick_m<S, TA, TDI, false>(ax);
+    return mix ? weno_pick_m<S, TA, TDI, true, COAL>(ax) : weno_pick_m<S, TA, TDI, false, COAL>(ax);
 }
-// coalesced sweeps (the contiguous axis: k in 3D, j in 2D) use 4 x 64 tiles, the others 16 x 16
+template <int S>
+static Pick weno_pick_v(bool coalesce, int variant, int ax, bool mix)
+{
+    if (coalesce) return variant ? weno_pick_s<S, 2, 128, true>(ax, mix) : weno_pick_s<S, 4, 64, true>(ax, mix);
+    return variant ? weno_pick_s<S, 8, 32, false>(ax, mix) : weno_pick_s<S, 16, 16, false>(ax, mix);
+}
 static Pick weno_pick(const shx_case *c, int dir)
 {
     const bool diag = (c->diag && c->gm.dims == 3);
     const bool mix = (c->d.precision == SHX_MIXED);
     const bool coalesce = (c->gm.st[dir] == 1);
     const int ax = diag ? dir : -1;
-    if (coalesce) return (c->d.space_order == 5) ? weno_pick_s<5, 4, 64>(ax, mix) : weno_pick_s<9, 4, 64>(ax, mix);
-    return (c->d.space_order == 5) ? weno_pick_s<5, 16, 16>(ax, mix) : weno_pick_s<9, 16, 16>(ax, mix);
+    const int v = c->wtile[dir];
+    return (c->d.space_order == 5) ? weno_pick_v<5>(coalesce, v, ax, mix) : weno_pick_v<9>(coalesce, v, ax, mix);
 }
 
 // transverse axes of a sweep: A = the contiguous non-sweep axis with extent > 1, B = the other
@@ -2177,7 +2191,8 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
 static bool same_group(const shx_case *a, const shx_case *c)
 {
     return a->gm.dims == c->gm.dims && a->d.space_order == c->d.space_order && a->diag == c->diag &&
-           a->d.precision == c->d.precision && a->d.inviscid == c->d.inviscid && a->d.time_order == c->d.time_order;
+           a->d.precision == c->d.precision && a->d.inviscid == c->d.inviscid && a->d.time_order == c->d.time_order &&
+           a->wtile[0] == c->wtile[0] && a->wtile[1] == c->wtile[1] && a->wtile[2] == c->wtile[2];
 }
 
 // (re)build the argument tables of every launch for the current time steps
@@ -2410,6 +2425,44 @@ extern "C" int shx_batch_destroy(shx_batch *b)
     if (b->done) hipEventDestroy(b->done);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
+    return SHX_OK;
+}
+
+// Tile autotuning: for every direction, time `reps` WENO sweeps of the current state with each
+// tile variant and keep the fastest.  The sweeps only write Q, which every stage recomputes, so
+// the state is unchanged.
+extern "C" int shx_tune(shx_case *c, int reps, int choice[3])
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (reps <= 0) reps = 3;
+    double *const *pr = c->prim[c->cur];
+    if ((rc = ghost_fill(c, pr, 5))) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int dir = 0; dir < c->gm.dims && !rc; dir++) {
+        float best = 1e30f;
+        int bestv = 0;
+        for (int v = 0; v < NWTILE && !rc; v++) {
+            c->wtile[dir] = v;
+            if ((rc = weno_sweep(c, pr, dir, 1))) break;          // warm-up
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps && !rc; r++) rc = weno_sweep(c, pr, dir, 1);
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bestv = v; }
+        }
+        c->wtile[dir] = bestv;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if (choice)
+        for (int d = 0; d < 3; d++) choice[d] = c->wtile[d];
+    if (c->timing) drain_events(c);
     return SHX_OK;
 }
 

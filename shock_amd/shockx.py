@@ -46,7 +46,7 @@ EXPORTS = [
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
-    "shx_batch_destroy",
+    "shx_batch_destroy", "shx_tune",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -84,6 +84,7 @@ def lib():
         L.shx_batch_tune.argtypes = [P, ctypes.c_int, P]
         L.shx_batch_synchronize.argtypes = [P]
         L.shx_batch_destroy.argtypes = [P]
+        L.shx_tune.argtypes = [P, ctypes.c_int, P]
         _lib = L
     return _lib
 
@@ -213,6 +214,12 @@ class Case:
         dt = ctypes.c_double()
         _check(lib().shx_cfl_dt(self.h, cfl, ctypes.byref(dt)))
         return dt.value
+
+    def tune(self, reps=3):
+        """Pick the fastest WENO tile variant per direction on the current state."""
+        ch = (ctypes.c_int * 3)()
+        _check(lib().shx_tune(self.h, reps, ch))
+        return list(ch)
 
     def copy_probe(self, reps=1):
         _check(lib().shx_copy_probe(self.h, reps))
