@@ -73,8 +73,8 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 // Its mirror node is layer l inside the face (StartReal + l, EndReal - l).
 struct BcArgs {
     Fields5 pr;              // stage input state; ghost rho,u,v,w,e are written
-    double *T, *mu;          // derived T and mu (viscous runs; null when inviscid)
-    const double *bs[6];     // exact rho,u,v,w,e,p for manufactured faces
+    gdp T, mu;               // derived T and mu (viscous runs; null when inviscid)
+    gcdp bs[6];              // exact rho,u,v,w,e,p for manufactured faces
     int kind, dir, upper, g;
     int t1, t2;              // tangential axes
     int lo1, n1, lo2, n2;    // tangential index ranges
@@ -145,7 +145,7 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
 }
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
-__device__ __forceinline__ void values_body(const CFields5 &pr, double *T, double *mu, const Geom &gm,
+__device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const Geom &gm,
                                             const Consts &k, const long t)
 {
     if (t >= gm.ntot) return;
@@ -164,7 +164,7 @@ __global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Const
 // ---- WENO sweep ---------------------------------------------------------------------------
 struct WenoArgs {
     CFields5 pr;          // rho, u, v, w, e of the stage input (ghosts filled)
-    const double *m0, *m1, *m2, *J;   // metric row of the sweep direction and the Jacobian
+    gcdp m0, m1, m2, J;               // metric row of the sweep direction and the Jacobian
                                       // (diagonal metrics: m0 is the row's diagonal entry)
     Fields5 q;            // stage residual, interior nodes
     long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
@@ -507,8 +507,8 @@ __device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const do
 }
 
 struct ViscArgs {
-    const double *u, *v, *w, *T, *mu;
-    const double *cr[8];          // cross derivatives (layout depends on DIAG, see host)
+    gcdp u, v, w, T, mu;
+    gcdp cr[8];                   // cross derivatives (layout depends on DIAG, see host)
     CFields10 met;
     Fields5 q;
     long sD, sA, sB;
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
 struct UpdArgs {
     Fields5 q;
-    const double *J;
+    gcdp J;
     Geom gm;
     RkArgs rk;
 };
@@ -888,7 +888,7 @@ __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
 // ---- viscous derivatives ------------------------------------------------------------------
 // grad order: u_xi,u_eta,u_zeta, v_xi,v_eta,v_zeta, w_xi,w_eta,w_zeta, T_xi,T_eta,T_zeta.
 // Direction d is evaluated where the node's d index is interior (all transverse nodes).
-__device__ __forceinline__ void grad_body(const CFields5 &pr, const double *T, const Fields12 &gr,
+__device__ __forceinline__ void grad_body(const CFields5 &pr, const gcdp T, const Fields12 &gr,
                                           const Geom &gm, const Consts &k, const long t)
 {
     if (t >= gm.ntot) return;
@@ -929,7 +929,7 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
 constexpr int GK = 64, GJ = 4;
 struct GradArgs {
     CFields5 pr;
-    const double *T;
+    gcdp T;
     Fields12 gr;
     Geom gm;
     Consts k;
@@ -939,7 +939,7 @@ template <int S>
 __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
 {
     const CFields5 &pr = a.pr;
-    const double *T = a.T;
+    const gcdp T = a.T;
     const Fields12 &gr = a.gr;
     const Geom &gm = a.gm;
     const Consts &k = a.k;
@@ -952,7 +952,7 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     const int kk = k0 + tx, jj = j0 + ty;
     const bool mine = (kk < gm.ng[2]) && (jj < gm.ng[1]);
     const int i0 = bz * ci, i1 = min(i0 + ci, gm.ng[0]);
-    const double *src[4] = {pr.f[1], pr.f[2], pr.f[3], T};
+    const gcdp src[4] = {pr.f[1], pr.f[2], pr.f[3], T};
     const long col = (long)jj * gm.st[1] + kk;
     double win[4][NL];
     // prime the window with planes i0-G .. i0+G-1 (clamped; out-of-block planes are never used)
@@ -1023,15 +1023,22 @@ __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
 }
 
 // ---- batched launches ----------------------------------------------------------------------------
-// One launch runs a kernel class over many independent cases.  Case i owns the blocks
-// [prefix[i], prefix[i+1]) of a flat work list, with its own argument record and grid shape; the
-// blocks of the launch walk the list with a grid stride, so the launch can be sized to the device
-// (persistent blocks) independently of the amount of work.
-// The per-case records, the case of every block (cob), the first block of every case (prefix)
-// and the case grids (gdim) are read-only for the launch: __restrict__ lets the compiler read
-// the records with scalar loads, as it does kernel arguments.
-#define BATCH_PARAMS(T) const T *__restrict__ A, const int *__restrict__ cob, const int *__restrict__ prefix, \
-                        const int *__restrict__ gdim, const int total
+// One launch runs a kernel class over many independent cases.  The launch's work list holds one
+// descriptor per block of every case: (case, tile x, tile y, tile z), or (case, block, 0, 0) for
+// the node-parallel kernels, precomputed on the host so that a block starts from one scalar load
+// and no integer division.  Blocks walk the list (one item per block, or with a grid stride when
+// the launch is sized to the device: persistent blocks).
+// The per-case records and the work list are read-only for the launch and are addressed in the
+// constant address space, as kernel arguments are: the compiler reads them with scalar loads and,
+// since nothing can write them, keeps the values across the LDS fences of the sweeps instead of
+// reloading them (a generic record pointer would be reloaded after every fence).
+#define KCONST __attribute__((address_space(4)))
+#define BATCH_PARAMS(T) const KCONST T *__restrict__ A, const KCONST int4 *__restrict__ work, const int total
+template <class T>
+__device__ __forceinline__ const T &rec(const KCONST T *A, const int i)
+{
+    return *(const T *)&A[i];
+}
 
 // P = false: one block per work item (the kernel then needs exactly the registers of its
 // single-case twin); P = true: persistent blocks walking the list with a grid stride.
@@ -1049,37 +1056,40 @@ __device__ __forceinline__ void batch_walk(const int total, F &&f)
 }
 
 struct GhostArgs { Fields10 f; int nf; Geom gm; };
-struct ValuesArgs { CFields5 pr; double *T, *mu; Geom gm; Consts k; };
+struct ValuesArgs { CFields5 pr; gdp T, mu; Geom gm; Consts k; };
 template <bool P>
 __global__ __launch_bounds__(256) void k_ghost_batch(BATCH_PARAMS(GhostArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        ghost_body(A[i].f, A[i].nf, A[i].gm, (long)(b - prefix[i]) * 256 + threadIdx.x);
+        const int4 w = rec(work, b);
+        const GhostArgs &a = rec(A, w.x);
+        ghost_body(a.f, a.nf, a.gm, (long)w.y * 256 + threadIdx.x);
     });
 }
 template <bool P>
 __global__ __launch_bounds__(256) void k_values_batch(BATCH_PARAMS(ValuesArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        values_body(A[i].pr, A[i].T, A[i].mu, A[i].gm, A[i].k, (long)(b - prefix[i]) * 256 + threadIdx.x);
+        const int4 w = rec(work, b);
+        const ValuesArgs &a = rec(A, w.x);
+        values_body(a.pr, a.T, a.mu, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
     });
 }
 template <bool P>
 __global__ __launch_bounds__(256) void k_rk_batch(BATCH_PARAMS(UpdArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        rk_body(A[i], (long)(b - prefix[i]) * 256 + threadIdx.x);
+        const int4 w = rec(work, b);
+        rk_body(rec(A, w.x), (long)w.y * 256 + threadIdx.x);
     });
 }
 template <bool P>
 __global__ __launch_bounds__(256) void k_grad2d_batch(BATCH_PARAMS(GradArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        grad_body(A[i].pr, A[i].T, A[i].gr, A[i].gm, A[i].k, (long)(b - prefix[i]) * 256 + threadIdx.x);
+        const int4 w = rec(work, b);
+        const GradArgs &a = rec(A, w.x);
+        grad_body(a.pr, a.T, a.gr, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
     });
 }
 
@@ -1087,9 +1097,8 @@ template <int S, bool P>
 __global__ __launch_bounds__(GK * GJ) void k_march_batch(BATCH_PARAMS(GradArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
-        march_body<S>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
+        const int4 w = rec(work, b);
+        march_body<S>(rec(A, w.x), w.y, w.z, w.w);
     });
 }
 
@@ -1097,9 +1106,8 @@ template <int S, int TA, int TDI, int AX, bool MIX, bool P>
 __global__ __launch_bounds__(TA * TDI) void k_weno_batch(BATCH_PARAMS(WenoArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
-        weno_body<S, TA, TDI, AX, MIX>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
+        const int4 w = rec(work, b);
+        weno_body<S, TA, TDI, AX, MIX>(rec(A, w.x), w.y, w.z, w.w);
     });
 }
 
@@ -1107,9 +1115,8 @@ template <int S, int DIR, int TA, int TDI, bool DIAG, bool P>
 __global__ __launch_bounds__(TA * TDI) void k_visc_batch(BATCH_PARAMS(ViscArgs))
 {
     batch_walk<P>(total, [&](const int b) {
-        const int i = cob[b];
-        const int l = b - prefix[i], gx = gdim[3 * i], gy = gdim[3 * i + 1];
-        visc_body<S, DIR, TA, TDI, DIAG>(A[i], l % gx, (l / gx) % gy, l / (gx * gy));
+        const int4 w = rec(work, b);
+        visc_body<S, DIR, TA, TDI, DIAG>(rec(A, w.x), w.y, w.z, w.w);
     });
 }
 
@@ -2010,9 +2017,10 @@ struct BatchLaunch {
     const void *fn, *fnp;          // flat and persistent kernels; launched: fn when grid == total
     const void *run;
     int threads, total, grid, klass;
-    size_t args_off, cob_off, prefix_off, gdim_off;
+    int group;                     // kernel-template group (one lane of the step graph)
+    size_t args_off, work_off;
     const void *A;
-    const int *cob, *prefix, *gdim;
+    const int4 *work;
 };
 
 enum { BK_GHOST, BK_VALUES, BK_GRAD, BK_WENO, BK_VISC, BK_RK, BK_NUM };
@@ -2031,6 +2039,13 @@ struct shx_batch {
     int factor[BK_NUM];          // resident blocks per CU per class (0: one block per work item)
     int ncu = 0;
     int use_graph = 1;           // 0: launch the step's kernels directly (SHX_BATCH_GRAPH=0)
+    // Groups share no data, so every group runs on a lane stream of its own between a fork and a
+    // join on `stream`: kernels of different groups (a memory-bound fill of one beside the sweeps
+    // of another, the tail of one launch beside the head of the next) overlap on the device.
+    int ngroups = 0, cur_group = 0;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2056,29 +2071,23 @@ static void batch_add(shx_batch *b, const void *fn, const void *fnp, int threads
 {
     BatchLaunch L;
     memset(&L, 0, sizeof L);
-    L.fn = fn; L.fnp = fnp; L.threads = threads; L.klass = klass;
+    L.fn = fn; L.fnp = fnp; L.threads = threads; L.klass = klass; L.group = b->cur_group;
     const int nc = (int)grids.size();
-    std::vector<int> prefix(nc + 1, 0), gdim(3 * nc);
+    std::vector<int4> work;
     for (int i = 0; i < nc; i++) {
-        prefix[i + 1] = prefix[i] + (int)(grids[i].x * grids[i].y * grids[i].z);
-        gdim[3 * i] = (int)grids[i].x; gdim[3 * i + 1] = (int)grids[i].y; gdim[3 * i + 2] = (int)grids[i].z;
+        const dim3 g = grids[i];      // node-parallel kernels: (blocks, 1, 1)
+        for (unsigned z = 0; z < g.z; z++)
+            for (unsigned y = 0; y < g.y; y++)
+                for (unsigned x = 0; x < g.x; x++) work.push_back(make_int4(i, (int)x, (int)y, (int)z));
     }
-    L.total = prefix[nc];
-    std::vector<int> cob(std::max(L.total, 1));
-    for (int i = 0; i < nc; i++)
-        for (int q = prefix[i]; q < prefix[i + 1]; q++) cob[q] = i;
+    L.total = (int)work.size();
+    if (work.empty()) work.push_back(make_int4(0, 0, 0, 0));
     L.args_off = align256(b->host.size());
     b->host.resize(L.args_off + recs.size());
     memcpy(b->host.data() + L.args_off, recs.data(), recs.size());
-    L.cob_off = align256(b->host.size());
-    b->host.resize(L.cob_off + cob.size() * sizeof(int));
-    memcpy(b->host.data() + L.cob_off, cob.data(), cob.size() * sizeof(int));
-    L.prefix_off = align256(b->host.size());
-    b->host.resize(L.prefix_off + prefix.size() * sizeof(int));
-    memcpy(b->host.data() + L.prefix_off, prefix.data(), prefix.size() * sizeof(int));
-    L.gdim_off = align256(b->host.size());
-    b->host.resize(L.gdim_off + gdim.size() * sizeof(int));
-    memcpy(b->host.data() + L.gdim_off, gdim.data(), gdim.size() * sizeof(int));
+    L.work_off = align256(b->host.size());
+    b->host.resize(L.work_off + work.size() * sizeof(int4));
+    memcpy(b->host.data() + L.work_off, work.data(), work.size() * sizeof(int4));
     (void)rsize;
     b->seq.push_back(L);
 }
@@ -2200,6 +2209,7 @@ static int batch_build(shx_batch *b)
 {
     b->seq.clear();
     b->host.clear();
+    b->ngroups = 0;
     std::vector<int> done(b->cases.size(), 0);
     for (size_t i = 0; i < b->cases.size(); i++) {
         if (done[i]) continue;
@@ -2211,6 +2221,7 @@ static int batch_build(shx_batch *b)
                 t.push_back(b->tau[j]);
                 done[j] = 1;
             }
+        b->cur_group = b->ngroups++;
         int rc = batch_group(b, g, t);
         if (rc) return rc;
     }
@@ -2230,20 +2241,56 @@ static int batch_grids(shx_batch *b)
     return SHX_OK;
 }
 
+// lane streams and events for the current number of groups
+static int batch_lanes(shx_batch *b)
+{
+    if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    while ((int)b->lanes.size() < b->ngroups) {
+        hipStream_t s;
+        hipEvent_t e;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->lanes.push_back(s);
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->joins.push_back(e);
+    }
+    return SHX_OK;
+}
+
+// one RK step of every group: fork from `stream`, each group's launches on its lane, join
+static hipError_t batch_enqueue(shx_batch *b)
+{
+    const bool fan = b->ngroups > 1;
+    hipError_t e = hipSuccess;
+    if (fan) {
+        e = hipEventRecord(b->fork, b->stream);
+        for (int g = 0; g < b->ngroups && e == hipSuccess; g++) e = hipStreamWaitEvent(b->lanes[g], b->fork, 0);
+    }
+    for (BatchLaunch &L : b->seq) {
+        if (e != hipSuccess) break;
+        void *args[] = {&L.A, &L.work, &L.total};
+        e = hipLaunchKernel(L.run, dim3(L.grid), dim3(L.threads), args, 0, fan ? b->lanes[L.group] : b->stream);
+    }
+    if (fan)
+        for (int g = 0; g < b->ngroups && e == hipSuccess; g++) {
+            e = hipEventRecord(b->joins[g], b->lanes[g]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->joins[g], 0);
+        }
+    return e;
+}
+
 static int batch_capture(shx_batch *b)
 {
     if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    int rc = batch_lanes(b);
+    if (rc) return rc;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (BatchLaunch &L : b->seq) {
-        void *args[] = {&L.A, &L.cob, &L.prefix, &L.gdim, &L.total};
-        hipError_t e = hipLaunchKernel(L.run, dim3(L.grid), dim3(L.threads), args, 0, b->stream);
-        if (e != hipSuccess) {
-            hipGraph_t g;
-            hipStreamEndCapture(b->stream, &g);
-            if (g) hipGraphDestroy(g);
-            return set_err(SHX_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
-        }
+    hipError_t e = batch_enqueue(b);
+    if (e != hipSuccess) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(b->stream, &g);
+        if (g) hipGraphDestroy(g);
+        return set_err(SHX_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
     }
     HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -2261,9 +2308,7 @@ static int batch_upload(shx_batch *b, bool fresh)
     HIPCHK(hipMemcpyAsync(b->dev, b->host.data(), b->host.size(), hipMemcpyHostToDevice, b->stream));
     for (BatchLaunch &L : b->seq) {
         L.A = b->dev + L.args_off;
-        L.cob = (const int *)(b->dev + L.cob_off);
-        L.prefix = (const int *)(b->dev + L.prefix_off);
-        L.gdim = (const int *)(b->dev + L.gdim_off);
+        L.work = (const int4 *)(b->dev + L.work_off);
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     return SHX_OK;
@@ -2333,10 +2378,7 @@ extern "C" int shx_batch_run(shx_batch *b, int nsteps)
         if (b->use_graph) {
             HIPCHK(hipGraphLaunch(b->exec, b->stream));
         } else {
-            for (BatchLaunch &L : b->seq) {
-                void *args[] = {&L.A, &L.cob, &L.prefix, &L.gdim, &L.total};
-                HIPCHK(hipLaunchKernel(L.run, dim3(L.grid), dim3(L.threads), args, 0, b->stream));
-            }
+            HIPCHK(batch_enqueue(b));
         }
     }
     HIPCHK(hipEventRecord(b->done, b->stream));
@@ -2423,6 +2465,9 @@ extern "C" int shx_batch_destroy(shx_batch *b)
     if (b->graph) hipGraphDestroy(b->graph);
     if (b->dev) hipFree(b->dev);
     if (b->done) hipEventDestroy(b->done);
+    for (hipStream_t l : b->lanes) { hipStreamSynchronize(l); hipStreamDestroy(l); }
+    for (hipEvent_t e : b->joins) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return SHX_OK;

@@ -16,12 +16,33 @@ struct Consts {
     int mms;
 };
 
-struct Fields5 { double *f[5]; };
-struct CFields5 { const double *f[5]; };
-struct Fields10 { double *f[10]; };
-struct CFields10 { const double *f[10]; };
-struct CFields12 { const double *f[12]; };
-struct Fields12 { double *f[12]; };
+// Pointer to a device array (hipMalloc'd, so in the global address space).  Element accesses
+// cast to that address space, so the compiler emits global loads and stores even where the
+// pointer itself was read from memory (the per-case records of the batched kernels); a plain
+// pointer read from memory is generic and gets flat instructions, which share the LDS wait
+// counter and serialise against the LDS traffic of the sweeps.  Host code assigns plain pointers.
+template <class T>
+struct gptr {
+    T *p;
+    gptr() = default;
+    __host__ __device__ gptr(T *q) : p(q) {}
+    template <class U>
+    __host__ __device__ gptr(const gptr<U> &o) : p(o.p) {}
+    __device__ __forceinline__ T &operator[](const long i) const
+    {
+        return *(T *)&((__attribute__((address_space(1))) T *)p)[i];
+    }
+    __host__ __device__ explicit operator bool() const { return p != nullptr; }
+};
+using gdp = gptr<double>;
+using gcdp = gptr<const double>;
+
+struct Fields5 { gdp f[5]; };
+struct CFields5 { gcdp f[5]; };
+struct Fields10 { gdp f[10]; };
+struct CFields10 { gcdp f[10]; };
+struct CFields12 { gcdp f[12]; };
+struct Fields12 { gdp f[12]; };
 
 struct Geom {
     int ng[3];        // padded extents
