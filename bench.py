@@ -187,7 +187,7 @@ def main():
         b = algo_bytes(dom, n, ng, time_order)
         ach = b / (avg * 1e-3) / 1e9
         traffic = None
-        tfile = os.path.join(ROOT, "profiles", "traffic_config1.json")
+        tfile = os.path.join(ROOT, "profiles", f"traffic_turbulence{n}_{args.precision}.json")
         if os.path.exists(tfile):
             with open(tfile) as f:
                 traffic = json.load(f).get(dom)
