@@ -146,9 +146,10 @@ int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[
                      int launches[SHX_NUM_KERNELS]);
 
 /* Batches (no reference counterpart: SHOCK runs one case per job).  A batch advances many
- * independent periodic cases, each created, uploaded and owned as above, with one launch per
- * kernel class and group of like cases (same dims, scheme, metric kind, precision, viscosity,
- * RK order) per stage over a tile work list, replayed from a captured HIP graph.  tau holds one
+ * independent cases, each created, uploaded and owned as above (any face kinds), with one launch
+ * per kernel class and group of like cases (same dims, scheme, metric kind, precision, viscosity,
+ * RK order, WENO tiles) per stage over a tile work list, replayed from a captured HIP graph; the
+ * face writers run as one launch per face index.  tau holds one
  * time step per case.  The cases keep their handles: their state can be read or replaced between
  * shx_batch_run calls (work on a case's stream is ordered after the batch).  Destroy the batch
  * before its cases. */

@@ -84,10 +84,9 @@ struct BcArgs {
     Consts k;
 };
 
-__global__ void k_face_bc(BcArgs a)
+__device__ __forceinline__ void face_body(const BcArgs &a, const long t)
 {
     const long per_layer = (long)a.n1 * a.n2;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per_layer * a.g) return;
     const int l = (int)(t / per_layer);
     const long r = t - (long)l * per_layer;
@@ -125,13 +124,16 @@ __global__ void k_face_bc(BcArgs a)
         if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
     }
 }
+__global__ void k_face_bc(BcArgs a)
+{
+    face_body(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // Ghost nodes of non-periodic directions carried from one stage buffer to the next: the
 // reference keeps one U_RK array whose ghost layers persist between stages, which matters where
 // a writer reads a ghost another face wrote in the previous stage.
-__global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
+__device__ __forceinline__ void carry_body(const CFields5 &from, const Fields5 &to, const Geom &gm, const long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= gm.ntot) return;
     int idx[3];
     const long n = padded_node(gm, t, idx);
@@ -142,6 +144,10 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
     if (!ghost) return;
 #pragma unroll
     for (int m = 0; m < 5; m++) to.f[m][n] = from.f[m][n];
+}
+__global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
+{
+    carry_body(from, to, gm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
@@ -1084,6 +1090,24 @@ __global__ __launch_bounds__(256) void k_rk_batch(BATCH_PARAMS(UpdArgs))
     });
 }
 template <bool P>
+__global__ __launch_bounds__(256) void k_face_batch(BATCH_PARAMS(BcArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        face_body(rec(A, w.x), (long)w.y * 256 + threadIdx.x);
+    });
+}
+struct CarryArgs { CFields5 from; Fields5 to; Geom gm; };
+template <bool P>
+__global__ __launch_bounds__(256) void k_carry_batch(BATCH_PARAMS(CarryArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        const CarryArgs &a = rec(A, w.x);
+        carry_body(a.from, a.to, a.gm, (long)w.y * 256 + threadIdx.x);
+    });
+}
+template <bool P>
 __global__ __launch_bounds__(256) void k_grad2d_batch(BATCH_PARAMS(GradArgs))
 {
     batch_walk<P>(total, [&](const int b) {
@@ -1545,46 +1569,50 @@ static int ghost_fill(shx_case *c, double *const *f, int nf)
 }
 
 // face writers of every non-periodic face, in face order (i-, i+, j-, j+, k-, k+)
+// argument record of the writer of face f (f = 2 * direction + upper); nodes: ghost nodes it owns
+static void face_args(const shx_case *c, double *const *pr, int f, BcArgs &a, long &nodes)
+{
+    const Geom &gm = c->gm;
+    const int kind = c->d.bc[f];
+    memset(&a, 0, sizeof a);
+    for (int m = 0; m < 5; m++) a.pr.f[m] = pr[m];
+    if (!c->d.inviscid) { a.T = c->T; a.mu = c->mu; }
+    for (int m = 0; m < 6; m++) a.bs[m] = c->bs[m];
+    a.kind = kind;
+    a.dir = f / 2;
+    a.upper = f % 2;
+    a.g = gm.g;
+    a.t1 = (a.dir == 0) ? 1 : 0;
+    a.t2 = (a.dir == 2) ? 1 : 2;
+    if (kind == SHX_BC_MANUFACTURED) {
+        a.lo1 = gm.lo[a.t1]; a.n1 = gm.n[a.t1]; a.lo2 = gm.lo[a.t2]; a.n2 = gm.n[a.t2];
+    } else {
+        a.lo1 = 0; a.n1 = gm.ng[a.t1]; a.lo2 = 0; a.n2 = gm.ng[a.t2];
+    }
+    a.first = gm.lo[a.dir];
+    a.last = gm.lo[a.dir] + gm.n[a.dir] - 1;
+    for (int d = 0; d < 3; d++) a.st[d] = gm.st[d];
+    a.rho_in = c->d.rho_inflow; a.u_in = c->d.u_inflow; a.v_in = c->d.v_inflow;
+    a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
+    a.cinf = 1.0 / c->d.mach;
+    a.k = c->k;
+    nodes = (kind == SHX_BC_PERIODIC) ? 0 : (long)a.n1 * a.n2 * a.g;
+}
 static int face_fill(shx_case *c, double *const *pr)
 {
     if (!c->faces) return SHX_OK;
-    const Geom &gm = c->gm;
     int rc = tbeg(c, KT_FACES);
     if (rc) return rc;
-    for (int f = 0; f < 2 * gm.dims; f++) {
-        const int kind = c->d.bc[f];
-        if (kind == SHX_BC_PERIODIC) continue;
+    for (int f = 0; f < 2 * c->gm.dims; f++) {
         BcArgs a;
-        memset(&a, 0, sizeof a);
-        for (int m = 0; m < 5; m++) a.pr.f[m] = pr[m];
-        if (!c->d.inviscid) { a.T = c->T; a.mu = c->mu; }
-        for (int m = 0; m < 6; m++) a.bs[m] = c->bs[m];
-        a.kind = kind;
-        a.dir = f / 2;
-        a.upper = f % 2;
-        a.g = gm.g;
-        a.t1 = (a.dir == 0) ? 1 : 0;
-        a.t2 = (a.dir == 2) ? 1 : 2;
-        if (kind == SHX_BC_MANUFACTURED) {
-            a.lo1 = gm.lo[a.t1]; a.n1 = gm.n[a.t1]; a.lo2 = gm.lo[a.t2]; a.n2 = gm.n[a.t2];
-        } else {
-            a.lo1 = 0; a.n1 = gm.ng[a.t1]; a.lo2 = 0; a.n2 = gm.ng[a.t2];
-        }
-        a.first = gm.lo[a.dir];
-        a.last = gm.lo[a.dir] + gm.n[a.dir] - 1;
-        for (int d = 0; d < 3; d++) a.st[d] = gm.st[d];
-        a.rho_in = c->d.rho_inflow; a.u_in = c->d.u_inflow; a.v_in = c->d.v_inflow;
-        a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
-        a.cinf = 1.0 / c->d.mach;
-        a.k = c->k;
-        const long nodes = (long)a.n1 * a.n2 * a.g;
+        long nodes;
+        face_args(c, pr, f, a, nodes);
+        if (nodes == 0) continue;
         k_face_bc<<<nblk(nodes, 256), 256, 0, c->stream>>>(a);
         LAUNCH_CHECK();
     }
     return tend(c);
 }
-
-// ghost layers of non-periodic directions from the stage input to the stage output buffer
 static int carry_ghosts(shx_case *c, double *const *from, double *const *to)
 {
     if (!c->faces) return SHX_OK;
@@ -2108,11 +2136,12 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
     const int nc = (int)g.size();
     for (int s = 0; s < c0->d.time_order; s++) {
         std::vector<RkArgs> rk(nc);
-        std::vector<double *const *> in(nc);
+        std::vector<double *const *> in(nc), out(nc);
         for (int i = 0; i < nc; i++) {
             int ib, ob;
             batch_buffers(g[i], s, ib, ob);
             in[i] = g[i]->prim[ib];
+            out[i] = g[i]->prim[ob];
             rk[i] = rk_args(g[i], g[i]->prim[ib], g[i]->prim[ob], s, tau[i]);
         }
         {   // periodic ghost fill of the stage input
@@ -2128,6 +2157,25 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             }
             batch_add(b, (const void *)&k_ghost_batch<false>, (const void *)&k_ghost_batch<true>, 256, BK_GHOST, recs, sizeof(GhostArgs), grids);
         }
+        // face writers, one launch per face in the reference's order (after CalcValues when viscous);
+        // a case contributes no blocks to the launch of a periodic face
+        auto add_faces = [&]() {
+            for (int f = 0; f < 2 * dims; f++) {
+                std::vector<char> recs;
+                std::vector<dim3> grids;
+                long any = 0;
+                for (int i = 0; i < nc; i++) {
+                    BcArgs a;
+                    long nodes;
+                    face_args(g[i], in[i], f, a, nodes);
+                    put(recs, a);
+                    grids.push_back(dim3(nblk(nodes, 256)));
+                    any += nodes;
+                }
+                if (any) batch_add(b, (const void *)&k_face_batch<false>, (const void *)&k_face_batch<true>, 256, BK_GHOST, recs, sizeof(BcArgs), grids);
+            }
+        };
+        if (c0->d.inviscid) add_faces();
         if (!c0->d.inviscid) {
             std::vector<char> recs, grecs;
             std::vector<dim3> grids, ggrids;
@@ -2149,6 +2197,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 else ggrids.push_back(dim3(nblk(c->gm.ntot, 256)));
             }
             batch_add(b, (const void *)&k_values_batch<false>, (const void *)&k_values_batch<true>, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
+            add_faces();
             if (dims == 3)
                 batch_add(b, S == 5 ? (const void *)&k_march_batch<5, false> : (const void *)&k_march_batch<9, false>,
                           S == 5 ? (const void *)&k_march_batch<5, true> : (const void *)&k_march_batch<9, true>, GK * GJ,
@@ -2192,6 +2241,23 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 grids.push_back(dim3(nblk((long)c->gm.n[0] * c->gm.n[1] * c->gm.n[2], 256)));
             }
             batch_add(b, (const void *)&k_rk_batch<false>, (const void *)&k_rk_batch<true>, 256, BK_RK, recs, sizeof(UpdArgs), grids);
+        }
+        {   // ghost layers of non-periodic directions into the next stage buffer (ghost nodes only,
+            // so the in-place last stage, whose update writes interior nodes of U_n, is unaffected)
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            long any = 0;
+            for (int i = 0; i < nc; i++) {
+                CarryArgs a;
+                memset(&a, 0, sizeof a);
+                for (int m = 0; m < 5; m++) { a.from.f[m] = in[i][m]; a.to.f[m] = out[i][m]; }
+                a.gm = g[i]->gm;
+                put(recs, a);
+                const long nodes = g[i]->faces ? g[i]->gm.ntot : 0;
+                grids.push_back(dim3(nblk(nodes, 256)));
+                any += nodes;
+            }
+            if (any) batch_add(b, (const void *)&k_carry_batch<false>, (const void *)&k_carry_batch<true>, 256, BK_GHOST, recs, sizeof(CarryArgs), grids);
         }
     }
     return SHX_OK;
@@ -2322,7 +2388,6 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
         shx_case *c = cases[i];
         int rc = check_ready(c);
         if (rc) return rc;
-        if (c->faces) return set_err(SHX_ERR_UNSUPPORTED, "case %d: batches take periodic blocks only", i);
         if (!(tau[i] > 0.0)) return set_err(SHX_ERR_ARG, "case %d: tau must be positive", i);
         for (int j = 0; j < i; j++)
             if (cases[j] == c) return set_err(SHX_ERR_ARG, "case %d appears twice", i);

@@ -5,8 +5,9 @@ Requests arrive as a Poisson stream.  Each asks for a 3D solve of size n^3 up to
 it is admitted when its HBM footprint fits the budget next to the cases already resident (and
 a slot is free), its state stays on the device, and every resident periodic case is advanced
 by the shared launches of one batch (shx_batch).  A request retires when it has taken its
-number of steps; the batch is rebuilt whenever the resident set changes.  Requests are periodic
-blocks (the batch takes periodic cases only).
+number of steps; the batch is rebuilt whenever the resident set changes.  A request is a
+periodic block of decaying turbulence or a normal shock meeting a vorticity field with
+inflow/outflow faces in x.
 
 Each request's steps and time step are fixed at admission (CFL on its initial state), so its
 result does not depend on what else is resident: served alone or in a mix, it runs the same
@@ -42,6 +43,7 @@ class Request:
     reynolds: float
     seed: int
     precision: int = shockx.FP64
+    kind: str = "turbulence"       # "turbulence" (periodic) or "shock" (inflow/outflow in x)
     # filled in by the server
     admitted: float = float("nan")
     done: float = float("nan")
@@ -50,35 +52,51 @@ class Request:
     result: list = field(default_factory=list)
 
 
-def make_requests(count, rate, seed, n_lo=64, n_hi=256, t_lo=0.5, t_hi=2.0, precision=shockx.FP64):
+def make_requests(count, rate, seed, n_lo=64, n_hi=256, t_lo=0.5, t_hi=2.0, precision=shockx.FP64,
+                  shock_frac=0.0):
     """Poisson arrivals (rate per second); sizes log-uniform in [n_lo, n_hi]; end times uniform in
-    [t_lo, t_hi]; Mach uniform 0.2-3 and Reynolds log-uniform 1e2-1e4 (random-phase turbulence on
-    the periodic curvilinear box); WENO5 or WENO9 at random."""
+    [t_lo, t_hi]; WENO5 or WENO9 at random; Reynolds log-uniform 1e2-1e4.  A fraction shock_frac
+    of the requests are shock cases (upstream Mach uniform 1.5-3.5), the others random-phase
+    turbulence on the periodic curvilinear box (turbulent Mach uniform 0.2-3)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(count):
         t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
         n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
-        out.append(Request(rid=i, arrival=t, n=n, S=int(rng.choice([5, 9])), t_end=float(rng.uniform(t_lo, t_hi)),
-                           mach=float(rng.uniform(0.2, 3.0)), reynolds=float(math.exp(rng.uniform(math.log(1e2), math.log(1e4)))),
-                           seed=int(rng.integers(1, 2 ** 31 - 1)), precision=precision))
+        S = int(rng.choice([5, 9]))
+        t_end = float(rng.uniform(t_lo, t_hi))
+        re = float(math.exp(rng.uniform(math.log(1e2), math.log(1e4))))
+        shock = bool(rng.uniform() < shock_frac)
+        mach = float(rng.uniform(1.5, 3.5)) if shock else float(rng.uniform(0.2, 3.0))
+        out.append(Request(rid=i, arrival=t, n=n, S=S, t_end=t_end, mach=mach, reynolds=re,
+                           seed=int(rng.integers(1, 2 ** 31 - 1)), precision=precision,
+                           kind="shock" if shock else "turbulence"))
     return out
 
 
 def prepare(r):
-    """Host inputs of one request (grid and initial state): built off the critical path."""
+    """Host inputs of one request (grid, initial state and inflow state): built off the critical
+    path."""
     nn = (r.n, r.n, r.n)
+    if r.kind == "shock":
+        return cases.shock_3d(r.S, nn, r.seed, mach=r.mach)
     coords = cases.periodic_ext_coords(3, r.S, nn, cases.sine_bump_3d(0.05))
-    return coords, cases.turbulence_3d_state(3, r.S, nn, r.seed, mach=r.mach)
+    return coords, cases.turbulence_3d_state(3, r.S, nn, r.seed, mach=r.mach), None
 
 
 def open_case(r, prep=None, cfl=0.5):
-    """Create, upload and size one request's case (periodic curvilinear box, turbulence IC)."""
+    """Create, upload and size one request's case: the periodic curvilinear box with turbulence,
+    or the shock case with supersonic inflow and Riemann outflow in x."""
     nn = (r.n, r.n, r.n)
-    coords, state = prep if prep is not None else prepare(r)
-    c = shockx.Case(dims=3, space_order=r.S, time_order=4, n=nn, mach=r.mach, reynolds=r.reynolds,
-                    precision=r.precision)
+    coords, state, inflow = prep if prep is not None else prepare(r)
+    if r.kind == "shock":
+        c = shockx.Case(dims=3, space_order=r.S, time_order=4, n=nn, gamma=1.4, mach=r.mach, reynolds=r.reynolds,
+                        quirks=shockx.QUIRKS_REFERENCE, bc=(shockx.BC_INFLOW_SUPERSONIC, shockx.BC_OUTFLOW_RIEMANN,
+                                                            0, 0, 0, 0), inflow=inflow, precision=r.precision)
+    else:
+        c = shockx.Case(dims=3, space_order=r.S, time_order=4, n=nn, mach=r.mach, reynolds=r.reynolds,
+                        precision=r.precision)
     c.build_metric(*coords)
     c.upload_state(state)
     tau = c.cfl_dt(cfl)

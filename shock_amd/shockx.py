@@ -239,7 +239,7 @@ class Case:
 
 
 class Batch:
-    """Many periodic cases advanced together (shx_batch): one launch per kernel class and group of
+    """Many cases advanced together (shx_batch): one launch per kernel class and group of
     like cases per stage, replayed from a captured HIP graph.  The cases stay usable through their
     own handles; close the batch before the cases."""
 

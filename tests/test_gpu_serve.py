@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("precision", [shockx.FP64, shockx.MIXED])
 def test_served_results_equal_alone(precision):
-    reqs = serve.make_requests(5, rate=50.0, seed=11, n_lo=12, n_hi=24, t_lo=0.02, t_hi=0.06, precision=precision)
+    reqs = serve.make_requests(6, rate=50.0, seed=11, n_lo=12, n_hi=24, t_lo=0.02, t_hi=0.06, precision=precision,
+                               shock_frac=0.5)
     srv = serve.Server(max_slots=3, chunk=2, keep_results=True)
     stats, done = srv.run(reqs)
     print(f"[serve] {stats}", flush=True)
@@ -20,5 +21,6 @@ def test_served_results_equal_alone(precision):
         g = (r.S + 1) // 2
         sl = (slice(g, g + r.n),) * 3
         diff = max(float(np.max(np.abs(a[sl] - b[sl]))) for a, b in zip(r.result, alone))
-        print(f"[serve] request {r.rid} n={r.n} S={r.S} steps={r.steps}: max |mix - alone| = {diff:.1e}", flush=True)
+        print(f"[serve] request {r.rid} {r.kind} n={r.n} S={r.S} steps={r.steps}: max |mix - alone| = {diff:.1e}",
+              flush=True)
         assert diff == 0.0
