@@ -42,6 +42,12 @@ extern "C" {
                                        (WriteInflowSupersonicNormalBoundary, src/BC.c:2505-2542) */
 #define SHX_BC_OUTFLOW_RIEMANN    3 /* characteristic outflow from the mirror node
                                        (WriteOutflowSubsonicRiemannBoundary, src/BC.c:2572-2622) */
+#define SHX_BC_OUTFLOW_NORMAL     4 /* p = p_out; rho, e, c, contravariant velocity from the mirror
+                                       node (WriteOutflowSubsonicNormalBoundary, src/BC.c:2544-2570) */
+#define SHX_BC_OUTFLOW_RUDY       5 /* non-reflecting: ghost p relaxed from the step-start ghost state
+                                       towards p_out (WriteOutflowSubsonicRudyBoundary, src/BC.c:2624-2655).
+                                       The step-start sound speed is the one of the state uploaded
+                                       last: the reference never recomputes U_lastStep's c. */
 
 /* Arithmetic forms. */
 #define SHX_FP64 0   /* every operation in FP64 (the reference's PRECISION=2) */
@@ -69,6 +75,10 @@ typedef struct shx_desc {
     /* inflow state (boundaryconditions:rho_inflow, p_inflow; u,v,w_inflow from the angle of
      * attack, src/Import.c:90-91,233-235); the Riemann outflow uses u_inflow as its far field */
     double rho_inflow, u_inflow, v_inflow, w_inflow, p_inflow;
+    /* subsonic outflow (SHX_BC_OUTFLOW_NORMAL / _RUDY): boundaryconditions:p_outflow
+     * (src/Import.c:94) and the Rudy relaxation rate AlphaNonRef = 0.25 L0/u0
+     * (src/Functions.c:1763), multiplied by the step's tau as in src/BC.c:2643-2646 */
+    double p_out, alpha_nonref;
 } shx_desc;
 
 typedef struct shx_case shx_case;

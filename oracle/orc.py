@@ -19,6 +19,7 @@ QUIRK_ZETA_VISC_J = 1
 QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MMS, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
+BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 
 
 class OrcCfg(ctypes.Structure):
@@ -30,6 +31,9 @@ class OrcCfg(ctypes.Structure):
         ("prandtl", ctypes.c_double), ("T0", ctypes.c_double), ("eps_override", ctypes.c_double),
         ("bc", ctypes.c_int * 6), ("rho_in", ctypes.c_double), ("u_in", ctypes.c_double),
         ("v_in", ctypes.c_double), ("w_in", ctypes.c_double), ("p_in", ctypes.c_double),
+        ("p_out", ctypes.c_double), ("alpha_nonref", ctypes.c_double), ("tau_step", ctypes.c_double),
+        ("p_last", ctypes.c_void_p), ("rho_last", ctypes.c_void_p), ("u_last", ctypes.c_void_p),
+        ("c_last", ctypes.c_void_p),
         ("g", ctypes.c_int), ("ng", ctypes.c_int * 3), ("ntot", ctypes.c_long),
         ("upsilon", ctypes.c_double), ("psi", ctypes.c_double), ("gam_heat", ctypes.c_double),
         ("suth", ctypes.c_double), ("eps", ctypes.c_double),
@@ -86,11 +90,14 @@ class Oracle:
 
     def __init__(self, dims=2, S=5, n=(16, 16, 1), time_order=4, gamma=1.4, mach=0.5,
                  reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, mms=False,
-                 quirks=QUIRKS_REFERENCE, eps_override=0.0, bc=(0, 0, 0, 0, 0, 0), inflow=(1.0, 1.0, 0.0, 0.0, 1.0)):
+                 quirks=QUIRKS_REFERENCE, eps_override=0.0, bc=(0, 0, 0, 0, 0, 0), inflow=(1.0, 1.0, 0.0, 0.0, 1.0),
+                 outflow=(1.0, 0.0)):
+        """outflow: (p_out, alpha_nonref) of the subsonic outflow writers."""
         c = OrcCfg()
         for f in range(6):
             c.bc[f] = bc[f]
         c.rho_in, c.u_in, c.v_in, c.w_in, c.p_in = inflow
+        c.p_out, c.alpha_nonref = outflow
         c.dims, c.S, c.time_order = dims, S, time_order
         for d in range(3):
             c.n[d] = n[d] if d < len(n) else 1
@@ -139,9 +146,24 @@ class Oracle:
         lib().orc_visc_dir(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
                            _bundle(grad), _bundle(q))
 
+    def _step_start(self, tau, met, un, p_last, c_last):
+        """Point the Rudy writer at the step-start state (kept alive on self)."""
+        self._last = (un[0], un[1], p_last, c_last)
+        self.c.tau_step = tau
+        self.c.rho_last, self.c.u_last = un[0].ctypes.data, un[1].ctypes.data
+        self.c.p_last, self.c.c_last = p_last.ctypes.data, c_last.ctypes.data
+
+    def eos_pc(self, met, prim):
+        """p and c of a state on every node (CalcValues, src/Functions.c:2445-2467)."""
+        der = self.calc_values(met, prim)
+        return der[0], der[2]
+
     def rhs(self, met, prim, src=None, bstate=None):
-        """Q (5 fields) of one stage for state prim; refills prim's ghosts (periodic + face BCs)."""
+        """Q (5 fields) of one stage for state prim; refills prim's ghosts (periodic + face BCs).
+        A Rudy face reads the input state as its step-start state, with tau = 0."""
         scratch = self.zeros(30)
+        p0, c0 = self.eos_pc(met, prim)
+        self._step_start(0.0, met, [a.copy() for a in prim[:2]], p0, c0)
         lib().orc_rhs_b(ctypes.byref(self.c), _bundle(met), _bundle(prim), _bundle(scratch),
                         _bundle(src) if src is not None else None,
                         _bundle(bstate) if bstate is not None else None)
@@ -157,14 +179,19 @@ class Oracle:
         return scratch
 
     def run(self, tau, met, prim0, steps, src=None, bstate=None):
-        """Advance prim0 by `steps` full steps (startSimulation semantics: U_lastStep <- U_RK)."""
+        """Advance prim0 by `steps` full steps (startSimulation semantics: U_lastStep <- U_RK, which
+        copies rho, p, u, v, w, e on every node, src/Functions.c:2484-2508).  The initial state's
+        p and c on every node (equation of state) start U_lastStep's p and c; c is never updated."""
         un = [a.copy() for a in prim0]
         urk = [a.copy() for a in prim0]
         scratch = self.zeros(30)
+        p_last, c_last = self.eos_pc(met, prim0)
         for _ in range(steps):
+            self._step_start(tau, met, un, p_last, c_last)
             self.step(tau, met, un, urk, src, scratch, bstate)
             for a, b in zip(un, urk):
                 a[...] = b
+            p_last[...] = scratch[0]
         return urk
 
     def rk_update(self, s, tau, met, un, urk, q, qsum):

@@ -30,7 +30,7 @@ int orc_define(orc_cfg *c)
     if (c->S != 5 && c->S != 9) return -1;
     if (c->time_order != 3 && c->time_order != 4) return -1;
     for (d = 0; d < 6; d++) {
-        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_OUTFLOW_RIEMANN) return -1;
+        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_OUTFLOW_RUDY) return -1;
         if ((c->bc[d] == ORC_BC_PERIODIC) != (c->bc[d ^ 1] == ORC_BC_PERIODIC)) return -1;
     }
     c->g = (c->S + 1) / 2;
@@ -652,6 +652,43 @@ static void bc_outflow_riemann(const orc_cfg *c, double *const met[10], double *
 /* One face: the ghost layers outside the face; tangential range all nodes for inflow/outflow
  * (src/BC.c:1344-1370, 2111-2140), interior for the manufactured solution
  * (src/ManufacturedSolution.c:202-338).  Mirror index: ghost StartReal-1-l <-> StartReal+l. */
+/* Cartesian velocity at ghost node n from the contravariant velocities th of its mirror node and
+   the ghost node's own metric rows: Cramer's rule for the 3x3 system, terms in the reference's
+   order (src/BC.c:2556-2558, 2633-2635) */
+static void velocity_from_theta(double *const met[10], long n, const double th[3], double *u, double *v,
+                                double *w)
+{
+    double xx = met[0][n], xy = met[1][n], xz = met[2][n], ex = met[3][n], ey = met[4][n], ez = met[5][n];
+    double zx = met[6][n], zy = met[7][n], zz = met[8][n];
+    double den = -xx * ey * zz + xx * ez * zy + xy * ex * zz - xy * ez * zx - xz * ex * zy + xz * ey * zx;
+    *u = -(th[0] * ey * zz - th[0] * ez * zy - th[1] * xy * zz + th[1] * xz * zy + th[2] * xy * ez - th[2] * xz * ey) / den;
+    *v = -(-th[0] * ex * zz + th[0] * ez * zx + th[1] * xx * zz - th[1] * xz * zx - th[2] * xx * ez + th[2] * xz * ex) / den;
+    *w = -(-th[0] * ex * zy + th[0] * ey * zx + th[1] * xx * zy - th[1] * xy * zx - th[2] * xx * ey + th[2] * xy * ex) / -den;
+}
+/* WriteOutflowSubsonicNormalBoundary (src/BC.c:2544-2570) and WriteOutflowSubsonicRudyBoundary
+   (src/BC.c:2624-2655): theta, rho, e, c, mu and T from the mirror node s; p = p_out (normal) or
+   relaxed from the step-start ghost state (Rudy); rho scaled by p/p_s (Rudy) */
+static void bc_outflow_subsonic(const orc_cfg *c, int rudy, double *const met[10], double *const prim[5],
+                                double *const der[8], long n, long s)
+{
+    double th[3] = {der[4][s], der[5][s], der[6][s]}, u, v, w, p;
+    velocity_from_theta(met, n, th, &u, &v, &w);
+    der[4][n] = th[0]; der[5][n] = th[1]; der[6][n] = th[2];
+    prim[1][n] = u; prim[2][n] = v; prim[3][n] = w;
+    if (rudy) {
+        double at = c->alpha_nonref * c->tau_step;
+        p = (c->p_last[n] + at * c->p_out + c->rho_last[n] * c->c_last[s] * (u - c->u_last[n]) / c->upsilon) / (1.0 + at);
+        der[0][n] = p;
+        prim[0][n] = p / der[0][s] * prim[0][s];
+    } else {
+        der[0][n] = c->p_out;
+        prim[0][n] = prim[0][s];
+    }
+    prim[4][n] = prim[4][s];
+    der[2][n] = der[2][s];
+    der[3][n] = der[3][s];
+    der[1][n] = der[1][s];
+}
 static void apply_face(const orc_cfg *c, int face, double *const met[10], double *const prim[5],
                        double *const der[8], double *const bs[6])
 {
@@ -676,6 +713,8 @@ static void apply_face(const orc_cfg *c, int face, double *const met[10], double
                 if (kind == ORC_BC_MMS) bc_mms(c, prim, der, bs, n);
                 else if (kind == ORC_BC_INFLOW_SUPERSONIC) bc_inflow_super(c, met, prim, der, n);
                 else if (kind == ORC_BC_OUTFLOW_RIEMANN) bc_outflow_riemann(c, met, prim, der, n, s);
+                else if (kind == ORC_BC_OUTFLOW_NORMAL) bc_outflow_subsonic(c, 0, met, prim, der, n, s);
+                else if (kind == ORC_BC_OUTFLOW_RUDY) bc_outflow_subsonic(c, 1, met, prim, der, n, s);
             }
     }
 }

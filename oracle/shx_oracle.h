@@ -47,6 +47,8 @@ extern "C" {
 #define ORC_BC_MMS 1               /* manufactured-solution Dirichlet (src/ManufacturedSolution.c:202-338) */
 #define ORC_BC_INFLOW_SUPERSONIC 2 /* src/BC.c:2505-2542 */
 #define ORC_BC_OUTFLOW_RIEMANN 3   /* src/BC.c:2572-2622 */
+#define ORC_BC_OUTFLOW_NORMAL 4    /* src/BC.c:2544-2570: p = p_out, rho, e, c from the mirror node */
+#define ORC_BC_OUTFLOW_RUDY 5      /* src/BC.c:2624-2655: non-reflecting relaxation towards p_out */
 
 typedef struct orc_cfg {
     /* inputs */
@@ -61,6 +63,13 @@ typedef struct orc_cfg {
     double eps_override; /* 0 -> reference epsilon 10*sqrt(DBL_MIN) */
     int bc[6];           /* ORC_BC_* per face; both faces of a direction periodic or neither */
     double rho_in, u_in, v_in, w_in, p_in;   /* boundaryconditions:{rho,p}_inflow, InitializeValues u0 (AoA) */
+    double p_out;        /* boundaryconditions:p_outflow (src/Import.c:94) */
+    double alpha_nonref; /* AlphaNonRef = 0.25 L0/u0 (src/Functions.c:1763) */
+    /* state at the start of the step, read by the Rudy writer (src/BC.c:2643-2646): the step's tau,
+       U_lastStep's stored p, rho and u, and U_lastStep's c (never recomputed after the
+       initialisation: CalcValues only runs on U_RK).  Unused unless a face is ORC_BC_OUTFLOW_RUDY. */
+    double tau_step;
+    const double *p_last, *rho_last, *u_last, *c_last;
     /* derived by orc_define */
     int g;
     int ng[3];

@@ -71,37 +71,95 @@ __global__ void k_ghost_periodic(Fields10 f, int nf, Geom gm)
 // One launch per non-periodic face, in the reference's face order, after CalcValues.  A thread
 // owns one ghost node of the face slab: layer l outside the face, tangential indices (i1, i2).
 // Its mirror node is layer l inside the face (StartReal + l, EndReal - l).
+// Cases with a subsonic outflow face keep the stored p and c of every node (P, C): CalcValues
+// values, overwritten by every face writer on its ghosts.  Those writers store p and c that the
+// equation of state does not give from rho,u,v,w,e, and the flux kernels read them there.
 struct BcArgs {
     Fields5 pr;              // stage input state; ghost rho,u,v,w,e are written
     gdp T, mu;               // derived T and mu (viscous runs; null when inviscid)
     gcdp bs[6];              // exact rho,u,v,w,e,p for manufactured faces
+    gcdp met[9];             // metric rows (subsonic outflow writers)
+    gdp P, C;                // stored p and c (null unless a face is a subsonic outflow)
+    gdp PL, RL, UL;          // Rudy: step-start ghost p, rho, u (written by face_save_body)
+    gcdp C0;                 // Rudy: sound speed of the uploaded state
     int kind, dir, upper, g;
     int t1, t2;              // tangential axes
     int lo1, n1, lo2, n2;    // tangential index ranges
     int first, last;         // first / last interior index along dir
     long st[3];
     double rho_in, u_in, v_in, w_in, p_in, cinf;
+    double p_out, alpha_tau; // subsonic outflow: p_out and AlphaNonRef * tau
     Consts k;
 };
 
-__device__ __forceinline__ void face_body(const BcArgs &a, const long t)
+// ghost node n and mirror node s of work item t of a face launch; false past the slab
+__device__ __forceinline__ bool face_node(const BcArgs &a, const long t, long &n, long &s)
 {
     const long per_layer = (long)a.n1 * a.n2;
-    if (t >= per_layer * a.g) return;
+    if (t >= per_layer * a.g) return false;
     const int l = (int)(t / per_layer);
     const long r = t - (long)l * per_layer;
     const int i1 = a.lo1 + (int)(r / a.n2), i2 = a.lo2 + (int)(r % a.n2);
     const int gi = a.upper ? a.last + 1 + l : a.first - 1 - l;
     const int si = a.upper ? a.last - l : a.first + l;
     const long base = (long)i1 * a.st[a.t1] + (long)i2 * a.st[a.t2];
-    const long n = base + (long)gi * a.st[a.dir], s = base + (long)si * a.st[a.dir];
+    n = base + (long)gi * a.st[a.dir];
+    s = base + (long)si * a.st[a.dir];
+    return true;
+}
+
+// Rudy faces at the start of a step: U_lastStep's ghost p, rho and u, which the writers of every
+// stage read (the first stage's writers overwrite the state buffer's ghosts)
+__device__ __forceinline__ void face_save_body(const BcArgs &a, const long t)
+{
+    long n, s;
+    if (!face_node(a, t, n, s)) return;
+    a.PL[n] = a.P[n];
+    a.RL[n] = a.pr.f[0][n];
+    a.UL[n] = a.pr.f[1][n];
+}
+
+__device__ __forceinline__ void face_body(const BcArgs &a, const long t)
+{
+    long n, s;
+    if (!face_node(a, t, n, s)) return;
     const Consts &k = a.k;
+    if (a.kind == SHX_BC_OUTFLOW_NORMAL || a.kind == SHX_BC_OUTFLOW_RUDY) {
+        // contravariant velocities of the mirror node (its CalcValues theta), then the Cartesian
+        // velocity that has them under the ghost node's metric rows: Cramer's rule, terms in the
+        // reference's order (src/BC.c:2556-2558)
+        const double us = a.pr.f[1][s], vs = a.pr.f[2][s], ws = a.pr.f[3][s];
+        const double t0 = us * a.met[0][s] + vs * a.met[1][s] + ws * a.met[2][s];
+        const double t1 = us * a.met[3][s] + vs * a.met[4][s] + ws * a.met[5][s];
+        const double t2 = us * a.met[6][s] + vs * a.met[7][s] + ws * a.met[8][s];
+        const double xx = a.met[0][n], xy = a.met[1][n], xz = a.met[2][n], ex = a.met[3][n], ey = a.met[4][n],
+                     ez = a.met[5][n], zx = a.met[6][n], zy = a.met[7][n], zz = a.met[8][n];
+        const double den = -xx * ey * zz + xx * ez * zy + xy * ex * zz - xy * ez * zx - xz * ex * zy + xz * ey * zx;
+        const double u = -(t0 * ey * zz - t0 * ez * zy - t1 * xy * zz + t1 * xz * zy + t2 * xy * ez - t2 * xz * ey) / den;
+        const double v = -(-t0 * ex * zz + t0 * ez * zx + t1 * xx * zz - t1 * xz * zx - t2 * xx * ez + t2 * xz * ex) / den;
+        const double w = -(-t0 * ex * zy + t0 * ey * zx + t1 * xx * zy - t1 * xy * zx - t2 * xx * ey + t2 * xy * ex) / -den;
+        double p, rho;
+        if (a.kind == SHX_BC_OUTFLOW_RUDY) {
+            p = (a.PL[n] + a.alpha_tau * a.p_out + a.RL[n] * a.C0[s] * (u - a.UL[n]) / k.upsilon) / (1.0 + a.alpha_tau);
+            rho = p / a.P[s] * a.pr.f[0][s];
+        } else {
+            p = a.p_out;
+            rho = a.pr.f[0][s];
+        }
+        a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
+        a.pr.f[4][n] = a.pr.f[4][s];
+        a.P[n] = p;
+        a.C[n] = a.C[s];
+        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        return;
+    }
     if (a.kind == SHX_BC_MANUFACTURED) {
         // WriteManufacturedSolution: exact state, T = |p/rho|, mu = 1 (c, p and theta are
         // recomputed from rho,u,v,w,e where the flux kernels need them)
         const double rho = a.bs[0][n];
         for (int m = 0; m < 5; m++) a.pr.f[m][n] = a.bs[m][n];
         if (a.T) { a.T[n] = fabs(a.bs[5][n] / rho); a.mu[n] = 1.0; }
+        if (a.P) { a.P[n] = a.bs[5][n]; a.C[n] = sqrt(k.upsilon * k.gamma * a.bs[5][n] / rho); }
     } else if (a.kind == SHX_BC_INFLOW_SUPERSONIC) {
         const double rho = a.rho_in, u = a.u_in, v = a.v_in, w = a.w_in, p = a.p_in;
         a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
@@ -110,11 +168,12 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
             a.T[n] = a.p_in / a.rho_in;
             a.mu[n] = ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
         }
+        if (a.P) { a.P[n] = p; a.C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
     } else {
         // Riemann invariants: outgoing from the mirror node, incoming from the inflow state
         const double rs = a.pr.f[0][s], us = a.pr.f[1][s], vs = a.pr.f[2][s], ws = a.pr.f[3][s];
-        const double ps = pressure(k, rs, us, vs, ws, a.pr.f[4][s]);
-        const double cii = sqrt(k.upsilon * k.gamma * ps / rs);
+        const double ps = a.P ? a.P[s] : pressure(k, rs, us, vs, ws, a.pr.f[4][s]);
+        const double cii = a.P ? a.C[s] : sqrt(k.upsilon * k.gamma * ps / rs);
         const double u = 0.5 * (us + a.u_in) - (a.cinf - cii) / (k.gamma - 1.);
         const double d1 = (0.5 * (cii + a.cinf) - 0.25 * (k.gamma - 1.) * (a.u_in - us)) / cii;
         const double rho = rs * pow(d1, 2. / (k.gamma - 1.));
@@ -122,7 +181,12 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = vs; a.pr.f[3][n] = ws;
         a.pr.f[4][n] = (0.5 * ((u * u) + (vs * vs) + (ws * ws)) + p / rho / (k.gamma - 1.0) * k.upsilon);
         if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        if (a.P) { a.P[n] = p; a.C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
     }
+}
+__global__ void k_face_save(BcArgs a)
+{
+    face_save_body(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 __global__ void k_face_bc(BcArgs a)
 {
@@ -150,21 +214,25 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
     carry_body(from, to, gm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes.
-__device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const Geom &gm,
-                                            const Consts &k, const long t)
+// T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes
+// (viscous runs), and the stored p and c (cases with a subsonic outflow face).
+__device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const gdp P,
+                                            const gdp C, const Geom &gm, const Consts &k, const long t)
 {
     if (t >= gm.ntot) return;
     int idx[3];
     const long n = padded_node(gm, t, idx);
     const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
     const double p = pressure(k, rho, u, v, w, e);
-    T[n] = fabs(p / rho);
-    mu[n] = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+    if (T) {
+        T[n] = fabs(p / rho);
+        mu[n] = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+    }
+    if (P) { P[n] = p; C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
 }
-__global__ void k_calc_values(CFields5 pr, double *T, double *mu, Geom gm, Consts k)
+__global__ void k_calc_values(CFields5 pr, gdp T, gdp mu, gdp P, gdp C, Geom gm, Consts k)
 {
-    values_body(pr, T, mu, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    values_body(pr, T, mu, P, C, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- WENO sweep ---------------------------------------------------------------------------
@@ -173,6 +241,8 @@ struct WenoArgs {
     gcdp m0, m1, m2, J;               // metric row of the sweep direction and the Jacobian
                                       // (diagonal metrics: m0 is the row's diagonal entry)
     Fields5 q;            // stage residual, interior nodes
+    gcdp P, C;            // stored p and c, read on the ghost nodes of subsonic outflow faces
+    int pc_lo, pc_hi;     // the lower / upper face of the sweep direction is a subsonic outflow
     long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
     int nD, nA, nB;       // interior counts
     int oD, oA, oB;       // padded index of the first interior node per axis
@@ -228,9 +298,10 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         const double rho = a.pr.f[0][n], u = a.pr.f[1][n], v = a.pr.f[2][n], w = a.pr.f[3][n],
                      e = a.pr.f[4][n];
         const double J = a.J[n];
-        const double p = pressure_fast(k, rho, u, v, w, e);
+        double p = pressure_fast(k, rho, u, v, w, e);
         const double ir = frcp(rho);
-        const double cs = sqrt(k.upsilon * k.gamma * p * ir);
+        double cs = sqrt(k.upsilon * k.gamma * p * ir);
+        if ((a.pc_lo && dp < a.oD) || (a.pc_hi && dp >= a.oD + a.nD)) { p = a.P[n]; cs = a.C[n]; }
         const double pu = p * k.upsilon;
         const int o = aa * WD + dd;
         if constexpr (AX < 0) {
@@ -1062,7 +1133,7 @@ __device__ __forceinline__ void batch_walk(const int total, F &&f)
 }
 
 struct GhostArgs { Fields10 f; int nf; Geom gm; };
-struct ValuesArgs { CFields5 pr; gdp T, mu; Geom gm; Consts k; };
+struct ValuesArgs { CFields5 pr; gdp T, mu, P, C; Geom gm; Consts k; };
 template <bool P>
 __global__ __launch_bounds__(256) void k_ghost_batch(BATCH_PARAMS(GhostArgs))
 {
@@ -1078,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_values_batch(BATCH_PARAMS(ValuesArgs))
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
         const ValuesArgs &a = rec(A, w.x);
-        values_body(a.pr, a.T, a.mu, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
+        values_body(a.pr, a.T, a.mu, a.P, a.C, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
     });
 }
 template <bool P>
@@ -1095,6 +1166,14 @@ __global__ __launch_bounds__(256) void k_face_batch(BATCH_PARAMS(BcArgs))
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
         face_body(rec(A, w.x), (long)w.y * 256 + threadIdx.x);
+    });
+}
+template <bool P>
+__global__ __launch_bounds__(256) void k_face_save_batch(BATCH_PARAMS(BcArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        face_save_body(rec(A, w.x), (long)w.y * 256 + threadIdx.x);
     });
 }
 struct CarryArgs { CFields5 from; Fields5 to; Geom gm; };
@@ -1319,6 +1398,9 @@ struct shx_case {
     int cur;
     double *q[5], *qsum[5], *met[10], *grad[12], *src[5], *T, *mu;
     double *bs[6];         // exact state for manufactured faces
+    double *P, *C;         // stored p and c (pc: some face is a subsonic outflow)
+    double *PL, *RL, *UL, *C0;   // Rudy faces: step-start ghost p, rho, u; uploaded sound speed
+    int pc, rudy;
     int has_src, has_metric, has_state, has_bs;
     int faces;             // number of non-periodic faces
     int needs_bs;          // some face is SHX_BC_MANUFACTURED
@@ -1398,13 +1480,15 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
         return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
-        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_OUTFLOW_RIEMANN)
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_OUTFLOW_RUDY)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
         if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
             return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
         if ((d.bc[f] == SHX_BC_INFLOW_SUPERSONIC || d.bc[f] == SHX_BC_OUTFLOW_RIEMANN) &&
             !(d.rho_inflow > 0.0 && d.p_inflow > 0.0))
             return set_err(SHX_ERR_ARG, "face %d: inflow/outflow faces need rho_inflow > 0 and p_inflow > 0", f);
+        if ((d.bc[f] == SHX_BC_OUTFLOW_NORMAL || d.bc[f] == SHX_BC_OUTFLOW_RUDY) && !(d.p_out > 0.0 && d.alpha_nonref >= 0.0))
+            return set_err(SHX_ERR_ARG, "face %d: subsonic outflow faces need p_out > 0 and alpha_nonref >= 0", f);
     }
     const int S = d.space_order, g = (S + 1) / 2;
     for (int a = 0; a < d.dims; a++)
@@ -1426,6 +1510,11 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         if (live && !gm.per[a]) {
             c->faces += 2;
             if (d.bc[2 * a] == SHX_BC_MANUFACTURED || d.bc[2 * a + 1] == SHX_BC_MANUFACTURED) c->needs_bs = 1;
+            for (int u = 0; u < 2; u++) {
+                const int kind = d.bc[2 * a + u];
+                if (kind == SHX_BC_OUTFLOW_NORMAL || kind == SHX_BC_OUTFLOW_RUDY) c->pc = 1;
+                if (kind == SHX_BC_OUTFLOW_RUDY) c->rudy = 1;
+            }
         }
     }
     // rows of the contiguous axis are padded to 16 doubles (128 B) so tiles start on cache lines
@@ -1492,6 +1581,8 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     for (int m = 0; m < 12; m++) all.push_back(&c->grad[m]);
     all.push_back(&c->T);
     all.push_back(&c->mu);
+    if (c->pc) { all.push_back(&c->P); all.push_back(&c->C); }
+    if (c->rudy) { all.push_back(&c->PL); all.push_back(&c->RL); all.push_back(&c->UL); all.push_back(&c->C0); }
     for (double **p : all) {
         e = hipMalloc((void **)p, bytes);
         if (e == hipSuccess) e = hipMemsetAsync(*p, 0, bytes, c->stream);
@@ -1519,6 +1610,7 @@ extern "C" int shx_destroy(shx_case *c)
     for (int m = 0; m < 12; m++) hipFree(c->grad[m]);
     for (int m = 0; m < 6; m++) hipFree(c->bs[m]);
     hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt);
+    hipFree(c->P); hipFree(c->C); hipFree(c->PL); hipFree(c->RL); hipFree(c->UL); hipFree(c->C0);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1569,8 +1661,9 @@ static int ghost_fill(shx_case *c, double *const *f, int nf)
 }
 
 // face writers of every non-periodic face, in face order (i-, i+, j-, j+, k-, k+)
-// argument record of the writer of face f (f = 2 * direction + upper); nodes: ghost nodes it owns
-static void face_args(const shx_case *c, double *const *pr, int f, BcArgs &a, long &nodes)
+// argument record of the writer of face f (f = 2 * direction + upper) in a step of length tau;
+// nodes: ghost nodes it owns
+static void face_args(const shx_case *c, double *const *pr, int f, double tau, BcArgs &a, long &nodes)
 {
     const Geom &gm = c->gm;
     const int kind = c->d.bc[f];
@@ -1595,10 +1688,15 @@ static void face_args(const shx_case *c, double *const *pr, int f, BcArgs &a, lo
     a.rho_in = c->d.rho_inflow; a.u_in = c->d.u_inflow; a.v_in = c->d.v_inflow;
     a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
     a.cinf = 1.0 / c->d.mach;
+    for (int m = 0; m < 9; m++) a.met[m] = c->met[m];
+    a.P = c->P; a.C = c->C;
+    a.PL = c->PL; a.RL = c->RL; a.UL = c->UL; a.C0 = c->C0;
+    a.p_out = c->d.p_out;
+    a.alpha_tau = c->d.alpha_nonref * tau;
     a.k = c->k;
     nodes = (kind == SHX_BC_PERIODIC) ? 0 : (long)a.n1 * a.n2 * a.g;
 }
-static int face_fill(shx_case *c, double *const *pr)
+static int face_fill(shx_case *c, double *const *pr, double tau)
 {
     if (!c->faces) return SHX_OK;
     int rc = tbeg(c, KT_FACES);
@@ -1606,12 +1704,26 @@ static int face_fill(shx_case *c, double *const *pr)
     for (int f = 0; f < 2 * c->gm.dims; f++) {
         BcArgs a;
         long nodes;
-        face_args(c, pr, f, a, nodes);
+        face_args(c, pr, f, tau, a, nodes);
         if (nodes == 0) continue;
         k_face_bc<<<nblk(nodes, 256), 256, 0, c->stream>>>(a);
         LAUNCH_CHECK();
     }
     return tend(c);
+}
+// start of a step on state pr: keep the ghost p, rho and u of every Rudy face
+static int rudy_save(shx_case *c, double *const *pr)
+{
+    if (!c->rudy) return SHX_OK;
+    for (int f = 0; f < 2 * c->gm.dims; f++) {
+        if (c->d.bc[f] != SHX_BC_OUTFLOW_RUDY) continue;
+        BcArgs a;
+        long nodes;
+        face_args(c, pr, f, 0.0, a, nodes);
+        k_face_save<<<nblk(nodes, 256), 256, 0, c->stream>>>(a);
+        LAUNCH_CHECK();
+    }
+    return SHX_OK;
 }
 static int carry_ghosts(shx_case *c, double *const *from, double *const *to)
 {
@@ -1692,6 +1804,15 @@ extern "C" int shx_upload_state(shx_case *c, const double *const prim[5])
         for (int b = 0; b < 3; b++)
             if (b != c->cur)
                 HIPCHK(hipMemcpyAsync(c->prim[b][m], c->prim[c->cur][m], bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (c->pc) {
+        // stored p and c of the uploaded state on every node; for Rudy faces also the step-start
+        // sound speed, which stays as uploaded (the reference never recomputes U_lastStep's c)
+        CFields5 cp;
+        for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+        k_calc_values<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(cp, nullptr, nullptr, c->P, c->C, c->gm, c->k);
+        LAUNCH_CHECK();
+        if (c->rudy) HIPCHK(hipMemcpyAsync(c->C0, c->C, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_state = 1;
@@ -1816,6 +1937,9 @@ static WenoArgs weno_args(const shx_case *c, double *const *pr, int dir, int fir
     a.first = first;
     a.coalesce_d = (a.sD == 1);
     a.alignA = (a.sA == 1);
+    a.P = c->P; a.C = c->C;
+    a.pc_lo = (c->d.bc[2 * dir] == SHX_BC_OUTFLOW_NORMAL || c->d.bc[2 * dir] == SHX_BC_OUTFLOW_RUDY);
+    a.pc_hi = (c->d.bc[2 * dir + 1] == SHX_BC_OUTFLOW_NORMAL || c->d.bc[2 * dir + 1] == SHX_BC_OUTFLOW_RUDY);
     a.k = c->k;
     return a;
 }
@@ -1851,21 +1975,30 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
 // src/Functions.c:2161-2272)
 static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk);
 // rk: when given, the last viscous sweep applies this stage update instead of storing Q
-static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
+static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double tau)
 {
     const Geom &gm = c->gm;
     int rc;
     if ((rc = ghost_fill(c, pr, 5))) return rc;
+    if (c->d.inviscid && c->pc) {
+        // stored p and c for the subsonic outflow writers and the sweeps
+        CFields5 cp;
+        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+        if ((rc = tbeg(c, KT_VALUES))) return rc;
+        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, nullptr, nullptr, c->P, c->C, gm, c->k);
+        LAUNCH_CHECK();
+        if ((rc = tend(c))) return rc;
+    }
     if (!c->d.inviscid) {
         CFields5 cp;
         Fields12 gf;
         for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
         for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         if ((rc = tbeg(c, KT_VALUES))) return rc;
-        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, gm, c->k);
+        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, c->P, c->C, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
-        if ((rc = face_fill(c, pr))) return rc;
+        if ((rc = face_fill(c, pr, tau))) return rc;
         if ((rc = tbeg(c, KT_GRAD))) return rc;
         if (gm.dims == 3) {
             constexpr int CI = 32;
@@ -1879,7 +2012,7 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk)
         }
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
-    } else if ((rc = face_fill(c, pr))) {
+    } else if ((rc = face_fill(c, pr, tau))) {
         return rc;
     }
     for (int dir = 0; dir < gm.dims; dir++)
@@ -2017,10 +2150,11 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
     const int un = c->cur;
     const int bx = (un + 1) % 3, by = (un + 2) % 3;
     int in = un, outb = bx;
+    if ((rc = rudy_save(c, c->prim[un]))) return rc;
     for (int s = 0; s < c->d.time_order; s++) {
         const RkArgs rk = rk_args(c, c->prim[in], c->prim[outb], s, tau);
         // viscous stages apply the update in their last sweep; inviscid ones in k_rk_update
-        if ((rc = stage_fluxes(c, c->prim[in], c->d.inviscid ? nullptr : &rk))) return rc;
+        if ((rc = stage_fluxes(c, c->prim[in], c->d.inviscid ? nullptr : &rk, tau))) return rc;
         if ((rc = carry_ghosts(c, c->prim[in], c->prim[outb]))) return rc;
         if (c->d.inviscid && (rc = rk_update(c, rk))) return rc;
         // stage s+1 reads what stage s wrote and writes the other scratch buffer
@@ -2144,6 +2278,23 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             out[i] = g[i]->prim[ob];
             rk[i] = rk_args(g[i], g[i]->prim[ib], g[i]->prim[ob], s, tau[i]);
         }
+        if (s == 0) {   // start of the step: step-start ghost state of every Rudy face
+            for (int f = 0; f < 2 * dims; f++) {
+                std::vector<char> recs;
+                std::vector<dim3> grids;
+                long any = 0;
+                for (int i = 0; i < nc; i++) {
+                    BcArgs a;
+                    long nodes;
+                    face_args(g[i], in[i], f, tau[i], a, nodes);
+                    if (g[i]->d.bc[f] != SHX_BC_OUTFLOW_RUDY) nodes = 0;
+                    put(recs, a);
+                    grids.push_back(dim3(nblk(nodes, 256)));
+                    any += nodes;
+                }
+                if (any) batch_add(b, (const void *)&k_face_save_batch<false>, (const void *)&k_face_save_batch<true>, 256, BK_GHOST, recs, sizeof(BcArgs), grids);
+            }
+        }
         {   // periodic ghost fill of the stage input
             std::vector<char> recs;
             std::vector<dim3> grids;
@@ -2167,7 +2318,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 for (int i = 0; i < nc; i++) {
                     BcArgs a;
                     long nodes;
-                    face_args(g[i], in[i], f, a, nodes);
+                    face_args(g[i], in[i], f, tau[i], a, nodes);
                     put(recs, a);
                     grids.push_back(dim3(nblk(nodes, 256)));
                     any += nodes;
@@ -2175,7 +2326,24 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 if (any) batch_add(b, (const void *)&k_face_batch<false>, (const void *)&k_face_batch<true>, 256, BK_GHOST, recs, sizeof(BcArgs), grids);
             }
         };
-        if (c0->d.inviscid) add_faces();
+        if (c0->d.inviscid) {
+            // stored p and c of the cases with subsonic outflow faces
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            long any = 0;
+            for (int i = 0; i < nc; i++) {
+                ValuesArgs a;
+                memset(&a, 0, sizeof a);
+                for (int m = 0; m < 5; m++) a.pr.f[m] = in[i][m];
+                a.P = g[i]->P; a.C = g[i]->C; a.gm = g[i]->gm; a.k = g[i]->k;
+                put(recs, a);
+                const long nodes = g[i]->pc ? g[i]->gm.ntot : 0;
+                grids.push_back(dim3(nblk(nodes, 256)));
+                any += nodes;
+            }
+            if (any) batch_add(b, (const void *)&k_values_batch<false>, (const void *)&k_values_batch<true>, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
+            add_faces();
+        }
         if (!c0->d.inviscid) {
             std::vector<char> recs, grecs;
             std::vector<dim3> grids, ggrids;
@@ -2184,7 +2352,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 ValuesArgs a;
                 memset(&a, 0, sizeof a);
                 for (int m = 0; m < 5; m++) a.pr.f[m] = in[i][m];
-                a.T = c->T; a.mu = c->mu; a.gm = c->gm; a.k = c->k;
+                a.T = c->T; a.mu = c->mu; a.P = c->P; a.C = c->C; a.gm = c->gm; a.k = c->k;
                 put(recs, a);
                 grids.push_back(dim3(nblk(c->gm.ntot, 256)));
                 GradArgs ga;
@@ -2594,7 +2762,9 @@ extern "C" int shx_rhs(shx_case *c, double *const q[5])
     if (!q) return set_err(SHX_ERR_ARG, "null output");
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    if ((rc = stage_fluxes(c, c->prim[c->cur], nullptr))) return rc;
+    // a Rudy face takes the current state as its step-start state, with tau = 0
+    if ((rc = rudy_save(c, c->prim[c->cur]))) return rc;
+    if ((rc = stage_fluxes(c, c->prim[c->cur], nullptr, 0.0))) return rc;
     if (c->has_src) {
         Fields5 qf;
         CFields5 sf;

@@ -1,4 +1,5 @@
-"""GPU parity of the boundary writers and the manufactured-solution path: libshockx.so against
+"""GPU parity of the boundary writers (supersonic inflow; Riemann, normal and Rudy subsonic
+outflow; manufactured-solution faces) and the manufactured-solution path: libshockx.so against
 the oracle on identical inputs (same bar as the periodic parity tests: relative L-infinity per
 conserved variable <= 1e-12 after one step, <= 1e-10 after many).
 
@@ -24,6 +25,9 @@ pytestmark = pytest.mark.gpu
 
 GAMMA, MACH = 1.4, 0.5
 INOUT = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN)
+IN_NORMAL = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_NORMAL)
+IN_RUDY = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RUDY)
+OUTFLOW = (0.95, 3.0)          # p_out, AlphaNonRef
 
 
 def _noise(o, dims, n, met, prim, steps, tau=0.0, src=None, bstate=None, trials=3):
@@ -57,14 +61,14 @@ def _report(tag, errs):
 def _setup(dims, S, n, bc, inflow, inviscid=False, quirks=orc.QUIRKS_REFERENCE, mms_on=False,
            mapping=None, reynolds=1000.0):
     o = orc.Oracle(dims=dims, S=S, n=n, bc=bc, inflow=inflow, mach=MACH, reynolds=reynolds,
-                   inviscid=int(inviscid), quirks=quirks, mms=int(mms_on))
+                   inviscid=int(inviscid), quirks=quirks, mms=int(mms_on), outflow=OUTFLOW)
     per = tuple(bc[2 * d] == orc.BC_PERIODIC for d in range(3))
     xe, ye, ze = cases.ext_coords(dims, S, n, mapping or cases.cartesian_unit(dims), periodic=per)
     met = o.metric_from_coords(xe, ye, ze)
     o.ghost_periodic(met)
     g = shockx.Case(dims=dims, space_order=S, time_order=4, n=n, gamma=GAMMA, mach=MACH,
                     reynolds=reynolds, inviscid=inviscid, manufactured=mms_on, quirks=quirks,
-                    bc=bc, inflow=inflow)
+                    bc=bc, inflow=inflow, outflow=OUTFLOW)
     g.build_metric(xe, ye, ze)
     return o, g, met, (xe, ye, ze)
 
@@ -89,6 +93,11 @@ CASES = [
     (3, 5, (14, 12, 12), INOUT + (0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
     (3, 9, (14, 12, 12), INOUT + (0, 0, 0, 0), True, orc.QUIRKS_REFERENCE, "bump"),
     (3, 5, (12, 12, 12), INOUT + INOUT + INOUT, False, 0, "bump"),
+    # subsonic outflow writers (stored p and c on their ghosts)
+    (2, 5, (24, 20, 1), IN_NORMAL + (0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (2, 9, (24, 20, 1), IN_RUDY + IN_NORMAL + (0, 0), True, orc.QUIRKS_REFERENCE, None),
+    (3, 5, (14, 12, 12), IN_RUDY + (0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    (3, 5, (12, 12, 12), IN_NORMAL + IN_RUDY + INOUT, False, 0, "bump"),
 ]
 
 

@@ -8,6 +8,11 @@ Checks (CPU only):
     invariant u_inflow - 2/(Ma (gamma-1)), the mirror's tangential velocity, T and mu
   * manufactured-solution faces (src/ManufacturedSolution.c:202-338): exact state on the ghost
     layers of the interior tangential range only
+  * subsonic outflow, normal (src/BC.c:2544-2570) and Rudy (src/BC.c:2624-2655): the ghost
+    velocity solves the ghost node's metric system for the mirror's contravariant velocities
+    (checked with a linear solve, not Cramer's rule); p = p_out (normal) or the relaxation of the
+    step-start ghost pressure towards p_out (Rudy, checked against the formula evaluated here);
+    rho, e, c, T and mu from the mirror (Rudy: rho scaled by p / p_mirror)
   * face order (src/BC.c:13-74): a later face overwrites an earlier one at shared edges
   * a uniform stream at the inflow state passes an inflow/outflow block unchanged
   * the manufactured source equals a symbolic (sympy) residual of the conservation laws, and
@@ -112,6 +117,55 @@ def test_riemann_outflow_carries_the_characteristic_invariants(dims):
         assert np.allclose(der[0][gh], pg, rtol=1e-13)
 
 
+def _ghost_velocity_by_solve(met, gh, theta):
+    """u, v, w at the ghost slab from the ghost metric rows and the mirror's theta (linear solve)"""
+    M = np.stack([np.stack([met[3 * r + col][gh] for col in range(3)], axis=-1) for r in range(3)], axis=-2)
+    th = np.stack(theta, axis=-1)[..., None]
+    return np.moveaxis(np.linalg.solve(M, th)[..., 0], -1, 0)
+
+
+@pytest.mark.parametrize("dims,rudy", [(2, False), (3, False), (2, True), (3, True)])
+def test_subsonic_outflow_writers(dims, rudy):
+    n = (9, 6, 5) if dims == 3 else (9, 6, 1)
+    kind = orc.BC_OUTFLOW_RUDY if rudy else orc.BC_OUTFLOW_NORMAL
+    bc = (orc.BC_INFLOW_SUPERSONIC, kind, 0, 0, 0, 0)
+    p_out, alpha, tau = 0.83, 2.0, 0.05
+    o, met, _ = _case(dims, 5, n, bc, mapping=cases.sine_bump_3d(0.05) if dims == 3 else None,
+                      quirks=0, outflow=(p_out, alpha))
+    prim = _random_state(o, 21)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    before = [a.copy() for a in prim]
+    last = _random_state(o, 22)                               # step-start state (U_lastStep)
+    rng = np.random.default_rng(23)
+    p_last, c_last = 1.0 + 0.1 * rng.random(o.shape), 2.0 + 0.1 * rng.random(o.shape)
+    o._step_start(tau, met, last, p_last, c_last)
+    o.apply_bc(met, prim, der)
+    g, ups = o.g, 1.0 / (GAMMA * MACH * MACH)
+    end = g + n[0] - 1
+    for l in range(g):
+        gh, mi = (end + 1 + l, slice(None), slice(None)), (end - l, slice(None), slice(None))
+        theta = [der0[4 + d][mi] for d in range(3)]
+        for d in range(3):
+            assert np.array_equal(der[4 + d][gh], theta[d])
+        vel = _ghost_velocity_by_solve(met, gh, theta)
+        for d in range(3 if dims == 3 else 2):
+            assert np.allclose(prim[1 + d][gh], vel[d], rtol=1e-12, atol=1e-14)
+        if rudy:
+            at = alpha * tau
+            p = (p_last[gh] + at * p_out + last[0][gh] * c_last[mi] * (prim[1][gh] - last[1][gh]) / ups) / (1 + at)
+            assert np.allclose(der[0][gh], p, rtol=1e-14, atol=0)
+            assert np.allclose(prim[0][gh], der[0][gh] / der0[0][mi] * before[0][mi], rtol=1e-14, atol=0)
+        else:
+            assert np.all(der[0][gh] == p_out)
+            assert np.array_equal(prim[0][gh], before[0][mi])
+        assert np.array_equal(prim[4][gh], before[4][mi])
+        for f in (1, 2, 3):                                   # T, c, mu from the mirror
+            assert np.array_equal(der[f][gh], der0[f][mi])
+    sl = interior(o, dims, n)
+    assert np.array_equal(prim[0][sl], before[0][sl])
+
+
 def test_manufactured_faces_fill_interior_tangential_range_only():
     n = (7, 8, 6)
     o, met, _ = _case(3, 5, n, bc=(orc.BC_MMS,) * 6, mms=1)
@@ -150,14 +204,19 @@ def test_later_faces_overwrite_shared_edges():
         assert np.all(prim[m][corner] == val)
 
 
-@pytest.mark.parametrize("dims,S,inviscid", [(2, 5, False), (3, 5, False), (2, 9, True)])
-def test_uniform_stream_through_inflow_outflow_block(dims, S, inviscid):
+@pytest.mark.parametrize("dims,S,inviscid,outflow", [(2, 5, False, orc.BC_OUTFLOW_RIEMANN),
+                                                     (3, 5, False, orc.BC_OUTFLOW_RIEMANN),
+                                                     (2, 9, True, orc.BC_OUTFLOW_RIEMANN),
+                                                     (2, 5, False, orc.BC_OUTFLOW_NORMAL),
+                                                     (3, 5, False, orc.BC_OUTFLOW_RUDY)])
+def test_uniform_stream_through_inflow_outflow_block(dims, S, inviscid, outflow):
     # p = rho = 1 makes c = 1/Ma, the Riemann far-field sound speed: the outflow ghost equals
-    # its mirror and the stream is a steady solution of the discrete operator
+    # its mirror and the stream is a steady solution of the discrete operator; the normal and
+    # Rudy writers with p_out = p keep the stream as well
     n = (12, 10, 8) if dims == 3 else (12, 10, 1)
     inflow = (1.0, 0.7, 0.0, 0.0, 1.0)
-    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
-    o, met, _ = _case(dims, S, n, bc, inflow=inflow, inviscid=int(inviscid))
+    bc = (orc.BC_INFLOW_SUPERSONIC, outflow, 0, 0, 0, 0)
+    o, met, _ = _case(dims, S, n, bc, inflow=inflow, inviscid=int(inviscid), outflow=(1.0, 3.0))
     prim = o.zeros(5)
     prim[0][...] = 1.0
     prim[1][...] = 0.7
