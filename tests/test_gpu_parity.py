@@ -107,7 +107,7 @@ def test_one_step_3d(S):
     assert max(errs) <= TOL_STEP
 
 
-def test_hundred_steps_config0():
+def test_hundred_steps_2d_64_mms():
     """2D 64x64 periodic manufactured field, WENO5 + viscous, RK4, fixed tau, 100 steps."""
     o, coords, met, prim, nn = setup_2d_mms(n=64, S=5)
     tau = o.cfl_dt(0.5, met, prim)
@@ -119,7 +119,7 @@ def test_hundred_steps_config0():
         out = g.download_state()
     sl = interior(o, 2, nn)
     errs = rel_linf(conserved(out), conserved(ref), sl)
-    _report("config0 100 steps", errs)
+    _report("2D 64x64 periodic MMS field, 100 steps", errs)
     assert max(errs[:3] + errs[4:]) <= TOL_100
 
 
