@@ -235,6 +235,34 @@ __global__ void k_calc_values(CFields5 pr, gdp T, gdp mu, gdp P, gdp C, Geom gm,
     values_body(pr, T, mu, P, C, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ---- block placement ------------------------------------------------------------------------
+// Blocks are dispatched round-robin over the eight XCDs (block p on XCD p % 8), each with its own
+// L2.  xcd_order hands XCD x a contiguous range of tiles in dispatch order, so tiles next to each
+// other in the tile order (which share halo nodes) run on one L2 at about the same time.
+__device__ __forceinline__ int xcd_order(const int p, const int total)
+{
+    const int per = total >> 3, rem = total & 7, x = p & 7, s = p >> 3;
+    return x * per + min(x, rem) + s;
+}
+// Tile of block p; tg = (A tiles, B tiles, sweep tiles, order).  order 1: sweep tiles fastest,
+// then A, then B, placed by xcd_order (neighbours along the sweep share halo nodes in one L2);
+// order 0: A fastest in launch order, which keeps the row segments that consecutive blocks read
+// adjacent in memory (measured faster where the sweep axis is the slowest one in memory).
+__device__ __forceinline__ void sweep_tile(const int p, const int tg[4], int &bx, int &by, int &bz)
+{
+    if (tg[3] == 0) {
+        bx = p % tg[0];
+        by = (p / tg[0]) % tg[1];
+        bz = p / (tg[0] * tg[1]);
+        return;
+    }
+    const int q = xcd_order(p, tg[0] * tg[1] * tg[2]);
+    bz = q % tg[2];
+    const int r = q / tg[2];
+    bx = r % tg[0];
+    by = r / tg[0];
+}
+
 // ---- WENO sweep ---------------------------------------------------------------------------
 struct WenoArgs {
     CFields5 pr;          // rho, u, v, w, e of the stage input (ghosts filled)
@@ -243,6 +271,7 @@ struct WenoArgs {
     Fields5 q;            // stage residual, interior nodes
     gcdp P, C;            // stored p and c, read on the ghost nodes of subsonic outflow faces
     int pc_lo, pc_hi;     // the lower / upper face of the sweep direction is a subsonic outflow
+    int tg[4];            // tile grid (A tiles, B tiles, sweep tiles) and order of a single-case launch
     long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
     int nD, nA, nB;       // interior counts
     int oD, oA, oB;       // padded index of the first interior node per axis
@@ -532,7 +561,9 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
 template <int S, int TA, int TDI, int AX, bool MIX>
 __global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
 {
-    weno_body<S, TA, TDI, AX, MIX>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, a.tg, bx, by, bz);
+    weno_body<S, TA, TDI, AX, MIX>(a, bx, by, bz);
 }
 
 // ---- viscous fluxes: face-based LDS sweep ------------------------------------------------
@@ -595,6 +626,7 @@ struct ViscArgs {
     int three;
     int bug;                      // zeta stencil k offset taken from j (DIR = 2 only)
     int fuse_rk;                  // last sweep of the stage: apply the stage update (rk)
+    int tg[4];                    // tile grid and order of a single-case launch
     RkArgs rk;
     Consts k;
 };
@@ -931,7 +963,9 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
 template <int S, int DIR, int TA, int TDI, bool DIAG>
 __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
 {
-    visc_body<S, DIR, TA, TDI, DIAG>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, a.tg, bx, by, bz);
+    visc_body<S, DIR, TA, TDI, DIAG>(a, bx, by, bz);
 }
 
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
@@ -1003,7 +1037,7 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
 // of 2g+1 planes per thread (every plane is read from HBM once per chunk), the j and k derivatives
 // from the current plane staged in LDS with a g-wide halo.  Outputs as k_visc_grad; the T
 // derivatives only when `tgrad` (the diagonal-metric viscous sweeps never read them).
-constexpr int GK = 64, GJ = 4;
+constexpr int GK = 64, GJ = 16;
 struct GradArgs {
     CFields5 pr;
     gcdp T;
@@ -1011,6 +1045,7 @@ struct GradArgs {
     Geom gm;
     Consts k;
     int ci, tgrad;
+    int tg[4];                    // tile grid (k columns, j columns, i chunks) of a single-case launch
 };
 template <int S>
 __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
@@ -1096,7 +1131,9 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
 template <int S>
 __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
 {
-    march_body<S>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, a.tg, bx, by, bz);
+    march_body<S>(a, bx, by, bz);
 }
 
 // ---- batched launches ----------------------------------------------------------------------------
@@ -1959,14 +1996,23 @@ static int launch_pick(const void *fn, dim3 grid, int threads, void *arg, hipStr
 }
 
 // one WENO sweep along axis dir of the state `pr`
+// one-dimensional launch of a tile grid g; tg receives its extents and the tile order
+static dim3 flat_grid(const dim3 g, int tg[4], int order)
+{
+    tg[0] = (int)g.x; tg[1] = (int)g.y; tg[2] = (int)g.z; tg[3] = order;
+    return dim3(g.x * g.y * g.z);
+}
+// tile order of a sweep (see sweep_tile): along the sweep, except the viscous sweep of the
+// slowest axis
+static int sweep_order(bool visc, int dir) { return (visc && dir == 0) ? 0 : 1; }
 static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
 {
     WenoArgs a = weno_args(c, pr, dir, first);
     const Pick p = weno_pick(c, dir);
     int rc = tbeg(c, KT_WENO0 + dir);
     if (rc) return rc;
-    if ((rc = launch_pick(p.single, sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), p.TA * p.TDI, &a, c->stream)))
-        return rc;
+    const dim3 grid = flat_grid(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), a.tg, sweep_order(false, dir));
+    if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &a, c->stream))) return rc;
     return tend(c);
 }
 
@@ -2002,9 +2048,9 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
         if ((rc = tbeg(c, KT_GRAD))) return rc;
         if (gm.dims == 3) {
             constexpr int CI = 32;
-            dim3 grid(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI));
             const int tgrad = c->diag ? 0 : 1;
-            const GradArgs ga = {cp, c->T, gf, gm, c->k, CI, tgrad};
+            GradArgs ga = {cp, c->T, gf, gm, c->k, CI, tgrad, {0, 0, 0, 0}};
+            const dim3 grid = flat_grid(dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI)), ga.tg, 0);
             if (c->d.space_order == 5) k_visc_grad_march<5><<<grid, GK * GJ, 0, c->stream>>>(ga);
             else k_visc_grad_march<9><<<grid, GK * GJ, 0, c->stream>>>(ga);
         } else {
@@ -2093,8 +2139,8 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
     const Pick p = visc_pick(c, dir);
     int rc = tbeg(c, KT_VISC0 + dir);
     if (rc) return rc;
-    if ((rc = launch_pick(p.single, sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), p.TA * p.TDI, &a, c->stream)))
-        return rc;
+    const dim3 grid = flat_grid(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), a.tg, sweep_order(true, dir));
+    if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &a, c->stream))) return rc;
     return tend(c);
 }
 
@@ -2228,20 +2274,39 @@ static void batch_buffers(const shx_case *c, int s, int &in, int &out)
 }
 
 // append one launch over `cs` with per-case argument records `recs` (rsize bytes each) and grids
+// Work-list order, as the single-case launches order their tiles (sweep_tile): ORDER_NODES keeps
+// every case's blocks in x order; ORDER_SWEEP lists sweep tiles fastest and places them on the
+// XCDs as xcd_order does.
+enum { ORDER_NODES, ORDER_SWEEP };
+static int host_xcd_order(const int p, const int total)
+{
+    const int per = total >> 3, rem = total & 7, x = p & 7, s = p >> 3;
+    return x * per + std::min(x, rem) + s;
+}
 static void batch_add(shx_batch *b, const void *fn, const void *fnp, int threads, int klass,
-                      const std::vector<char> &recs, size_t rsize, const std::vector<dim3> &grids)
+                      const std::vector<char> &recs, size_t rsize, const std::vector<dim3> &grids,
+                      int order = ORDER_NODES)
 {
     BatchLaunch L;
     memset(&L, 0, sizeof L);
     L.fn = fn; L.fnp = fnp; L.threads = threads; L.klass = klass; L.group = b->cur_group;
     const int nc = (int)grids.size();
-    std::vector<int4> work;
+    std::vector<int4> tiles;
     for (int i = 0; i < nc; i++) {
         const dim3 g = grids[i];      // node-parallel kernels: (blocks, 1, 1)
-        for (unsigned z = 0; z < g.z; z++)
+        if (order == ORDER_SWEEP) {
             for (unsigned y = 0; y < g.y; y++)
-                for (unsigned x = 0; x < g.x; x++) work.push_back(make_int4(i, (int)x, (int)y, (int)z));
+                for (unsigned x = 0; x < g.x; x++)
+                    for (unsigned z = 0; z < g.z; z++) tiles.push_back(make_int4(i, (int)x, (int)y, (int)z));
+        } else {
+            for (unsigned z = 0; z < g.z; z++)
+                for (unsigned y = 0; y < g.y; y++)
+                    for (unsigned x = 0; x < g.x; x++) tiles.push_back(make_int4(i, (int)x, (int)y, (int)z));
+        }
     }
+    std::vector<int4> work(tiles.size());
+    const int nt = (int)tiles.size();
+    for (int q = 0; q < nt; q++) work[q] = (order == ORDER_NODES) ? tiles[q] : tiles[host_xcd_order(q, nt)];
     L.total = (int)work.size();
     if (work.empty()) work.push_back(make_int4(0, 0, 0, 0));
     L.args_off = align256(b->host.size());
@@ -2369,7 +2434,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             if (dims == 3)
                 batch_add(b, S == 5 ? (const void *)&k_march_batch<5, false> : (const void *)&k_march_batch<9, false>,
                           S == 5 ? (const void *)&k_march_batch<5, true> : (const void *)&k_march_batch<9, true>, GK * GJ,
-                          BK_GRAD, grecs, sizeof(GradArgs), ggrids);
+                          BK_GRAD, grecs, sizeof(GradArgs), ggrids, ORDER_NODES);
             else
                 batch_add(b, (const void *)&k_grad2d_batch<false>, (const void *)&k_grad2d_batch<true>, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
         }
@@ -2382,7 +2447,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 put(recs, a);
                 grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
             }
-            batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids);
+            batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids, ORDER_SWEEP);
         }
         if (!c0->d.inviscid) {
             for (int dir = 0; dir < dims; dir++) {
@@ -2394,7 +2459,8 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                     put(recs, a);
                     grids.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p));
                 }
-                batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_VISC, recs, sizeof(ViscArgs), grids);
+                batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_VISC, recs, sizeof(ViscArgs), grids,
+                          sweep_order(true, dir) ? ORDER_SWEEP : ORDER_NODES);
             }
         } else {
             std::vector<char> recs;
