@@ -302,8 +302,11 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     constexpr int WD = TDI - 1 + 2 * G;   // window nodes along the sweep
     constexpr int NW = WD * TA;
     constexpr int NT = TA * TDI;
-    constexpr int NVK = (AX < 0) ? NV : NV - 2;   // diagonal: V_NX holds the sign, J moves to V_NY
-    constexpr int VJ = (AX < 0) ? V_J : V_NY;
+    // diagonal metric: no normal components; the sign of the metric entry rides on V_LAM (the
+    // spectral radius is positive) and J moves to V_NX, which keeps the 8 x 32 tile's window
+    // under 40 KB (four blocks per CU)
+    constexpr int NVK = (AX < 0) ? NV : NV - 3;
+    constexpr int VJ = (AX < 0) ? V_J : V_NX;
     __shared__ double sm[NVK * NW];
     __shared__ float dfu[MIX ? 10 * NW : 1];      // FP32 differences of F (0..4) and U (5..9)
 
@@ -357,8 +360,8 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
 #pragma unroll
             for (int q = 0; q < 3; q++) sm[(V_F1 + q) * NW + o] = (q == AX) ? J * (rt * uu[q] + md * pu) : J * (rt * uu[q]);
             sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
-            sm[V_LAM * NW + o] = fabs(th) + cs * fabs(md);
-            sm[V_NX * NW + o] = (md > 0.0) ? 1.0 : -1.0;
+            const double lam = fabs(th) + cs * fabs(md);
+            sm[V_LAM * NW + o] = (md > 0.0) ? lam : -lam;
         }
         sm[V_U0 * NW + o] = rho;
         sm[V_U1 * NW + o] = rho * u;
@@ -403,7 +406,7 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             ny = 0.5 * (sm[V_NY * NW + oc] + sm[V_NY * NW + oc + 1]);
             nz = 0.5 * (sm[V_NZ * NW + oc] + sm[V_NZ * NW + oc + 1]);
         } else {
-            const double ns = 0.5 * (sm[V_NX * NW + oc] + sm[V_NX * NW + oc + 1]);
+            const double ns = 0.5 * (copysign(1.0, sm[V_LAM * NW + oc]) + copysign(1.0, sm[V_LAM * NW + oc + 1]));
             nx = (AX == 0) ? ns : 0.0;
             ny = (AX == 1) ? ns : 0.0;
             nz = (AX == 2) ? ns : 0.0;
@@ -412,7 +415,7 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         const Eig e = make_eig(nx, ny, nz, u, v, w, rho, cs, k.gamma);
         double lam = 0.0;
 #pragma unroll
-        for (int s = 0; s < 2 * G; s++) lam = fmax(lam, sm[V_LAM * NW + ob + s]);
+        for (int s = 0; s < 2 * G; s++) lam = fmax(lam, (AX < 0) ? sm[V_LAM * NW + ob + s] : fabs(sm[V_LAM * NW + ob + s]));
         const double lj = lam * Jh;
 
         // Characteristic projection per field c: a_s = L_c . F_s and b_s = L_c . U_s on the 2G
@@ -558,8 +561,14 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         }
     }
 }
+// Waves per SIMD asked of the compiler: the diagonal W5 eta sweep otherwise takes 134 VGPRs
+// (three waves per SIMD); at 128 it runs four and is measured 9 % faster.  The other variants are
+// left to the compiler (the zeta sweep measured slower under the same bound).
+template <int S, int AX, bool MIX>
+constexpr int weno_min_waves() { return (S == 5 && AX == 1 && !MIX) ? 4 : 1; }
+
 template <int S, int TA, int TDI, int AX, bool MIX>
-__global__ __launch_bounds__(TA * TDI) void k_weno(WenoArgs a)
+__global__ __launch_bounds__(TA * TDI, (weno_min_waves<S, AX, MIX>())) void k_weno(WenoArgs a)
 {
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
@@ -1243,7 +1252,7 @@ __global__ __launch_bounds__(GK * GJ) void k_march_batch(BATCH_PARAMS(GradArgs))
 }
 
 template <int S, int TA, int TDI, int AX, bool MIX, bool P>
-__global__ __launch_bounds__(TA * TDI) void k_weno_batch(BATCH_PARAMS(WenoArgs))
+__global__ __launch_bounds__(TA * TDI, (weno_min_waves<S, AX, MIX>())) void k_weno_batch(BATCH_PARAMS(WenoArgs))
 {
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
