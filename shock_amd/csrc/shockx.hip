@@ -780,7 +780,11 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
     constexpr int NP = 2 * G + 1;                    // line positions used by the zeta-index quirk
     constexpr int NLV = 4 + NC;                      // values per position: u,v,w,T + cross
     __shared__ double sm[NVS * NW];
-    __shared__ double fo[8 * NT];                    // face outputs (minus/plus sides)
+    // Plus-side face values go from each thread to the previous one (the minus side stays in
+    // registers).  Where a tile row never straddles two wavefronts (TDI divides or is a multiple of
+    // the 64-lane wavefront) the hand-over is a lane shuffle; otherwise it goes through LDS.
+    constexpr bool SHFL = (64 % TDI == 0) || (TDI % 64 == 0);
+    __shared__ double fo[SHFL ? 1 : 4 * NT];
     __shared__ double line[(DIR == 2) ? TA * NP * NLV : 1];
     __shared__ double lined[(DIR == 2) ? TA * 2 * NLV : 1];
 
@@ -906,9 +910,17 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
             for (int m = 0; m < 4; m++) { out[m] = FA[m]; out[4 + m] = FB[m]; }
         }
     }
+    double nxt[4];
+    if constexpr (SHFL) {
 #pragma unroll
-    for (int m = 0; m < 8; m++) fo[m * NT + t] = out[m];
-    __syncthreads();
+        for (int m = 0; m < 4; m++) nxt[m] = __shfl_down(out[4 + m], 1);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 4; m++) fo[m * NT + t] = out[4 + m];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 4; m++) nxt[m] = (di < TDI - 1) ? fo[m * NT + t + 1] : 0.0;
+    }
 
     if (di < TDI - 1) {
         const int dn = d0 + di, pa = pa0 + ta;
@@ -918,7 +930,7 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
             // faces: minus = this thread's face (A values), plus = next thread's face (B values)
             double dF[4];
 #pragma unroll
-            for (int m = 0; m < 4; m++) dF[m] = fo[(4 + m) * NT + t + 1] - fo[m * NT + t];
+            for (int m = 0; m < 4; m++) dF[m] = nxt[m] - out[m];
             // nodal gradients: own-direction derivatives from the window, cross ones stored
             double g[9];
 #pragma unroll
