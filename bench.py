@@ -34,7 +34,8 @@ def algo_bytes(name, n, ng, time_order=4, diag=True):
     extent ng^3 (row padding excluded): each input array element read once, each output element
     written once.  diag: the metric is diagonal (the reference's 3D metric), so a sweep reads one
     metric entry of its direction plus J.  The last viscous sweep of a stage also applies the
-    stage update (U_n, Q, Q_sum, U_s+1)."""
+    stage update (U_n, Q, Q_sum, U_s+1); the fused zeta kernel (WENO and viscous zeta sweeps plus
+    the update) reads the viscous sweep's inputs plus rho and e of the stage input."""
     nint = n ** 3
     npad = ng ** 3
     ghost = npad - nint
@@ -53,6 +54,8 @@ def algo_bytes(name, n, ng, time_order=4, diag=True):
         return nint * (vin + 8) * FP64
     if name == "visc_zeta":                      # + Q (5) read, U_n (5) read, Q_sum, U_s+1 (5) written
         return nint * (vin + 5 + 5 + qsum + 5) * FP64
+    if name == "weno_visc_zeta":                 # fused zeta sweeps: the viscous inputs + rho, e
+        return nint * (vin + 2 + 5 + 5 + qsum + 5) * FP64
     if name == "rk_update":          # U_n, Q, J in (Qsum r/w on RK4 stages), U_s+1 out
         return nint * (5 + 5 + 1 + 5 + qsum) * FP64
     if name == "ghost_periodic":     # 5 fields read + written on ghost nodes

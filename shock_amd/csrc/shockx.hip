@@ -294,21 +294,33 @@ enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V
 // formed in FP64 and rounded once), the smoothness indicators, weights and the deviation Phi, and
 // R . Phi are FP32.  Phi is a correction on top of Theta, so its rounding enters at the size of
 // the correction rather than of the flux.
+// LDS of one WENO sweep block, in doubles: the node window (NVK fields) and, in the mixed form,
+// the FP32 differences of F and U.  Diagonal metric: no normal components; the sign of the
+// metric entry rides on V_LAM (the spectral radius is positive) and J moves to V_NX, which keeps
+// the 8 x 32 tile's window under 40 KB (four blocks per CU).
 template <int S, int TA, int TDI, int AX, bool MIX>
-__device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const int by, const int bz)
+constexpr int weno_lds()
+{
+    constexpr int NW = (TDI - 1 + (S + 1)) * TA;
+    constexpr int NVK = (AX < 0) ? NV : NV - 3;
+    return NVK * NW + (MIX ? 5 * NW : 0);
+}
+
+// KEEP: leave Q alone and return this sweep's contribution of the thread's node in keep[5] (a
+// fused kernel adds it later); otherwise Q = -div (first sweep of the stage) or Q -= div.
+template <int S, int TA, int TDI, int AX, bool MIX, bool KEEP = false>
+__device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const int by, const int bz,
+                                          double *lds, double *keep = nullptr)
 {
     constexpr int G = (S + 1) / 2;
     constexpr int R = G - 1;
     constexpr int WD = TDI - 1 + 2 * G;   // window nodes along the sweep
     constexpr int NW = WD * TA;
     constexpr int NT = TA * TDI;
-    // diagonal metric: no normal components; the sign of the metric entry rides on V_LAM (the
-    // spectral radius is positive) and J moves to V_NX, which keeps the 8 x 32 tile's window
-    // under 40 KB (four blocks per CU)
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
     constexpr int VJ = (AX < 0) ? V_J : V_NX;
-    __shared__ double sm[NVK * NW];
-    __shared__ float dfu[MIX ? 10 * NW : 1];      // FP32 differences of F (0..4) and U (5..9)
+    double *const sm = lds;                       // NVK * NW
+    float *const dfu = (float *)(lds + NVK * NW); // mixed form: FP32 differences of F (0..4) and U (5..9)
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
@@ -556,7 +568,8 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 const double dv = sm[m * NT + t + 1] - sm[m * NT + t];
-                a.q.f[m][n] = (a.first ? 0.0 : a.q.f[m][n]) - dv;
+                if constexpr (KEEP) keep[m] = -dv;
+                else a.q.f[m][n] = (a.first ? 0.0 : a.q.f[m][n]) - dv;
             }
         }
     }
@@ -570,9 +583,10 @@ constexpr int weno_min_waves() { return (S == 5 && AX == 1 && !MIX) ? 4 : 1; }
 template <int S, int TA, int TDI, int AX, bool MIX>
 __global__ __launch_bounds__(TA * TDI, (weno_min_waves<S, AX, MIX>())) void k_weno(WenoArgs a)
 {
+    __shared__ double lds[weno_lds<S, TA, TDI, AX, MIX>()];
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
-    weno_body<S, TA, TDI, AX, MIX>(a, bx, by, bz);
+    weno_body<S, TA, TDI, AX, MIX>(a, bx, by, bz, lds);
 }
 
 // ---- viscous fluxes: face-based LDS sweep ------------------------------------------------
@@ -765,8 +779,26 @@ __device__ __forceinline__ double node_prod_diag(const double g[9], const double
 // LDS slots
 enum { VS_U, VS_V, VS_W, VS_T, VS_MU, VS_CR };
 
+// LDS of one viscous sweep block, in doubles: the node window (NVS fields), the face hand-over
+// buffer where a tile row straddles wavefronts, and the zeta-index quirk's line values.  Plus-side
+// face values go from each thread to the previous one (the minus side stays in registers); where a
+// tile row never straddles two wavefronts (TDI divides or is a multiple of the 64-lane wavefront)
+// the hand-over is a lane shuffle.
 template <int S, int DIR, int TA, int TDI, bool DIAG>
-__device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const int by, const int bz)
+constexpr int visc_lds()
+{
+    constexpr int G = (S + 1) / 2, NW = (TDI - 1 + 2 * G) * TA, NT = TA * TDI;
+    constexpr int NC = DIAG ? 4 : 8, NM = DIAG ? 4 : 10;
+    constexpr int NLV = 4 + NC, NP = 2 * G + 1;
+    constexpr bool SHFL = (64 % TDI == 0) || (TDI % 64 == 0);
+    return (VS_CR + NC + NM) * NW + (SHFL ? 0 : 4 * NT) + ((DIR == 2) ? TA * NP * NLV + TA * 2 * NLV : 0);
+}
+
+// ADD: add add[5] (a fused WENO sweep's contribution of the thread's node) to Q before the
+// viscous terms.
+template <int S, int DIR, int TA, int TDI, bool DIAG, bool ADD = false>
+__device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const int by, const int bz,
+                                          double *lds, const double *add = nullptr)
 {
     constexpr int G = (S + 1) / 2;
     constexpr int WD = TDI - 1 + 2 * G;
@@ -779,14 +811,11 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
     constexpr int b1 = Others<DIR>::b1, b2 = Others<DIR>::b2;
     constexpr int NP = 2 * G + 1;                    // line positions used by the zeta-index quirk
     constexpr int NLV = 4 + NC;                      // values per position: u,v,w,T + cross
-    __shared__ double sm[NVS * NW];
-    // Plus-side face values go from each thread to the previous one (the minus side stays in
-    // registers).  Where a tile row never straddles two wavefronts (TDI divides or is a multiple of
-    // the 64-lane wavefront) the hand-over is a lane shuffle; otherwise it goes through LDS.
     constexpr bool SHFL = (64 % TDI == 0) || (TDI % 64 == 0);
-    __shared__ double fo[SHFL ? 1 : 4 * NT];
-    __shared__ double line[(DIR == 2) ? TA * NP * NLV : 1];
-    __shared__ double lined[(DIR == 2) ? TA * 2 * NLV : 1];
+    double *const sm = lds;
+    double *const fo = sm + NVS * NW;
+    double *const line = fo + (SHFL ? 0 : 4 * NT);
+    double *const lined = line + TA * NP * NLV;
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
@@ -963,20 +992,25 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
                 prod = node_prod_generic<DIR>(g, m, mu);
             }
             const double work = sm[VS_U * NW + p] * dF[0] + sm[VS_V * NW + p] * dF[1] + sm[VS_W * NW + p] * dF[2];
+            double q0[5];
+#pragma unroll
+            for (int m = 0; m < 5; m++) q0[m] = ADD ? a.q.f[m][n] + add[m] : a.q.f[m][n];
             if (a.fuse_rk) {
                 double qv[5];
-                qv[0] = a.q.f[0][n];
-                qv[1] = a.q.f[1][n] + k.psi * dF[0];
-                qv[2] = a.q.f[2][n] + k.psi * dF[1];
-                qv[3] = a.three ? a.q.f[3][n] + k.psi * dF[2] : a.q.f[3][n];
-                qv[4] = a.q.f[4][n] + (k.gam_heat * dF[3] + k.psi * (work + prod));
+                qv[0] = q0[0];
+                qv[1] = q0[1] + k.psi * dF[0];
+                qv[2] = q0[2] + k.psi * dF[1];
+                qv[3] = a.three ? q0[3] + k.psi * dF[2] : q0[3];
+                qv[4] = q0[4] + (k.gam_heat * dF[3] + k.psi * (work + prod));
                 const double J = DIAG ? sm[(VS_M + 3) * NW + p] : sm[(VS_M + 9) * NW + p];
                 rk_point(a.rk, n, J, qv);
             } else {
-                a.q.f[1][n] += k.psi * dF[0];
-                a.q.f[2][n] += k.psi * dF[1];
-                if (a.three) a.q.f[3][n] += k.psi * dF[2];
-                a.q.f[4][n] += k.gam_heat * dF[3] + k.psi * (work + prod);
+                if (ADD) a.q.f[0][n] = q0[0];
+                a.q.f[1][n] = q0[1] + k.psi * dF[0];
+                a.q.f[2][n] = q0[2] + k.psi * dF[1];
+                if (a.three) a.q.f[3][n] = q0[3] + k.psi * dF[2];
+                else if (ADD) a.q.f[3][n] = q0[3];
+                a.q.f[4][n] = q0[4] + (k.gam_heat * dF[3] + k.psi * (work + prod));
             }
         }
     }
@@ -984,9 +1018,43 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
 template <int S, int DIR, int TA, int TDI, bool DIAG>
 __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
 {
+    __shared__ double lds[visc_lds<S, DIR, TA, TDI, DIAG>()];
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
-    visc_body<S, DIR, TA, TDI, DIAG>(a, bx, by, bz);
+    visc_body<S, DIR, TA, TDI, DIAG>(a, bx, by, bz, lds);
+}
+
+// ---- fused zeta direction: WENO and viscous sweeps and the stage update ----------------------
+// One block sweeps a 4 x 64 tile along zeta twice.  The WENO phase keeps each node's inviscid
+// divergence in registers; after a barrier the viscous phase reloads its window into the same LDS,
+// adds that divergence and its own terms to Q and applies the stage update (or stores Q).  Q of
+// this direction is read once instead of read-written by two kernels, and blocks in their WENO
+// phase (VALU-bound) overlap blocks in their viscous phase (memory-bound).  Used for 3D cases with
+// the diagonal metric whose zeta WENO tile is 4 x 64 (the viscous zeta tile), viscous runs only.
+struct ZetaArgs { WenoArgs w; ViscArgs v; };
+template <int S, bool MIX>
+constexpr int zeta_lds()
+{
+    return weno_lds<S, 4, 64, 2, MIX>() > visc_lds<S, 2, 4, 64, true>() ? weno_lds<S, 4, 64, 2, MIX>()
+                                                                       : visc_lds<S, 2, 4, 64, true>();
+}
+template <int S, bool MIX>
+__device__ __forceinline__ void zeta_body(const ZetaArgs &z, const int bx, const int by, const int bz, double *lds)
+{
+    double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    weno_body<S, 4, 64, 2, MIX, true>(z.w, bx, by, bz, lds, inv);
+    __syncthreads();
+    visc_body<S, 2, 4, 64, true, true>(z.v, bx, by, bz, lds, inv);
+}
+// FP64 W5: held to 128 VGPRs (four waves per SIMD, as the LDS allows); measured 11 % faster than
+// the compiler's own 136
+template <int S, bool MIX>
+__global__ __launch_bounds__(256, ((S == 5 && !MIX) ? 4 : 1)) void k_zeta(ZetaArgs z)
+{
+    __shared__ double lds[zeta_lds<S, MIX>()];
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, z.w.tg, bx, by, bz);
+    zeta_body<S, MIX>(z, bx, by, bz, lds);
 }
 
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
@@ -1266,18 +1334,29 @@ __global__ __launch_bounds__(GK * GJ) void k_march_batch(BATCH_PARAMS(GradArgs))
 template <int S, int TA, int TDI, int AX, bool MIX, bool P>
 __global__ __launch_bounds__(TA * TDI, (weno_min_waves<S, AX, MIX>())) void k_weno_batch(BATCH_PARAMS(WenoArgs))
 {
+    __shared__ double lds[weno_lds<S, TA, TDI, AX, MIX>()];
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
-        weno_body<S, TA, TDI, AX, MIX>(rec(A, w.x), w.y, w.z, w.w);
+        weno_body<S, TA, TDI, AX, MIX>(rec(A, w.x), w.y, w.z, w.w, lds);
     });
 }
 
 template <int S, int DIR, int TA, int TDI, bool DIAG, bool P>
 __global__ __launch_bounds__(TA * TDI) void k_visc_batch(BATCH_PARAMS(ViscArgs))
 {
+    __shared__ double lds[visc_lds<S, DIR, TA, TDI, DIAG>()];
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
-        visc_body<S, DIR, TA, TDI, DIAG>(rec(A, w.x), w.y, w.z, w.w);
+        visc_body<S, DIR, TA, TDI, DIAG>(rec(A, w.x), w.y, w.z, w.w, lds);
+    });
+}
+template <int S, bool MIX, bool P>
+__global__ __launch_bounds__(256, ((S == 5 && !MIX) ? 4 : 1)) void k_zeta_batch(BATCH_PARAMS(ZetaArgs))
+{
+    __shared__ double lds[zeta_lds<S, MIX>()];
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        zeta_body<S, MIX>(rec(A, w.x), w.y, w.z, w.w, lds);
     });
 }
 
@@ -1441,10 +1520,11 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                               \
     } while (0)
 
-enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES };
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA };
 static const char *g_knames[SHX_NUM_KERNELS] = {
     "ghost_periodic", "calc_values", "weno_xi", "weno_eta", "weno_zeta",
-    "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces"};
+    "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces",
+    "weno_visc_zeta"};
 
 struct shx_case {
     shx_desc d;
@@ -2041,6 +2121,12 @@ static int weno_sweep(shx_case *c, double *const *pr, int dir, int first)
 // sweeps of every direction, then the viscous sweeps (the reference's accumulation order,
 // src/Functions.c:2161-2272)
 static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk);
+static int zeta_sweep(shx_case *c, double *const *pr, const RkArgs *rk);
+// the zeta direction runs as one fused kernel (k_zeta)
+static bool fuse_zeta(const shx_case *c)
+{
+    return c->gm.dims == 3 && c->diag && !c->d.inviscid && c->wtile[2] == 0;
+}
 // rk: when given, the last viscous sweep applies this stage update instead of storing Q
 static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double tau)
 {
@@ -2082,11 +2168,16 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
     } else if ((rc = face_fill(c, pr, tau))) {
         return rc;
     }
-    for (int dir = 0; dir < gm.dims; dir++)
+    // Q accumulates the WENO divergences, then the viscous terms; with the fused zeta direction
+    // the order is WENO xi, eta, viscous xi, eta, then WENO zeta and viscous zeta together
+    const bool fz = fuse_zeta(c);
+    const int nsep = fz ? gm.dims - 1 : gm.dims;
+    for (int dir = 0; dir < nsep; dir++)
         if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
     if (!c->d.inviscid)
-        for (int dir = 0; dir < gm.dims; dir++)
+        for (int dir = 0; dir < nsep; dir++)
             if ((rc = visc_sweep(c, pr, dir, dir == gm.dims - 1 ? rk : nullptr))) return rc;
+    if (fz && (rc = zeta_sweep(c, pr, rk))) return rc;
     return SHX_OK;
 }
 
@@ -2162,6 +2253,35 @@ static int visc_sweep(shx_case *c, double *const *pr, int dir, const RkArgs *rk)
     if (rc) return rc;
     const dim3 grid = flat_grid(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, p), a.tg, sweep_order(true, dir));
     if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &a, c->stream))) return rc;
+    return tend(c);
+}
+template <int S, bool MIX>
+static Pick zeta_pick_t()
+{
+    return {(const void *)&k_zeta<S, MIX>, (const void *)&k_zeta_batch<S, MIX, false>,
+            (const void *)&k_zeta_batch<S, MIX, true>, 4, 64};
+}
+static Pick zeta_pick(const shx_case *c)
+{
+    const bool mix = (c->d.precision == SHX_MIXED);
+    if (c->d.space_order == 5) return mix ? zeta_pick_t<5, true>() : zeta_pick_t<5, false>();
+    return mix ? zeta_pick_t<9, true>() : zeta_pick_t<9, false>();
+}
+static ZetaArgs zeta_args(const shx_case *c, double *const *pr, const RkArgs *rk)
+{
+    ZetaArgs z;
+    z.w = weno_args(c, pr, 2, 0);
+    z.v = visc_args(c, pr, 2, rk);
+    return z;
+}
+static int zeta_sweep(shx_case *c, double *const *pr, const RkArgs *rk)
+{
+    ZetaArgs z = zeta_args(c, pr, rk);
+    const Pick p = zeta_pick(c);
+    int rc = tbeg(c, KT_ZETA);
+    if (rc) return rc;
+    const dim3 grid = flat_grid(sweep_grid(z.w.nA, z.w.oA, z.w.alignA, z.w.nB, z.w.nD, p), z.w.tg, 1);
+    if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &z, c->stream))) return rc;
     return tend(c);
 }
 
@@ -2459,7 +2579,10 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             else
                 batch_add(b, (const void *)&k_grad2d_batch<false>, (const void *)&k_grad2d_batch<true>, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
         }
-        for (int dir = 0; dir < dims; dir++) {
+        // the same sweep sequence as stage_fluxes (fused zeta direction where it applies)
+        const bool fz = fuse_zeta(c0);
+        const int nsep = fz ? dims - 1 : dims;
+        for (int dir = 0; dir < nsep; dir++) {
             const Pick p = weno_pick(c0, dir);
             std::vector<char> recs;
             std::vector<dim3> grids;
@@ -2470,7 +2593,30 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             }
             batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids, ORDER_SWEEP);
         }
-        if (!c0->d.inviscid) {
+        if (fz) {
+            const Pick p = zeta_pick(c0);
+            std::vector<char> zrecs;
+            std::vector<dim3> grids;
+            // viscous xi and eta first, then the fused zeta kernel
+            for (int dir = 0; dir < nsep; dir++) {
+                const Pick pv = visc_pick(c0, dir);
+                std::vector<char> recs;
+                std::vector<dim3> vg;
+                for (int i = 0; i < nc; i++) {
+                    const ViscArgs a = visc_args(g[i], in[i], dir, nullptr);
+                    put(recs, a);
+                    vg.push_back(sweep_grid(a.nA, a.oA, a.alignA, a.nB, a.nD, pv));
+                }
+                batch_add(b, pv.batch, pv.batchp, pv.TA * pv.TDI, BK_VISC, recs, sizeof(ViscArgs), vg,
+                          sweep_order(true, dir) ? ORDER_SWEEP : ORDER_NODES);
+            }
+            for (int i = 0; i < nc; i++) {
+                const ZetaArgs z = zeta_args(g[i], in[i], &rk[i]);
+                put(zrecs, z);
+                grids.push_back(sweep_grid(z.w.nA, z.w.oA, z.w.alignA, z.w.nB, z.w.nD, p));
+            }
+            batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_VISC, zrecs, sizeof(ZetaArgs), grids, ORDER_SWEEP);
+        } else if (!c0->d.inviscid) {
             for (int dir = 0; dir < dims; dir++) {
                 const Pick p = visc_pick(c0, dir);
                 std::vector<char> recs;
@@ -2807,6 +2953,8 @@ extern "C" int shx_tune(shx_case *c, int reps, int choice[3])
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     for (int dir = 0; dir < c->gm.dims && !rc; dir++) {
+        // the zeta sweep of a fusable case stays on the 4 x 64 tile of the fused kernel (k_zeta)
+        if (dir == 2 && c->gm.dims == 3 && c->diag && !c->d.inviscid) { c->wtile[2] = 0; continue; }
         float best = 1e30f;
         int bestv = 0;
         for (int v = 0; v < NWTILE && !rc; v++) {

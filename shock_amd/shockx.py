@@ -20,7 +20,7 @@ QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MANUFACTURED, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 FP64, MIXED = 0, 1
-NUM_KERNELS = 13
+NUM_KERNELS = 14
 
 
 class ShxError(RuntimeError):
@@ -239,7 +239,7 @@ class Case:
         ms = (ctypes.c_double * NUM_KERNELS)()
         cnt = (ctypes.c_int * NUM_KERNELS)()
         _check(lib().shx_kernel_times(self.h, names, ms, cnt))
-        return {names[i].decode(): (ms[i], cnt[i]) for i in range(NUM_KERNELS)}
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(NUM_KERNELS) if names[i] is not None}
 
 
 class Batch:
