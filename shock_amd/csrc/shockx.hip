@@ -461,16 +461,16 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
                     da[s] = x;
                     db[s] = y;
                 }
-                // differences of the split fluxes: f+ on nodes 0..S-1, f- mirrored
+                // differences of the split fluxes (times 2, see below): f+ on nodes 0..S-1, f- mirrored
 #pragma unroll
                 for (int s = 0; s < S - 1; s++) {
-                    dp[s] = 0.5f * (da[s] + ljf * db[s]);
-                    dm[s] = 0.5f * (da[S - 1 - s] - ljf * db[S - 1 - s]);
+                    dp[s] = da[s] + ljf * db[s];
+                    dm[s] = da[S - 1 - s] - ljf * db[S - 1 - s];
                 }
                 if constexpr (S == 5) {
-                    phif[c] = -phi_w5_f(dp[0], dp[1], dp[2], dp[3]) + phi_w5_f(dm[0], dm[1], dm[2], dm[3]);
+                    phif[c] = 0.5f * (-phi_w5_f(dp[0], dp[1], dp[2], dp[3]) + phi_w5_f(dm[0], dm[1], dm[2], dm[3]));
                 } else {
-                    phif[c] = -phi_w9_f(dp) + phi_w9_f(dm);
+                    phif[c] = 0.5f * (-phi_w9_f(dp) + phi_w9_f(dm));
                 }
                 return;
             } else {
@@ -494,20 +494,24 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
                 av[s] = aa;
                 bv[s] = bb;
             }
+            // The split fluxes are formed without their factor 1/2, which is applied to Phi
+            // instead: Phi is homogeneous of degree one in its inputs (the weights depend on
+            // ratios of the indicators, rescaled by exact powers of two), so halving the inputs
+            // or the output gives the same bits wherever epsilon is far below the indicators.
             double fp[S], fm[S];
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                fp[s] = 0.5 * (av[s] + lj * bv[s]);
-                fm[s] = 0.5 * (av[2 * G - 1 - s] - lj * bv[2 * G - 1 - s]);
+                fp[s] = av[s] + lj * bv[s];
+                fm[s] = av[2 * G - 1 - s] - lj * bv[2 * G - 1 - s];
             }
             if constexpr (S == 5) {
-                phis[c] = -phi_w5_fast(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
-                          + phi_w5_fast(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]);
+                phis[c] = 0.5 * (-phi_w5_fast(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
+                                 + phi_w5_fast(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]));
             } else {
                 double dp_[8], dm_[8];
 #pragma unroll
                 for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
-                phis[c] = -phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_);
+                phis[c] = 0.5 * (-phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_));
             }
             }
         };
