@@ -127,6 +127,10 @@ int shx_run(shx_case *c, double tau, int nsteps);
  * assembly, src/Functions.c:2111-2293).  For parity tests. */
 int shx_rhs(shx_case *c, double *const q[5]);
 
+/* check_Metric (src/Functions.c:6198-6243): number of padded nodes whose Jacobian is negative
+ * (the reference prints it and continues; a left-handed grid or crossing ghost cells). */
+int shx_check_metric(shx_case *c, int *negative);
+
 /* checkNAN (src/Functions.c:5419-5467) on U_lastStep: count of non-finite interior rho;
  * those nodes get rho = u = v = w = 1e6 as in the reference. */
 int shx_check_nan(shx_case *c, int *count);

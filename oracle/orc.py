@@ -67,6 +67,7 @@ def lib():
         _lib.orc_cfl_dt.restype = ctypes.c_double
         _lib.orc_cfl_dt.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P]
         _lib.orc_check_nan.restype = ctypes.c_int
+        _lib.orc_check_metric.restype = ctypes.c_int
         _lib.orc_visc_fac.argtypes = [ctypes.c_int, P, P]
         _lib.orc_step.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P]
         _lib.orc_step_b.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P, P]
@@ -197,6 +198,10 @@ class Oracle:
     def rk_update(self, s, tau, met, un, urk, q, qsum):
         lib().orc_rk_update(ctypes.byref(self.c), s, ctypes.c_double(tau), _bundle(met), _bundle(un),
                             _bundle(urk), _bundle(q), _bundle(qsum))
+
+    def check_metric(self, met):
+        """Padded nodes with a negative Jacobian (check_Metric)."""
+        return lib().orc_check_metric(ctypes.byref(self.c), _bundle(met))
 
     def cfl_dt(self, cfl, met, prim):
         return lib().orc_cfl_dt(ctypes.byref(self.c), cfl, _bundle(met), _bundle(prim))

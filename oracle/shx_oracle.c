@@ -899,3 +899,13 @@ void orc_metric_from_coords(const orc_cfg *c, const double *xe, const double *ye
                 met[8][n] = (x_xi * y_eta - x_eta * y_xi) / J;
             }
 }
+
+/* check_Metric (src/Functions.c:6198-6243): padded nodes whose Jacobian is negative */
+int orc_check_metric(const orc_cfg *c, double *const met[10])
+{
+    long n;
+    int bad = 0;
+    for (n = 0; n < c->ntot; n++)
+        if (met[9][n] < 0.0) bad++;
+    return bad;
+}

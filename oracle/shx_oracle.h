@@ -149,6 +149,9 @@ double orc_cfl_dt(const orc_cfg *c, double cfl, double *const met[10], double *c
 void orc_metric_from_coords(const orc_cfg *c, const double *xe, const double *ye,
                             const double *ze, double *const met[10]);
 
+/* check_Metric (src/Functions.c:6198-6243): number of padded nodes with a negative Jacobian. */
+int orc_check_metric(const orc_cfg *c, double *const met[10]);
+
 #ifdef __cplusplus
 }
 #endif

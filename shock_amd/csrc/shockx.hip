@@ -1494,6 +1494,20 @@ __global__ void k_check_nan(Fields5 pr, Geom gm, int *count)
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
 }
 
+// check_Metric (src/Functions.c:6198-6243): count the padded nodes with a negative Jacobian.
+__global__ void k_check_metric(gcdp J, Geom gm, int *count)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (t < gm.ntot) {
+        int idx[3];
+        const long n = padded_node(gm, t, idx);
+        bad = (J[n] < 0.0) ? 1 : 0;
+    }
+    bad = wave_sum(bad);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
 // Plain streaming copy of n doubles (calibration of hardware byte counters).
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n)
 {
@@ -3012,6 +3026,18 @@ extern "C" int shx_rhs(shx_case *c, double *const q[5])
         LAUNCH_CHECK();
     }
     for (int m = 0; m < 5; m++) { rc = copy_out(c, q[m], c->q[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_check_metric(shx_case *c, int *negative)
+{
+    if (!c || !negative) return set_err(SHX_ERR_ARG, "null argument");
+    if (!c->has_metric) return set_err(SHX_ERR_STATE, "no metric yet");
+    HIPCHK(hipMemsetAsync(c->d_cnt, 0, sizeof(int), c->stream));
+    k_check_metric<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(c->met[9], c->gm, c->d_cnt);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(negative, c->d_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SHX_OK;
 }

@@ -48,7 +48,7 @@ EXPORTS = [
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
-    "shx_batch_destroy", "shx_tune",
+    "shx_batch_destroy", "shx_tune", "shx_check_metric",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -74,6 +74,7 @@ def lib():
         L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
         L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.shx_check_metric.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.shx_synchronize.argtypes = [P]
         L.shx_copy_probe.argtypes = [P, ctypes.c_int]
@@ -212,6 +213,12 @@ class Case:
     def check_nan(self):
         c = ctypes.c_int()
         _check(lib().shx_check_nan(self.h, ctypes.byref(c)))
+        return c.value
+
+    def check_metric(self):
+        """Number of padded nodes with a negative Jacobian (check_Metric)."""
+        c = ctypes.c_int()
+        _check(lib().shx_check_metric(self.h, ctypes.byref(c)))
         return c.value
 
     def cfl_dt(self, cfl):

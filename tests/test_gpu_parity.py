@@ -8,7 +8,7 @@ import pytest
 
 import orc
 from helpers import conserved, interior, rel_linf, setup_2d_mms, setup_3d_turb
-from shock_amd import shockx
+from shock_amd import cases, shockx
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +121,22 @@ def test_hundred_steps_2d_64_mms():
     errs = rel_linf(conserved(out), conserved(ref), sl)
     _report("2D 64x64 periodic MMS field, 100 steps", errs)
     assert max(errs[:3] + errs[4:]) <= TOL_100
+
+
+def test_check_metric_on_a_folded_grid():
+    # x = xi + a sin(2 pi xi) with a > 1/(2 pi) folds the grid: the Jacobian is negative on a band
+    nn = (24, 20, 1)
+    o = orc.Oracle(dims=2, S=5, n=nn, bc=(orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0))
+    fold = lambda xi, eta, zeta: (xi + 0.25 * np.sin(2 * np.pi * xi), eta, zeta)
+    xe, ye, ze = cases.ext_coords(2, 5, nn, fold, periodic=(False, True, False))
+    met = o.metric_from_coords(xe, ye, ze)
+    want = o.check_metric(met)
+    assert 0 < want < o.c.ntot
+    with shockx.Case(dims=2, space_order=5, n=nn, bc=(shockx.BC_INFLOW_SUPERSONIC, shockx.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)) as g:
+        g.build_metric(xe, ye, ze)
+        got = g.check_metric()
+    print(f"[parity] negative Jacobians: GPU {got}, oracle {want}", flush=True)
+    assert got == want
 
 
 def test_cfl_and_nan():

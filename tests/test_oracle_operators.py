@@ -229,6 +229,18 @@ def test_cfl_and_nan_against_brute_force():
     assert bad[0][g + 1, g + 2, g + 3] == 1e6 and bad[3][g + 4, g + 4, g + 4] == 1e6
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_check_metric_counts_negative_jacobians(dims):
+    # a right-handed grid has none; mirroring one axis makes every Jacobian negative
+    n = (10, 9, 8) if dims == 3 else (10, 9, 1)
+    o = orc.Oracle(dims=dims, S=5, n=n)
+    ident = cases.cartesian_unit(dims)
+    for mp, want in ((ident, 0), (lambda a, b, c: (-ident(a, b, c)[0],) + tuple(ident(a, b, c)[1:]), o.c.ntot)):
+        xe, ye, ze = cases.ext_coords(dims, 5, n, mp, periodic=(False, False, False))
+        met = o.metric_from_coords(xe, ye, ze)
+        assert o.check_metric(met) == want
+
+
 def test_rk_step_keeps_uniform_state():
     o = orc.Oracle(dims=2, S=5, n=(12, 12, 1))
     met, _ = _metric(o, 2, 5, (12, 12, 1), cases.cartesian_unit(2))
