@@ -91,20 +91,38 @@ def test_one_step_2d(S, time_order):
     assert max(errs[:3] + errs[4:]) <= TOL_STEP
 
 
-@pytest.mark.parametrize("S", [5, 9])
-def test_one_step_3d(S):
-    o, coords, met, prim, nn = setup_3d_turb(n=12, S=S)
+# 3D viscous cases with the reference's diagonal metric run the zeta direction as one fused
+# kernel (WENO, viscous terms and the stage update), for both RK orders
+@pytest.mark.parametrize("S,time_order", [(5, 4), (9, 4), (5, 3), (9, 3)])
+def test_one_step_3d(S, time_order):
+    o, coords, met, prim, nn = setup_3d_turb(n=12, S=S, time_order=time_order)
     tau = o.cfl_dt(0.5, met, prim)
     ref = o.run(tau, met, prim, 1)
-    with _case(o, 3, S, 4, nn) as g:
+    with _case(o, 3, S, time_order, nn) as g:
         g.upload_metric(met)
         g.upload_state(prim)
         g.calc_runge_kutta(tau)
         out = g.download_state()
     sl = interior(o, 3, nn)
     errs = rel_linf(conserved(out), conserved(ref), sl)
-    _report(f"step3d S={S}", errs)
+    _report(f"step3d S={S} rk={time_order}", errs)
     assert max(errs) <= TOL_STEP
+
+
+@pytest.mark.parametrize("S", [5, 9])
+def test_ten_steps_3d(S):
+    o, coords, met, prim, nn = setup_3d_turb(n=14, S=S, seed=19)
+    tau = o.cfl_dt(0.5, met, prim)
+    ref = o.run(tau, met, prim, 10)
+    with _case(o, 3, S, 4, nn) as g:
+        g.upload_metric(met)
+        g.upload_state(prim)
+        g.run(tau, 10)
+        out = g.download_state()
+    sl = interior(o, 3, nn)
+    errs = rel_linf(conserved(out), conserved(ref), sl)
+    _report(f"10 steps 3d S={S}", errs)
+    assert max(errs) <= 1e-11
 
 
 def test_hundred_steps_2d_64_mms():
