@@ -214,8 +214,16 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
     carry_body(from, to, gm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) on all padded nodes
-// (viscous runs), and the stored p and c (cases with a subsonic outflow face).
+// T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) of one node, CalcValues
+// (src/Functions.c:2462-2476)
+__device__ __forceinline__ void node_tmu(const Consts &k, const double p, const double rho, double &T, double &mu)
+{
+    T = fabs(p / rho);
+    mu = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+}
+
+// T and mu on all padded nodes (viscous runs), and the stored p and c (cases with a subsonic
+// outflow face).
 __device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const gdp P,
                                             const gdp C, const Geom &gm, const Consts &k, const long t)
 {
@@ -225,8 +233,10 @@ __device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, con
     const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
     const double p = pressure(k, rho, u, v, w, e);
     if (T) {
-        T[n] = fabs(p / rho);
-        mu[n] = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
+        double t, m;
+        node_tmu(k, p, rho, t, m);
+        T[n] = t;
+        mu[n] = m;
     }
     if (P) { P[n] = p; C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
 }
@@ -1161,11 +1171,12 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     const long col = (long)jj * gm.st[1] + kk;
     double win[4][NL];
     // prime the window with planes i0-G .. i0+G-1 (clamped; out-of-block planes are never used)
+    // T (field 3) is loaded only where its derivatives are written (tgrad)
 #pragma unroll
     for (int l = 0; l < NL - 1; l++) {
         const int ip = max(0, min(i0 - G + l, gm.ng[0] - 1));
 #pragma unroll
-        for (int f = 0; f < 4; f++) win[f][l + 1] = mine ? src[f][(long)ip * gm.st[0] + col] : 0.0;
+        for (int f = 0; f < 4; f++) win[f][l + 1] = (mine && (f < 3 || tgrad)) ? src[f][(long)ip * gm.st[0] + col] : 0.0;
     }
     for (int i = i0; i < i1; i++) {
         // shift the window and read plane i+G
@@ -1175,7 +1186,7 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
             for (int f = 0; f < 4; f++) win[f][l] = win[f][l + 1];
         const int ip = min(i + G, gm.ng[0] - 1);
 #pragma unroll
-        for (int f = 0; f < 4; f++) win[f][NL - 1] = mine ? src[f][(long)ip * gm.st[0] + col] : 0.0;
+        for (int f = 0; f < 4; f++) win[f][NL - 1] = (mine && (f < 3 || tgrad)) ? src[f][(long)ip * gm.st[0] + col] : 0.0;
         // stage plane i with its halo
         const long pbase = (long)i * gm.st[0];
         for (int e = threadIdx.x; e < LP; e += GK * GJ) {
@@ -1184,7 +1195,7 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
             const bool in = (j >= 0) && (j < gm.ng[1]) && (kq >= 0) && (kq < gm.ng[2]);
             const long m = pbase + (long)j * gm.st[1] + kq;
 #pragma unroll
-            for (int f = 0; f < 4; f++) sm[f * LP + e] = in ? src[f][m] : 0.0;
+            for (int f = 0; f < 4; f++) sm[f * LP + e] = (in && (f < 3 || tgrad)) ? src[f][m] : 0.0;
         }
         __syncthreads();
         if (mine) {
