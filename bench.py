@@ -45,8 +45,8 @@ def algo_bytes(name, n, ng, time_order=4, diag=True):
         return nint * (5 + met + 5) * FP64
     if name in ("weno_eta", "weno_zeta"):   # same + Q read-modify-write
         return nint * (5 + met + 5 + 5) * FP64
-    if name == "calc_values":        # rho,u,v,w,e in, T, mu out (all padded nodes)
-        return npad * (5 + 2) * FP64
+    if name == "ghost_values":       # rho,u,v,w,e in (interior), T, mu out (all padded nodes), ghosts filled
+        return (nint * 5 + npad * 2 + ghost * 5) * FP64
     if name == "visc_grad":          # u,v,w (+T for the general metric) in (padded), 9 or 12 derivatives out
         return npad * (3 if diag else 4) * FP64 + (9 if diag else 12) * n * ng * ng * FP64
     vin = 5 + (4 + 4 if diag else 8 + 10)       # u,v,w,T,mu + cross derivatives + metric

@@ -48,6 +48,17 @@ extern "C" {
                                        towards p_out (WriteOutflowSubsonicRudyBoundary, src/BC.c:2624-2655).
                                        The step-start sound speed is the one of the state uploaded
                                        last: the reference never recomputes U_lastStep's c. */
+#define SHX_BC_WALL_SLIP          6 /* WallInviscid: the mirror's contravariant velocity normal to
+                                       the face reversed; rho, e, p, c, T, mu from the mirror node;
+                                       interior tangential range (WriteWallSlipBoundary,
+                                       src/BC.c:2688-2719, face loops :988-1172) */
+#define SHX_BC_WALL_NOSLIP        7 /* WallViscous: velocities reversed; rho, e, p, c, T, mu from the
+                                       mirror node; interior tangential range
+                                       (WriteWallNoSlipBoundary, src/BC.c:2657-2686, loops :804-986).
+                                       On the ghost nodes of both wall kinds the WENO sweeps apply the
+                                       reference's BC_Corrector to the convective momentum terms
+                                       (src/WENO.c:265-277), and shx_build_metric mirrors the ghost
+                                       metric of the face slab (src/Functions.c:6849-7014). */
 
 /* Arithmetic forms. */
 #define SHX_FP64 0   /* every operation in FP64 (the reference's PRECISION=2) */
@@ -95,8 +106,9 @@ int shx_padded_extent(const shx_case *c, int ng[3]);
 /* Geometry (once per run).  shx_build_metric: CreateMetric (src/Functions.c:6388-6509) on the
  * extended coordinate arrays (extent ng+2 per direction, 1 in 2D z; the reference's
  * x_extrapolate), followed by the periodic ghost transfer of the metric
- * (TransferMeshParameter, src/Functions.c:3481-3578).  shx_upload_metric takes metric arrays a
- * caller already has (strct_mesh xi_x ... zeta_z, jacobian), order
+ * (TransferMeshParameter, src/Functions.c:3481-3578) and, on wall faces, the mirrored ghost
+ * metric (mirrorGhostCellsMetric, src/Functions.c:6849-7014).  shx_upload_metric takes metric
+ * arrays a caller already has (strct_mesh xi_x ... zeta_z, jacobian, mirrored at walls), order
  * xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z,J. */
 int shx_build_metric(shx_case *c, const double *xe, const double *ye, const double *ze);
 int shx_upload_metric(shx_case *c, const double *const met[10]);

@@ -20,6 +20,8 @@ QUIRK_METRIC_DIAG3D = 2
 QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MMS, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
+BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
+WALLS = (BC_WALL_SLIP, BC_WALL_NOSLIP)
 
 
 class OrcCfg(ctypes.Structure):
@@ -33,7 +35,7 @@ class OrcCfg(ctypes.Structure):
         ("v_in", ctypes.c_double), ("w_in", ctypes.c_double), ("p_in", ctypes.c_double),
         ("p_out", ctypes.c_double), ("alpha_nonref", ctypes.c_double), ("tau_step", ctypes.c_double),
         ("p_last", ctypes.c_void_p), ("rho_last", ctypes.c_void_p), ("u_last", ctypes.c_void_p),
-        ("c_last", ctypes.c_void_p),
+        ("c_last", ctypes.c_void_p), ("corr", ctypes.c_void_p * 3),
         ("g", ctypes.c_int), ("ng", ctypes.c_int * 3), ("ntot", ctypes.c_long),
         ("upsilon", ctypes.c_double), ("psi", ctypes.c_double), ("gam_heat", ctypes.c_double),
         ("suth", ctypes.c_double), ("eps", ctypes.c_double),
@@ -110,6 +112,12 @@ class Oracle:
         self.c = c
         self.shape = (c.ng[0], c.ng[1], c.ng[2])
         self.g = c.g
+        self.corr = None
+        if any(b in WALLS for b in bc[:2 * dims]):
+            # BC_Corrector arrays: 1 everywhere, set on wall ghosts by the wall writers
+            self.corr = [np.ones(self.shape) for _ in range(3)]
+            for m in range(3):
+                c.corr[m] = self.corr[m].ctypes.data
 
     # ---- allocation
     def zeros(self, nf):
@@ -136,6 +144,12 @@ class Oracle:
         out = np.zeros(5)
         lib().orc_weno_interface(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
                                  i, j, k, _ptr(out))
+        return out
+
+    def point_flux(self, d, met, prim, der, i, j, k):
+        out = np.zeros(5)
+        lib().orc_point_flux_at(ctypes.byref(self.c), d, _bundle(met), _bundle(prim), _bundle(der),
+                                i, j, k, _ptr(out))
         return out
 
     def visc_derivs(self, prim, der):
@@ -208,6 +222,10 @@ class Oracle:
 
     def check_nan(self, prim):
         return lib().orc_check_nan(ctypes.byref(self.c), _bundle(prim))
+
+    def mirror_metric(self, met):
+        """Wall faces: ghost metric copied from the mirror nodes (mirrorGhostCellsMetric), in place."""
+        lib().orc_mirror_metric(ctypes.byref(self.c), _bundle(met))
 
     def metric_from_coords(self, xe, ye, ze):
         met = self.zeros(10)

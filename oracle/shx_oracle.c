@@ -30,7 +30,7 @@ int orc_define(orc_cfg *c)
     if (c->S != 5 && c->S != 9) return -1;
     if (c->time_order != 3 && c->time_order != 4) return -1;
     for (d = 0; d < 6; d++) {
-        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_OUTFLOW_RUDY) return -1;
+        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_WALL_NOSLIP) return -1;
         if ((c->bc[d] == ORC_BC_PERIODIC) != (c->bc[d ^ 1] == ORC_BC_PERIODIC)) return -1;
     }
     c->g = (c->S + 1) / 2;
@@ -246,15 +246,18 @@ double orc_phi_w9(double eps, const double *f, const double *d)
          + 1. / 60. * w4 * (3. * q2 - 2. * q3 + 3. * q4) - 1. / 840. * (39. * q2 - 14. * q3 + 3. * q4);
 }
 
-/* pointwise flux in direction dir (src/WENO.c:251-290, BC_Corrector = 1 away from walls) */
+/* pointwise flux in direction dir (src/WENO.c:251-290); the convective momentum terms carry the
+ * BC_Corrector factors (1 away from wall ghosts) */
 static void point_flux(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
                        double *const der[8], long n, double F[5])
 {
     double J = met[9][n], rho = prim[0][n], th = der[4 + dir][n], p = der[0][n];
+    double c0 = c->corr[0] ? c->corr[0][n] : 1.0, c1 = c->corr[1] ? c->corr[1][n] : 1.0,
+           c2 = c->corr[2] ? c->corr[2][n] : 1.0;
     F[0] = J * (rho * th);
-    F[1] = J * (rho * th * prim[1][n] * 1.0 + met[3 * dir + 0][n] * p * c->upsilon);
-    F[2] = J * (rho * th * prim[2][n] * 1.0 + met[3 * dir + 1][n] * p * c->upsilon);
-    F[3] = J * (rho * th * prim[3][n] * 1.0 + met[3 * dir + 2][n] * p * c->upsilon);
+    F[1] = J * (rho * th * prim[1][n] * c0 + met[3 * dir + 0][n] * p * c->upsilon);
+    F[2] = J * (rho * th * prim[2][n] * c1 + met[3 * dir + 1][n] * p * c->upsilon);
+    F[3] = J * (rho * th * prim[3][n] * c2 + met[3 * dir + 2][n] * p * c->upsilon);
     F[4] = J * (rho * th * (prim[4][n] + p / rho * c->upsilon));
 }
 
@@ -404,6 +407,12 @@ void orc_weno_interface(const orc_cfg *c, int dir, double *const met[10], double
     fill_point_flux(c, dir, met, prim, der, F);
     interface_flux(c, dir, met, prim, der, F, orc_idx(c, i, j, k), out);
     free_fields(F, 5);
+}
+
+void orc_point_flux_at(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                       double *const der[8], int i, int j, int k, double F[5])
+{
+    point_flux(c, dir, met, prim, der, orc_idx(c, i, j, k), F);
 }
 
 /* CreateViscidMetric, src/Functions.c:6245-6386: 30 factors of one node for direction dir.
@@ -689,14 +698,47 @@ static void bc_outflow_subsonic(const orc_cfg *c, int rudy, double *const met[10
     der[3][n] = der[3][s];
     der[1][n] = der[1][s];
 }
+/* WriteWallSlipBoundary (src/BC.c:2688-2719): the mirror's contravariant velocities times the
+ * face's corrector (the normal one reversed), the Cartesian velocity that has them under the ghost
+ * node's metric rows, the corrector stored for the momentum fluxes; rho, p, e, c, mu, T copied.
+ * WriteWallNoSlipBoundary (src/BC.c:2657-2686): velocities and contravariant velocities reversed,
+ * corrector -1 on all three momentum fluxes; rho, p, e, c, mu, T copied. */
+static void bc_wall(const orc_cfg *c, int slip, const double corr[3], double *const met[10],
+                    double *const prim[5], double *const der[8], long n, long s)
+{
+    int m;
+    if (slip) {
+        double th[3] = {corr[0] * der[4][s], corr[1] * der[5][s], corr[2] * der[6][s]}, u, v, w;
+        for (m = 0; m < 3; m++) der[4 + m][n] = th[m];
+        velocity_from_theta(met, n, th, &u, &v, &w);
+        prim[1][n] = u; prim[2][n] = v; prim[3][n] = w;
+        for (m = 0; m < 3; m++) if (c->corr[m]) c->corr[m][n] = corr[m];
+    } else {
+        for (m = 0; m < 3; m++) der[4 + m][n] = -der[4 + m][s];
+        for (m = 1; m < 4; m++) prim[m][n] = -prim[m][s];
+        for (m = 0; m < 3; m++) if (c->corr[m]) c->corr[m][n] = -1.0;
+    }
+    prim[0][n] = prim[0][s];
+    der[0][n] = der[0][s];
+    prim[4][n] = prim[4][s];
+    der[2][n] = der[2][s];
+    der[3][n] = der[3][s];
+    der[1][n] = der[1][s];
+}
+
 static void apply_face(const orc_cfg *c, int face, double *const met[10], double *const prim[5],
                        double *const der[8], double *const bs[6])
 {
     int d = face / 2, upper = face % 2, kind = c->bc[face];
     int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2, l, a, b;
     int lo1, hi1, lo2, hi2;
+    /* the wall writers' corrector: the momentum component along the face's index direction */
+    double corr[3] = {1.0, 1.0, 1.0};
+    corr[d] = -1.0;
     if (kind == ORC_BC_PERIODIC) return;
-    if (kind == ORC_BC_MMS) { lo1 = rs(c, t1); hi1 = re(c, t1); lo2 = rs(c, t2); hi2 = re(c, t2); }
+    if (kind == ORC_BC_MMS || kind == ORC_BC_WALL_SLIP || kind == ORC_BC_WALL_NOSLIP) {
+        lo1 = rs(c, t1); hi1 = re(c, t1); lo2 = rs(c, t2); hi2 = re(c, t2);
+    }
     else { lo1 = 0; hi1 = ge(c, t1); lo2 = 0; hi2 = ge(c, t2); }
     for (l = 0; l < c->g; l++) {
         int gi = upper ? re(c, d) + 1 + l : rs(c, d) - 1 - l;
@@ -715,7 +757,34 @@ static void apply_face(const orc_cfg *c, int face, double *const met[10], double
                 else if (kind == ORC_BC_OUTFLOW_RIEMANN) bc_outflow_riemann(c, met, prim, der, n, s);
                 else if (kind == ORC_BC_OUTFLOW_NORMAL) bc_outflow_subsonic(c, 0, met, prim, der, n, s);
                 else if (kind == ORC_BC_OUTFLOW_RUDY) bc_outflow_subsonic(c, 1, met, prim, der, n, s);
+                else if (kind == ORC_BC_WALL_SLIP) bc_wall(c, 1, corr, met, prim, der, n, s);
+                else if (kind == ORC_BC_WALL_NOSLIP) bc_wall(c, 0, corr, met, prim, der, n, s);
             }
+    }
+}
+
+void orc_mirror_metric(const orc_cfg *c, double *const met[10])
+{
+    int face;
+    for (face = 0; face < 2 * c->dims; face++) {
+        int d = face / 2, upper = face % 2, kind = c->bc[face];
+        int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2, l, a, b, m;
+        if (kind != ORC_BC_WALL_SLIP && kind != ORC_BC_WALL_NOSLIP) continue;
+        for (l = 0; l < c->g; l++) {
+            int gi = upper ? re(c, d) + 1 + l : rs(c, d) - 1 - l;
+            int si = upper ? re(c, d) - l : rs(c, d) + l;
+            for (a = 0; a <= ge(c, t1); a++)
+                for (b = 0; b <= ge(c, t2); b++) {
+                    int idx[3], sdx[3];
+                    long n, s;
+                    idx[d] = gi; sdx[d] = si;
+                    idx[t1] = a; sdx[t1] = a;
+                    idx[t2] = b; sdx[t2] = b;
+                    n = orc_idx(c, idx[0], idx[1], idx[2]);
+                    s = orc_idx(c, sdx[0], sdx[1], sdx[2]);
+                    for (m = 0; m < 10; m++) met[m][n] = met[m][s];
+                }
+        }
     }
 }
 

@@ -26,6 +26,9 @@
  *   orc_rk_*              closed form (linear ODE per node reproduces RK3/RK4 polynomials)
  *   orc_metric_*          exact on affine grids; 2nd-order central differences
  *   orc_ghost_periodic    permutation identity
+ *   wall writers          ghost state against an independent linear solve (slip) and sign
+ *                         flips (no-slip); quiescent gas and wall-parallel stream stay exact
+ *   orc_mirror_metric     copy identity on every ghost node of the slab
  *   orc_cfl_dt, orc_check_nan   brute force
  * The reference itself cannot be compiled here (it needs mpi.h and cgnslib.h,
  * which the image lacks), so no compiled-reference fixture exists.
@@ -49,6 +52,8 @@ extern "C" {
 #define ORC_BC_OUTFLOW_RIEMANN 3   /* src/BC.c:2572-2622 */
 #define ORC_BC_OUTFLOW_NORMAL 4    /* src/BC.c:2544-2570: p = p_out, rho, e, c from the mirror node */
 #define ORC_BC_OUTFLOW_RUDY 5      /* src/BC.c:2624-2655: non-reflecting relaxation towards p_out */
+#define ORC_BC_WALL_SLIP 6         /* WallInviscid, src/BC.c:2688-2719 (face loops :988-1172) */
+#define ORC_BC_WALL_NOSLIP 7       /* WallViscous, src/BC.c:2657-2686 (face loops :804-986) */
 
 typedef struct orc_cfg {
     /* inputs */
@@ -70,6 +75,10 @@ typedef struct orc_cfg {
        initialisation: CalcValues only runs on U_RK).  Unused unless a face is ORC_BC_OUTFLOW_RUDY. */
     double tau_step;
     const double *p_last, *rho_last, *u_last, *c_last;
+    /* BC_Corrector_{xi,eta,zeta}Momentum (src/Functions.c:1128-1130): padded arrays, 1 everywhere
+       after check_Metric (src/Functions.c:6212-6214), set on wall ghosts by the wall writers and
+       multiplying the convective momentum fluxes (src/WENO.c:265-277).  NULL: 1 everywhere. */
+    double *corr[3];
     /* derived by orc_define */
     int g;
     int ng[3];
@@ -108,10 +117,18 @@ void orc_weno_dir(const orc_cfg *c, int dir, double *const met[10], double *cons
 void orc_weno_interface(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
                         double *const der[8], int i, int j, int k, double out[5]);
 
+/* pointwise flux of node (i,j,k) in direction dir, as the WENO sweep forms it */
+void orc_point_flux_at(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
+                       double *const der[8], int i, int j, int k, double F[5]);
+
 void orc_visc_fac(int dir, const double m[10], double fac[30]);
 void orc_visc_derivs(const orc_cfg *c, double *const prim[5], double *const der[8], double *const grad[12]);
 void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
                   double *const der[8], double *const grad[12], double *const q[5]);
+
+/* mirrorGhostCellsMetric (src/Functions.c:6849-7014): on wall faces, the metric of every ghost
+ * node of the face slab (full tangential range) is copied from its mirror node, faces in order. */
+void orc_mirror_metric(const orc_cfg *c, double *const met[10]);
 
 void orc_zero(const orc_cfg *c, double *const f[], int nf);
 void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);

@@ -41,12 +41,10 @@ using namespace shx;
 // the node at the wrapped index of each periodic direction (fixed point of the ghost-inclusive
 // halo passes, src/Functions.c:2117-2136; indices along non-periodic directions are kept, so a
 // face slab's ghosts wrap along the periodic tangential directions only).
-__device__ __forceinline__ void ghost_body(const Fields10 &f, const int nf, const Geom &gm, const long t)
+// node index of the periodic image of padded node idx; ghost: idx is a ghost in a periodic direction
+__device__ __forceinline__ long periodic_image(const Geom &gm, const int idx[3], bool &ghost)
 {
-    if (t >= gm.ntot) return;
-    int idx[3];
-    const long n = padded_node(gm, t, idx);
-    bool ghost = false;
+    ghost = false;
     long src = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -57,6 +55,15 @@ __device__ __forceinline__ void ghost_body(const Fields10 &f, const int nf, cons
         }
         src += (long)s * gm.st[d];
     }
+    return src;
+}
+__device__ __forceinline__ void ghost_body(const Fields10 &f, const int nf, const Geom &gm, const long t)
+{
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    bool ghost;
+    const long src = periodic_image(gm, idx, ghost);
     if (!ghost) return;
 #pragma unroll
     for (int m = 0; m < 10; m++)
@@ -119,25 +126,52 @@ __device__ __forceinline__ void face_save_body(const BcArgs &a, const long t)
     a.UL[n] = a.pr.f[1][n];
 }
 
+// contravariant velocities of the mirror node s (its CalcValues theta), the one along `flip`
+// reversed (flip < 0: none), then the Cartesian velocity that has them under the metric rows of
+// ghost node n: Cramer's rule, terms in the reference's order (src/BC.c:2556-2558, 2697-2699)
+__device__ __forceinline__ void ghost_velocity(const BcArgs &a, const long n, const long s, const int flip,
+                                               double &u, double &v, double &w)
+{
+    const double us = a.pr.f[1][s], vs = a.pr.f[2][s], ws = a.pr.f[3][s];
+    double t0 = us * a.met[0][s] + vs * a.met[1][s] + ws * a.met[2][s];
+    double t1 = us * a.met[3][s] + vs * a.met[4][s] + ws * a.met[5][s];
+    double t2 = us * a.met[6][s] + vs * a.met[7][s] + ws * a.met[8][s];
+    if (flip == 0) t0 = -t0;
+    if (flip == 1) t1 = -t1;
+    if (flip == 2) t2 = -t2;
+    const double xx = a.met[0][n], xy = a.met[1][n], xz = a.met[2][n], ex = a.met[3][n], ey = a.met[4][n],
+                 ez = a.met[5][n], zx = a.met[6][n], zy = a.met[7][n], zz = a.met[8][n];
+    const double den = -xx * ey * zz + xx * ez * zy + xy * ex * zz - xy * ez * zx - xz * ex * zy + xz * ey * zx;
+    u = -(t0 * ey * zz - t0 * ez * zy - t1 * xy * zz + t1 * xz * zy + t2 * xy * ez - t2 * xz * ey) / den;
+    v = -(-t0 * ex * zz + t0 * ez * zx + t1 * xx * zz - t1 * xz * zx - t2 * xx * ez + t2 * xz * ex) / den;
+    w = -(-t0 * ex * zy + t0 * ey * zx + t1 * xx * zy - t1 * xy * zx - t2 * xx * ey + t2 * xy * ex) / -den;
+}
+
 __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
 {
     long n, s;
     if (!face_node(a, t, n, s)) return;
     const Consts &k = a.k;
+    if (a.kind == SHX_BC_WALL_SLIP || a.kind == SHX_BC_WALL_NOSLIP) {
+        // WriteWallSlipBoundary (src/BC.c:2688-2719): the mirror's contravariant velocity along
+        // the face's direction reversed; WriteWallNoSlipBoundary (src/BC.c:2657-2686): all
+        // velocities reversed.  rho, e, p, c, T and mu from the mirror node.  The corrector these
+        // writers store is applied by the WENO sweeps on the ghost nodes of wall faces.
+        double u, v, w;
+        if (a.kind == SHX_BC_WALL_SLIP) {
+            ghost_velocity(a, n, s, a.dir, u, v, w);
+        } else {
+            u = -a.pr.f[1][s]; v = -a.pr.f[2][s]; w = -a.pr.f[3][s];
+        }
+        a.pr.f[0][n] = a.pr.f[0][s]; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
+        a.pr.f[4][n] = a.pr.f[4][s];
+        if (a.P) { a.P[n] = a.P[s]; a.C[n] = a.C[s]; }
+        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        return;
+    }
     if (a.kind == SHX_BC_OUTFLOW_NORMAL || a.kind == SHX_BC_OUTFLOW_RUDY) {
-        // contravariant velocities of the mirror node (its CalcValues theta), then the Cartesian
-        // velocity that has them under the ghost node's metric rows: Cramer's rule, terms in the
-        // reference's order (src/BC.c:2556-2558)
-        const double us = a.pr.f[1][s], vs = a.pr.f[2][s], ws = a.pr.f[3][s];
-        const double t0 = us * a.met[0][s] + vs * a.met[1][s] + ws * a.met[2][s];
-        const double t1 = us * a.met[3][s] + vs * a.met[4][s] + ws * a.met[5][s];
-        const double t2 = us * a.met[6][s] + vs * a.met[7][s] + ws * a.met[8][s];
-        const double xx = a.met[0][n], xy = a.met[1][n], xz = a.met[2][n], ex = a.met[3][n], ey = a.met[4][n],
-                     ez = a.met[5][n], zx = a.met[6][n], zy = a.met[7][n], zz = a.met[8][n];
-        const double den = -xx * ey * zz + xx * ez * zy + xy * ex * zz - xy * ez * zx - xz * ex * zy + xz * ey * zx;
-        const double u = -(t0 * ey * zz - t0 * ez * zy - t1 * xy * zz + t1 * xz * zy + t2 * xy * ez - t2 * xz * ey) / den;
-        const double v = -(-t0 * ex * zz + t0 * ez * zx + t1 * xx * zz - t1 * xz * zx - t2 * xx * ez + t2 * xz * ex) / den;
-        const double w = -(-t0 * ex * zy + t0 * ey * zx + t1 * xx * zy - t1 * xy * zx - t2 * xx * ey + t2 * xy * ex) / -den;
+        double u, v, w;
+        ghost_velocity(a, n, s, -1, u, v, w);
         double p, rho;
         if (a.kind == SHX_BC_OUTFLOW_RUDY) {
             p = (a.PL[n] + a.alpha_tau * a.p_out + a.RL[n] * a.C0[s] * (u - a.UL[n]) / k.upsilon) / (1.0 + a.alpha_tau);
@@ -184,6 +218,15 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         if (a.P) { a.P[n] = p; a.C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
     }
 }
+// mirrorGhostCellsMetric (src/Functions.c:6849-7014): every ghost node of a wall face's slab (the
+// whole tangential range) takes the metric of its mirror node
+__global__ void k_mirror_metric(BcArgs a, Fields10 met)
+{
+    long n, s;
+    if (!face_node(a, (long)blockIdx.x * blockDim.x + threadIdx.x, n, s)) return;
+#pragma unroll
+    for (int m = 0; m < 10; m++) met.f[m][n] = met.f[m][s];
+}
 __global__ void k_face_save(BcArgs a)
 {
     face_save_body(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
@@ -222,15 +265,11 @@ __device__ __forceinline__ void node_tmu(const Consts &k, const double p, const 
     mu = k.mms ? 1.0 : ((1.0 + k.suth) * pow(p / rho, 1.5) / (p / rho + k.suth));
 }
 
-// T and mu on all padded nodes (viscous runs), and the stored p and c (cases with a subsonic
-// outflow face).
-__device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const gdp P,
-                                            const gdp C, const Geom &gm, const Consts &k, const long t)
+// T and mu of node n (viscous runs), and its stored p and c (cases with a subsonic outflow face)
+__device__ __forceinline__ void node_values(const double rho, const double u, const double v, const double w,
+                                            const double e, const gdp T, const gdp mu, const gdp P, const gdp C,
+                                            const Consts &k, const long n)
 {
-    if (t >= gm.ntot) return;
-    int idx[3];
-    const long n = padded_node(gm, t, idx);
-    const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
     const double p = pressure(k, rho, u, v, w, e);
     if (T) {
         double t, m;
@@ -240,9 +279,43 @@ __device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, con
     }
     if (P) { P[n] = p; C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
 }
+// CalcValues on all padded nodes
+__device__ __forceinline__ void values_body(const CFields5 &pr, const gdp T, const gdp mu, const gdp P,
+                                            const gdp C, const Geom &gm, const Consts &k, const long t)
+{
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    node_values(pr.f[0][n], pr.f[1][n], pr.f[2][n], pr.f[3][n], pr.f[4][n], T, mu, P, C, k, n);
+}
+// Periodic ghost fill of the primitives and CalcValues in one pass: a ghost node copies its
+// periodic image and evaluates the values from the copied primitives, which gives the bits of the
+// fill followed by CalcValues.  Only ghost nodes are written to pr, and they are never an image,
+// so no thread reads a node another thread writes.
+__device__ __forceinline__ void ghost_values_body(const Fields5 &pr, const gdp T, const gdp mu, const gdp P,
+                                                  const gdp C, const Geom &gm, const Consts &k, const long t)
+{
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    bool ghost;
+    const long src = periodic_image(gm, idx, ghost);
+    double q[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) q[m] = pr.f[m][src];
+    if (ghost) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) pr.f[m][n] = q[m];
+    }
+    node_values(q[0], q[1], q[2], q[3], q[4], T, mu, P, C, k, n);
+}
 __global__ void k_calc_values(CFields5 pr, gdp T, gdp mu, gdp P, gdp C, Geom gm, Consts k)
 {
     values_body(pr, T, mu, P, C, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+__global__ void k_ghost_values(Fields5 pr, gdp T, gdp mu, gdp P, gdp C, Geom gm, Consts k)
+{
+    ghost_values_body(pr, T, mu, P, C, gm, k, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---- block placement ------------------------------------------------------------------------
@@ -280,7 +353,9 @@ struct WenoArgs {
                                       // (diagonal metrics: m0 is the row's diagonal entry)
     Fields5 q;            // stage residual, interior nodes
     gcdp P, C;            // stored p and c, read on the ghost nodes of subsonic outflow faces
-    int pc_lo, pc_hi;     // the lower / upper face of the sweep direction is a subsonic outflow
+    int pc_lo, pc_hi;     // the lower / upper face of the sweep direction stores p and c on its ghosts
+    int w_lo, w_hi;       // the lower / upper face of the sweep direction is a wall
+    double c_lo[3], c_hi[3];  // its corrector of the convective momentum terms (src/WENO.c:265-277)
     int tg[4];            // tile grid (A tiles, B tiles, sweep tiles) and order of a single-case launch
     long sD, sA, sB;      // strides: sweep, fast transverse, slow transverse
     int nD, nA, nB;       // interior counts
@@ -358,14 +433,22 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         if ((a.pc_lo && dp < a.oD) || (a.pc_hi && dp >= a.oD + a.nD)) { p = a.P[n]; cs = a.C[n]; }
         const double pu = p * k.upsilon;
         const int o = aa * WD + dd;
+        // velocities of the convective momentum terms: times the wall corrector (+-1, exact) on
+        // the ghost nodes of a wall face of this direction, the velocities themselves elsewhere
+        double uc[3] = {u, v, w};
+        if ((a.w_lo && dp < a.oD) || (a.w_hi && dp >= a.oD + a.nD)) {
+            const bool lo = dp < a.oD;
+#pragma unroll
+            for (int q = 0; q < 3; q++) uc[q] *= lo ? a.c_lo[q] : a.c_hi[q];
+        }
         if constexpr (AX < 0) {
             const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n];
             const double th = u * mx + v * my + w * mz;
             const double rt = rho * th;
             sm[V_F0 * NW + o] = J * rt;
-            sm[V_F1 * NW + o] = J * (rt * u + mx * pu);
-            sm[V_F2 * NW + o] = J * (rt * v + my * pu);
-            sm[V_F3 * NW + o] = J * (rt * w + mz * pu);
+            sm[V_F1 * NW + o] = J * (rt * uc[0] + mx * pu);
+            sm[V_F2 * NW + o] = J * (rt * uc[1] + my * pu);
+            sm[V_F3 * NW + o] = J * (rt * uc[2] + mz * pu);
             sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
             const double mn = sqrt(mx * mx + my * my + mz * mz);
             const double imn = frcp(mn);
@@ -380,7 +463,7 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             const double rt = rho * th;
             sm[V_F0 * NW + o] = J * rt;
 #pragma unroll
-            for (int q = 0; q < 3; q++) sm[(V_F1 + q) * NW + o] = (q == AX) ? J * (rt * uu[q] + md * pu) : J * (rt * uu[q]);
+            for (int q = 0; q < 3; q++) sm[(V_F1 + q) * NW + o] = (q == AX) ? J * (rt * uc[q] + md * pu) : J * (rt * uc[q]);
             sm[V_F4 * NW + o] = J * (rt * (e + pu * ir));
             const double lam = fabs(th) + cs * fabs(md);
             sm[V_LAM * NW + o] = (md > 0.0) ? lam : -lam;
@@ -1293,6 +1376,16 @@ __global__ __launch_bounds__(256) void k_values_batch(BATCH_PARAMS(ValuesArgs))
         values_body(a.pr, a.T, a.mu, a.P, a.C, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
     });
 }
+struct GhostValuesArgs { Fields5 pr; gdp T, mu, P, C; Geom gm; Consts k; };
+template <bool P>
+__global__ __launch_bounds__(256) void k_ghost_values_batch(BATCH_PARAMS(GhostValuesArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        const GhostValuesArgs &a = rec(A, w.x);
+        ghost_values_body(a.pr, a.T, a.mu, a.P, a.C, a.gm, a.k, (long)w.y * 256 + threadIdx.x);
+    });
+}
 template <bool P>
 __global__ __launch_bounds__(256) void k_rk_batch(BATCH_PARAMS(UpdArgs))
 {
@@ -1551,9 +1644,25 @@ static int set_err(int code, const char *fmt, ...)
 
 enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA };
 static const char *g_knames[SHX_NUM_KERNELS] = {
-    "ghost_periodic", "calc_values", "weno_xi", "weno_eta", "weno_zeta",
+    "ghost_periodic", "ghost_values", "weno_xi", "weno_eta", "weno_zeta",
     "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces",
     "weno_visc_zeta"};
+
+// face kinds whose writers store a p and c that the equation of state does not give from the
+// ghost's rho,u,v,w,e (subsonic outflow; the slip wall, whose ghost velocity is not the mirror's)
+static bool stores_pc(const int kind)
+{
+    return kind == SHX_BC_OUTFLOW_NORMAL || kind == SHX_BC_OUTFLOW_RUDY || kind == SHX_BC_WALL_SLIP;
+}
+static bool is_wall(const int kind) { return kind == SHX_BC_WALL_SLIP || kind == SHX_BC_WALL_NOSLIP; }
+// BC_Corrector of a face's ghost nodes: slip walls reverse the momentum component along the
+// face's index direction (src/BC.c:988-1172), no-slip walls all three (src/BC.c:2657-2686)
+static void wall_corrector(const int kind, const int dir, double cr[3])
+{
+    for (int q = 0; q < 3; q++) cr[q] = 1.0;
+    if (kind == SHX_BC_WALL_SLIP) cr[dir] = -1.0;
+    if (kind == SHX_BC_WALL_NOSLIP) cr[0] = cr[1] = cr[2] = -1.0;
+}
 
 struct shx_case {
     shx_desc d;
@@ -1647,7 +1756,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
         return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
-        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_OUTFLOW_RUDY)
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_WALL_NOSLIP)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
         if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
             return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
@@ -1679,7 +1788,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
             if (d.bc[2 * a] == SHX_BC_MANUFACTURED || d.bc[2 * a + 1] == SHX_BC_MANUFACTURED) c->needs_bs = 1;
             for (int u = 0; u < 2; u++) {
                 const int kind = d.bc[2 * a + u];
-                if (kind == SHX_BC_OUTFLOW_NORMAL || kind == SHX_BC_OUTFLOW_RUDY) c->pc = 1;
+                if (stores_pc(kind)) c->pc = 1;
                 if (kind == SHX_BC_OUTFLOW_RUDY) c->rudy = 1;
             }
         }
@@ -1844,7 +1953,8 @@ static void face_args(const shx_case *c, double *const *pr, int f, double tau, B
     a.g = gm.g;
     a.t1 = (a.dir == 0) ? 1 : 0;
     a.t2 = (a.dir == 2) ? 1 : 2;
-    if (kind == SHX_BC_MANUFACTURED) {
+    if (kind == SHX_BC_MANUFACTURED || is_wall(kind)) {
+        // interior tangential range (src/ManufacturedSolution.c:202-338, src/BC.c:804-1172)
         a.lo1 = gm.lo[a.t1]; a.n1 = gm.n[a.t1]; a.lo2 = gm.lo[a.t2]; a.n2 = gm.n[a.t2];
     } else {
         a.lo1 = 0; a.n1 = gm.ng[a.t1]; a.lo2 = 0; a.n2 = gm.ng[a.t2];
@@ -1928,6 +2038,17 @@ extern "C" int shx_build_metric(shx_case *c, const double *xe, const double *ye,
     LAUNCH_CHECK();
     if ((rc = tend(c))) return rc;
     if ((rc = ghost_fill(c, c->met, 10))) return rc;
+    // wall faces, in face order: the ghost metric mirrored (full tangential range)
+    for (int f = 0; f < 2 * gm.dims; f++) {
+        if (!is_wall(c->d.bc[f])) continue;
+        BcArgs a;
+        long nodes;
+        face_args(c, c->q, f, 0.0, a, nodes);
+        a.lo1 = 0; a.n1 = gm.ng[a.t1]; a.lo2 = 0; a.n2 = gm.ng[a.t2];
+        nodes = (long)a.n1 * a.n2 * a.g;
+        k_mirror_metric<<<nblk(nodes, 256), 256, 0, c->stream>>>(a, mf);
+        LAUNCH_CHECK();
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     hipFree(dx); hipFree(dy); hipFree(dz);
     c->has_metric = 1;
@@ -2105,8 +2226,12 @@ static WenoArgs weno_args(const shx_case *c, double *const *pr, int dir, int fir
     a.coalesce_d = (a.sD == 1);
     a.alignA = (a.sA == 1);
     a.P = c->P; a.C = c->C;
-    a.pc_lo = (c->d.bc[2 * dir] == SHX_BC_OUTFLOW_NORMAL || c->d.bc[2 * dir] == SHX_BC_OUTFLOW_RUDY);
-    a.pc_hi = (c->d.bc[2 * dir + 1] == SHX_BC_OUTFLOW_NORMAL || c->d.bc[2 * dir + 1] == SHX_BC_OUTFLOW_RUDY);
+    a.pc_lo = stores_pc(c->d.bc[2 * dir]);
+    a.pc_hi = stores_pc(c->d.bc[2 * dir + 1]);
+    a.w_lo = is_wall(c->d.bc[2 * dir]);
+    a.w_hi = is_wall(c->d.bc[2 * dir + 1]);
+    wall_corrector(c->d.bc[2 * dir], dir, a.c_lo);
+    wall_corrector(c->d.bc[2 * dir + 1], dir, a.c_hi);
     a.k = c->k;
     return a;
 }
@@ -2161,13 +2286,16 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
 {
     const Geom &gm = c->gm;
     int rc;
-    if ((rc = ghost_fill(c, pr, 5))) return rc;
-    if (c->d.inviscid && c->pc) {
-        // stored p and c for the subsonic outflow writers and the sweeps
-        CFields5 cp;
-        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+    if (c->d.inviscid && !c->pc) {
+        if ((rc = ghost_fill(c, pr, 5))) return rc;
+    } else {
+        // periodic ghosts, then T and mu (viscous) and the stored p and c (subsonic outflow faces)
+        Fields5 fp;
+        for (int m = 0; m < 5; m++) fp.f[m] = pr[m];
+        const bool visc = !c->d.inviscid;
         if ((rc = tbeg(c, KT_VALUES))) return rc;
-        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, nullptr, nullptr, c->P, c->C, gm, c->k);
+        k_ghost_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(fp, visc ? c->T : nullptr, visc ? c->mu : nullptr,
+                                                                    c->P, c->C, gm, c->k);
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
     }
@@ -2176,10 +2304,6 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
         Fields12 gf;
         for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
         for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
-        if ((rc = tbeg(c, KT_VALUES))) return rc;
-        k_calc_values<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, c->mu, c->P, c->C, gm, c->k);
-        LAUNCH_CHECK();
-        if ((rc = tend(c))) return rc;
         if ((rc = face_fill(c, pr, tau))) return rc;
         if ((rc = tbeg(c, KT_GRAD))) return rc;
         if (gm.dims == 3) {
@@ -2530,7 +2654,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 if (any) batch_add(b, (const void *)&k_face_save_batch<false>, (const void *)&k_face_save_batch<true>, 256, BK_GHOST, recs, sizeof(BcArgs), grids);
             }
         }
-        {   // periodic ghost fill of the stage input
+        if (c0->d.inviscid) {   // periodic ghost fill of the stage input
             std::vector<char> recs;
             std::vector<dim3> grids;
             for (int i = 0; i < nc; i++) {
@@ -2580,11 +2704,12 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             add_faces();
         }
         if (!c0->d.inviscid) {
+            // periodic ghost fill of the stage input with CalcValues
             std::vector<char> recs, grecs;
             std::vector<dim3> grids, ggrids;
             for (int i = 0; i < nc; i++) {
                 shx_case *c = g[i];
-                ValuesArgs a;
+                GhostValuesArgs a;
                 memset(&a, 0, sizeof a);
                 for (int m = 0; m < 5; m++) a.pr.f[m] = in[i][m];
                 a.T = c->T; a.mu = c->mu; a.P = c->P; a.C = c->C; a.gm = c->gm; a.k = c->k;
@@ -2592,14 +2717,15 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 grids.push_back(dim3(nblk(c->gm.ntot, 256)));
                 GradArgs ga;
                 memset(&ga, 0, sizeof ga);
-                ga.pr = a.pr; ga.T = c->T;
+                for (int m = 0; m < 5; m++) ga.pr.f[m] = in[i][m];
+                ga.T = c->T;
                 for (int m = 0; m < 12; m++) ga.gr.f[m] = c->grad[m];
                 ga.gm = c->gm; ga.k = c->k; ga.ci = 32; ga.tgrad = c->diag ? 0 : 1;
                 put(grecs, ga);
                 if (dims == 3) ggrids.push_back(dim3(nblk(c->gm.ng[2], GK), nblk(c->gm.ng[1], GJ), nblk(c->gm.ng[0], 32)));
                 else ggrids.push_back(dim3(nblk(c->gm.ntot, 256)));
             }
-            batch_add(b, (const void *)&k_values_batch<false>, (const void *)&k_values_batch<true>, 256, BK_VALUES, recs, sizeof(ValuesArgs), grids);
+            batch_add(b, (const void *)&k_ghost_values_batch<false>, (const void *)&k_ghost_values_batch<true>, 256, BK_VALUES, recs, sizeof(GhostValuesArgs), grids);
             add_faces();
             if (dims == 3)
                 batch_add(b, S == 5 ? (const void *)&k_march_batch<5, false> : (const void *)&k_march_batch<9, false>,

@@ -1,5 +1,5 @@
-"""Batches with boundary faces: supersonic inflow, Riemann, normal and Rudy outflow and
-manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
+"""Batches with boundary faces: supersonic inflow, Riemann, normal and Rudy outflow, slip and
+no-slip walls and manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
 must reach exactly the state it reaches when stepped alone (the single-case path's parity against
 the oracle is covered by the boundary tests); three of the cases are also held against the oracle
 directly."""
@@ -9,7 +9,7 @@ import pytest
 import orc
 from helpers import conserved, interior, rel_linf, setup_3d_turb
 from shock_amd import cases, mms, shockx
-from test_gpu_boundaries import IN_NORMAL, IN_RUDY, INOUT, _mapping, _setup, _stream
+from test_gpu_boundaries import IN_NORMAL, IN_RUDY, INOUT, NOSLIP, SLIP, _mapping, _setup, _stream
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +52,8 @@ def test_batch_with_faces_equals_cases_alone():
         _faces_case(3, 9, (14, 12, 12), INOUT + (0, 0, 0, 0), True, orc.QUIRKS_REFERENCE, "bump"),
         _faces_case(3, 5, (14, 12, 12), IN_RUDY + (0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
         _faces_case(2, 5, (24, 20, 1), IN_RUDY + IN_NORMAL + (0, 0), True, orc.QUIRKS_REFERENCE, None),
+        _faces_case(3, 5, (14, 12, 12), INOUT + SLIP + (0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+        _faces_case(2, 9, (24, 20, 1), INOUT + NOSLIP + (0, 0), False, orc.QUIRKS_REFERENCE, None),
         _mms_case(5, (16, 16, 16), orc.QUIRK_METRIC_DIAG3D),
         _periodic_case(12, 5, 11),
     ]
@@ -72,7 +74,7 @@ def test_batch_with_faces_equals_cases_alone():
             diff = max(float(np.max(np.abs(a[sl] - r[sl]))) for a, r in zip(got, ref))
             print(f"[batch-faces] case {k} {m['tag']}: max |batch - alone| = {diff:.2e}", flush=True)
             assert diff == 0.0
-            if k in (0, 4, 6):
+            if k in (0, 4, 6, 8):
                 want = o.run(tau, m["met"], m["prim"], steps, src=m["src"], bstate=m["bs"])
                 errs = rel_linf(conserved(got), conserved(want), sl)
                 if dims == 2:

@@ -1,5 +1,5 @@
 """GPU parity of the boundary writers (supersonic inflow; Riemann, normal and Rudy subsonic
-outflow; manufactured-solution faces) and the manufactured-solution path: libshockx.so against
+outflow; slip and no-slip walls; manufactured-solution faces) and the manufactured-solution path: libshockx.so against
 the oracle on identical inputs (same bar as the periodic parity tests: relative L-infinity per
 conserved variable <= 1e-12 after one step, <= 1e-10 after many).
 
@@ -27,6 +27,8 @@ GAMMA, MACH = 1.4, 0.5
 INOUT = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN)
 IN_NORMAL = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_NORMAL)
 IN_RUDY = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RUDY)
+SLIP = (orc.BC_WALL_SLIP, orc.BC_WALL_SLIP)
+NOSLIP = (orc.BC_WALL_NOSLIP, orc.BC_WALL_NOSLIP)
 OUTFLOW = (0.95, 3.0)          # p_out, AlphaNonRef
 
 
@@ -66,6 +68,7 @@ def _setup(dims, S, n, bc, inflow, inviscid=False, quirks=orc.QUIRKS_REFERENCE, 
     xe, ye, ze = cases.ext_coords(dims, S, n, mapping or cases.cartesian_unit(dims), periodic=per)
     met = o.metric_from_coords(xe, ye, ze)
     o.ghost_periodic(met)
+    o.mirror_metric(met)
     g = shockx.Case(dims=dims, space_order=S, time_order=4, n=n, gamma=GAMMA, mach=MACH,
                     reynolds=reynolds, inviscid=inviscid, manufactured=mms_on, quirks=quirks,
                     bc=bc, inflow=inflow, outflow=OUTFLOW)
@@ -98,6 +101,14 @@ CASES = [
     (2, 9, (24, 20, 1), IN_RUDY + IN_NORMAL + (0, 0), True, orc.QUIRKS_REFERENCE, None),
     (3, 5, (14, 12, 12), IN_RUDY + (0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
     (3, 5, (12, 12, 12), IN_NORMAL + IN_RUDY + INOUT, False, 0, "bump"),
+    # walls (mirrored ghost metric, corrector on the ghosts' convective momentum terms)
+    (2, 5, (24, 20, 1), INOUT + SLIP + (0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (2, 9, (24, 20, 1), INOUT + NOSLIP + (0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (2, 5, (20, 18, 1), SLIP + SLIP + (0, 0), True, orc.QUIRKS_REFERENCE, None),
+    (3, 5, (14, 12, 12), INOUT + SLIP + (0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    (3, 9, (14, 12, 12), INOUT + NOSLIP + (0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    (3, 5, (12, 12, 12), INOUT + (orc.BC_WALL_NOSLIP, orc.BC_WALL_SLIP, orc.BC_WALL_SLIP, orc.BC_WALL_NOSLIP),
+     False, 0, "bump"),
 ]
 
 

@@ -141,6 +141,49 @@ def test_hundred_steps_2d_64_mms():
     assert max(errs[:3] + errs[4:]) <= TOL_100
 
 
+@pytest.mark.parametrize("S", [5, 9])
+def test_hundred_steps_3d_turbulence(S):
+    """16^3 periodic curvilinear box with random-phase turbulence, viscous, RK4, 100 steps (the
+    oracle's own spread under one ulp of input noise: 1e-15 for W5, 1.3e-13 for W9)."""
+    o, coords, met, prim, nn = setup_3d_turb(n=16, S=S, seed=23)
+    tau = o.cfl_dt(0.5, met, prim)
+    ref = o.run(tau, met, prim, 100)
+    with _case(o, 3, S, 4, nn) as g:
+        g.upload_metric(met)
+        g.upload_state(prim)
+        g.run(tau, 100)
+        out = g.download_state()
+    sl = interior(o, 3, nn)
+    errs = rel_linf(conserved(out), conserved(ref), sl)
+    _report(f"100 steps 3d turbulence S={S}", errs)
+    assert max(errs) <= TOL_100
+
+
+def test_hundred_steps_3d_shock():
+    """16^3 Mach-3 normal shock meeting a vorticity field on a perturbed curvilinear box, WENO9,
+    Re 5000, supersonic inflow and Riemann outflow in x, periodic in y and z, 100 steps (the
+    oracle's own spread under one ulp of input noise: up to 3e-11)."""
+    nn = (16, 16, 16)
+    coords, prim, inflow = cases.shock_3d(9, nn, 5)
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    o = orc.Oracle(dims=3, S=9, n=nn, bc=bc, inflow=inflow, mach=3.0, reynolds=5000.0)
+    met = o.metric_from_coords(*coords)
+    o.ghost_periodic(met)
+    o.ghost_periodic(prim)
+    tau = o.cfl_dt(0.5, met, prim)
+    ref = o.run(tau, met, prim, 100)
+    with shockx.Case(dims=3, space_order=9, time_order=4, n=nn, gamma=1.4, mach=3.0, reynolds=5000.0,
+                     bc=bc, inflow=inflow) as g:
+        g.build_metric(*coords)
+        g.upload_state(prim)
+        g.run(tau, 100)
+        out = g.download_state()
+    sl = interior(o, 3, nn)
+    errs = rel_linf(conserved(out), conserved(ref), sl)
+    _report("100 steps 3d shock S=9", errs)
+    assert max(errs) <= TOL_100
+
+
 def test_check_metric_on_a_folded_grid():
     # x = xi + a sin(2 pi xi) with a > 1/(2 pi) folds the grid: the Jacobian is negative on a band
     nn = (24, 20, 1)

@@ -13,6 +13,11 @@ Checks (CPU only):
     (checked with a linear solve, not Cramer's rule); p = p_out (normal) or the relaxation of the
     step-start ghost pressure towards p_out (Rudy, checked against the formula evaluated here);
     rho, e, c, T and mu from the mirror (Rudy: rho scaled by p / p_mirror)
+  * slip and no-slip walls (src/BC.c:2657-2719): ghost contravariant velocities reflected (slip:
+    the normal one, velocity by a linear solve) or all velocities reversed (no-slip), rho, p, e,
+    c, T, mu from the mirror, the corrector set on the interior tangential range only; the ghost
+    metric mirrored over the whole slab (src/Functions.c:6849-7014); ghost point fluxes minus
+    their mirror's except for pressure; gas at rest and a stream along slip walls stay steady
   * face order (src/BC.c:13-74): a later face overwrites an earlier one at shared edges
   * a uniform stream at the inflow state passes an inflow/outflow block unchanged
   * the manufactured source equals a symbolic (sympy) residual of the conservation laws, and
@@ -164,6 +169,122 @@ def test_subsonic_outflow_writers(dims, rudy):
             assert np.array_equal(der[f][gh], der0[f][mi])
     sl = interior(o, dims, n)
     assert np.array_equal(prim[0][sl], before[0][sl])
+
+
+WALL_BC = {"slip": orc.BC_WALL_SLIP, "noslip": orc.BC_WALL_NOSLIP}
+
+
+@pytest.mark.parametrize("dims,kind", [(2, "slip"), (3, "slip"), (2, "noslip"), (3, "noslip")])
+def test_wall_writers_and_mirrored_metric(dims, kind):
+    # walls at j- and j+, inflow/outflow in i, periodic in k; curvilinear grid
+    n = (9, 7, 6) if dims == 3 else (9, 7, 1)
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, WALL_BC[kind], WALL_BC[kind], 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, mapping=cases.sine_bump_3d(0.05) if dims == 3 else None, quirks=0)
+    raw = [a.copy() for a in met]
+    o.mirror_metric(met)
+    g, end = o.g, o.g + n[1] - 1
+    for l in range(g):
+        for gj, mj in ((g - 1 - l, g + l), (end + 1 + l, end - l)):
+            for m in range(10):                                # every ghost node of the slab
+                assert np.array_equal(met[m][:, gj], raw[m][:, mj])
+    inner = (slice(None), slice(g, g + n[1]))
+    for m in range(10):
+        assert np.array_equal(met[m][inner], raw[m][inner])
+    prim = _random_state(o, 31)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    before = [a.copy() for a in prim]
+    o.apply_bc(met, prim, der)
+    ti = slice(g, g + n[0])                                    # interior tangential range
+    tk = slice(g, g + n[2]) if dims == 3 else slice(None)
+    for l in range(g):
+        for gj, mj in ((g - 1 - l, g + l), (end + 1 + l, end - l)):
+            gh, mi = (ti, gj, tk), (ti, mj, tk)
+            if kind == "slip":
+                theta = [der0[4][mi], -der0[5][mi], der0[6][mi]]
+                vel = _ghost_velocity_by_solve(met, gh, theta)
+                for d in range(3 if dims == 3 else 2):
+                    assert np.allclose(prim[1 + d][gh], vel[d], rtol=1e-12, atol=1e-14)
+                want_corr = (1.0, -1.0, 1.0)
+            else:
+                theta = [-der0[4 + d][mi] for d in range(3)]
+                for d in range(3):
+                    assert np.array_equal(prim[1 + d][gh], -before[1 + d][mi])
+                want_corr = (-1.0, -1.0, -1.0)
+            for d in range(3):
+                assert np.array_equal(der[4 + d][gh], theta[d])
+                assert np.all(o.corr[d][gh] == want_corr[d])
+            for m in (0, 4):
+                assert np.array_equal(prim[m][gh], before[m][mi])
+            for f in (0, 1, 2, 3):                             # p, T, c, mu from the mirror
+                assert np.array_equal(der[f][gh], der0[f][mi])
+    # the corrector stays 1 away from the wall ghosts of the interior tangential range
+    wall = np.zeros(o.shape, dtype=bool)
+    wall[ti, :g, tk] = True
+    wall[ti, end + 1:, tk] = True
+    for d in range(3):
+        assert np.all(o.corr[d][~wall] == 1.0)
+
+
+@pytest.mark.parametrize("kind", ["slip", "noslip"])
+def test_wall_ghost_fluxes_are_antisymmetric_but_for_pressure(kind):
+    # Cartesian grid: with the mirrored metric, the wall writers and the corrector, the point
+    # flux of a ghost node is minus its mirror's for mass, energy and the convective momentum
+    # terms, while the pressure terms are equal (the purpose the reference states for the
+    # corrector, src/SHOCK.c:213-217)
+    n = (8, 7, 1)
+    bc = (0, 0, WALL_BC[kind], WALL_BC[kind], 0, 0)
+    o, met, _ = _case(2, 5, n, bc)
+    o.mirror_metric(met)
+    prim = _random_state(o, 41)
+    der = o.calc_values(met, prim)
+    o.apply_bc(met, prim, der)
+    g, end, ups = o.g, o.g + n[1] - 1, 1.0 / (GAMMA * MACH * MACH)
+    for i in range(g, g + n[0]):
+        for l in range(g):
+            for gj, mj in ((g - 1 - l, g + l), (end + 1 + l, end - l)):
+                Fg = o.point_flux(1, met, prim, der, i, gj, 0)
+                Fm = o.point_flux(1, met, prim, der, i, mj, 0)
+                J, p = met[9][i, mj, 0], der[0][i, mj, 0]
+                press = [0.0] + [2.0 * J * met[3 + q][i, mj, 0] * p * ups for q in range(3)] + [0.0]
+                scale = np.max(np.abs(Fm)) + abs(J * p * ups)
+                assert np.allclose(Fg + Fm, press, rtol=0, atol=4e-16 * scale)
+
+
+@pytest.mark.parametrize("dims,kind", [(2, "slip"), (3, "slip"), (2, "noslip"), (3, "noslip")])
+def test_gas_at_rest_in_a_walled_box_stays_at_rest(dims, kind):
+    n = (10, 9, 8) if dims == 3 else (10, 9, 1)
+    w = WALL_BC[kind]
+    bc = (w, w, w, w, w, w) if dims == 3 else (w, w, w, w, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc)
+    o.mirror_metric(met)
+    prim = o.zeros(5)
+    prim[0][...] = 1.3
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], 0.9, prim[0], GAMMA, MACH)
+    out = o.run(0.01, met, prim, 3)
+    sl = interior(o, dims, n)
+    assert np.max(np.abs(out[0][sl] - 1.3)) < 1e-14
+    for m in (1, 2, 3):
+        assert np.max(np.abs(out[m][sl])) < 1e-14
+    assert np.max(np.abs(out[4][sl] - prim[4][sl])) < 1e-14 * np.max(prim[4][sl])
+
+
+@pytest.mark.parametrize("dims,inviscid", [(2, True), (3, True), (2, False)])
+def test_stream_along_slip_walls_stays_uniform(dims, inviscid):
+    # slip walls at j-/j+ of a Cartesian block, periodic elsewhere, a stream along i: the normal
+    # contravariant velocity vanishes, the ghosts equal their mirrors, and the stream is steady
+    n = (10, 9, 6) if dims == 3 else (10, 9, 1)
+    bc = (0, 0, orc.BC_WALL_SLIP, orc.BC_WALL_SLIP, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, inviscid=int(inviscid))
+    o.mirror_metric(met)
+    prim = o.zeros(5)
+    prim[0][...] = 1.0
+    prim[1][...] = 0.6
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], 1.0, prim[0], GAMMA, MACH)
+    out = o.run(0.01, met, prim, 3)
+    sl = interior(o, dims, n)
+    for a, b in zip(out, prim):
+        assert np.max(np.abs(a[sl] - b[sl])) < 1e-13 * max(1.0, np.max(np.abs(b[sl])))
 
 
 def test_manufactured_faces_fill_interior_tangential_range_only():
