@@ -59,6 +59,15 @@ extern "C" {
                                        reference's BC_Corrector to the convective momentum terms
                                        (src/WENO.c:265-277), and shx_build_metric mirrors the ghost
                                        metric of the face slab (src/Functions.c:6849-7014). */
+#define SHX_BC_INFLOW_SUB_NORMAL  8 /* subsonic inflow: inflow velocity, rho and p; T, mu from the
+                                       mirror (WriteInflowSubsonicNormalBoundary, src/BC.c:2457-2503) */
+#define SHX_BC_INFLOW_SUB_RIEMANN 9 /* subsonic inflow from the far field's incoming invariant and
+                                       entropy and the mirror's outgoing invariant
+                                       (WriteInflowSubsonicRiemannBoundary, src/BC.c:2404-2455) */
+#define SHX_BC_INFLOW_SUB_ISENTROPIC 10 /* subsonic inflow: the mirror's contravariant velocity, rho
+                                       from Bernoulli along the far field's isentrope
+                                       (WriteInflowSubsonicIsentropBoundary, src/BC.c:2363-2402).
+                                       All three cover every ghost node of the face slab. */
 
 /* Arithmetic forms. */
 #define SHX_FP64 0   /* every operation in FP64 (the reference's PRECISION=2) */

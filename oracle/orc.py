@@ -22,6 +22,7 @@ BC_PERIODIC, BC_MMS, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
 WALLS = (BC_WALL_SLIP, BC_WALL_NOSLIP)
+BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
 
 
 class OrcCfg(ctypes.Structure):

@@ -30,7 +30,7 @@ int orc_define(orc_cfg *c)
     if (c->S != 5 && c->S != 9) return -1;
     if (c->time_order != 3 && c->time_order != 4) return -1;
     for (d = 0; d < 6; d++) {
-        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_WALL_NOSLIP) return -1;
+        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_INFLOW_SUB_ISENTROPIC) return -1;
         if ((c->bc[d] == ORC_BC_PERIODIC) != (c->bc[d ^ 1] == ORC_BC_PERIODIC)) return -1;
     }
     c->g = (c->S + 1) / 2;
@@ -726,6 +726,52 @@ static void bc_wall(const orc_cfg *c, int slip, const double corr[3], double *co
     der[1][n] = der[1][s];
 }
 
+/* Subsonic inflow (src/BC.c:2363-2503), mirror node s.  Normal (WriteInflowSubsonicNormalBoundary):
+ * the inflow velocity, p and rho.  Riemann (WriteInflowSubsonicRiemannBoundary): the velocity
+ * normal part from the far field's and the mirror's invariants, v and w of the inflow, rho and p
+ * on the far field's isentrope.  Isentropic (WriteInflowSubsonicIsentropBoundary): the mirror's
+ * contravariant velocities under the ghost metric, rho from the isentropic Bernoulli relation in
+ * u, p on the far field's isentrope.  e, c and theta follow; T and mu come from the mirror. */
+static void bc_inflow_sub(const orc_cfg *c, int kind, double *const met[10], double *const prim[5],
+                          double *const der[8], long n, long s)
+{
+    double gm = c->gamma, u, v, w, rho, p;
+    if (kind == ORC_BC_INFLOW_SUB_NORMAL) {
+        u = c->u_in; v = c->v_in; w = c->w_in;
+        p = c->p_in; rho = c->rho_in;
+    } else if (kind == ORC_BC_INFLOW_SUB_RIEMANN) {
+        double cinf = 1.0 / c->mach, cii = der[2][s], d1, d2, rrr;
+        u = 0.5 * (prim[1][s] + c->u_in) + (cinf - cii) / (gm - 1.);
+        v = c->v_in; w = c->w_in;
+        d1 = (0.5 * (cii + cinf) + 0.25 * (gm - 1.) * (c->u_in - prim[1][s])) / cinf;
+        d2 = 2. / (gm - 1.);
+        rrr = pow(d1, d2);
+        rho = c->rho_in * rrr;
+        d2 = 2. * gm / (gm - 1.);
+        rrr = pow(d1, d2);
+        p = c->p_in * rrr;
+    } else {
+        double th[3] = {der[4][s], der[5][s], der[6][s]};
+        velocity_from_theta(met, n, th, &u, &v, &w);
+        rho = pow(pow(c->rho_in, (gm - 1.0)) - pow(c->rho_in, gm) / c->p_in * (gm - 1.0) / gm * gm * c->mach * c->mach
+                  * (u * u / 2.0), 1.0 / (gm - 1.0));
+        p = c->p_in * pow(rho / c->rho_in, gm);
+    }
+    prim[0][n] = rho; prim[1][n] = u; prim[2][n] = v; prim[3][n] = w;
+    der[0][n] = p;
+    if (kind == ORC_BC_INFLOW_SUB_ISENTROPIC) {
+        der[4][n] = der[4][s]; der[5][n] = der[5][s]; der[6][n] = der[6][s];
+    } else {
+        der[4][n] = u * met[0][n] + v * met[1][n] + w * met[2][n];
+        der[5][n] = u * met[3][n] + v * met[4][n] + w * met[5][n];
+        der[6][n] = u * met[6][n] + v * met[7][n] + w * met[8][n];
+    }
+    prim[4][n] = (0.5 * ((u * u) + (v * v) + (w * w)) + p / rho / (gm - 1.0) * c->upsilon);
+    der[2][n] = sqrt(c->upsilon * gm * p / rho);
+    der[3][n] = der[3][s];
+    der[1][n] = der[1][s];
+}
+
 static void apply_face(const orc_cfg *c, int face, double *const met[10], double *const prim[5],
                        double *const der[8], double *const bs[6])
 {
@@ -759,6 +805,7 @@ static void apply_face(const orc_cfg *c, int face, double *const met[10], double
                 else if (kind == ORC_BC_OUTFLOW_RUDY) bc_outflow_subsonic(c, 1, met, prim, der, n, s);
                 else if (kind == ORC_BC_WALL_SLIP) bc_wall(c, 1, corr, met, prim, der, n, s);
                 else if (kind == ORC_BC_WALL_NOSLIP) bc_wall(c, 0, corr, met, prim, der, n, s);
+                else bc_inflow_sub(c, kind, met, prim, der, n, s);
             }
     }
 }

@@ -29,6 +29,9 @@
  *   wall writers          ghost state against an independent linear solve (slip) and sign
  *                         flips (no-slip); quiescent gas and wall-parallel stream stay exact
  *   orc_mirror_metric     copy identity on every ghost node of the slab
+ *   subsonic inflow writers  normal: inflow state; Riemann: the far field's u + 2c/(gamma-1) and
+ *                         entropy, the mirror's u - 2c/(gamma-1); isentropic: linear solve for
+ *                         the velocity, far-field entropy and Bernoulli's h + u^2/2 = h_inflow
  *   orc_cfl_dt, orc_check_nan   brute force
  * The reference itself cannot be compiled here (it needs mpi.h and cgnslib.h,
  * which the image lacks), so no compiled-reference fixture exists.
@@ -54,6 +57,9 @@ extern "C" {
 #define ORC_BC_OUTFLOW_RUDY 5      /* src/BC.c:2624-2655: non-reflecting relaxation towards p_out */
 #define ORC_BC_WALL_SLIP 6         /* WallInviscid, src/BC.c:2688-2719 (face loops :988-1172) */
 #define ORC_BC_WALL_NOSLIP 7       /* WallViscous, src/BC.c:2657-2686 (face loops :804-986) */
+#define ORC_BC_INFLOW_SUB_NORMAL 8     /* src/BC.c:2457-2503: inflow state, T and mu from the mirror */
+#define ORC_BC_INFLOW_SUB_RIEMANN 9    /* src/BC.c:2404-2455: Riemann invariants, far-field entropy */
+#define ORC_BC_INFLOW_SUB_ISENTROPIC 10 /* src/BC.c:2363-2402: mirror theta, isentropic Bernoulli density */
 
 typedef struct orc_cfg {
     /* inputs */

@@ -94,7 +94,7 @@ struct BcArgs {
     int lo1, n1, lo2, n2;    // tangential index ranges
     int first, last;         // first / last interior index along dir
     long st[3];
-    double rho_in, u_in, v_in, w_in, p_in, cinf;
+    double rho_in, u_in, v_in, w_in, p_in, cinf, mach;
     double p_out, alpha_tau; // subsonic outflow: p_out and AlphaNonRef * tau
     Consts k;
 };
@@ -185,6 +185,38 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         a.P[n] = p;
         a.C[n] = a.C[s];
         if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        return;
+    }
+    if (a.kind == SHX_BC_INFLOW_SUB_NORMAL || a.kind == SHX_BC_INFLOW_SUB_RIEMANN ||
+        a.kind == SHX_BC_INFLOW_SUB_ISENTROPIC) {
+        // subsonic inflow (src/BC.c:2363-2503): velocity, rho and p by the writer's rule, e and c
+        // from them, T and mu from the mirror node
+        const double gm = k.gamma;
+        double u, v, w, rho, p;
+        if (a.kind == SHX_BC_INFLOW_SUB_NORMAL) {
+            u = a.u_in; v = a.v_in; w = a.w_in;
+            p = a.p_in; rho = a.rho_in;
+        } else if (a.kind == SHX_BC_INFLOW_SUB_RIEMANN) {
+            // incoming invariant and entropy of the far field, outgoing invariant of the mirror
+            const double rs = a.pr.f[0][s], us = a.pr.f[1][s];
+            const double cii = a.P ? a.C[s]
+                                   : sqrt(k.upsilon * gm * pressure(k, rs, us, a.pr.f[2][s], a.pr.f[3][s], a.pr.f[4][s]) / rs);
+            u = 0.5 * (us + a.u_in) + (a.cinf - cii) / (gm - 1.);
+            v = a.v_in; w = a.w_in;
+            const double d1 = (0.5 * (cii + a.cinf) + 0.25 * (gm - 1.) * (a.u_in - us)) / a.cinf;
+            rho = a.rho_in * pow(d1, 2. / (gm - 1.));
+            p = a.p_in * pow(d1, 2. * gm / (gm - 1.));
+        } else {
+            // the mirror's contravariant velocities, rho from Bernoulli along the far field's isentrope
+            ghost_velocity(a, n, s, -1, u, v, w);
+            rho = pow(pow(a.rho_in, (gm - 1.0)) - pow(a.rho_in, gm) / a.p_in * (gm - 1.0) / gm * gm * a.mach * a.mach
+                      * (u * u / 2.0), 1.0 / (gm - 1.0));
+            p = a.p_in * pow(rho / a.rho_in, gm);
+        }
+        a.pr.f[0][n] = rho; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
+        a.pr.f[4][n] = (0.5 * ((u * u) + (v * v) + (w * w)) + p / rho / (gm - 1.0) * k.upsilon);
+        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        if (a.P) { a.P[n] = p; a.C[n] = sqrt(k.upsilon * gm * p / rho); }
         return;
     }
     if (a.kind == SHX_BC_MANUFACTURED) {
@@ -1756,7 +1788,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
         return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
-        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_WALL_NOSLIP)
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_INFLOW_SUB_ISENTROPIC)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
         if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
             return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
@@ -1965,6 +1997,7 @@ static void face_args(const shx_case *c, double *const *pr, int f, double tau, B
     a.rho_in = c->d.rho_inflow; a.u_in = c->d.u_inflow; a.v_in = c->d.v_inflow;
     a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
     a.cinf = 1.0 / c->d.mach;
+    a.mach = c->d.mach;
     for (int m = 0; m < 9; m++) a.met[m] = c->met[m];
     a.P = c->P; a.C = c->C;
     a.PL = c->PL; a.RL = c->RL; a.UL = c->UL; a.C0 = c->C0;

@@ -20,6 +20,7 @@ QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MANUFACTURED, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
+BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
 FP64, MIXED = 0, 1
 NUM_KERNELS = 14
 

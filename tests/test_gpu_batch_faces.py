@@ -1,5 +1,5 @@
-"""Batches with boundary faces: supersonic inflow, Riemann, normal and Rudy outflow, slip and
-no-slip walls and manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
+"""Batches with boundary faces: supersonic and subsonic inflow, Riemann, normal and Rudy outflow,
+slip and no-slip walls and manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
 must reach exactly the state it reaches when stepped alone (the single-case path's parity against
 the oracle is covered by the boundary tests); three of the cases are also held against the oracle
 directly."""
@@ -56,6 +56,8 @@ def test_batch_with_faces_equals_cases_alone():
         _faces_case(2, 9, (24, 20, 1), INOUT + NOSLIP + (0, 0), False, orc.QUIRKS_REFERENCE, None),
         _mms_case(5, (16, 16, 16), orc.QUIRK_METRIC_DIAG3D),
         _periodic_case(12, 5, 11),
+        _faces_case(2, 5, (24, 20, 1), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0), False,
+                    orc.QUIRKS_REFERENCE, None),
     ]
     steps = 3
     taus = [0.2 / (m["n"][0] * 5.0) for m in made]
@@ -74,7 +76,7 @@ def test_batch_with_faces_equals_cases_alone():
             diff = max(float(np.max(np.abs(a[sl] - r[sl]))) for a, r in zip(got, ref))
             print(f"[batch-faces] case {k} {m['tag']}: max |batch - alone| = {diff:.2e}", flush=True)
             assert diff == 0.0
-            if k in (0, 4, 6, 8):
+            if k in (0, 4, 6, 8, 10):
                 want = o.run(tau, m["met"], m["prim"], steps, src=m["src"], bstate=m["bs"])
                 errs = rel_linf(conserved(got), conserved(want), sl)
                 if dims == 2:

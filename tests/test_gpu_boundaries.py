@@ -1,5 +1,6 @@
-"""GPU parity of the boundary writers (supersonic inflow; Riemann, normal and Rudy subsonic
-outflow; slip and no-slip walls; manufactured-solution faces) and the manufactured-solution path: libshockx.so against
+"""GPU parity of the boundary writers (supersonic inflow; normal, Riemann and isentropic subsonic
+inflow; Riemann, normal and Rudy subsonic outflow; slip and no-slip walls; manufactured-solution
+faces) and the manufactured-solution path: libshockx.so against
 the oracle on identical inputs (same bar as the periodic parity tests: relative L-infinity per
 conserved variable <= 1e-12 after one step, <= 1e-10 after many).
 
@@ -109,6 +110,14 @@ CASES = [
     (3, 9, (14, 12, 12), INOUT + NOSLIP + (0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
     (3, 5, (12, 12, 12), INOUT + (orc.BC_WALL_NOSLIP, orc.BC_WALL_SLIP, orc.BC_WALL_SLIP, orc.BC_WALL_NOSLIP),
      False, 0, "bump"),
+    # subsonic inflow writers
+    (2, 5, (24, 20, 1), (orc.BC_INFLOW_SUB_NORMAL, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (2, 9, (24, 20, 1), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_NORMAL, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (3, 5, (14, 12, 12), (orc.BC_INFLOW_SUB_ISENTROPIC, orc.BC_OUTFLOW_RUDY, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    # (an isentropic inflow on a face along the stream is not well posed: its density formula
+    # leaves the real numbers within ten steps, in the oracle as on the GPU)
+    (3, 5, (12, 12, 12), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_RIEMANN, orc.BC_WALL_SLIP,
+                          orc.BC_WALL_SLIP, orc.BC_INFLOW_SUB_NORMAL, orc.BC_WALL_NOSLIP), False, 0, "bump"),
 ]
 
 

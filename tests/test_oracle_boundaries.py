@@ -18,6 +18,9 @@ Checks (CPU only):
     c, T, mu from the mirror, the corrector set on the interior tangential range only; the ghost
     metric mirrored over the whole slab (src/Functions.c:6849-7014); ghost point fluxes minus
     their mirror's except for pressure; gas at rest and a stream along slip walls stay steady
+  * subsonic inflow, normal, Riemann and isentropic (src/BC.c:2363-2503): the inflow state; the
+    far field's u + 2c/(gamma-1) and entropy with the mirror's u - 2c/(gamma-1); the mirror's
+    contravariant velocities (linear solve), far-field entropy and Bernoulli's h + u^2/2
   * face order (src/BC.c:13-74): a later face overwrites an earlier one at shared edges
   * a uniform stream at the inflow state passes an inflow/outflow block unchanged
   * the manufactured source equals a symbolic (sympy) residual of the conservation laws, and
@@ -120,6 +123,57 @@ def test_riemann_outflow_carries_the_characteristic_invariants(dims):
         assert np.array_equal(prim[3][gh], mirror_prim[3][mi])
         assert np.array_equal(der[1][gh], der0[1][mi]) and np.array_equal(der[3][gh], der0[3][mi])
         assert np.allclose(der[0][gh], pg, rtol=1e-13)
+
+
+@pytest.mark.parametrize("dims,kind", [(2, "normal"), (3, "normal"), (2, "riemann"), (3, "riemann"),
+                                       (2, "isentropic"), (3, "isentropic")])
+def test_subsonic_inflow_writers(dims, kind):
+    n = (9, 6, 5) if dims == 3 else (9, 6, 1)
+    # the Riemann writer takes the far-field sound speed as 1/Ma, i.e. p = rho at the inflow
+    inflow = (1.1, 0.45, 0.05, -0.03 if dims == 3 else 0.0, 1.1 if kind == "riemann" else 0.9)
+    code = {"normal": orc.BC_INFLOW_SUB_NORMAL, "riemann": orc.BC_INFLOW_SUB_RIEMANN,
+            "isentropic": orc.BC_INFLOW_SUB_ISENTROPIC}[kind]
+    bc = (code, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, inflow=inflow, mapping=cases.sine_bump_3d(0.05) if dims == 3 else None,
+                      quirks=0)
+    prim = _random_state(o, 7)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    mirror = [a.copy() for a in prim]
+    o.apply_bc(met, prim, der)
+    g, gm1, ups = o.g, GAMMA - 1.0, 1.0 / (GAMMA * MACH * MACH)
+    rin, uin, vin, win, pin = inflow
+    for l in range(g):
+        gh, mi = (g - 1 - l, slice(None), slice(None)), (g + l, slice(None), slice(None))   # whole slab
+        rg, ug, vg, wg = prim[0][gh], prim[1][gh], prim[2][gh], prim[3][gh]
+        pg = der[0][gh]
+        # e and c follow from the written rho, u, v, w, p; T and mu come from the mirror
+        assert np.allclose(prim[4][gh], 0.5 * (ug * ug + vg * vg + wg * wg) + pg / rg / gm1 * ups, rtol=1e-14, atol=0)
+        assert np.allclose(der[2][gh], np.sqrt(ups * GAMMA * pg / rg), rtol=1e-14, atol=0)
+        for f in (1, 3):
+            assert np.array_equal(der[f][gh], der0[f][mi])
+        if kind == "normal":
+            for m, val in enumerate((rin, uin, vin, win)):
+                assert np.all(prim[m][gh] == val)
+            assert np.all(pg == pin)
+        elif kind == "riemann":
+            cg, cs, us = der[2][gh], der0[2][mi], mirror[1][mi]
+            assert np.allclose(ug + 2 * cg / gm1, uin + 2 * (1.0 / MACH) / gm1, rtol=1e-13)   # from the far field
+            assert np.allclose(ug - 2 * cg / gm1, us - 2 * cs / gm1, rtol=1e-13)            # from the interior
+            assert np.allclose(pg / rg ** GAMMA, pin / rin ** GAMMA, rtol=1e-13)
+            assert np.all(vg == vin) and np.all(wg == win)
+        else:
+            theta = [der0[4 + d][mi] for d in range(3)]
+            vel = _ghost_velocity_by_solve(met, gh, theta)
+            for d in range(3 if dims == 3 else 2):
+                assert np.allclose(prim[1 + d][gh], vel[d], rtol=1e-12, atol=1e-14)
+            for d in range(3):
+                assert np.array_equal(der[4 + d][gh], theta[d])
+            assert np.allclose(pg / rg ** GAMMA, pin / rin ** GAMMA, rtol=1e-13)
+            # Bernoulli along the isentrope in the x velocity: h + u^2/2 = h(inflow state),
+            # h = p / (rho (gamma - 1) Ma^2)
+            h = lambda p_, r_: p_ / (r_ * gm1 * MACH * MACH)
+            assert np.allclose(h(pg, rg) + 0.5 * ug * ug, h(pin, rin), rtol=1e-13)
 
 
 def _ghost_velocity_by_solve(met, gh, theta):
