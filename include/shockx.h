@@ -68,6 +68,16 @@ extern "C" {
                                        from Bernoulli along the far field's isentrope
                                        (WriteInflowSubsonicIsentropBoundary, src/BC.c:2363-2402).
                                        All three cover every ghost node of the face slab. */
+#define SHX_BC_WALL_ISOTHERMAL   12 /* WallViscousIsothermal: as the no-slip wall with T = T_wall on
+                                       the ghosts, whose heat-conduction factor is the one of
+                                       Prandtl 0.72; the viscous energy term of a node takes the
+                                       mean factor of the node and its upper neighbour, so the last
+                                       node below an upper isothermal face sees it
+                                       (WriteWallNoSlipIsothermalBoundary, src/BC.c:2943-2978;
+                                       src/ZD.c:193-194, 422-423, 617-618) */
+#define SHX_BC_FARFIELD          11 /* every value of the mirror node, interior tangential range;
+                                       the ghost metric is mirrored as at walls
+                                       (WriteFarfieldBoundary, src/BC.c:2817-2852, loops :638-803) */
 
 /* Arithmetic forms. */
 #define SHX_FP64 0   /* every operation in FP64 (the reference's PRECISION=2) */
@@ -99,6 +109,8 @@ typedef struct shx_desc {
      * (src/Import.c:94) and the Rudy relaxation rate AlphaNonRef = 0.25 L0/u0
      * (src/Functions.c:1763), multiplied by the step's tau as in src/BC.c:2643-2646 */
     double p_out, alpha_nonref;
+    /* isothermal wall (SHX_BC_WALL_ISOTHERMAL): boundaryconditions:T_wall (src/Import.c:96) */
+    double T_wall;
 } shx_desc;
 
 typedef struct shx_case shx_case;
@@ -115,7 +127,7 @@ int shx_padded_extent(const shx_case *c, int ng[3]);
 /* Geometry (once per run).  shx_build_metric: CreateMetric (src/Functions.c:6388-6509) on the
  * extended coordinate arrays (extent ng+2 per direction, 1 in 2D z; the reference's
  * x_extrapolate), followed by the periodic ghost transfer of the metric
- * (TransferMeshParameter, src/Functions.c:3481-3578) and, on wall faces, the mirrored ghost
+ * (TransferMeshParameter, src/Functions.c:3481-3578) and, on wall and farfield faces, the mirrored ghost
  * metric (mirrorGhostCellsMetric, src/Functions.c:6849-7014).  shx_upload_metric takes metric
  * arrays a caller already has (strct_mesh xi_x ... zeta_z, jacobian, mirrored at walls), order
  * xi_x,xi_y,xi_z,eta_x,eta_y,eta_z,zeta_x,zeta_y,zeta_z,J. */

@@ -21,8 +21,9 @@ QUIRKS_REFERENCE = QUIRK_ZETA_VISC_J | QUIRK_METRIC_DIAG3D
 BC_PERIODIC, BC_MMS, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2, 3
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
-WALLS = (BC_WALL_SLIP, BC_WALL_NOSLIP)
 BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
+BC_FARFIELD, BC_WALL_ISOTHERMAL = 11, 12
+WALLS = (BC_WALL_SLIP, BC_WALL_NOSLIP, BC_WALL_ISOTHERMAL)
 
 
 class OrcCfg(ctypes.Structure):
@@ -36,7 +37,8 @@ class OrcCfg(ctypes.Structure):
         ("v_in", ctypes.c_double), ("w_in", ctypes.c_double), ("p_in", ctypes.c_double),
         ("p_out", ctypes.c_double), ("alpha_nonref", ctypes.c_double), ("tau_step", ctypes.c_double),
         ("p_last", ctypes.c_void_p), ("rho_last", ctypes.c_void_p), ("u_last", ctypes.c_void_p),
-        ("c_last", ctypes.c_void_p), ("corr", ctypes.c_void_p * 3),
+        ("c_last", ctypes.c_void_p), ("corr", ctypes.c_void_p * 3), ("T_wall", ctypes.c_double),
+        ("gam", ctypes.c_void_p),
         ("g", ctypes.c_int), ("ng", ctypes.c_int * 3), ("ntot", ctypes.c_long),
         ("upsilon", ctypes.c_double), ("psi", ctypes.c_double), ("gam_heat", ctypes.c_double),
         ("suth", ctypes.c_double), ("eps", ctypes.c_double),
@@ -95,13 +97,14 @@ class Oracle:
     def __init__(self, dims=2, S=5, n=(16, 16, 1), time_order=4, gamma=1.4, mach=0.5,
                  reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, mms=False,
                  quirks=QUIRKS_REFERENCE, eps_override=0.0, bc=(0, 0, 0, 0, 0, 0), inflow=(1.0, 1.0, 0.0, 0.0, 1.0),
-                 outflow=(1.0, 0.0)):
-        """outflow: (p_out, alpha_nonref) of the subsonic outflow writers."""
+                 outflow=(1.0, 0.0), T_wall=1.0):
+        """outflow: (p_out, alpha_nonref) of the subsonic outflow writers; T_wall of isothermal walls."""
         c = OrcCfg()
         for f in range(6):
             c.bc[f] = bc[f]
         c.rho_in, c.u_in, c.v_in, c.w_in, c.p_in = inflow
         c.p_out, c.alpha_nonref = outflow
+        c.T_wall = T_wall
         c.dims, c.S, c.time_order = dims, S, time_order
         for d in range(3):
             c.n[d] = n[d] if d < len(n) else 1
@@ -119,6 +122,11 @@ class Oracle:
             self.corr = [np.ones(self.shape) for _ in range(3)]
             for m in range(3):
                 c.corr[m] = self.corr[m].ctypes.data
+        self.gam = None
+        if BC_WALL_ISOTHERMAL in bc[:2 * dims]:
+            # per-node heat-conduction factor (pnt_config->Gamma), set on isothermal ghosts
+            self.gam = np.full(self.shape, c.gam_heat)
+            c.gam = self.gam.ctypes.data
 
     # ---- allocation
     def zeros(self, nf):
@@ -225,7 +233,7 @@ class Oracle:
         return lib().orc_check_nan(ctypes.byref(self.c), _bundle(prim))
 
     def mirror_metric(self, met):
-        """Wall faces: ghost metric copied from the mirror nodes (mirrorGhostCellsMetric), in place."""
+        """Wall and farfield faces: ghost metric copied from the mirror nodes (mirrorGhostCellsMetric), in place."""
         lib().orc_mirror_metric(ctypes.byref(self.c), _bundle(met))
 
     def metric_from_coords(self, xe, ye, ze):

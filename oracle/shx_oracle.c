@@ -30,7 +30,7 @@ int orc_define(orc_cfg *c)
     if (c->S != 5 && c->S != 9) return -1;
     if (c->time_order != 3 && c->time_order != 4) return -1;
     for (d = 0; d < 6; d++) {
-        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_INFLOW_SUB_ISENTROPIC) return -1;
+        if (c->bc[d] < ORC_BC_PERIODIC || c->bc[d] > ORC_BC_WALL_ISOTHERMAL) return -1;
         if ((c->bc[d] == ORC_BC_PERIODIC) != (c->bc[d ^ 1] == ORC_BC_PERIODIC)) return -1;
     }
     c->g = (c->S + 1) / 2;
@@ -566,7 +566,9 @@ void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *cons
                             }
                         }
                     }
-                    q[4][n] = q[4][n] + c->gam_heat * heat + c->psi * (work + prod);
+                    /* the factor of the node and its upper neighbour (src/ZD.c:193-194) */
+                    double gh = c->gam ? 0.5 * (c->gam[np] + c->gam[n]) : c->gam_heat;
+                    q[4][n] = q[4][n] + gh * heat + c->psi * (work + prod);
                 }
             }
 }
@@ -704,7 +706,7 @@ static void bc_outflow_subsonic(const orc_cfg *c, int rudy, double *const met[10
  * WriteWallNoSlipBoundary (src/BC.c:2657-2686): velocities and contravariant velocities reversed,
  * corrector -1 on all three momentum fluxes; rho, p, e, c, mu, T copied. */
 static void bc_wall(const orc_cfg *c, int slip, const double corr[3], double *const met[10],
-                    double *const prim[5], double *const der[8], long n, long s)
+                    double *const prim[5], double *const der[8], long n, long s, int iso)
 {
     int m;
     if (slip) {
@@ -724,6 +726,12 @@ static void bc_wall(const orc_cfg *c, int slip, const double corr[3], double *co
     der[2][n] = der[2][s];
     der[3][n] = der[3][s];
     der[1][n] = der[1][s];
+    if (iso) {
+        /* WriteWallNoSlipIsothermalBoundary (src/BC.c:2943-2978): T = T_wall and the heat
+           conduction factor of Prandtl 0.72 on the ghost */
+        der[1][n] = c->T_wall;
+        if (c->gam) c->gam[n] = 1.0 / ((c->gamma - 1.0) * pow(c->mach, 2.0) * c->reynolds * 0.72);
+    }
 }
 
 /* Subsonic inflow (src/BC.c:2363-2503), mirror node s.  Normal (WriteInflowSubsonicNormalBoundary):
@@ -782,7 +790,8 @@ static void apply_face(const orc_cfg *c, int face, double *const met[10], double
     double corr[3] = {1.0, 1.0, 1.0};
     corr[d] = -1.0;
     if (kind == ORC_BC_PERIODIC) return;
-    if (kind == ORC_BC_MMS || kind == ORC_BC_WALL_SLIP || kind == ORC_BC_WALL_NOSLIP) {
+    if (kind == ORC_BC_MMS || kind == ORC_BC_WALL_SLIP || kind == ORC_BC_WALL_NOSLIP || kind == ORC_BC_FARFIELD ||
+        kind == ORC_BC_WALL_ISOTHERMAL) {
         lo1 = rs(c, t1); hi1 = re(c, t1); lo2 = rs(c, t2); hi2 = re(c, t2);
     }
     else { lo1 = 0; hi1 = ge(c, t1); lo2 = 0; hi2 = ge(c, t2); }
@@ -803,8 +812,15 @@ static void apply_face(const orc_cfg *c, int face, double *const met[10], double
                 else if (kind == ORC_BC_OUTFLOW_RIEMANN) bc_outflow_riemann(c, met, prim, der, n, s);
                 else if (kind == ORC_BC_OUTFLOW_NORMAL) bc_outflow_subsonic(c, 0, met, prim, der, n, s);
                 else if (kind == ORC_BC_OUTFLOW_RUDY) bc_outflow_subsonic(c, 1, met, prim, der, n, s);
-                else if (kind == ORC_BC_WALL_SLIP) bc_wall(c, 1, corr, met, prim, der, n, s);
-                else if (kind == ORC_BC_WALL_NOSLIP) bc_wall(c, 0, corr, met, prim, der, n, s);
+                else if (kind == ORC_BC_WALL_SLIP) bc_wall(c, 1, corr, met, prim, der, n, s, 0);
+                else if (kind == ORC_BC_WALL_NOSLIP) bc_wall(c, 0, corr, met, prim, der, n, s, 0);
+                else if (kind == ORC_BC_WALL_ISOTHERMAL) bc_wall(c, 0, corr, met, prim, der, n, s, 1);
+                else if (kind == ORC_BC_FARFIELD) {
+                    /* WriteFarfieldBoundary (src/BC.c:2817-2852): every value of the mirror node */
+                    int m;
+                    for (m = 0; m < 5; m++) prim[m][n] = prim[m][s];
+                    for (m = 0; m < 7; m++) der[m][n] = der[m][s];
+                }
                 else bc_inflow_sub(c, kind, met, prim, der, n, s);
             }
     }
@@ -816,7 +832,8 @@ void orc_mirror_metric(const orc_cfg *c, double *const met[10])
     for (face = 0; face < 2 * c->dims; face++) {
         int d = face / 2, upper = face % 2, kind = c->bc[face];
         int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2, l, a, b, m;
-        if (kind != ORC_BC_WALL_SLIP && kind != ORC_BC_WALL_NOSLIP) continue;
+        if (kind != ORC_BC_WALL_SLIP && kind != ORC_BC_WALL_NOSLIP && kind != ORC_BC_FARFIELD &&
+            kind != ORC_BC_WALL_ISOTHERMAL) continue;
         for (l = 0; l < c->g; l++) {
             int gi = upper ? re(c, d) + 1 + l : rs(c, d) - 1 - l;
             int si = upper ? re(c, d) - l : rs(c, d) + l;

@@ -29,6 +29,9 @@
  *   wall writers          ghost state against an independent linear solve (slip) and sign
  *                         flips (no-slip); quiescent gas and wall-parallel stream stay exact
  *   orc_mirror_metric     copy identity on every ghost node of the slab
+ *   farfield writer       copy identity on the interior tangential range
+ *   isothermal wall       no-slip ghost with T = T_wall and the Prandtl-0.72 factor; the energy
+ *                         term's mean factor checked against a hand-set factor array
  *   subsonic inflow writers  normal: inflow state; Riemann: the far field's u + 2c/(gamma-1) and
  *                         entropy, the mirror's u - 2c/(gamma-1); isentropic: linear solve for
  *                         the velocity, far-field entropy and Bernoulli's h + u^2/2 = h_inflow
@@ -60,6 +63,8 @@ extern "C" {
 #define ORC_BC_INFLOW_SUB_NORMAL 8     /* src/BC.c:2457-2503: inflow state, T and mu from the mirror */
 #define ORC_BC_INFLOW_SUB_RIEMANN 9    /* src/BC.c:2404-2455: Riemann invariants, far-field entropy */
 #define ORC_BC_INFLOW_SUB_ISENTROPIC 10 /* src/BC.c:2363-2402: mirror theta, isentropic Bernoulli density */
+#define ORC_BC_FARFIELD 11         /* src/BC.c:2817-2852 (face loops :638-803): the mirror node copied */
+#define ORC_BC_WALL_ISOTHERMAL 12  /* WallViscousIsothermal, src/BC.c:2943-2978 (loops :2980-3151) */
 
 typedef struct orc_cfg {
     /* inputs */
@@ -85,6 +90,11 @@ typedef struct orc_cfg {
        after check_Metric (src/Functions.c:6212-6214), set on wall ghosts by the wall writers and
        multiplying the convective momentum fluxes (src/WENO.c:265-277).  NULL: 1 everywhere. */
     double *corr[3];
+    /* boundaryconditions:T_wall (src/Import.c:96) and the per-node heat-conduction factor
+       pnt_config->Gamma (src/Functions.c:141-152; the isothermal writer sets it on its ghosts,
+       src/BC.c:2974-2976).  gam NULL: gam_heat everywhere. */
+    double T_wall;
+    double *gam;
     /* derived by orc_define */
     int g;
     int ng[3];
@@ -132,7 +142,7 @@ void orc_visc_derivs(const orc_cfg *c, double *const prim[5], double *const der[
 void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *const prim[5],
                   double *const der[8], double *const grad[12], double *const q[5]);
 
-/* mirrorGhostCellsMetric (src/Functions.c:6849-7014): on wall faces, the metric of every ghost
+/* mirrorGhostCellsMetric (src/Functions.c:6849-7014): on wall and farfield faces, the metric of every ghost
  * node of the face slab (full tangential range) is copied from its mirror node, faces in order. */
 void orc_mirror_metric(const orc_cfg *c, double *const met[10]);
 

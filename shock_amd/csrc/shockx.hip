@@ -95,6 +95,7 @@ struct BcArgs {
     int first, last;         // first / last interior index along dir
     long st[3];
     double rho_in, u_in, v_in, w_in, p_in, cinf, mach;
+    double T_wall;           // isothermal walls
     double p_out, alpha_tau; // subsonic outflow: p_out and AlphaNonRef * tau
     Consts k;
 };
@@ -152,7 +153,7 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
     long n, s;
     if (!face_node(a, t, n, s)) return;
     const Consts &k = a.k;
-    if (a.kind == SHX_BC_WALL_SLIP || a.kind == SHX_BC_WALL_NOSLIP) {
+    if (a.kind == SHX_BC_WALL_SLIP || a.kind == SHX_BC_WALL_NOSLIP || a.kind == SHX_BC_WALL_ISOTHERMAL) {
         // WriteWallSlipBoundary (src/BC.c:2688-2719): the mirror's contravariant velocity along
         // the face's direction reversed; WriteWallNoSlipBoundary (src/BC.c:2657-2686): all
         // velocities reversed.  rho, e, p, c, T and mu from the mirror node.  The corrector these
@@ -166,7 +167,8 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         a.pr.f[0][n] = a.pr.f[0][s]; a.pr.f[1][n] = u; a.pr.f[2][n] = v; a.pr.f[3][n] = w;
         a.pr.f[4][n] = a.pr.f[4][s];
         if (a.P) { a.P[n] = a.P[s]; a.C[n] = a.C[s]; }
-        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        // WriteWallNoSlipIsothermalBoundary (src/BC.c:2943-2978): T = T_wall on the ghost
+        if (a.T) { a.T[n] = (a.kind == SHX_BC_WALL_ISOTHERMAL) ? a.T_wall : a.T[s]; a.mu[n] = a.mu[s]; }
         return;
     }
     if (a.kind == SHX_BC_OUTFLOW_NORMAL || a.kind == SHX_BC_OUTFLOW_RUDY) {
@@ -184,6 +186,14 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         a.pr.f[4][n] = a.pr.f[4][s];
         a.P[n] = p;
         a.C[n] = a.C[s];
+        if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
+        return;
+    }
+    if (a.kind == SHX_BC_FARFIELD) {
+        // WriteFarfieldBoundary (src/BC.c:2817-2852): the mirror node's values
+#pragma unroll
+        for (int m = 0; m < 5; m++) a.pr.f[m][n] = a.pr.f[m][s];
+        if (a.P) { a.P[n] = a.P[s]; a.C[n] = a.C[s]; }
         if (a.T) { a.T[n] = a.T[s]; a.mu[n] = a.mu[s]; }
         return;
     }
@@ -250,8 +260,8 @@ __device__ __forceinline__ void face_body(const BcArgs &a, const long t)
         if (a.P) { a.P[n] = p; a.C[n] = sqrt(k.upsilon * k.gamma * p / rho); }
     }
 }
-// mirrorGhostCellsMetric (src/Functions.c:6849-7014): every ghost node of a wall face's slab (the
-// whole tangential range) takes the metric of its mirror node
+// mirrorGhostCellsMetric (src/Functions.c:6849-7014): every ghost node of a wall or farfield face's
+// slab (the whole tangential range) takes the metric of its mirror node
 __global__ void k_mirror_metric(BcArgs a, Fields10 met)
 {
     long n, s;
@@ -778,6 +788,9 @@ struct ViscArgs {
     int three;
     int bug;                      // zeta stencil k offset taken from j (DIR = 2 only)
     int fuse_rk;                  // last sweep of the stage: apply the stage update (rk)
+    int iso_hi;                   // the upper face of the sweep direction is an isothermal wall:
+    double gam_iso;               // the last interior node's heat-conduction factor (the mean of
+                                  // its own and the wall ghost's, src/ZD.c:193-194)
     int tg[4];                    // tile grid and order of a single-case launch
     RkArgs rk;
     Consts k;
@@ -1130,7 +1143,8 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
                 qv[1] = q0[1] + k.psi * dF[0];
                 qv[2] = q0[2] + k.psi * dF[1];
                 qv[3] = a.three ? q0[3] + k.psi * dF[2] : q0[3];
-                qv[4] = q0[4] + (k.gam_heat * dF[3] + k.psi * (work + prod));
+                const double gh = (a.iso_hi && dn == a.nD - 1) ? a.gam_iso : k.gam_heat;
+                qv[4] = q0[4] + (gh * dF[3] + k.psi * (work + prod));
                 const double J = DIAG ? sm[(VS_M + 3) * NW + p] : sm[(VS_M + 9) * NW + p];
                 rk_point(a.rk, n, J, qv);
             } else {
@@ -1139,7 +1153,8 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
                 a.q.f[2][n] = q0[2] + k.psi * dF[1];
                 if (a.three) a.q.f[3][n] = q0[3] + k.psi * dF[2];
                 else if (ADD) a.q.f[3][n] = q0[3];
-                a.q.f[4][n] = q0[4] + (k.gam_heat * dF[3] + k.psi * (work + prod));
+                const double gh = (a.iso_hi && dn == a.nD - 1) ? a.gam_iso : k.gam_heat;
+                a.q.f[4][n] = q0[4] + (gh * dF[3] + k.psi * (work + prod));
             }
         }
     }
@@ -1686,14 +1701,20 @@ static bool stores_pc(const int kind)
 {
     return kind == SHX_BC_OUTFLOW_NORMAL || kind == SHX_BC_OUTFLOW_RUDY || kind == SHX_BC_WALL_SLIP;
 }
-static bool is_wall(const int kind) { return kind == SHX_BC_WALL_SLIP || kind == SHX_BC_WALL_NOSLIP; }
+static bool is_wall(const int kind)
+{
+    return kind == SHX_BC_WALL_SLIP || kind == SHX_BC_WALL_NOSLIP || kind == SHX_BC_WALL_ISOTHERMAL;
+}
+// faces whose ghost metric is the mirror's (mirrorGhostCellsMetric, src/Functions.c:6849-7014) and
+// whose writers cover the interior tangential range only
+static bool mirrors_metric(const int kind) { return is_wall(kind) || kind == SHX_BC_FARFIELD; }
 // BC_Corrector of a face's ghost nodes: slip walls reverse the momentum component along the
 // face's index direction (src/BC.c:988-1172), no-slip walls all three (src/BC.c:2657-2686)
 static void wall_corrector(const int kind, const int dir, double cr[3])
 {
     for (int q = 0; q < 3; q++) cr[q] = 1.0;
     if (kind == SHX_BC_WALL_SLIP) cr[dir] = -1.0;
-    if (kind == SHX_BC_WALL_NOSLIP) cr[0] = cr[1] = cr[2] = -1.0;
+    if (kind == SHX_BC_WALL_NOSLIP || kind == SHX_BC_WALL_ISOTHERMAL) cr[0] = cr[1] = cr[2] = -1.0;
 }
 
 struct shx_case {
@@ -1788,7 +1809,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
         return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
-        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_INFLOW_SUB_ISENTROPIC)
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_WALL_ISOTHERMAL)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
         if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
             return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
@@ -1797,6 +1818,8 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
             return set_err(SHX_ERR_ARG, "face %d: inflow/outflow faces need rho_inflow > 0 and p_inflow > 0", f);
         if ((d.bc[f] == SHX_BC_OUTFLOW_NORMAL || d.bc[f] == SHX_BC_OUTFLOW_RUDY) && !(d.p_out > 0.0 && d.alpha_nonref >= 0.0))
             return set_err(SHX_ERR_ARG, "face %d: subsonic outflow faces need p_out > 0 and alpha_nonref >= 0", f);
+        if (d.bc[f] == SHX_BC_WALL_ISOTHERMAL && !(d.T_wall > 0.0))
+            return set_err(SHX_ERR_ARG, "face %d: isothermal walls need T_wall > 0", f);
     }
     const int S = d.space_order, g = (S + 1) / 2;
     for (int a = 0; a < d.dims; a++)
@@ -1985,7 +2008,7 @@ static void face_args(const shx_case *c, double *const *pr, int f, double tau, B
     a.g = gm.g;
     a.t1 = (a.dir == 0) ? 1 : 0;
     a.t2 = (a.dir == 2) ? 1 : 2;
-    if (kind == SHX_BC_MANUFACTURED || is_wall(kind)) {
+    if (kind == SHX_BC_MANUFACTURED || mirrors_metric(kind)) {
         // interior tangential range (src/ManufacturedSolution.c:202-338, src/BC.c:804-1172)
         a.lo1 = gm.lo[a.t1]; a.n1 = gm.n[a.t1]; a.lo2 = gm.lo[a.t2]; a.n2 = gm.n[a.t2];
     } else {
@@ -1998,6 +2021,7 @@ static void face_args(const shx_case *c, double *const *pr, int f, double tau, B
     a.w_in = c->d.w_inflow; a.p_in = c->d.p_inflow;
     a.cinf = 1.0 / c->d.mach;
     a.mach = c->d.mach;
+    a.T_wall = c->d.T_wall;
     for (int m = 0; m < 9; m++) a.met[m] = c->met[m];
     a.P = c->P; a.C = c->C;
     a.PL = c->PL; a.RL = c->RL; a.UL = c->UL; a.C0 = c->C0;
@@ -2071,9 +2095,9 @@ extern "C" int shx_build_metric(shx_case *c, const double *xe, const double *ye,
     LAUNCH_CHECK();
     if ((rc = tend(c))) return rc;
     if ((rc = ghost_fill(c, c->met, 10))) return rc;
-    // wall faces, in face order: the ghost metric mirrored (full tangential range)
+    // wall and farfield faces, in face order: the ghost metric mirrored (full tangential range)
     for (int f = 0; f < 2 * gm.dims; f++) {
-        if (!is_wall(c->d.bc[f])) continue;
+        if (!mirrors_metric(c->d.bc[f])) continue;
         BcArgs a;
         long nodes;
         face_args(c, c->q, f, 0.0, a, nodes);
@@ -2425,6 +2449,14 @@ static ViscArgs visc_args(const shx_case *c, double *const *pr, int dir, const R
     a.three = (gm.dims == 3);
     a.bug = (dir == 2 && (c->d.quirks & SHX_QUIRK_ZETA_VISC_J)) ? 1 : 0;
     a.k = c->k;
+    a.iso_hi = (c->d.bc[2 * dir + 1] == SHX_BC_WALL_ISOTHERMAL) ? 1 : 0;
+    {
+        // heat-conduction factor the isothermal writer gives its ghosts (Prandtl 0.72,
+        // src/BC.c:2974-2976), averaged with the interior one as src/ZD.c:193-194
+        const shx_desc &d = c->d;
+        const double gw = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * 0.72);
+        a.gam_iso = 0.5 * (gw + c->k.gam_heat);
+    }
     a.fuse_rk = rk ? 1 : 0;
     if (rk) a.rk = *rk;
     return a;

@@ -21,6 +21,7 @@ BC_PERIODIC, BC_MANUFACTURED, BC_INFLOW_SUPERSONIC, BC_OUTFLOW_RIEMANN = 0, 1, 2
 BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
 BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
+BC_FARFIELD, BC_WALL_ISOTHERMAL = 11, 12
 FP64, MIXED = 0, 1
 NUM_KERNELS = 14
 
@@ -39,7 +40,7 @@ class Desc(ctypes.Structure):
         ("bc", ctypes.c_int * 6), ("precision", ctypes.c_int), ("eps_override", ctypes.c_double),
         ("rho_inflow", ctypes.c_double), ("u_inflow", ctypes.c_double), ("v_inflow", ctypes.c_double),
         ("w_inflow", ctypes.c_double), ("p_inflow", ctypes.c_double),
-        ("p_out", ctypes.c_double), ("alpha_nonref", ctypes.c_double),
+        ("p_out", ctypes.c_double), ("alpha_nonref", ctypes.c_double), ("T_wall", ctypes.c_double),
     ]
 
 
@@ -113,9 +114,9 @@ class Case:
     def __init__(self, dims=2, space_order=5, time_order=4, n=(64, 64, 1), gamma=1.4, mach=0.5,
                  reynolds=1000.0, prandtl=0.72, T0=300.0, inviscid=False, manufactured=False,
                  quirks=QUIRKS_REFERENCE, bc=(BC_PERIODIC,) * 6, precision=FP64, eps_override=0.0,
-                 inflow=(1.0, 1.0, 0.0, 0.0, 1.0), outflow=(1.0, 0.0)):
+                 inflow=(1.0, 1.0, 0.0, 0.0, 1.0), outflow=(1.0, 0.0), T_wall=1.0):
         """inflow = (rho, u, v, w, p) of the inflow state (also the Riemann outflow's far field);
-        outflow = (p_out, alpha_nonref) of the subsonic outflow writers."""
+        outflow = (p_out, alpha_nonref) of the subsonic outflow writers; T_wall of isothermal walls."""
         d = Desc()
         d.dims, d.space_order, d.time_order = dims, space_order, time_order
         for a in range(3):
@@ -127,6 +128,7 @@ class Case:
         d.precision, d.eps_override = precision, eps_override
         d.rho_inflow, d.u_inflow, d.v_inflow, d.w_inflow, d.p_inflow = inflow
         d.p_out, d.alpha_nonref = outflow
+        d.T_wall = T_wall
         h = ctypes.c_void_p()
         _check(lib().shx_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

@@ -1,5 +1,5 @@
 """Batches with boundary faces: supersonic and subsonic inflow, Riemann, normal and Rudy outflow,
-slip and no-slip walls and manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
+slip, no-slip and isothermal walls, farfield and manufactured-solution faces (with the source) mixed with a periodic case in one batch.  Each case
 must reach exactly the state it reaches when stepped alone (the single-case path's parity against
 the oracle is covered by the boundary tests); three of the cases are also held against the oracle
 directly."""
@@ -58,6 +58,8 @@ def test_batch_with_faces_equals_cases_alone():
         _periodic_case(12, 5, 11),
         _faces_case(2, 5, (24, 20, 1), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0), False,
                     orc.QUIRKS_REFERENCE, None),
+        _faces_case(3, 5, (14, 12, 12), INOUT + (orc.BC_FARFIELD, orc.BC_WALL_ISOTHERMAL, 0, 0), False,
+                    orc.QUIRKS_REFERENCE, "bump"),
     ]
     steps = 3
     taus = [0.2 / (m["n"][0] * 5.0) for m in made]
@@ -76,7 +78,7 @@ def test_batch_with_faces_equals_cases_alone():
             diff = max(float(np.max(np.abs(a[sl] - r[sl]))) for a, r in zip(got, ref))
             print(f"[batch-faces] case {k} {m['tag']}: max |batch - alone| = {diff:.2e}", flush=True)
             assert diff == 0.0
-            if k in (0, 4, 6, 8, 10):
+            if k in (0, 4, 6, 8, 10, 11):
                 want = o.run(tau, m["met"], m["prim"], steps, src=m["src"], bstate=m["bs"])
                 errs = rel_linf(conserved(got), conserved(want), sl)
                 if dims == 2:

@@ -1,6 +1,6 @@
 """GPU parity of the boundary writers (supersonic inflow; normal, Riemann and isentropic subsonic
-inflow; Riemann, normal and Rudy subsonic outflow; slip and no-slip walls; manufactured-solution
-faces) and the manufactured-solution path: libshockx.so against
+inflow; Riemann, normal and Rudy subsonic outflow; slip, no-slip and isothermal walls; farfield;
+manufactured-solution faces) and the manufactured-solution path: libshockx.so against
 the oracle on identical inputs (same bar as the periodic parity tests: relative L-infinity per
 conserved variable <= 1e-12 after one step, <= 1e-10 after many).
 
@@ -31,6 +31,7 @@ IN_RUDY = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RUDY)
 SLIP = (orc.BC_WALL_SLIP, orc.BC_WALL_SLIP)
 NOSLIP = (orc.BC_WALL_NOSLIP, orc.BC_WALL_NOSLIP)
 OUTFLOW = (0.95, 3.0)          # p_out, AlphaNonRef
+PRANDTL, T_WALL = 0.9, 1.3
 
 
 def _noise(o, dims, n, met, prim, steps, tau=0.0, src=None, bstate=None, trials=3):
@@ -62,17 +63,18 @@ def _report(tag, errs):
 
 
 def _setup(dims, S, n, bc, inflow, inviscid=False, quirks=orc.QUIRKS_REFERENCE, mms_on=False,
-           mapping=None, reynolds=1000.0):
+           mapping=None, reynolds=1000.0, prandtl=0.72):
     o = orc.Oracle(dims=dims, S=S, n=n, bc=bc, inflow=inflow, mach=MACH, reynolds=reynolds,
-                   inviscid=int(inviscid), quirks=quirks, mms=int(mms_on), outflow=OUTFLOW)
+                   inviscid=int(inviscid), quirks=quirks, mms=int(mms_on), outflow=OUTFLOW,
+                   prandtl=prandtl, T_wall=T_WALL)
     per = tuple(bc[2 * d] == orc.BC_PERIODIC for d in range(3))
     xe, ye, ze = cases.ext_coords(dims, S, n, mapping or cases.cartesian_unit(dims), periodic=per)
     met = o.metric_from_coords(xe, ye, ze)
     o.ghost_periodic(met)
     o.mirror_metric(met)
     g = shockx.Case(dims=dims, space_order=S, time_order=4, n=n, gamma=GAMMA, mach=MACH,
-                    reynolds=reynolds, inviscid=inviscid, manufactured=mms_on, quirks=quirks,
-                    bc=bc, inflow=inflow, outflow=OUTFLOW)
+                    reynolds=reynolds, prandtl=prandtl, inviscid=inviscid, manufactured=mms_on, quirks=quirks,
+                    bc=bc, inflow=inflow, outflow=OUTFLOW, T_wall=T_WALL)
     g.build_metric(xe, ye, ze)
     return o, g, met, (xe, ye, ze)
 
@@ -114,6 +116,17 @@ CASES = [
     (2, 5, (24, 20, 1), (orc.BC_INFLOW_SUB_NORMAL, orc.BC_OUTFLOW_RIEMANN, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, None),
     (2, 9, (24, 20, 1), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_NORMAL, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, None),
     (3, 5, (14, 12, 12), (orc.BC_INFLOW_SUB_ISENTROPIC, orc.BC_OUTFLOW_RUDY, 0, 0, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    # farfield faces
+    (2, 5, (24, 20, 1), INOUT + (orc.BC_FARFIELD, orc.BC_FARFIELD, 0, 0), False, orc.QUIRKS_REFERENCE, None),
+    (3, 9, (14, 12, 12), INOUT + (orc.BC_FARFIELD, orc.BC_FARFIELD, 0, 0), False, orc.QUIRKS_REFERENCE, "bump"),
+    # isothermal walls (T_wall on the ghosts; the heat-conduction factor of Prandtl 0.72 against
+    # the case's 0.9 below the upper wall)
+    (2, 5, (24, 20, 1), INOUT + (orc.BC_WALL_ISOTHERMAL, orc.BC_WALL_ISOTHERMAL, 0, 0), False,
+     orc.QUIRKS_REFERENCE, None),
+    (3, 5, (14, 12, 12), INOUT + (orc.BC_WALL_ISOTHERMAL, orc.BC_WALL_ISOTHERMAL, 0, 0), False,
+     orc.QUIRKS_REFERENCE, "bump"),
+    (3, 5, (12, 12, 12), INOUT + (orc.BC_WALL_SLIP, orc.BC_WALL_ISOTHERMAL, orc.BC_WALL_ISOTHERMAL,
+                                  orc.BC_WALL_NOSLIP), False, 0, "bump"),
     # (an isentropic inflow on a face along the stream is not well posed: its density formula
     # leaves the real numbers within ten steps, in the oracle as on the GPU)
     (3, 5, (12, 12, 12), (orc.BC_INFLOW_SUB_RIEMANN, orc.BC_OUTFLOW_RIEMANN, orc.BC_WALL_SLIP,
@@ -128,7 +141,8 @@ def _mapping(dims, tag):
 @pytest.mark.parametrize("dims,S,n,bc,inviscid,quirks,mp", CASES)
 def test_inflow_outflow_rhs_and_steps(dims, S, n, bc, inviscid, quirks, mp):
     inflow = (1.0, 2.5, 0.0, 0.0, 1.0)
-    o, g, met, coords = _setup(dims, S, n, bc, inflow, inviscid, quirks, mapping=_mapping(dims, mp))
+    # Prandtl 0.9: the isothermal ghosts' heat-conduction factor (Prandtl 0.72) differs from the interior's
+    o, g, met, coords = _setup(dims, S, n, bc, inflow, inviscid, quirks, mapping=_mapping(dims, mp), prandtl=PRANDTL)
     tag = f"bc{bc[:2 * dims]} dims={dims} S={S} inviscid={inviscid} quirks={quirks}"
     with g:
         gm = g.download_metric()

@@ -21,6 +21,11 @@ Checks (CPU only):
   * subsonic inflow, normal, Riemann and isentropic (src/BC.c:2363-2503): the inflow state; the
     far field's u + 2c/(gamma-1) and entropy with the mirror's u - 2c/(gamma-1); the mirror's
     contravariant velocities (linear solve), far-field entropy and Bernoulli's h + u^2/2
+  * farfield (src/BC.c:2817-2852): the mirror node copied on the interior tangential range, the
+    ghost metric mirrored
+  * isothermal wall (src/BC.c:2943-2978): the no-slip ghost with T = T_wall and the heat
+    conduction factor of Prandtl 0.72; the viscous energy term of a node takes the mean factor
+    of the node and its upper neighbour (src/ZD.c:193-194)
   * face order (src/BC.c:13-74): a later face overwrites an earlier one at shared edges
   * a uniform stream at the inflow state passes an inflow/outflow block unchanged
   * the manufactured source equals a symbolic (sympy) residual of the conservation laws, and
@@ -278,6 +283,108 @@ def test_wall_writers_and_mirrored_metric(dims, kind):
     wall[ti, end + 1:, tk] = True
     for d in range(3):
         assert np.all(o.corr[d][~wall] == 1.0)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_isothermal_wall_writer(dims):
+    n = (9, 7, 6) if dims == 3 else (9, 7, 1)
+    iso = orc.BC_WALL_ISOTHERMAL
+    bc = (orc.BC_INFLOW_SUPERSONIC, orc.BC_OUTFLOW_RIEMANN, iso, iso, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, mapping=cases.sine_bump_3d(0.05) if dims == 3 else None, quirks=0,
+                      prandtl=0.9, reynolds=700.0, T_wall=1.7)
+    o.mirror_metric(met)
+    prim = _random_state(o, 61)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    before = [a.copy() for a in prim]
+    o.apply_bc(met, prim, der)
+    g, end = o.g, o.g + n[1] - 1
+    ti = slice(g, g + n[0])
+    tk = slice(g, g + n[2]) if dims == 3 else slice(None)
+    gam_int = 1.0 / ((GAMMA - 1.0) * MACH ** 2 * 700.0 * 0.9)
+    gam_wall = 1.0 / ((GAMMA - 1.0) * MACH ** 2 * 700.0 * 0.72)
+    ghosts = np.zeros(o.shape, dtype=bool)
+    for l in range(g):
+        for gj, mj in ((g - 1 - l, g + l), (end + 1 + l, end - l)):
+            gh, mi = (ti, gj, tk), (ti, mj, tk)
+            ghosts[gh] = True
+            for d in range(3):                                  # no-slip part
+                assert np.array_equal(prim[1 + d][gh], -before[1 + d][mi])
+                assert np.array_equal(der[4 + d][gh], -der0[4 + d][mi])
+                assert np.all(o.corr[d][gh] == -1.0)
+            for m in (0, 4):
+                assert np.array_equal(prim[m][gh], before[m][mi])
+            for f in (0, 2, 3):                                 # p, c, mu from the mirror
+                assert np.array_equal(der[f][gh], der0[f][mi])
+            assert np.all(der[1][gh] == 1.7)                    # T = T_wall
+    assert np.allclose(o.gam[ghosts], gam_wall, rtol=1e-15, atol=0)
+    assert np.allclose(o.gam[~ghosts], gam_int, rtol=1e-15, atol=0)
+
+
+def test_heat_conduction_factor_is_the_mean_of_a_node_and_its_upper_neighbour():
+    # The viscous energy term of node i uses (Gamma_i + Gamma_{i+1}) / 2 (src/ZD.c:193-194):
+    # a factor changed at one node moves the energy term of that node and of the node below it
+    # only, by the mean's change times the node's heat-flux divergence
+    n = (10, 8, 1)
+    bc = (0, 0, orc.BC_WALL_ISOTHERMAL, orc.BC_WALL_ISOTHERMAL, 0, 0)
+    o, met, _ = _case(2, 5, n, bc, quirks=0, mapping=cases.cartesian_unit(2))
+    prim = _random_state(o, 71)
+    der = o.calc_values(met, prim)
+    grad = o.visc_derivs(prim, der)
+    G = o.c.gam_heat
+
+    def energy(gam):
+        o.gam[...] = gam
+        q = o.zeros(5)
+        o.visc_dir(1, met, prim, der, grad, q)
+        return q[4]
+
+    e1 = energy(G)
+    heat = (energy(2.0 * G) - e1) / G                            # heat-flux divergence per node
+    spike = np.full(o.shape, G)
+    i, j = o.g + 4, o.g + 3
+    spike[i, j, 0] = 3.0 * G
+    diff = energy(spike) - e1
+    want = np.zeros(o.shape)
+    want[i, j, 0] = (0.5 * (G + 3.0 * G) - G) * heat[i, j, 0]     # the node itself
+    want[i, j - 1, 0] = (0.5 * (3.0 * G + G) - G) * heat[i, j - 1, 0]   # the node below it
+    sl = interior(o, 2, n)
+    assert np.allclose(diff[sl], want[sl], rtol=1e-9, atol=1e-12 * np.max(np.abs(e1[sl])))
+    assert abs(diff[i, j + 1, 0]) <= 1e-12 * np.max(np.abs(e1[sl]))
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_farfield_copies_the_mirror_node(dims):
+    n = (9, 7, 6) if dims == 3 else (9, 7, 1)
+    bc = (orc.BC_FARFIELD, orc.BC_FARFIELD, orc.BC_FARFIELD, orc.BC_FARFIELD, 0, 0)
+    o, met, _ = _case(dims, 5, n, bc, mapping=cases.sine_bump_3d(0.05) if dims == 3 else None, quirks=0)
+    raw = [a.copy() for a in met]
+    o.mirror_metric(met)
+    g = o.g
+    tj = slice(g, g + n[1])
+    for l in range(g):
+        for m in range(10):                       # i faces (the j faces overwrite the shared edges)
+            assert np.array_equal(met[m][g - 1 - l, tj], raw[m][g + l, tj])
+            # j faces, whole slab, mirroring the i-mirrored values on the edges
+            assert np.array_equal(met[m][:, g - 1 - l], met[m][:, g + l])
+    prim = _random_state(o, 51)
+    der0 = o.calc_values(met, prim)
+    der = [a.copy() for a in der0]
+    before = [a.copy() for a in prim]
+    o.apply_bc(met, prim, der)
+    end = g + n[1] - 1
+    ti = slice(g, g + n[0])
+    tk = slice(g, g + n[2]) if dims == 3 else slice(None)
+    for l in range(g):
+        gh, mi = (ti, end + 1 + l, tk), (ti, end - l, tk)
+        for m in range(5):
+            assert np.array_equal(prim[m][gh], before[m][mi])
+        for f in range(7):
+            assert np.array_equal(der[f][gh], der0[f][mi])
+    assert np.all(o.corr is None or all(np.all(cr == 1.0) for cr in o.corr))
+    edge = (slice(0, g), slice(0, g), slice(None))                  # i-ghost x j-ghost edge untouched
+    for m in range(5):
+        assert np.array_equal(prim[m][edge], before[m][edge])
 
 
 @pytest.mark.parametrize("kind", ["slip", "noslip"])
