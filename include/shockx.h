@@ -155,6 +155,36 @@ int shx_calc_runge_kutta(shx_case *c, double tau);
 /* nsteps of the above (the startSimulation loop body, src/SHOCK.c:557-666, without I/O). */
 int shx_run(shx_case *c, double tau, int nsteps);
 
+/* startSimulation's iteration loop with its NaN rollback and time-step control
+ * (src/SHOCK.c:531-691, without sampling and I/O).  Per iteration: time += tau L0/u0, one step,
+ * checkNAN on the new state.  On NaN: stop if number_resets rollbacks were taken (the case then
+ * holds the failed step's state, NaN nodes marked as checkNAN marks them); otherwise
+ * tau *= decel_factor, distance_forward restarts, and the older of two device-resident backups
+ * (state, stored p, iteration and time) is reloaded.  Without NaN and with the accelerator on, the
+ * tau resetter (back to the starting tau after distance_nan steps' worth of time) and the tau
+ * accelerator (tau *= accel_factor after distance_forward steps' worth, which then quadruples) act
+ * as in the reference.  Backups: slot 1 every distance_backward iterations, slot 2 half-way
+ * between; slot 1 starts as the initial state at iteration 1, slot 2 empty (iteration 999999). */
+typedef struct shx_sim {
+    int start_iteration, end_iteration;   /* int_StartIteration, int_EndIteration */
+    double tau;                           /* numerics:tau (numericalTauStart) */
+    double time, time_last_action;        /* time_dim, time_dim_lastAction at the start */
+    double L0, u0;                        /* L0_dim, u0_dim (time_dim += tau L0 / u0) */
+    int accelerator;                      /* general:TauAccelerator */
+    double accel_factor, decel_factor;    /* tau:tauaccelerator_factor, tau:taudecelerator_factor */
+    int number_resets;                    /* tau:NumberResets */
+    int distance_nan, distance_forward, distance_backward;  /* tau:DistanceNAN, DistanceForward, DistanceBackward */
+} shx_sim;
+typedef struct shx_sim_report {
+    int iteration;           /* last iteration the loop ran (the failing one after a stop) */
+    double time, tau;        /* time_dim and numericalTau at the end */
+    int resets;              /* rollbacks taken */
+    int nan_steps;           /* steps whose checkNAN found a NaN or inf density */
+    int accelerations, tau_resets;
+    int aborted;             /* 1: stopped at number_resets */
+} shx_sim_report;
+int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *report);
+
 /* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
  * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux
  * assembly, src/Functions.c:2111-2293).  For parity tests. */

@@ -218,6 +218,77 @@ class Oracle:
             p_last[...] = scratch[0]
         return urk
 
+    def simulate(self, met, prim0, sim, src=None, bstate=None):
+        """startSimulation's iteration loop (src/SHOCK.c:493-768) with its NaN rollback and time-step
+        control: time += tau L0/u0; one step; checkNAN on the new state; on NaN, unless NumberResets
+        rollbacks were taken (then stop), tau *= the decelerator, DistanceForward restarts and the
+        older of two backups (iteration, time, state with p) is reloaded; otherwise the tau resetter
+        and the tau accelerator as configured; then U_lastStep <- U_RK and a backup every
+        DistanceBackward iterations (slot 1) and half-way between (slot 2), with slot 1 initially the
+        starting state at iteration 1 and slot 2 at iteration 999999 (src/SHOCK.c:531-540).
+        sim: dict with start, end, tau, time, time_last_action, L0, u0, accelerator, accel_factor,
+        decel_factor, number_resets, distance_nan, distance_forward, distance_backward.
+        Returns (state, report)."""
+        un = [a.copy() for a in prim0]
+        urk = [a.copy() for a in prim0]
+        scratch = self.zeros(30)
+        p_last, c_last = self.eos_pc(met, prim0)
+        tau = tau_start = sim["tau"]
+        time, last = sim["time"], sim["time_last_action"]
+        L0, u0 = sim["L0"], sim["u0"]
+        dfwd, dback, dnan = sim["distance_forward"], sim["distance_backward"], sim["distance_nan"]
+        end = sim["end"]
+        backup = [([a.copy() for a in un], p_last.copy()), None]
+        it_b, time_b = [1, 999999], [time, 0.0]
+        rep = {"resets": 0, "nan_steps": 0, "accelerations": 0, "tau_resets": 0, "aborted": 0}
+        it = sim["start"] + 1
+        last_run = sim["start"]
+        while it <= end:
+            last_run = it
+            time += tau * L0 / u0
+            self._step_start(tau, met, un, p_last, c_last)
+            self.step(tau, met, un, urk, src, scratch, bstate)
+            p_rk = scratch[0]
+            if self.check_nan(urk) > 0:
+                rep["nan_steps"] += 1
+                if rep["resets"] >= sim["number_resets"]:
+                    rep["aborted"] = 1
+                    break
+                rep["resets"] += 1
+                tau *= sim["decel_factor"]
+                dfwd = sim["distance_forward"]
+                b = 0 if it_b[0] < it_b[1] else 1
+                for a, bk in zip(urk, backup[b][0]):
+                    a[...] = bk
+                p_rk = backup[b][1]
+                last = time
+                time = time_b[b]
+                it = it_b[b]
+            elif (time - last > dnan * tau * L0 / u0 and sim["accelerator"] and it + dback < end
+                  and tau < tau_start):
+                dfwd = sim["distance_forward"]
+                tau = tau_start
+                last = time
+                rep["tau_resets"] += 1
+            elif (time - last > dfwd * tau * L0 / u0 and sim["accelerator"] and it + dback < end
+                  and tau >= tau_start):
+                dfwd = int(dfwd * 4.0)
+                tau *= sim["accel_factor"]
+                last = time
+                rep["accelerations"] += 1
+            for a, b_ in zip(un, urk):
+                a[...] = b_
+            p_last = p_rk.copy()
+            if it % dback == 0:
+                backup[0] = ([a.copy() for a in un], p_last.copy())
+                time_b[0], it_b[0] = time, it
+            elif (it + dback // 2) % dback == 0:
+                backup[1] = ([a.copy() for a in un], p_last.copy())
+                time_b[1], it_b[1] = time, it
+            it += 1
+        rep.update(iteration=last_run, time=time, tau=tau)
+        return urk, rep
+
     def rk_update(self, s, tau, met, un, urk, q, qsum):
         lib().orc_rk_update(ctypes.byref(self.c), s, ctypes.c_double(tau), _bundle(met), _bundle(un),
                             _bundle(urk), _bundle(q), _bundle(qsum))

@@ -1728,6 +1728,8 @@ struct shx_case {
     double *q[5], *qsum[5], *met[10], *grad[12], *src[5], *T, *mu;
     double *bs[6];         // exact state for manufactured faces
     double *P, *C;         // stored p and c (pc: some face is a subsonic outflow)
+    double *bk[2][5];      // shx_simulate's two backups of the state (and of P), allocated on use
+    double *bkP[2];
     double *PL, *RL, *UL, *C0;   // Rudy faces: step-start ghost p, rho, u; uploaded sound speed
     int pc, rudy;
     int has_src, has_metric, has_state, has_bs;
@@ -1942,6 +1944,10 @@ extern "C" int shx_destroy(shx_case *c)
     for (int m = 0; m < 6; m++) hipFree(c->bs[m]);
     hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt);
     hipFree(c->P); hipFree(c->C); hipFree(c->PL); hipFree(c->RL); hipFree(c->UL); hipFree(c->C0);
+    for (int b = 0; b < 2; b++) {
+        for (int m = 0; m < 5; m++) hipFree(c->bk[b][m]);
+        hipFree(c->bkP[b]);
+    }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -3207,6 +3213,93 @@ extern "C" int shx_run(shx_case *c, double tau, int nsteps)
         if (rc) return rc;
         if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
     }
+    return SHX_OK;
+}
+
+// state (and stored p) of the case into / out of backup slot b (WriteValuesFromU1ToU2,
+// src/Functions.c:2484-2508: every padded node); a reload writes every stage buffer, as an upload does
+static int backup_io(shx_case *c, int b, bool save)
+{
+    const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+    if (!c->bk[b][0]) {
+        for (int m = 0; m < 5; m++) HIPCHK(hipMalloc((void **)&c->bk[b][m], bytes));
+        if (c->pc) HIPCHK(hipMalloc((void **)&c->bkP[b], bytes));
+    }
+    for (int m = 0; m < 5; m++) {
+        if (save) {
+            HIPCHK(hipMemcpyAsync(c->bk[b][m], c->prim[c->cur][m], bytes, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            for (int s = 0; s < 3; s++)
+                HIPCHK(hipMemcpyAsync(c->prim[s][m], c->bk[b][m], bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    if (c->pc) {
+        if (save) HIPCHK(hipMemcpyAsync(c->bkP[b], c->P, bytes, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(hipMemcpyAsync(c->P, c->bkP[b], bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return SHX_OK;
+}
+
+extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!sim || !rep) return set_err(SHX_ERR_ARG, "null argument");
+    if (!(sim->tau > 0.0) || !(sim->L0 > 0.0) || !(sim->u0 > 0.0) || sim->end_iteration < sim->start_iteration ||
+        sim->distance_backward < 1 || sim->number_resets < 0 || !(sim->decel_factor > 0.0) ||
+        (sim->accelerator && !(sim->accel_factor > 0.0)))
+        return set_err(SHX_ERR_ARG, "bad simulation parameters");
+    memset(rep, 0, sizeof *rep);
+    const double tau_start = sim->tau, L0 = sim->L0, u0 = sim->u0;
+    const int end = sim->end_iteration, dback = sim->distance_backward, dnan = sim->distance_nan;
+    double tau = tau_start, time = sim->time, last = sim->time_last_action;
+    int dfwd = sim->distance_forward;
+    // slot 0 is the reference's backup1, slot 1 its backup2 (src/SHOCK.c:531-540)
+    int it_b[2] = {1, 999999};
+    double time_b[2] = {time, 0.0};
+    if ((rc = backup_io(c, 0, true))) return rc;
+    int last_run = sim->start_iteration;
+    for (int it = sim->start_iteration + 1; it <= end; it++) {
+        last_run = it;
+        time += tau * L0 / u0;
+        if ((rc = shx_calc_runge_kutta(c, tau))) return rc;
+        int bad = 0;
+        if ((rc = shx_check_nan(c, &bad))) return rc;
+        if (bad > 0) {
+            rep->nan_steps++;
+            if (rep->resets >= sim->number_resets) { rep->aborted = 1; break; }
+            rep->resets++;
+            tau *= sim->decel_factor;
+            dfwd = sim->distance_forward;
+            const int b = (it_b[0] < it_b[1]) ? 0 : 1;
+            if ((rc = backup_io(c, b, false))) return rc;
+            last = time;
+            time = time_b[b];
+            it = it_b[b];
+        } else if ((time - last) > dnan * tau * L0 / u0 && sim->accelerator && it + dback < end && tau < tau_start) {
+            dfwd = sim->distance_forward;
+            tau = tau_start;
+            last = time;
+            rep->tau_resets++;
+        } else if ((time - last) > dfwd * tau * L0 / u0 && sim->accelerator && it + dback < end && tau >= tau_start) {
+            dfwd = (int)(dfwd * 4.);
+            tau *= sim->accel_factor;
+            last = time;
+            rep->accelerations++;
+        }
+        if (it % dback == 0) {
+            if ((rc = backup_io(c, 0, true))) return rc;
+            time_b[0] = time; it_b[0] = it;
+        } else if ((it + dback / 2) % dback == 0) {
+            if ((rc = backup_io(c, 1, true))) return rc;
+            time_b[1] = time; it_b[1] = it;
+        }
+        if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    rep->iteration = last_run;
+    rep->time = time;
+    rep->tau = tau;
     return SHX_OK;
 }
 

@@ -44,6 +44,24 @@ class Desc(ctypes.Structure):
     ]
 
 
+class Sim(ctypes.Structure):
+    _fields_ = [
+        ("start_iteration", ctypes.c_int), ("end_iteration", ctypes.c_int), ("tau", ctypes.c_double),
+        ("time", ctypes.c_double), ("time_last_action", ctypes.c_double), ("L0", ctypes.c_double),
+        ("u0", ctypes.c_double), ("accelerator", ctypes.c_int), ("accel_factor", ctypes.c_double),
+        ("decel_factor", ctypes.c_double), ("number_resets", ctypes.c_int), ("distance_nan", ctypes.c_int),
+        ("distance_forward", ctypes.c_int), ("distance_backward", ctypes.c_int),
+    ]
+
+
+class SimReport(ctypes.Structure):
+    _fields_ = [
+        ("iteration", ctypes.c_int), ("time", ctypes.c_double), ("tau", ctypes.c_double),
+        ("resets", ctypes.c_int), ("nan_steps", ctypes.c_int), ("accelerations", ctypes.c_int),
+        ("tau_resets", ctypes.c_int), ("aborted", ctypes.c_int),
+    ]
+
+
 EXPORTS = [
     "shx_last_error", "shx_device_count", "shx_create", "shx_destroy", "shx_padded_extent",
     "shx_build_metric", "shx_upload_metric", "shx_download_metric", "shx_upload_state",
@@ -51,7 +69,7 @@ EXPORTS = [
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
-    "shx_batch_destroy", "shx_tune", "shx_check_metric",
+    "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -76,6 +94,7 @@ def lib():
             getattr(L, f).argtypes = [P, P]
         L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
         L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
+        L.shx_simulate.argtypes = [P, ctypes.POINTER(Sim), ctypes.POINTER(SimReport)]
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_check_metric.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -208,6 +227,17 @@ class Case:
 
     def run(self, tau, nsteps):
         _check(lib().shx_run(self.h, tau, nsteps))
+
+    def simulate(self, start, end, tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=False,
+                 accel_factor=1.1, decel_factor=0.7, number_resets=10, distance_nan=5000, distance_forward=1000,
+                 distance_backward=200):
+        """Iterations start+1 .. end with the reference's NaN rollback and time-step control
+        (defaults: the reference's tau:* defaults, src/Import.c:53-58).  Returns the report as a dict."""
+        sim = Sim(start, end, tau, time, time_last_action, L0, u0, int(accelerator), accel_factor, decel_factor,
+                  number_resets, distance_nan, distance_forward, distance_backward)
+        rep = SimReport()
+        _check(lib().shx_simulate(self.h, ctypes.byref(sim), ctypes.byref(rep)))
+        return {f: getattr(rep, f) for f, _ in SimReport._fields_}
 
     def rhs(self):
         out, p = self._outs(5)

@@ -1,0 +1,102 @@
+"""shx_simulate (startSimulation's loop with the NaN rollback and time-step control,
+src/SHOCK.c:531-691) against the oracle's restatement of the same loop: the same decisions
+(rollbacks, stops, tau resets and accelerations, final iteration, time and tau) and the same
+final state within the many-step bar."""
+import numpy as np
+import pytest
+
+import orc
+from helpers import conserved, interior, rel_linf, setup_2d_mms
+from shock_amd import cases, shockx
+from test_gpu_boundaries import GAMMA, IN_RUDY, MACH, OUTFLOW, _stream
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("resets", "nan_steps", "accelerations", "tau_resets", "aborted", "iteration")
+
+
+def _compare(o, nn, dims, want_state, want, got_state, got, tag):
+    print(f"[simulate] {tag}: oracle {want}", flush=True)
+    print(f"[simulate] {tag}: GPU    {got}", flush=True)
+    for k in KEYS:
+        assert got[k] == want[k], k
+    assert got["time"] == pytest.approx(want["time"], rel=1e-15, abs=0)
+    assert got["tau"] == pytest.approx(want["tau"], rel=1e-15, abs=0)
+    if not want["aborted"]:
+        sl = interior(o, dims, nn)
+        errs = rel_linf(conserved(got_state), conserved(want_state), sl)
+        print(f"[simulate] {tag}: state " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+        assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= 1e-10
+
+
+def _sim(tau, **kw):
+    base = dict(start=0, end=40, tau=tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=1,
+                accel_factor=1.1, decel_factor=0.5, number_resets=10, distance_nan=12, distance_forward=5,
+                distance_backward=4)
+    base.update(kw)
+    return base
+
+
+def _gpu(g, s):
+    return g.simulate(s["start"], s["end"], s["tau"], s["time"], s["time_last_action"], s["L0"], s["u0"],
+                      bool(s["accelerator"]), s["accel_factor"], s["decel_factor"], s["number_resets"],
+                      s["distance_nan"], s["distance_forward"], s["distance_backward"])
+
+
+def test_unstable_tau_recovers_periodic_2d():
+    # tau at six times the CFL-0.5 step: the state blows up, rollbacks halve tau, the resetter
+    # tries the starting tau again
+    o, coords, met, prim, nn = setup_2d_mms(n=32, S=5)
+    s = _sim(6.0 * o.cfl_dt(0.5, met, prim))
+    want_state, want = o.simulate(met, prim, s)
+    assert want["resets"] >= 1 and want["tau_resets"] >= 1
+    c = o.c
+    with shockx.Case(dims=2, space_order=5, time_order=4, n=nn, gamma=c.gamma, mach=c.mach,
+                     reynolds=c.reynolds, prandtl=c.prandtl, T0=c.T0) as g:
+        g.upload_metric(met)
+        g.upload_state(prim)
+        got = _gpu(g, s)
+        got_state = g.download_state()
+    _compare(o, nn, 2, want_state, want, got_state, got, "periodic 2D")
+
+
+def test_unstable_tau_recovers_with_rudy_outflow():
+    # the backups hold the stored p of the Rudy ghosts, which the next step's writers read
+    n = (24, 20, 1)
+    inflow = (1.0, 2.5, 0.0, 0.0, 1.0)
+    bc = IN_RUDY + (0, 0, 0, 0)
+    o = orc.Oracle(dims=2, S=5, n=n, bc=bc, inflow=inflow, mach=MACH, reynolds=1000.0, outflow=OUTFLOW)
+    xe, ye, ze = cases.ext_coords(2, 5, n, cases.cartesian_unit(2), periodic=(False, True, True))
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    prim = _stream(o, 2, (xe, ye, ze), inflow)
+    s = _sim(10.0 * 0.2 / (n[0] * 5.0))
+    want_state, want = o.simulate(met, prim, s)
+    assert want["resets"] >= 1
+    with shockx.Case(dims=2, space_order=5, time_order=4, n=n, gamma=GAMMA, mach=MACH, reynolds=1000.0,
+                     bc=bc, inflow=inflow, outflow=OUTFLOW) as g:
+        g.build_metric(xe, ye, ze)
+        g.upload_state(prim)
+        got = _gpu(g, s)
+        got_state = g.download_state()
+    _compare(o, n, 2, want_state, want, got_state, got, "Rudy outflow 2D")
+
+
+def test_nan_state_stops_after_the_resets():
+    o, coords, met, prim, nn = setup_2d_mms(n=16, S=5)
+    bad = [a.copy() for a in prim]
+    bad[0][o.g + 3, o.g + 5, 0] = np.nan
+    s = _sim(o.cfl_dt(0.5, met, prim), end=30, number_resets=3)
+    want_state, want = o.simulate(met, bad, s)
+    assert want["aborted"] == 1
+    c = o.c
+    with shockx.Case(dims=2, space_order=5, time_order=4, n=nn, gamma=c.gamma, mach=c.mach,
+                     reynolds=c.reynolds, prandtl=c.prandtl, T0=c.T0) as g:
+        g.upload_metric(met)
+        g.upload_state(bad)
+        got = _gpu(g, s)
+        got_state = g.download_state()
+    _compare(o, nn, 2, want_state, want, got_state, got, "NaN start")
+    sl = interior(o, 2, nn)
+    assert np.array_equal(np.isnan(got_state[0][sl]) | (got_state[0][sl] == 1e6),
+                          np.isnan(want_state[0][sl]) | (want_state[0][sl] == 1e6))
