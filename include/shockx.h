@@ -174,6 +174,10 @@ typedef struct shx_sim {
     double accel_factor, decel_factor;    /* tau:tauaccelerator_factor, tau:taudecelerator_factor */
     int number_resets;                    /* tau:NumberResets */
     int distance_nan, distance_forward, distance_backward;  /* tau:DistanceNAN, DistanceForward, DistanceBackward */
+    /* sampling into the film (shx_film_open first): int_Samples, IterationsBetweenSamples,
+       StartSampling; samples = 0: none.  Each sample is CalcValuesForPost + WriteValuesFromUToFilm
+       of the step's state; a rollback rewinds the sample index with the state. */
+    int samples, iterations_between_samples, start_sampling;
 } shx_sim;
 typedef struct shx_sim_report {
     int iteration;           /* last iteration the loop ran (the failing one after a stop) */
@@ -182,8 +186,26 @@ typedef struct shx_sim_report {
     int nan_steps;           /* steps whose checkNAN found a NaN or inf density */
     int accelerations, tau_resets;
     int aborted;             /* 1: stopped at number_resets */
+    int samples;             /* samples in the film at the end */
 } shx_sim_report;
 int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *report);
+
+/* Post-processing on the device.  shx_post: CalcValuesForPost with CalcLambda2
+ * (src/Functions.c:3687-3761, 5846-6070) on the interior nodes of the current state: Mach number,
+ * density gradient and lambda2 (3D) into host arrays on the padded grid (NULL entries skipped).
+ * The Mach number uses U_lastStep's stored pressure, which the reference's RK update leaves one
+ * stage behind the state; the library keeps that pressure from the first shx_post or
+ * shx_film_open call on (from the start for cases with subsonic outflow or slip wall faces), so
+ * call one of them before stepping for the reference's values.
+ * The film keeps samples on the device (WriteValuesFromUToFilm, src/Functions.c:2510-2553):
+ * shx_film_open allocates room for `samples`, shx_film_sample appends one sample of the current
+ * state with its time, shx_film_download copies sample s out as rho, u, v, w, p, gradRho, Mach,
+ * Lambda2 over the interior nodes (n0 x n1 x n2, k fastest). */
+int shx_post(shx_case *c, double *const out[3]);
+int shx_film_open(shx_case *c, int samples);
+int shx_film_sample(shx_case *c, double time);
+int shx_film_count(const shx_case *c, int *count);
+int shx_film_download(shx_case *c, int sample, double *const out[8], double *time);
 
 /* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
  * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux

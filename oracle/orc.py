@@ -227,8 +227,10 @@ class Oracle:
         DistanceBackward iterations (slot 1) and half-way between (slot 2), with slot 1 initially the
         starting state at iteration 1 and slot 2 at iteration 999999 (src/SHOCK.c:531-540).
         sim: dict with start, end, tau, time, time_last_action, L0, u0, accelerator, accel_factor,
-        decel_factor, number_resets, distance_nan, distance_forward, distance_backward.
-        Returns (state, report)."""
+        decel_factor, number_resets, distance_nan, distance_forward, distance_backward, and for
+        sampling (CalcValuesForPost + WriteValuesFromUToFilm of U_lastStep) samples,
+        iterations_between_samples, start_sampling.  Returns (state, report); report["film"] holds
+        (time, [rho, u, v, w, p, gradRho, Mach, Lambda2] over the interior) per sample."""
         un = [a.copy() for a in prim0]
         urk = [a.copy() for a in prim0]
         scratch = self.zeros(30)
@@ -241,6 +243,14 @@ class Oracle:
         backup = [([a.copy() for a in un], p_last.copy()), None]
         it_b, time_b = [1, 999999], [time, 0.0]
         rep = {"resets": 0, "nan_steps": 0, "accelerations": 0, "tau_resets": 0, "aborted": 0}
+        # sampling (src/SHOCK.c:694-717): counter and sample index, rewound with the backups
+        nsamp = sim.get("samples", 0)
+        between, start_s = sim.get("iterations_between_samples", 1), sim.get("start_sampling", 0)
+        counter, sample = 0, 0
+        counter_b, sample_b = [0, 0], [0, 0]
+        film = [None] * nsamp
+        sl = tuple(slice(self.g, self.g + self.c.n[d]) if (d < 2 or self.c.dims == 3) else slice(None)
+                   for d in range(3))
         it = sim["start"] + 1
         last_run = sim["start"]
         while it <= end:
@@ -264,6 +274,7 @@ class Oracle:
                 last = time
                 time = time_b[b]
                 it = it_b[b]
+                sample, counter = sample_b[b], counter_b[b]
             elif (time - last > dnan * tau * L0 / u0 and sim["accelerator"] and it + dback < end
                   and tau < tau_start):
                 dfwd = sim["distance_forward"]
@@ -281,12 +292,20 @@ class Oracle:
             p_last = p_rk.copy()
             if it % dback == 0:
                 backup[0] = ([a.copy() for a in un], p_last.copy())
-                time_b[0], it_b[0] = time, it
+                time_b[0], it_b[0], sample_b[0], counter_b[0] = time, it, sample, counter
             elif (it + dback // 2) % dback == 0:
                 backup[1] = ([a.copy() for a in un], p_last.copy())
-                time_b[1], it_b[1] = time, it
+                time_b[1], it_b[1], sample_b[1], counter_b[1] = time, it, sample, counter
+            counter += 1
+            if nsamp and counter % between == 0 and sample < nsamp and it >= start_s:
+                post = self.calc_post(met, un, p_last)
+                fields = list(un[:4]) + [p_last] + [post[1], post[0], post[2]]
+                film[sample] = (time, [f[sl].copy() for f in fields])
+                sample += 1
             it += 1
-        rep.update(iteration=last_run, time=time, tau=tau)
+        rep.update(iteration=last_run, time=time, tau=tau, samples=sample)
+        if nsamp:
+            rep["film"] = film[:sample]
         return urk, rep
 
     def rk_update(self, s, tau, met, un, urk, q, qsum):
@@ -302,6 +321,12 @@ class Oracle:
 
     def check_nan(self, prim):
         return lib().orc_check_nan(ctypes.byref(self.c), _bundle(prim))
+
+    def calc_post(self, met, prim, p):
+        """Mach, gradRho, Lambda2 on interior nodes (CalcValuesForPost); p is U_lastStep's stored p."""
+        out = self.zeros(3)
+        lib().orc_calc_post(ctypes.byref(self.c), _bundle(met), _bundle(prim), _ptr(p), _bundle(out))
+        return out
 
     def mirror_metric(self, met):
         """Wall and farfield faces: ghost metric copied from the mirror nodes (mirrorGhostCellsMetric), in place."""

@@ -1042,3 +1042,98 @@ int orc_check_metric(const orc_cfg *c, double *const met[10])
         if (met[9][n] < 0.0) bad++;
     return bad;
 }
+
+/* lambda2 of node n (CalcLambda2, src/Functions.c:5846-6070): velocity derivatives as differences
+ * of midpoint-interpolated values (the viscous interpolation table), Cartesian gradient through
+ * the metric, the middle eigenvalue of S^2 + Omega^2 by the trigonometric cubic solution, capped
+ * at 0.  A degenerate case where no root lies strictly between the others keeps 0. */
+static double lambda2_at(const orc_cfg *c, double *const met[10], double *const prim[5], int i, int j, int k)
+{
+    long n = orc_idx(c, i, j, k), st[3] = {stride(c, 0), stride(c, 1), stride(c, 2)};
+    int half = (c->S + 1) / 2, m, d, q;
+    double dv[3][3];           /* dv[q][d]: velocity component q along index direction d */
+    double lo[3][3], hi[3][3];
+    for (q = 0; q < 3; q++)
+        for (d = 0; d < 3; d++) { lo[q][d] = 0.; hi[q][d] = 0.; }
+    for (m = 0; m <= c->S; m++)
+        for (d = 0; d < 3; d++) {
+            long i0 = n + (long)(m - half) * st[d], i1 = n + (long)(m - half + 1) * st[d];
+            for (q = 0; q < 3; q++) {
+                lo[q][d] += c->zd_int[m] * prim[1 + q][i0];
+                hi[q][d] += c->zd_int[m] * prim[1 + q][i1];
+            }
+        }
+    for (q = 0; q < 3; q++)
+        for (d = 0; d < 3; d++) dv[q][d] = hi[q][d] - lo[q][d];
+    {
+        double xx = met[0][n], xy = met[1][n], xz = met[2][n], ex = met[3][n], ey = met[4][n], ez = met[5][n];
+        double zx = met[6][n], zy = met[7][n], zz = met[8][n];
+        double udx = dv[0][0] * xx + dv[0][1] * ex + dv[0][2] * zx, udy = dv[0][0] * xy + dv[0][1] * ey + dv[0][2] * zy;
+        double udz = dv[0][0] * xz + dv[0][1] * ez + dv[0][2] * zz;
+        double vdx = dv[1][0] * xx + dv[1][1] * ex + dv[1][2] * zx, vdy = dv[1][0] * xy + dv[1][1] * ey + dv[1][2] * zy;
+        double vdz = dv[1][0] * xz + dv[1][1] * ez + dv[1][2] * zz;
+        double wdx = dv[2][0] * xx + dv[2][1] * ex + dv[2][2] * zx, wdy = dv[2][0] * xy + dv[2][1] * ey + dv[2][2] * zy;
+        double wdz = dv[2][0] * xz + dv[2][1] * ez + dv[2][2] * zz;
+        double om12 = 0.5 * (udy - vdx), om13 = 0.5 * (udz - wdx), om23 = 0.5 * (vdz - wdy);
+        double s11 = 0.5 * (udx), s12 = 0.5 * (udy + vdx), s13 = 0.5 * (udz + wdx);
+        double s22 = 0.5 * (vdy), s23 = 0.5 * (vdz + wdy), s33 = 0.5 * (wdz);
+        double oo11 = -om12 * om12 - om13 * om13, oo12 = -om13 * om23, oo13 = om12 * om23;
+        double oo22 = -om12 * om12 - om23 * om23, oo23 = -om12 * om13, oo33 = -om13 * om13 - om23 * om23;
+        double ss11 = s11 * s11 + s12 * s12 + s13 * s13, ss12 = s11 * s12 + s12 * s22 + s13 * s23;
+        double ss13 = s11 * s13 + s12 * s23 + s13 * s33, ss22 = s12 * s12 + s22 * s22 + s23 * s23;
+        double ss23 = s12 * s13 + s22 * s23 + s23 * s33, ss33 = s13 * s13 + s23 * s23 + s33 * s33;
+        double a11 = ss11 + oo11, a12 = ss12 + oo12, a13 = ss13 + oo13, a22 = ss22 + oo22, a23 = ss23 + oo23;
+        double a33 = ss33 + oo33;
+        double f1 = -a11 - a22 - a33;
+        double f2 = -a23 * a23 - a12 * a12 - a13 * a13 + a11 * a33 + a11 * a22 + a22 * a33;
+        double f3 = -a11 * a22 * a33 + a11 * a23 * a23 + a22 * a13 * a13 + a33 * a12 * a12 - 2. * a12 * a13 * a23;
+        double p = -f1 * f1 / 3. + f2, qq = 2. * f1 * f1 * f1 / 27. - f1 * f2 / 3. + f3;
+        double rrr = sqrt((p < 0. ? -p : p) / 3.), al, l1, l2, l3, lmax, lmin, lam = 0.;
+        const double pi = 3.14159265358979323846264338327950288;
+        al = acos(qq / (2. * rrr * rrr * rrr));
+        l1 = -2. * rrr * cos(al / 3.) - f1 / 3;
+        l2 = 2. * rrr * cos(pi / 3. - al / 3.) - f1 / 3;
+        l3 = 2. * rrr * cos(pi / 3. + al / 3.) - f1 / 3;
+        lmax = (l2 >= l1) ? l2 : l1;
+        if (l3 >= lmax) lmax = l3;
+        lmin = (l2 <= l1) ? l2 : l1;
+        if (l3 <= lmin) lmin = l3;
+        if (l1 == l2 || l2 == l3) lam = l2;
+        if (l1 > lmin && l1 < lmax) lam = l1;
+        else if (l2 > lmin && l2 < lmax) lam = l2;
+        else if (l3 > lmin && l3 < lmax) lam = l3;
+        if (lam >= 0.) lam = 0.;
+        return lam;
+    }
+}
+
+/* CalcValuesForPost (src/Functions.c:3687-3761) on interior nodes: Mach number from the stored
+ * pressure p (U_lastStep's, one RK stage behind the state: the RK update does not renew it), the
+ * density "gradient" as the root of the sum of squares of every (central difference x metric
+ * entry) product, and lambda2 (3D; untouched in 2D).  out = Mach, gradRho, Lambda2. */
+void orc_calc_post(const orc_cfg *c, double *const met[10], double *const prim[5], const double *p,
+                   double *const out[3])
+{
+    int i, j, k;
+    long si = stride(c, 0), sj = stride(c, 1), sk = stride(c, 2);
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++)
+            for (k = rs(c, 2); k <= re(c, 2); k++) {
+                long n = orc_idx(c, i, j, k);
+                const double *r = prim[0];
+                double u = prim[1][n], v = prim[2][n], w = prim[3][n];
+                double dxi = (0.5 * r[n + si] - 0.5 * r[n - si]) / 1.0;
+                double deta = (0.5 * r[n + sj] - 0.5 * r[n - sj]) / 1.0;
+                out[0][n] = (sqrt(u * u + v * v + w * w)) / sqrt(p[n] / r[n]) * c->mach;
+                if (c->dims == 2) {
+                    out[1][n] = sqrt(pow(dxi * met[0][n], 2.0) + pow(deta * met[3][n], 2.0)
+                                     + pow(dxi * met[1][n], 2.0) + pow(deta * met[4][n], 2.0));
+                } else {
+                    double dze = (0.5 * r[n + sk] - 0.5 * r[n - sk]) / 1.0;
+                    out[1][n] = sqrt(pow(dxi * met[0][n], 2.0) + pow(deta * met[3][n], 2.0) + pow(dze * met[6][n], 2.0)
+                                     + pow(dxi * met[1][n], 2.0) + pow(deta * met[4][n], 2.0) + pow(dze * met[7][n], 2.0)
+                                     + pow(dxi * met[2][n], 2.0) + pow(deta * met[5][n], 2.0) + pow(dze * met[8][n], 2.0));
+                    out[2][n] = lambda2_at(c, met, prim, i, j, k);
+                }
+            }
+}

@@ -30,6 +30,9 @@
  *                         flips (no-slip); quiescent gas and wall-parallel stream stay exact
  *   orc_mirror_metric     copy identity on every ghost node of the slab
  *   farfield writer       copy identity on the interior tangential range
+ *   orc_calc_post         Mach closed form; gradRho exact on an anisotropic affine grid for a
+ *                         linear density; lambda2 against numpy's eigenvalues of S^2 + Omega^2
+ *                         for a linear velocity field on a sheared affine grid
  *   isothermal wall       no-slip ghost with T = T_wall and the Prandtl-0.72 factor; the energy
  *                         term's mean factor checked against a hand-set factor array
  *   subsonic inflow writers  normal: inflow state; Riemann: the far field's u + 2c/(gamma-1) and
@@ -145,6 +148,11 @@ void orc_visc_dir(const orc_cfg *c, int dir, double *const met[10], double *cons
 /* mirrorGhostCellsMetric (src/Functions.c:6849-7014): on wall and farfield faces, the metric of every ghost
  * node of the face slab (full tangential range) is copied from its mirror node, faces in order. */
 void orc_mirror_metric(const orc_cfg *c, double *const met[10]);
+
+/* CalcValuesForPost with CalcLambda2 (src/Functions.c:3687-3761, 5846-6070) on interior nodes;
+ * p is the stored pressure of U_lastStep; out = Mach, gradRho, Lambda2 (3D only). */
+void orc_calc_post(const orc_cfg *c, double *const met[10], double *const prim[5], const double *p,
+                   double *const out[3]);
 
 void orc_zero(const orc_cfg *c, double *const f[], int nf);
 void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);

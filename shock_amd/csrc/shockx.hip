@@ -298,6 +298,21 @@ __global__ void k_carry_ghosts(CFields5 from, Fields5 to, Geom gm)
 {
     carry_body(from, to, gm, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
+// every ghost node (periodic directions included) copied from one buffer to another
+__global__ void k_copy_ghosts(CFields5 from, Fields5 to, Geom gm)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= gm.ntot) return;
+    int idx[3];
+    const long n = padded_node(gm, t, idx);
+    bool ghost = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (gm.ng[d] > 1 && (idx[d] < gm.lo[d] || idx[d] >= gm.lo[d] + gm.n[d])) ghost = true;
+    if (!ghost) return;
+#pragma unroll
+    for (int m = 0; m < 5; m++) to.f[m][n] = from.f[m][n];
+}
 
 // T = |p/rho| and mu (Sutherland, or 1 under the manufactured solution) of one node, CalcValues
 // (src/Functions.c:2462-2476)
@@ -1623,8 +1638,9 @@ __global__ void k_cfl_max(CFields5 pr, CFields10 met, Geom gm, Consts k, unsigne
     }
 }
 
-// checkNAN (src/Functions.c:5419-5467): count and overwrite non-finite interior rho.
-__global__ void k_check_nan(Fields5 pr, Geom gm, int *count)
+// checkNAN (src/Functions.c:5419-5467): count and overwrite non-finite interior rho (and the
+// stored p, where it is kept).
+__global__ void k_check_nan(Fields5 pr, gdp P, Geom gm, int *count)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
@@ -1639,10 +1655,126 @@ __global__ void k_check_nan(Fields5 pr, Geom gm, int *count)
         if (!isfinite(rho)) {
             bad = 1;
             pr.f[0][n] = 1000000; pr.f[1][n] = 1000000; pr.f[2][n] = 1000000; pr.f[3][n] = 1000000;
+            if (P) P[n] = 1000000;
         }
     }
     bad = wave_sum(bad);
     if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
+// ---- post-processing ---------------------------------------------------------------------
+// lambda2 of interior node n (CalcLambda2, src/Functions.c:5846-6070): velocity derivatives as
+// differences of midpoint-interpolated values (the viscous interpolation table), the Cartesian
+// gradient through the metric, the middle eigenvalue of S^2 + Omega^2 by the trigonometric
+// solution of the characteristic cubic, capped at 0.  The reference's strain tensor carries half
+// the derivative on its diagonal (s11 = 0.5 u_x, src/Functions.c:5952-5957); kept.
+__device__ double lambda2_node(const CFields5 &pr, const CFields10 &met, const Geom &gm, const Consts &k,
+                               const int S, const long n)
+{
+    const int half = (S + 1) / 2;
+    double dv[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double lo[3] = {0., 0., 0.}, hi[3] = {0., 0., 0.};
+        for (int m = 0; m <= S; m++) {
+            const long i0 = n + (long)(m - half) * gm.st[d], i1 = i0 + gm.st[d];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                lo[q] += k.zd_int[m] * pr.f[1 + q][i0];
+                hi[q] += k.zd_int[m] * pr.f[1 + q][i1];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) dv[q][d] = hi[q] - lo[q];
+    }
+    const double xx = met.f[0][n], xy = met.f[1][n], xz = met.f[2][n], ex = met.f[3][n], ey = met.f[4][n];
+    const double ez = met.f[5][n], zx = met.f[6][n], zy = met.f[7][n], zz = met.f[8][n];
+    const double udx = dv[0][0] * xx + dv[0][1] * ex + dv[0][2] * zx, udy = dv[0][0] * xy + dv[0][1] * ey + dv[0][2] * zy;
+    const double udz = dv[0][0] * xz + dv[0][1] * ez + dv[0][2] * zz;
+    const double vdx = dv[1][0] * xx + dv[1][1] * ex + dv[1][2] * zx, vdy = dv[1][0] * xy + dv[1][1] * ey + dv[1][2] * zy;
+    const double vdz = dv[1][0] * xz + dv[1][1] * ez + dv[1][2] * zz;
+    const double wdx = dv[2][0] * xx + dv[2][1] * ex + dv[2][2] * zx, wdy = dv[2][0] * xy + dv[2][1] * ey + dv[2][2] * zy;
+    const double wdz = dv[2][0] * xz + dv[2][1] * ez + dv[2][2] * zz;
+    const double om12 = 0.5 * (udy - vdx), om13 = 0.5 * (udz - wdx), om23 = 0.5 * (vdz - wdy);
+    const double s11 = 0.5 * (udx), s12 = 0.5 * (udy + vdx), s13 = 0.5 * (udz + wdx);
+    const double s22 = 0.5 * (vdy), s23 = 0.5 * (vdz + wdy), s33 = 0.5 * (wdz);
+    const double a11 = (s11 * s11 + s12 * s12 + s13 * s13) + (-om12 * om12 - om13 * om13);
+    const double a12 = (s11 * s12 + s12 * s22 + s13 * s23) + (-om13 * om23);
+    const double a13 = (s11 * s13 + s12 * s23 + s13 * s33) + (om12 * om23);
+    const double a22 = (s12 * s12 + s22 * s22 + s23 * s23) + (-om12 * om12 - om23 * om23);
+    const double a23 = (s12 * s13 + s22 * s23 + s23 * s33) + (-om12 * om13);
+    const double a33 = (s13 * s13 + s23 * s23 + s33 * s33) + (-om13 * om13 - om23 * om23);
+    const double f1 = -a11 - a22 - a33;
+    const double f2 = -a23 * a23 - a12 * a12 - a13 * a13 + a11 * a33 + a11 * a22 + a22 * a33;
+    const double f3 = -a11 * a22 * a33 + a11 * a23 * a23 + a22 * a13 * a13 + a33 * a12 * a12 - 2. * a12 * a13 * a23;
+    const double p = -f1 * f1 / 3. + f2, q = 2. * f1 * f1 * f1 / 27. - f1 * f2 / 3. + f3;
+    const double rrr = sqrt(fabs(p) / 3.);
+    const double pi = 3.14159265358979323846;
+    const double al = acos(q / (2. * rrr * rrr * rrr));
+    const double l1 = -2. * rrr * cos(al / 3.) - f1 / 3, l2 = 2. * rrr * cos(pi / 3. - al / 3.) - f1 / 3;
+    const double l3 = 2. * rrr * cos(pi / 3. + al / 3.) - f1 / 3;
+    double lmax = (l2 >= l1) ? l2 : l1;
+    if (l3 >= lmax) lmax = l3;
+    double lmin = (l2 <= l1) ? l2 : l1;
+    if (l3 <= lmin) lmin = l3;
+    double lam = 0.;
+    if (l1 == l2 || l2 == l3) lam = l2;
+    if (l1 > lmin && l1 < lmax) lam = l1;
+    else if (l2 > lmin && l2 < lmax) lam = l2;
+    else if (l3 > lmin && l3 < lmax) lam = l3;
+    return (lam >= 0.) ? 0. : lam;
+}
+
+// CalcValuesForPost (src/Functions.c:3687-3761) on interior nodes: Mach number from the stored
+// pressure (U_lastStep's, which the RK update leaves one stage behind), the density "gradient" as
+// the root of the sum of squares of every (central difference x metric entry) product, and
+// lambda2 (3D).  out: Mach, gradRho, Lambda2 on the padded grid.
+__global__ void k_post(CFields5 pr, gcdp P, CFields10 met, Fields5 out, Geom gm, Consts k, int S)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    if (t >= nint) return;
+    const int kk = (int)(t % gm.n[2]);
+    const long r = t / gm.n[2];
+    const int jj = (int)(r % gm.n[1]);
+    const int ii = (int)(r / gm.n[1]);
+    const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+    const double rho = pr.f[0][n], u = pr.f[1][n], v = pr.f[2][n], w = pr.f[3][n];
+    out.f[0][n] = (sqrt(u * u + v * v + w * w)) / sqrt(P[n] / rho) * k.mach;
+    const double dxi = (0.5 * pr.f[0][n + gm.st[0]] - 0.5 * pr.f[0][n - gm.st[0]]) / 1.0;
+    const double deta = (0.5 * pr.f[0][n + gm.st[1]] - 0.5 * pr.f[0][n - gm.st[1]]) / 1.0;
+    auto sq = [](const double x) { return pow(x, 2.0); };
+    if (gm.dims == 2) {
+        out.f[1][n] = sqrt(sq(dxi * met.f[0][n]) + sq(deta * met.f[3][n]) + sq(dxi * met.f[1][n]) + sq(deta * met.f[4][n]));
+    } else {
+        const double dze = (0.5 * pr.f[0][n + gm.st[2]] - 0.5 * pr.f[0][n - gm.st[2]]) / 1.0;
+        out.f[1][n] = sqrt(sq(dxi * met.f[0][n]) + sq(deta * met.f[3][n]) + sq(dze * met.f[6][n])
+                           + sq(dxi * met.f[1][n]) + sq(deta * met.f[4][n]) + sq(dze * met.f[7][n])
+                           + sq(dxi * met.f[2][n]) + sq(deta * met.f[5][n]) + sq(dze * met.f[8][n]));
+        out.f[2][n] = lambda2_node(pr, met, gm, k, S, n);
+    }
+}
+
+// WriteValuesFromUToFilm (src/Functions.c:2510-2553): rho, u, v, w, p, gradRho, Mach, Lambda2 of
+// the interior nodes into sample `s` of the device-resident film (interior-compact, field-major)
+__global__ void k_film(CFields5 pr, gcdp P, CFields5 post, gdp film, long nint, int s, Geom gm)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nint) return;
+    const int kk = (int)(t % gm.n[2]);
+    const long r = t / gm.n[2];
+    const int jj = (int)(r % gm.n[1]);
+    const int ii = (int)(r / gm.n[1]);
+    const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+    const long base = (long)s * 8 * nint + t;
+    film[base + 0 * nint] = pr.f[0][n];
+    film[base + 1 * nint] = pr.f[1][n];
+    film[base + 2 * nint] = pr.f[2][n];
+    film[base + 3 * nint] = pr.f[3][n];
+    film[base + 4 * nint] = P[n];
+    film[base + 5 * nint] = post.f[1][n];
+    film[base + 6 * nint] = post.f[0][n];
+    film[base + 7 * nint] = post.f[2][n];
 }
 
 // check_Metric (src/Functions.c:6198-6243): count the padded nodes with a negative Jacobian.
@@ -1730,6 +1862,11 @@ struct shx_case {
     double *P, *C;         // stored p and c (pc: some face is a subsonic outflow)
     double *bk[2][5];      // shx_simulate's two backups of the state (and of P), allocated on use
     double *bkP[2];
+    double *post[3];       // Mach, gradRho, Lambda2 (padded), allocated on use
+    double *film;          // device-resident samples: 8 interior fields each
+    int last_in;           // buffer of the last stage's input of the last step (-1: none)
+    int film_cap, film_n;
+    std::vector<double> film_time;
     double *PL, *RL, *UL, *C0;   // Rudy faces: step-start ghost p, rho, u; uploaded sound speed
     int pc, rudy;
     int has_src, has_metric, has_state, has_bs;
@@ -1830,6 +1967,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         return set_err(SHX_ERR_ARG, "non-physical fluid properties");
 
     shx_case *c = new shx_case();
+    c->last_in = -1;
     c->d = d;
     Geom &gm = c->gm;
     gm.dims = d.dims;
@@ -1872,6 +2010,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     memset(&k, 0, sizeof k);
     k.gamma = d.gamma;
     k.upsilon = 1.0 / (d.gamma * (d.mach * d.mach));
+    k.mach = d.mach;
     k.psi = 1.0 / d.reynolds;
     k.gam_heat = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * d.prandtl);
     k.suth = 100.4 / d.T0;
@@ -1948,6 +2087,8 @@ extern "C" int shx_destroy(shx_case *c)
         for (int m = 0; m < 5; m++) hipFree(c->bk[b][m]);
         hipFree(c->bkP[b]);
     }
+    for (int m = 0; m < 3; m++) hipFree(c->post[m]);
+    hipFree(c->film);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -2167,6 +2308,7 @@ extern "C" int shx_upload_state(shx_case *c, const double *const prim[5])
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_state = 1;
+    c->last_in = -1;
     return SHX_OK;
 }
 
@@ -2187,9 +2329,13 @@ extern "C" int shx_upload_boundary_state(shx_case *c, const double *const bs[6])
     return SHX_OK;
 }
 
+static int sync_ghosts(shx_case *c);
 extern "C" int shx_download_state(shx_case *c, double *const prim[5])
 {
     if (!c || !prim) return set_err(SHX_ERR_ARG, "null argument");
+    // the ghost values the reference's U_lastStep holds after a step
+    int rc0 = sync_ghosts(c);
+    if (rc0) return rc0;
     for (int m = 0; m < 5; m++) { int rc = copy_out(c, prim[m], c->prim[c->cur][m]); if (rc) return rc; }
     HIPCHK(hipStreamSynchronize(c->stream));
     return SHX_OK;
@@ -2572,6 +2718,9 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
         in = outb;
         outb = (outb == bx) ? by : bx;
     }
+    // the last stage's input holds the ghost values the reference's U_RK (and so U_lastStep) keeps
+    // after the step; post-processing copies them over when it needs them
+    c->last_in = (in == bx) ? by : bx;
     c->cur = in;   // U_lastStep <- U_RK (pointer swap instead of WriteValuesFromU1ToU2)
     return SHX_OK;
 }
@@ -3216,15 +3365,148 @@ extern "C" int shx_run(shx_case *c, double tau, int nsteps)
     return SHX_OK;
 }
 
+// The stored pressure of U_lastStep (the last stage's CalcValues value, which the RK update does
+// not renew) is what the reference's post-processing reads.  Cases with subsonic outflow or slip
+// wall faces keep it anyway; others start keeping it here, from the current state's own p.
+static int keep_pressure(shx_case *c)
+{
+    if (c->pc) return SHX_OK;
+    const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
+    HIPCHK(hipMalloc((void **)&c->P, bytes));
+    HIPCHK(hipMalloc((void **)&c->C, bytes));
+    CFields5 cp;
+    for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+    k_calc_values<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(cp, nullptr, nullptr, c->P, c->C, c->gm, c->k);
+    LAUNCH_CHECK();
+    c->pc = 1;
+    return SHX_OK;
+}
+
+// U_lastStep's ghosts in the reference are those of the last stage's input (filled at that stage's
+// start); the state buffer gets them before it is read whole (post-processing, backups).  The next
+// step refills the periodic ghosts and carries the others, so stepping is unaffected.
+static int sync_ghosts(shx_case *c)
+{
+    if (c->last_in < 0 || c->last_in == c->cur) return SHX_OK;
+    CFields5 a;
+    Fields5 b;
+    for (int m = 0; m < 5; m++) { a.f[m] = c->prim[c->last_in][m]; b.f[m] = c->prim[c->cur][m]; }
+    k_copy_ghosts<<<nblk(c->gm.ntot, 256), 256, 0, c->stream>>>(a, b, c->gm);
+    LAUNCH_CHECK();
+    c->last_in = -1;
+    return SHX_OK;
+}
+
+static int post_compute(shx_case *c)
+{
+    int rc = keep_pressure(c);
+    if (rc) return rc;
+    if ((rc = sync_ghosts(c))) return rc;
+    const Geom &gm = c->gm;
+    const size_t bytes = sizeof(double) * (size_t)gm.nalloc;
+    for (int m = 0; m < 3; m++)
+        if (!c->post[m]) {
+            HIPCHK(hipMalloc((void **)&c->post[m], bytes));
+            HIPCHK(hipMemsetAsync(c->post[m], 0, bytes, c->stream));
+        }
+    CFields5 cp;
+    CFields10 mf;
+    Fields5 out;
+    memset(&out, 0, sizeof out);
+    for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+    for (int m = 0; m < 10; m++) mf.f[m] = c->met[m];
+    for (int m = 0; m < 3; m++) out.f[m] = c->post[m];
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    k_post<<<nblk(nint, 256), 256, 0, c->stream>>>(cp, c->P, mf, out, gm, c->k, c->d.space_order);
+    LAUNCH_CHECK();
+    return SHX_OK;
+}
+
+extern "C" int shx_post(shx_case *c, double *const out[3])
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return set_err(SHX_ERR_ARG, "null output");
+    if ((rc = post_compute(c))) return rc;
+    for (int m = 0; m < 3; m++)
+        if (out[m]) { rc = copy_out(c, out[m], c->post[m]); if (rc) return rc; }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_film_open(shx_case *c, int samples)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (samples < 1) return set_err(SHX_ERR_ARG, "samples must be positive");
+    if ((rc = keep_pressure(c))) return rc;
+    const long nint = (long)c->gm.n[0] * c->gm.n[1] * c->gm.n[2];
+    hipFree(c->film);
+    c->film = nullptr;
+    HIPCHK(hipMalloc((void **)&c->film, sizeof(double) * (size_t)8 * nint * samples));
+    c->film_cap = samples;
+    c->film_n = 0;
+    c->film_time.assign(samples, 0.0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SHX_OK;
+}
+
+// one sample of the current state into the film (CalcValuesForPost + WriteValuesFromUToFilm)
+static int film_sample(shx_case *c, double time)
+{
+    if (c->film_n >= c->film_cap) return set_err(SHX_ERR_STATE, "film full (%d samples)", c->film_cap);
+    int rc = post_compute(c);
+    if (rc) return rc;
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    CFields5 cp, po;
+    memset(&po, 0, sizeof po);
+    for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+    for (int m = 0; m < 3; m++) po.f[m] = c->post[m];
+    k_film<<<nblk(nint, 256), 256, 0, c->stream>>>(cp, c->P, po, c->film, nint, c->film_n, gm);
+    LAUNCH_CHECK();
+    c->film_time[c->film_n++] = time;
+    return SHX_OK;
+}
+
+extern "C" int shx_film_sample(shx_case *c, double time)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->film) return set_err(SHX_ERR_STATE, "no film: call shx_film_open");
+    return film_sample(c, time);
+}
+
+extern "C" int shx_film_download(shx_case *c, int sample, double *const out[8], double *time)
+{
+    if (!c || !out) return set_err(SHX_ERR_ARG, "null argument");
+    if (!c->film || sample < 0 || sample >= c->film_n) return set_err(SHX_ERR_ARG, "no such sample");
+    const long nint = (long)c->gm.n[0] * c->gm.n[1] * c->gm.n[2];
+    for (int m = 0; m < 8; m++)
+        if (out[m])
+            HIPCHK(hipMemcpyAsync(out[m], c->film + ((long)sample * 8 + m) * nint, sizeof(double) * nint,
+                                  hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (time) *time = c->film_time[sample];
+    return SHX_OK;
+}
+
+extern "C" int shx_film_count(const shx_case *c, int *count)
+{
+    if (!c || !count) return set_err(SHX_ERR_ARG, "null argument");
+    *count = c->film_n;
+    return SHX_OK;
+}
+
 // state (and stored p) of the case into / out of backup slot b (WriteValuesFromU1ToU2,
 // src/Functions.c:2484-2508: every padded node); a reload writes every stage buffer, as an upload does
 static int backup_io(shx_case *c, int b, bool save)
 {
     const size_t bytes = sizeof(double) * (size_t)c->gm.nalloc;
-    if (!c->bk[b][0]) {
+    if (save) { int rc = sync_ghosts(c); if (rc) return rc; }
+    if (!c->bk[b][0])
         for (int m = 0; m < 5; m++) HIPCHK(hipMalloc((void **)&c->bk[b][m], bytes));
-        if (c->pc) HIPCHK(hipMalloc((void **)&c->bkP[b], bytes));
-    }
+    if (c->pc && !c->bkP[b]) HIPCHK(hipMalloc((void **)&c->bkP[b], bytes));
     for (int m = 0; m < 5; m++) {
         if (save) {
             HIPCHK(hipMemcpyAsync(c->bk[b][m], c->prim[c->cur][m], bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -3237,6 +3519,7 @@ static int backup_io(shx_case *c, int b, bool save)
         if (save) HIPCHK(hipMemcpyAsync(c->bkP[b], c->P, bytes, hipMemcpyDeviceToDevice, c->stream));
         else HIPCHK(hipMemcpyAsync(c->P, c->bkP[b], bytes, hipMemcpyDeviceToDevice, c->stream));
     }
+    if (!save) c->last_in = -1;
     return SHX_OK;
 }
 
@@ -3249,7 +3532,11 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
         sim->distance_backward < 1 || sim->number_resets < 0 || !(sim->decel_factor > 0.0) ||
         (sim->accelerator && !(sim->accel_factor > 0.0)))
         return set_err(SHX_ERR_ARG, "bad simulation parameters");
+    const bool sampling = sim->samples > 0;
+    if (sampling && (sim->iterations_between_samples < 1 || !c->film || c->film_cap < sim->samples))
+        return set_err(SHX_ERR_ARG, "sampling needs iterations_between_samples >= 1 and a film of that many samples");
     memset(rep, 0, sizeof *rep);
+    if (sampling && (rc = keep_pressure(c))) return rc;
     const double tau_start = sim->tau, L0 = sim->L0, u0 = sim->u0;
     const int end = sim->end_iteration, dback = sim->distance_backward, dnan = sim->distance_nan;
     double tau = tau_start, time = sim->time, last = sim->time_last_action;
@@ -3257,6 +3544,9 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
     // slot 0 is the reference's backup1, slot 1 its backup2 (src/SHOCK.c:531-540)
     int it_b[2] = {1, 999999};
     double time_b[2] = {time, 0.0};
+    // sampling counters (int_iterationCounterSamples, int_actualSample) and their backups
+    int counter = 0, sample = c->film_n;
+    int counter_b[2] = {0, 0}, sample_b[2] = {sample, 0};
     if ((rc = backup_io(c, 0, true))) return rc;
     int last_run = sim->start_iteration;
     for (int it = sim->start_iteration + 1; it <= end; it++) {
@@ -3276,6 +3566,8 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
             last = time;
             time = time_b[b];
             it = it_b[b];
+            sample = sample_b[b];
+            counter = counter_b[b];
         } else if ((time - last) > dnan * tau * L0 / u0 && sim->accelerator && it + dback < end && tau < tau_start) {
             dfwd = sim->distance_forward;
             tau = tau_start;
@@ -3289,10 +3581,19 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
         }
         if (it % dback == 0) {
             if ((rc = backup_io(c, 0, true))) return rc;
-            time_b[0] = time; it_b[0] = it;
+            time_b[0] = time; it_b[0] = it; sample_b[0] = sample; counter_b[0] = counter;
         } else if ((it + dback / 2) % dback == 0) {
             if ((rc = backup_io(c, 1, true))) return rc;
-            time_b[1] = time; it_b[1] = it;
+            time_b[1] = time; it_b[1] = it; sample_b[1] = sample; counter_b[1] = counter;
+        }
+        // sampling (src/SHOCK.c:694-717): every iterations_between_samples iterations, from
+        // start_sampling on, while samples remain; a rollback rewinds the sample index
+        counter++;
+        if (sampling && counter % sim->iterations_between_samples == 0 && sample < sim->samples &&
+            it >= sim->start_sampling) {
+            c->film_n = sample;
+            if ((rc = film_sample(c, time))) return rc;
+            sample++;
         }
         if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
     }
@@ -3300,6 +3601,8 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
     rep->iteration = last_run;
     rep->time = time;
     rep->tau = tau;
+    if (sampling) c->film_n = sample;
+    rep->samples = sampling ? sample : c->film_n;
     return SHX_OK;
 }
 
@@ -3348,7 +3651,7 @@ extern "C" int shx_check_nan(shx_case *c, int *count)
     for (int m = 0; m < 5; m++) pf.f[m] = c->prim[c->cur][m];
     HIPCHK(hipMemsetAsync(c->d_cnt, 0, sizeof(int), c->stream));
     if ((rc = tbeg(c, KT_REDUCE))) return rc;
-    k_check_nan<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, gm, c->d_cnt);
+    k_check_nan<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, c->P, gm, c->d_cnt);
     LAUNCH_CHECK();
     if ((rc = tend(c))) return rc;
     HIPCHK(hipMemcpyAsync(count, c->d_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -12,6 +12,7 @@ namespace shx {
 struct Consts {
     double gamma, upsilon, psi, gam_heat, suth, eps;
     double gm1_ups;   // (gamma - 1) / Upsilon
+    double mach;      // reference Mach number (post-processing)
     double zd_mid[10], zd_int[10], zd_der[11];
     int mms;
 };

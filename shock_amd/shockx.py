@@ -50,7 +50,8 @@ class Sim(ctypes.Structure):
         ("time", ctypes.c_double), ("time_last_action", ctypes.c_double), ("L0", ctypes.c_double),
         ("u0", ctypes.c_double), ("accelerator", ctypes.c_int), ("accel_factor", ctypes.c_double),
         ("decel_factor", ctypes.c_double), ("number_resets", ctypes.c_int), ("distance_nan", ctypes.c_int),
-        ("distance_forward", ctypes.c_int), ("distance_backward", ctypes.c_int),
+        ("distance_forward", ctypes.c_int), ("distance_backward", ctypes.c_int), ("samples", ctypes.c_int),
+        ("iterations_between_samples", ctypes.c_int), ("start_sampling", ctypes.c_int),
     ]
 
 
@@ -58,7 +59,7 @@ class SimReport(ctypes.Structure):
     _fields_ = [
         ("iteration", ctypes.c_int), ("time", ctypes.c_double), ("tau", ctypes.c_double),
         ("resets", ctypes.c_int), ("nan_steps", ctypes.c_int), ("accelerations", ctypes.c_int),
-        ("tau_resets", ctypes.c_int), ("aborted", ctypes.c_int),
+        ("tau_resets", ctypes.c_int), ("aborted", ctypes.c_int), ("samples", ctypes.c_int),
     ]
 
 
@@ -69,7 +70,8 @@ EXPORTS = [
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
-    "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate",
+    "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
+    "shx_film_sample", "shx_film_count", "shx_film_download",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -95,6 +97,11 @@ def lib():
         L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
         L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
         L.shx_simulate.argtypes = [P, ctypes.POINTER(Sim), ctypes.POINTER(SimReport)]
+        L.shx_post.argtypes = [P, P]
+        L.shx_film_open.argtypes = [P, ctypes.c_int]
+        L.shx_film_sample.argtypes = [P, ctypes.c_double]
+        L.shx_film_count.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.shx_film_download.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_check_metric.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -230,14 +237,43 @@ class Case:
 
     def simulate(self, start, end, tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=False,
                  accel_factor=1.1, decel_factor=0.7, number_resets=10, distance_nan=5000, distance_forward=1000,
-                 distance_backward=200):
+                 distance_backward=200, samples=0, iterations_between_samples=1, start_sampling=0):
         """Iterations start+1 .. end with the reference's NaN rollback and time-step control
-        (defaults: the reference's tau:* defaults, src/Import.c:53-58).  Returns the report as a dict."""
+        (defaults: the reference's tau:* defaults, src/Import.c:53-58), sampling into the film when
+        samples > 0 (film_open first).  Returns the report as a dict."""
         sim = Sim(start, end, tau, time, time_last_action, L0, u0, int(accelerator), accel_factor, decel_factor,
-                  number_resets, distance_nan, distance_forward, distance_backward)
+                  number_resets, distance_nan, distance_forward, distance_backward, samples,
+                  iterations_between_samples, start_sampling)
         rep = SimReport()
         _check(lib().shx_simulate(self.h, ctypes.byref(sim), ctypes.byref(rep)))
         return {f: getattr(rep, f) for f, _ in SimReport._fields_}
+
+    def post(self):
+        """Mach, gradRho, Lambda2 (padded arrays; interior nodes filled) of the current state."""
+        out, p = self._outs(3)
+        _check(lib().shx_post(self.h, p))
+        return out
+
+    def film_open(self, samples):
+        _check(lib().shx_film_open(self.h, samples))
+
+    def film_sample(self, time):
+        _check(lib().shx_film_sample(self.h, time))
+
+    def film_count(self):
+        n = ctypes.c_int()
+        _check(lib().shx_film_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def film_download(self, s):
+        """Sample s: (time, [rho, u, v, w, p, gradRho, Mach, Lambda2]) over the interior nodes."""
+        d = self.desc
+        n = (d.n[0], d.n[1], d.n[2] if d.dims == 3 else 1)
+        out = [np.zeros(n) for _ in range(8)]
+        ptrs = (ctypes.c_void_p * 8)(*[a.ctypes.data for a in out])
+        t = ctypes.c_double()
+        _check(lib().shx_film_download(self.h, s, ptrs, ctypes.byref(t)))
+        return t.value, out
 
     def rhs(self):
         out, p = self._outs(5)
