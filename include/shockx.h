@@ -178,6 +178,12 @@ typedef struct shx_sim {
        StartSampling; samples = 0: none.  Each sample is CalcValuesForPost + WriteValuesFromUToFilm
        of the step's state; a rollback rewinds the sample index with the state. */
     int samples, iterations_between_samples, start_sampling;
+    /* probe histories (WriteValuesForPressureHistory / WriteValuesForVelocityHistory,
+       src/Functions.c:5381-5417): the stored p and u, v, w of nprobes interior nodes (padded
+       linear indices) every iteration, kept on the device under index iteration - (start + 1),
+       which a rollback rewinds; read them with shx_history_download.  nprobes = 0: none. */
+    const long *probes;
+    int nprobes;
 } shx_sim;
 typedef struct shx_sim_report {
     int iteration;           /* last iteration the loop ran (the failing one after a stop) */
@@ -206,6 +212,9 @@ int shx_film_open(shx_case *c, int samples);
 int shx_film_sample(shx_case *c, double time);
 int shx_film_count(const shx_case *c, int *count);
 int shx_film_download(shx_case *c, int sample, double *const out[8], double *time);
+/* probe histories of the last shx_simulate: time[iterations], then p, u, v, w each
+ * [nprobes][iterations] (iterations = end - start); NULL entries skipped */
+int shx_history_download(shx_case *c, double *time, double *p, double *u, double *v, double *w);
 
 /* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
  * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux

@@ -249,6 +249,11 @@ class Oracle:
         counter, sample = 0, 0
         counter_b, sample_b = [0, 0], [0, 0]
         film = [None] * nsamp
+        # probe histories (src/Functions.c:5381-5417) under index iteration - (start + 1)
+        probes = list(sim.get("probes", ()))
+        iters = sim["end"] - sim["start"]
+        hist_t = np.zeros(iters)
+        hist = np.zeros((4, len(probes), iters))
         sl = tuple(slice(self.g, self.g + self.c.n[d]) if (d < 2 or self.c.dims == 3) else slice(None)
                    for d in range(3))
         it = sim["start"] + 1
@@ -302,10 +307,18 @@ class Oracle:
                 fields = list(un[:4]) + [p_last] + [post[1], post[0], post[2]]
                 film[sample] = (time, [f[sl].copy() for f in fields])
                 sample += 1
+            if probes:
+                row = it - (sim["start"] + 1)
+                hist_t[row] = time
+                for q, n in enumerate(probes):
+                    idx = np.unravel_index(n, self.shape)
+                    hist[:, q, row] = (p_last[idx], un[1][idx], un[2][idx], un[3][idx])
             it += 1
         rep.update(iteration=last_run, time=time, tau=tau, samples=sample)
         if nsamp:
             rep["film"] = film[:sample]
+        if probes:
+            rep["history"] = (hist_t, hist)
         return urk, rep
 
     def rk_update(self, s, tau, met, un, urk, q, qsum):

@@ -472,7 +472,11 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     const long bbase = (long)(b + a.oB) * a.sB;
     const Consts &k = a.k;
 
-    // phase 1: per-node flux, conserved vector, spectral radius and eigenvector inputs
+    // phase 1: per-node flux, conserved vector, spectral radius and eigenvector inputs.  Sweeps
+    // of a direction with a wall face take a second copy of the loop (WALL), so the others keep
+    // the plain node flux.
+    auto node_pass = [&](auto wall_c) {
+    constexpr bool WALL = decltype(wall_c)::value;
     for (int idx = t; idx < NW; idx += NT) {
         int aa, dd;
         if (a.coalesce_d) { aa = idx / WD; dd = idx - aa * WD; }
@@ -493,10 +497,12 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         // velocities of the convective momentum terms: times the wall corrector (+-1, exact) on
         // the ghost nodes of a wall face of this direction, the velocities themselves elsewhere
         double uc[3] = {u, v, w};
-        if ((a.w_lo && dp < a.oD) || (a.w_hi && dp >= a.oD + a.nD)) {
-            const bool lo = dp < a.oD;
+        if constexpr (WALL) {
+            if ((a.w_lo && dp < a.oD) || (a.w_hi && dp >= a.oD + a.nD)) {
+                const bool lo = dp < a.oD;
 #pragma unroll
-            for (int q = 0; q < 3; q++) uc[q] *= lo ? a.c_lo[q] : a.c_hi[q];
+                for (int q = 0; q < 3; q++) uc[q] *= lo ? a.c_lo[q] : a.c_hi[q];
+            }
         }
         if constexpr (AX < 0) {
             const double mx = a.m0[n], my = a.m1[n], mz = a.m2[n];
@@ -536,6 +542,9 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         sm[V_C * NW + o] = cs;
         sm[VJ * NW + o] = J;
     }
+    };
+    if (a.w_lo || a.w_hi) node_pass(std::true_type{});
+    else node_pass(std::false_type{});
     __syncthreads();
     if constexpr (MIX) {
         // node-to-node differences of F and U, rounded once to FP32 (position o: node o+1 - node o)
@@ -1777,6 +1786,20 @@ __global__ void k_film(CFields5 pr, gcdp P, CFields5 post, gdp film, long nint, 
     film[base + 7 * nint] = post.f[2][n];
 }
 
+// probe histories (src/Functions.c:5381-5417): stored p and u, v, w of each probe node into row
+// `row` of the history (field-major: p, u, v, w, each [nprobes][iters])
+__global__ void k_probe(CFields5 pr, gcdp P, const long *idx, int np, gdp hist, int row, int iters)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const long n = idx[t];
+    const long base = (long)t * iters + row, fs = (long)np * iters;
+    hist[base] = P[n];
+    hist[base + fs] = pr.f[1][n];
+    hist[base + 2 * fs] = pr.f[2][n];
+    hist[base + 3 * fs] = pr.f[3][n];
+}
+
 // check_Metric (src/Functions.c:6198-6243): count the padded nodes with a negative Jacobian.
 __global__ void k_check_metric(gcdp J, Geom gm, int *count)
 {
@@ -1865,6 +1888,10 @@ struct shx_case {
     double *post[3];       // Mach, gradRho, Lambda2 (padded), allocated on use
     double *film;          // device-resident samples: 8 interior fields each
     int last_in;           // buffer of the last stage's input of the last step (-1: none)
+    long *d_probes;        // probe histories of shx_simulate
+    double *hist;
+    int hist_np, hist_iters;
+    std::vector<double> hist_time;
     int film_cap, film_n;
     std::vector<double> film_time;
     double *PL, *RL, *UL, *C0;   // Rudy faces: step-start ghost p, rho, u; uploaded sound speed
@@ -2089,6 +2116,8 @@ extern "C" int shx_destroy(shx_case *c)
     }
     for (int m = 0; m < 3; m++) hipFree(c->post[m]);
     hipFree(c->film);
+    hipFree(c->d_probes);
+    hipFree(c->hist);
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -3491,6 +3520,20 @@ extern "C" int shx_film_download(shx_case *c, int sample, double *const out[8], 
     return SHX_OK;
 }
 
+extern "C" int shx_history_download(shx_case *c, double *time, double *p, double *u, double *v, double *w)
+{
+    if (!c) return set_err(SHX_ERR_ARG, "null argument");
+    if (!c->hist) return set_err(SHX_ERR_STATE, "no probe history: run shx_simulate with probes");
+    const size_t fs = sizeof(double) * (size_t)c->hist_np * c->hist_iters;
+    double *out[4] = {p, u, v, w};
+    for (int m = 0; m < 4; m++)
+        if (out[m]) HIPCHK(hipMemcpyAsync(out[m], c->hist + (size_t)m * c->hist_np * c->hist_iters, fs,
+                                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (time) memcpy(time, c->hist_time.data(), sizeof(double) * c->hist_iters);
+    return SHX_OK;
+}
+
 extern "C" int shx_film_count(const shx_case *c, int *count)
 {
     if (!c || !count) return set_err(SHX_ERR_ARG, "null argument");
@@ -3535,8 +3578,35 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
     const bool sampling = sim->samples > 0;
     if (sampling && (sim->iterations_between_samples < 1 || !c->film || c->film_cap < sim->samples))
         return set_err(SHX_ERR_ARG, "sampling needs iterations_between_samples >= 1 and a film of that many samples");
+    if (sim->nprobes < 0 || (sim->nprobes > 0 && !sim->probes)) return set_err(SHX_ERR_ARG, "bad probe list");
+    const Geom &gm = c->gm;
+    // probes are the reference's linear indices (i ng1 ng2 + j ng2 + k); the device arrays use
+    // padded row pitches
+    std::vector<long> dev_probes(sim->nprobes);
+    for (int q = 0; q < sim->nprobes; q++) {
+        const long n = sim->probes[q];
+        if (n < 0 || n >= gm.ntot) return set_err(SHX_ERR_ARG, "probe %d outside the grid", q);
+        const int ijk[3] = {(int)(n / ((long)gm.ng[1] * gm.ng[2])), (int)((n / gm.ng[2]) % gm.ng[1]), (int)(n % gm.ng[2])};
+        for (int d = 0; d < gm.dims; d++)
+            if (ijk[d] < gm.lo[d] || ijk[d] >= gm.lo[d] + gm.n[d]) return set_err(SHX_ERR_ARG, "probe %d is not an interior node", q);
+        dev_probes[q] = (long)ijk[0] * gm.st[0] + (long)ijk[1] * gm.st[1] + (long)ijk[2] * gm.st[2];
+    }
     memset(rep, 0, sizeof *rep);
-    if (sampling && (rc = keep_pressure(c))) return rc;
+    if ((sampling || sim->nprobes > 0) && (rc = keep_pressure(c))) return rc;
+    const int iters = sim->end_iteration - sim->start_iteration;
+    if (sim->nprobes > 0) {
+        hipFree(c->d_probes); hipFree(c->hist);
+        c->d_probes = nullptr; c->hist = nullptr;
+        HIPCHK(hipMalloc((void **)&c->d_probes, sizeof(long) * sim->nprobes));
+        HIPCHK(hipMemcpy(c->d_probes, dev_probes.data(), sizeof(long) * sim->nprobes, hipMemcpyHostToDevice));
+        if (iters > 0) {
+            HIPCHK(hipMalloc((void **)&c->hist, sizeof(double) * 4 * (size_t)sim->nprobes * iters));
+            HIPCHK(hipMemsetAsync(c->hist, 0, sizeof(double) * 4 * (size_t)sim->nprobes * iters, c->stream));
+        }
+        c->hist_np = sim->nprobes;
+        c->hist_iters = iters;
+        c->hist_time.assign(iters, 0.0);
+    }
     const double tau_start = sim->tau, L0 = sim->L0, u0 = sim->u0;
     const int end = sim->end_iteration, dback = sim->distance_backward, dnan = sim->distance_nan;
     double tau = tau_start, time = sim->time, last = sim->time_last_action;
@@ -3594,6 +3664,14 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
             c->film_n = sample;
             if ((rc = film_sample(c, time))) return rc;
             sample++;
+        }
+        if (sim->nprobes > 0) {
+            const int row = it - (sim->start_iteration + 1);
+            CFields5 cp;
+            for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+            k_probe<<<nblk(sim->nprobes, 64), 64, 0, c->stream>>>(cp, c->P, c->d_probes, sim->nprobes, c->hist, row, iters);
+            LAUNCH_CHECK();
+            c->hist_time[row] = time;
         }
         if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
     }

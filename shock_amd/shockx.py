@@ -52,6 +52,7 @@ class Sim(ctypes.Structure):
         ("decel_factor", ctypes.c_double), ("number_resets", ctypes.c_int), ("distance_nan", ctypes.c_int),
         ("distance_forward", ctypes.c_int), ("distance_backward", ctypes.c_int), ("samples", ctypes.c_int),
         ("iterations_between_samples", ctypes.c_int), ("start_sampling", ctypes.c_int),
+        ("probes", ctypes.c_void_p), ("nprobes", ctypes.c_int),
     ]
 
 
@@ -71,7 +72,7 @@ EXPORTS = [
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
     "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
-    "shx_film_sample", "shx_film_count", "shx_film_download",
+    "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -102,6 +103,7 @@ def lib():
         L.shx_film_sample.argtypes = [P, ctypes.c_double]
         L.shx_film_count.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_film_download.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
+        L.shx_history_download.argtypes = [P, P, P, P, P, P]
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_check_metric.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -237,13 +239,14 @@ class Case:
 
     def simulate(self, start, end, tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=False,
                  accel_factor=1.1, decel_factor=0.7, number_resets=10, distance_nan=5000, distance_forward=1000,
-                 distance_backward=200, samples=0, iterations_between_samples=1, start_sampling=0):
+                 distance_backward=200, samples=0, iterations_between_samples=1, start_sampling=0, probes=()):
         """Iterations start+1 .. end with the reference's NaN rollback and time-step control
         (defaults: the reference's tau:* defaults, src/Import.c:53-58), sampling into the film when
         samples > 0 (film_open first).  Returns the report as a dict."""
+        pr = np.ascontiguousarray(probes, dtype=np.int64)
         sim = Sim(start, end, tau, time, time_last_action, L0, u0, int(accelerator), accel_factor, decel_factor,
                   number_resets, distance_nan, distance_forward, distance_backward, samples,
-                  iterations_between_samples, start_sampling)
+                  iterations_between_samples, start_sampling, pr.ctypes.data if pr.size else None, int(pr.size))
         rep = SimReport()
         _check(lib().shx_simulate(self.h, ctypes.byref(sim), ctypes.byref(rep)))
         return {f: getattr(rep, f) for f, _ in SimReport._fields_}
@@ -253,6 +256,13 @@ class Case:
         out, p = self._outs(3)
         _check(lib().shx_post(self.h, p))
         return out
+
+    def history(self, nprobes, iterations):
+        """Probe histories of the last simulate(): time[iterations], and p, u, v, w [nprobes, iterations]."""
+        t = np.zeros(iterations)
+        f = [np.zeros((nprobes, iterations)) for _ in range(4)]
+        _check(lib().shx_history_download(self.h, t.ctypes.data, *[a.ctypes.data for a in f]))
+        return t, f
 
     def film_open(self, samples):
         _check(lib().shx_film_open(self.h, samples))

@@ -1,7 +1,7 @@
 """Post-processing on the device against the oracle: shx_post (CalcValuesForPost with
-CalcLambda2, src/Functions.c:3687-3761, 5846-6070) and the device-resident film filled by
-shx_simulate's sampling (WriteValuesFromUToFilm, src/Functions.c:2510-2553), including samples
-rewound by a NaN rollback."""
+CalcLambda2, src/Functions.c:3687-3761, 5846-6070), the device-resident film filled by
+shx_simulate's sampling (WriteValuesFromUToFilm, src/Functions.c:2510-2553) and the probe
+histories (src/Functions.c:5381-5417), including samples and rows rewound by a NaN rollback."""
 import numpy as np
 import pytest
 
@@ -62,7 +62,9 @@ def test_film_of_a_3d_run():
 def test_film_rewound_by_a_rollback():
     # an unstable tau: samples taken before a NaN are rewritten after the rollback
     o, coords, met, prim, nn = setup_2d_mms(n=32, S=5)
-    s = _sim(6.0 * o.cfl_dt(0.5, met, prim), samples=6, iterations_between_samples=3, start_sampling=0)
+    probes = [int(np.ravel_multi_index((o.g + i, o.g + j, 0), o.shape)) for i, j in ((0, 0), (13, 21), (31, 5))]
+    s = _sim(6.0 * o.cfl_dt(0.5, met, prim), samples=6, iterations_between_samples=3, start_sampling=0,
+             probes=probes)
     want_state, want = o.simulate(met, prim, s)
     assert want["resets"] >= 1
     c = o.c
@@ -73,7 +75,12 @@ def test_film_rewound_by_a_rollback():
         g.film_open(6)
         rep = g.simulate(s["start"], s["end"], s["tau"], accelerator=True, accel_factor=1.1, decel_factor=0.5,
                          number_resets=10, distance_nan=12, distance_forward=5, distance_backward=4, samples=6,
-                         iterations_between_samples=3, start_sampling=0)
+                         iterations_between_samples=3, start_sampling=0, probes=s["probes"])
+        t_got, h_got = g.history(len(s["probes"]), s["end"] - s["start"])
+        t_want, h_want = want["history"]
+        assert np.array_equal(t_got, t_want)
+        for m, name in enumerate(("p", "u", "v", "w")):
+            _close(h_got[m], h_want[m], 1e-10, f"probe history {name}")
         print(f"[post] rollback run: oracle resets {want['resets']} samples {want['samples']}, "
               f"GPU resets {rep['resets']} samples {rep['samples']}", flush=True)
         assert rep["resets"] == want["resets"] and rep["samples"] == want["samples"]

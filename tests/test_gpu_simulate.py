@@ -32,7 +32,7 @@ def _compare(o, nn, dims, want_state, want, got_state, got, tag):
 def _sim(tau, **kw):
     base = dict(start=0, end=40, tau=tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=1,
                 accel_factor=1.1, decel_factor=0.5, number_resets=10, distance_nan=12, distance_forward=5,
-                distance_backward=4, samples=0, iterations_between_samples=1, start_sampling=0)
+                distance_backward=4, samples=0, iterations_between_samples=1, start_sampling=0, probes=())
     base.update(kw)
     return base
 

@@ -10,6 +10,7 @@ rollback and the time-step control, against what the loop's rules fix in closed 
     NaN, with tau multiplied by the decelerator once per rollback
   * an unstable tau recovers: rollbacks halve tau until the run is stable, and the state that
     the loop ends with is finite
+  * probe histories hold each iteration's state at the probe nodes under iteration - (start + 1)
 """
 import math
 
@@ -22,7 +23,7 @@ from helpers import interior, setup_2d_mms
 def _sim(**kw):
     base = dict(start=0, end=12, tau=1.0, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=0,
                 accel_factor=1.1, decel_factor=0.7, number_resets=10, distance_nan=5000, distance_forward=1000,
-                distance_backward=200)
+                distance_backward=200, probes=())
     base.update(kw)
     return base
 
@@ -84,3 +85,20 @@ def test_unstable_tau_recovers_by_rollback():
     assert rep["aborted"] == 0 and rep["resets"] >= 1 and rep["nan_steps"] == rep["resets"]
     assert rep["iteration"] == 40
     assert all(np.all(np.isfinite(a)) for a in st)
+
+
+def test_probe_history_records_each_iteration():
+    # history row k holds u, v, w of iteration k + 1's state (plain stepping here) and its time
+    o, coords, met, prim, nn = setup_2d_mms(n=16, S=5)
+    tau = o.cfl_dt(0.5, met, prim)
+    probes = [int(np.ravel_multi_index((o.g + 3, o.g + 7, 0), o.shape)),
+              int(np.ravel_multi_index((o.g + 11, o.g + 2, 0), o.shape))]
+    st, rep = o.simulate(met, prim, _sim(end=5, tau=tau, probes=probes))
+    t, hist = rep["history"]
+    for k in range(5):
+        ref = o.run(tau, met, prim, k + 1)
+        for q, n in enumerate(probes):
+            idx = np.unravel_index(n, o.shape)
+            for m in range(3):
+                assert hist[1 + m, q, k] == ref[1 + m][idx]
+    assert np.all(np.diff(t) > 0) and t[-1] == rep["time"]
