@@ -32,6 +32,28 @@ def build_driver(verbose=False):
     return DRIVER
 
 
+COMPAT_DIR = os.path.join(HERE, "compat")
+COMPAT_LIB = os.path.join(OUT_DIR, "libshock_compat.so")
+COMPAT_RUN = os.path.join(COMPAT_DIR, "compat_run")
+
+
+def build_compat(verbose=False):
+    """The reference-named entry points (shock_compat.h) over libshockx.so, and their step-loop
+    driver (plain gcc)."""
+    cmds = [
+        ["gcc", "-O2", "-Wall", "-fPIC", "-shared", "-I", INCLUDE, "-I", COMPAT_DIR,
+         os.path.join(COMPAT_DIR, "shock_compat.c"), "-L", OUT_DIR, "-lshockx", "-Wl,-rpath,$ORIGIN",
+         "-o", COMPAT_LIB],
+        ["gcc", "-O2", "-Wall", "-I", COMPAT_DIR, os.path.join(COMPAT_DIR, "compat_run.c"), "-L", OUT_DIR,
+         "-lshock_compat", "-lshockx", "-Wl,-rpath,$ORIGIN/../lib", "-o", COMPAT_RUN],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return COMPAT_LIB
+
+
 def build(verbose=False, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -44,6 +66,7 @@ def build(verbose=False, extra=()):
         raise RuntimeError("hipcc failed building libshockx.so")
     os.replace(OUT + ".tmp", OUT)
     build_driver(verbose)
+    build_compat(verbose)
     return OUT
 
 

@@ -1,0 +1,189 @@
+/*
+ * shock_compat.c -- the reference's RHS / time-step entry points over libshockx.so (see
+ * shock_compat.h).  The device case is created by the first CalcRungeKutta of a configuration:
+ * face kinds from the reference's BC strings and option flags (SetAllBoundaryConditions,
+ * src/BC.c:75-637), the metric from the mesh arrays.  The state is uploaded when U_lastStep
+ * differs from what the previous call wrote back (a shadow copy is kept), so the usual loop
+ * -- CalcRungeKutta, checkNAN, WriteValuesFromU1ToU2 -- moves each state over PCIe once per step
+ * and the Rudy writer keeps the uploaded sound speed as the reference keeps U_lastStep's.
+ */
+#include "shock_compat.h"
+#include "shockx.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct compat_ctx {
+    shx_case *c;
+    long ntot;
+    double *shadow[5];   /* rho, u, v, w, e as last written to the host */
+    int have_shadow;
+};
+
+static void compat_abort(const char *what, int rc)
+{
+    printf("SHOCK: GPU path: %s failed (%d): %s\n", what, rc, shx_last_error());
+    exit(4242);
+}
+#define CALL(x)                          \
+    do {                                 \
+        int rc_ = (x);                   \
+        if (rc_ != SHX_OK) compat_abort(#x, rc_); \
+    } while (0)
+
+void compat_define(struct strct_configuration *cfg)
+{
+    const int g = (cfg->SPACEORDER + 1) / 2;
+    cfg->int_iMeshPointsGhostCells = cfg->int_iMeshPoints + cfg->SPACEORDER + 1;
+    cfg->int_jMeshPointsGhostCells = cfg->int_jMeshPoints + cfg->SPACEORDER + 1;
+    cfg->int_iStartReal = g; cfg->int_iEndReal = g + cfg->int_iMeshPoints - 1;
+    cfg->int_jStartReal = g; cfg->int_jEndReal = g + cfg->int_jMeshPoints - 1;
+    if (cfg->MESHDIMENSIONS == 3) {
+        cfg->int_kMeshPointsGhostCells = cfg->int_kMeshPoints + cfg->SPACEORDER + 1;
+        cfg->int_kStartReal = g; cfg->int_kEndReal = g + cfg->int_kMeshPoints - 1;
+    } else {
+        cfg->int_kMeshPoints = 1;
+        cfg->int_kMeshPointsGhostCells = 1;
+        cfg->int_kStartReal = 0; cfg->int_kEndReal = 0;
+    }
+    cfg->compat_quirks = SHX_QUIRKS_REFERENCE;
+}
+
+/* face kind of one face (SetAllBoundaryConditions' selection, src/BC.c:75-637) */
+static int face_kind(const struct strct_configuration *cfg, const char *bc, int neighbour)
+{
+    const int inflow = !strcmp(bc, cfg->BCInflow) || !strcmp(bc, cfg->BCInflowSubsonic) ||
+                       !strcmp(bc, cfg->BCInflowSupersonic);
+    const int outflow = !strcmp(bc, cfg->BCOutflow) || !strcmp(bc, cfg->BCOutflowSubsonic);
+    if (neighbour != NO_NEIGHBOUR) return SHX_BC_PERIODIC;
+    if (!strcmp(bc, cfg->BCManufacturedSolution)) return SHX_BC_MANUFACTURED;
+    if (!strcmp(bc, cfg->BCWallInviscid)) return SHX_BC_WALL_SLIP;
+    if (!strcmp(bc, cfg->BCWallViscous)) return SHX_BC_WALL_NOSLIP;
+    if (!strcmp(bc, cfg->BCWallViscousIsothermal)) return SHX_BC_WALL_ISOTHERMAL;
+    if (!strcmp(bc, cfg->BCFarfield)) return SHX_BC_FARFIELD;
+    if (inflow && cfg->flag_BC_option_inflow_normal_super) return SHX_BC_INFLOW_SUPERSONIC;
+    if (inflow && cfg->flag_BC_option_inflow_normal_sub) return SHX_BC_INFLOW_SUB_NORMAL;
+    if (inflow && cfg->flag_BC_option_inflow_riemann_sub) return SHX_BC_INFLOW_SUB_RIEMANN;
+    if (inflow && cfg->flag_BC_option_inflow_isentrop_sub) return SHX_BC_INFLOW_SUB_ISENTROPIC;
+    if (outflow && cfg->flag_BC_option_outflow_riemann_sub) return SHX_BC_OUTFLOW_RIEMANN;
+    if (outflow && cfg->flag_BC_option_outflow_normal_sub) return SHX_BC_OUTFLOW_NORMAL;
+    if (outflow && cfg->flag_BC_option_outflow_rudy_sub) return SHX_BC_OUTFLOW_RUDY;
+    printf("SHOCK: GPU path: boundary %s not available\n", bc);
+    exit(4242);
+}
+
+static struct compat_ctx *ctx_of(struct strct_configuration *cfg, struct strct_mesh *mesh)
+{
+    struct compat_ctx *x = (struct compat_ctx *)cfg->compat_case;
+    if (x) return x;
+    if (cfg->flag_ManufacturedSolution) {
+        printf("SHOCK: GPU path: the manufactured solution needs shx_upload_source / shx_upload_boundary_state\n");
+        exit(4242);
+    }
+    shx_desc d;
+    memset(&d, 0, sizeof d);
+    d.dims = cfg->MESHDIMENSIONS; d.space_order = cfg->SPACEORDER; d.time_order = cfg->int_TimeOrder;
+    d.n[0] = cfg->int_iMeshPoints; d.n[1] = cfg->int_jMeshPoints; d.n[2] = cfg->int_kMeshPoints;
+    d.gamma = cfg->gammaNumber; d.mach = cfg->machNumber; d.reynolds = cfg->reynoldsNumber;
+    d.prandtl = cfg->prandtlNumber; d.T0 = cfg->T0_dim;
+    d.inviscid = cfg->flag_Inviscid; d.manufactured = 0; d.quirks = cfg->compat_quirks;
+    const char *bcs[6] = {cfg->BC_Left, cfg->BC_Right, cfg->BC_Bottom, cfg->BC_Top, cfg->BC_Behind, cfg->BC_InFront};
+    const int nbr[6] = {cfg->InterfaceNeighbourLeft, cfg->InterfaceNeighbourRight, cfg->InterfaceNeighbourBottom,
+                        cfg->InterfaceNeighbourTop, cfg->InterfaceNeighbourBehind, cfg->InterfaceNeighbourInFront};
+    for (int f = 0; f < 6; f++)
+        d.bc[f] = (f < 2 * cfg->MESHDIMENSIONS) ? face_kind(cfg, bcs[f], nbr[f]) : SHX_BC_PERIODIC;
+    d.precision = SHX_FP64;
+    d.rho_inflow = cfg->rho_inflow; d.u_inflow = cfg->u_inflow; d.v_inflow = cfg->v_inflow;
+    d.w_inflow = cfg->w_inflow; d.p_inflow = cfg->p_inflow;
+    d.p_out = cfg->p_out; d.alpha_nonref = cfg->AlphaNonRef; d.T_wall = cfg->T_wall;
+    x = (struct compat_ctx *)calloc(1, sizeof *x);
+    CALL(shx_create(&d, &x->c));
+    const double *met[10] = {mesh->xi_x, mesh->xi_y, mesh->xi_z, mesh->eta_x, mesh->eta_y,
+                             mesh->eta_z, mesh->zeta_x, mesh->zeta_y, mesh->zeta_z, mesh->jacobian};
+    CALL(shx_upload_metric(x->c, met));
+    x->ntot = (long)cfg->int_iMeshPointsGhostCells * cfg->int_jMeshPointsGhostCells * cfg->int_kMeshPointsGhostCells;
+    for (int m = 0; m < 5; m++) x->shadow[m] = (double *)malloc(sizeof(double) * x->ntot);
+    cfg->compat_case = x;
+    return x;
+}
+
+void compat_release(struct strct_configuration *cfg)
+{
+    struct compat_ctx *x = (struct compat_ctx *)cfg->compat_case;
+    if (!x) return;
+    shx_destroy(x->c);
+    for (int m = 0; m < 5; m++) free(x->shadow[m]);
+    free(x);
+    cfg->compat_case = NULL;
+}
+
+static void fields(struct strct_U *U, double *f[5])
+{
+    f[0] = U->rho; f[1] = U->u; f[2] = U->v; f[3] = U->w; f[4] = U->e;
+}
+
+/* the device holds U unless U differs from what was last written back to the host */
+static void ensure_on_device(struct compat_ctx *x, struct strct_U *U)
+{
+    double *f[5];
+    fields(U, f);
+    int same = x->have_shadow;
+    for (int m = 0; m < 5 && same; m++) same = !memcmp(f[m], x->shadow[m], sizeof(double) * x->ntot);
+    if (same) return;
+    const double *cf[5] = {f[0], f[1], f[2], f[3], f[4]};
+    CALL(shx_upload_state(x->c, cf));
+    for (int m = 0; m < 5; m++) memcpy(x->shadow[m], f[m], sizeof(double) * x->ntot);
+    x->have_shadow = 1;
+}
+
+static void write_back(struct compat_ctx *x, struct strct_U *U)
+{
+    double *f[5];
+    fields(U, f);
+    CALL(shx_download_state(x->c, f));
+    for (int m = 0; m < 5; m++) memcpy(x->shadow[m], f[m], sizeof(double) * x->ntot);
+    x->have_shadow = 1;
+}
+
+void CalcRungeKutta(struct strct_configuration *pnt_config, struct strct_mesh *pnt_mesh,
+                    struct strct_U *pnt_U_lastStep, struct strct_U *pnt_U_RK, struct strct_Flux *pnt_Q,
+                    struct strct_Flux *pnt_Q_sum, struct strct_Flux *pnt_Flux,
+                    struct strct_Flux *pnt_Flux_PlusHalf)
+{
+    (void)pnt_Q; (void)pnt_Q_sum; (void)pnt_Flux; (void)pnt_Flux_PlusHalf;
+    struct compat_ctx *x = ctx_of(pnt_config, pnt_mesh);
+    ensure_on_device(x, pnt_U_lastStep);
+    CALL(shx_calc_runge_kutta(x->c, pnt_config->numericalTau));
+    write_back(x, pnt_U_RK);
+}
+
+int checkNAN(struct strct_configuration *pnt_config, struct strct_U *pnt_U, struct strct_mesh *pnt_mesh)
+{
+    struct compat_ctx *x = ctx_of(pnt_config, pnt_mesh);
+    ensure_on_device(x, pnt_U);
+    int count = 0;
+    CALL(shx_check_nan(x->c, &count));
+    if (count > 0) {
+        /* the marks checkNAN leaves in the state (src/Functions.c:5443-5445) */
+        write_back(x, pnt_U);
+        for (long n = 0; n < x->ntot; n++)
+            if (pnt_U->rho[n] == 1000000 && pnt_U->p) pnt_U->p[n] = 1000000;
+        pnt_config->flag_NAN = 1;
+        return 1;
+    }
+    return 0;
+}
+
+void WriteValuesFromU1ToU2(struct strct_configuration *pnt_config, struct strct_U *pnt_U1, struct strct_U *pnt_U2)
+{
+    const long ntot = (long)pnt_config->int_iMeshPointsGhostCells * pnt_config->int_jMeshPointsGhostCells *
+                      pnt_config->int_kMeshPointsGhostCells;
+    const size_t bytes = sizeof(double) * (size_t)ntot;
+    memcpy(pnt_U2->rho, pnt_U1->rho, bytes);
+    memcpy(pnt_U2->u, pnt_U1->u, bytes);
+    memcpy(pnt_U2->v, pnt_U1->v, bytes);
+    memcpy(pnt_U2->w, pnt_U1->w, bytes);
+    memcpy(pnt_U2->e, pnt_U1->e, bytes);
+    if (pnt_U1->p && pnt_U2->p) memcpy(pnt_U2->p, pnt_U1->p, bytes);
+}

@@ -1,0 +1,81 @@
+/*
+ * shock_compat.h -- the reference's entry points of the RHS / time-step path under their own
+ * names, argument order and meaning, over restated structs that hold only the fields the path
+ * reads (field names as in src/SHOCK.h).  The work runs in libshockx.so on the GPU: every
+ * CalcRungeKutta call uploads U_lastStep, takes one step on the device and writes U_RK back, so
+ * a caller written against the reference's calls runs unchanged (at PCIe cost per step; callers
+ * that keep the state on the device use shockx.h directly).
+ *
+ *   CalcRungeKutta         src/Functions.h:206-214 (called at src/SHOCK.c:565)
+ *   checkNAN               src/Functions.h:361-364
+ *   WriteValuesFromU1ToU2  src/Functions.h:216-219
+ *
+ * The reference's compile-time MESHDIMENSIONS and SPACEORDER are fields of the configuration
+ * here; compat_define() fills the ghost-padded extents DefineParameters derives
+ * (src/Functions.c:56-103).  Errors in the configuration print a message and end the process
+ * with status 4242, where the reference calls MPI_Abort.
+ */
+#ifndef SHOCK_COMPAT_H
+#define SHOCK_COMPAT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NO_NEIGHBOUR -1
+typedef double FLT;
+
+struct strct_configuration {
+    int MESHDIMENSIONS;                       /* 2 or 3 */
+    int SPACEORDER;                           /* 5 or 9 */
+    int int_iMeshPoints, int_jMeshPoints, int_kMeshPoints;
+    int int_iMeshPointsGhostCells, int_jMeshPointsGhostCells, int_kMeshPointsGhostCells;  /* compat_define */
+    int int_iStartReal, int_iEndReal, int_jStartReal, int_jEndReal, int_kStartReal, int_kEndReal;
+    int int_TimeOrder;
+    FLT numericalTau;
+    FLT machNumber, reynoldsNumber, prandtlNumber, gammaNumber, T0_dim;
+    int flag_Inviscid, flag_ManufacturedSolution, flag_NAN;
+    char BC_Left[30], BC_Right[30], BC_Top[30], BC_Bottom[30], BC_Behind[30], BC_InFront[30];
+    char BCFarfield[30], BCInflow[30], BCOutflow[30], BCOutflowSubsonic[30], BCWallInviscid[30];
+    char BCWallViscous[30], BCInflowSupersonic[30], BCInflowSubsonic[30], BCWallViscousIsothermal[30];
+    char BCManufacturedSolution[30];
+    int flag_BC_option_inflow_normal_sub, flag_BC_option_inflow_riemann_sub;
+    int flag_BC_option_inflow_isentrop_sub, flag_BC_option_inflow_normal_super;
+    int flag_BC_option_outflow_normal_sub, flag_BC_option_outflow_riemann_sub, flag_BC_option_outflow_rudy_sub;
+    FLT rho_inflow, p_inflow, u_inflow, v_inflow, w_inflow;
+    FLT p_out, AlphaNonRef, T_wall;
+    int InterfaceNeighbourLeft, InterfaceNeighbourRight, InterfaceNeighbourBottom;
+    int InterfaceNeighbourTop, InterfaceNeighbourBehind, InterfaceNeighbourInFront;
+    unsigned compat_quirks;                   /* SHX_QUIRK_* bits; compat_define sets the reference's */
+    void *compat_case;                        /* the device case, created by the first step */
+};
+
+/* metric on the padded grid, already mirrored at walls as the reference leaves it */
+struct strct_mesh {
+    FLT *xi_x, *xi_y, *xi_z, *eta_x, *eta_y, *eta_z, *zeta_x, *zeta_y, *zeta_z, *jacobian;
+};
+
+/* state on the padded grid: rho, u, v, w, e are the path's; the derived fields are not written */
+struct strct_U {
+    FLT *rho, *u, *v, *w, *p, *e, *c, *T, *mue, *theta1, *theta2, *theta3;
+};
+
+/* the reference's flux scratch; accepted and unused (the device keeps its own) */
+struct strct_Flux {
+    FLT *Mass, *xiMomentum, *etaMomentum, *zetaMomentum, *Energy;
+};
+
+void compat_define(struct strct_configuration *cfg);
+void compat_release(struct strct_configuration *cfg);
+
+void CalcRungeKutta(struct strct_configuration *pnt_config, struct strct_mesh *pnt_mesh,
+                    struct strct_U *pnt_U_lastStep, struct strct_U *pnt_U_RK, struct strct_Flux *pnt_Q,
+                    struct strct_Flux *pnt_Q_sum, struct strct_Flux *pnt_Flux,
+                    struct strct_Flux *pnt_Flux_PlusHalf);
+int checkNAN(struct strct_configuration *pnt_config, struct strct_U *pnt_U, struct strct_mesh *pnt_mesh);
+void WriteValuesFromU1ToU2(struct strct_configuration *pnt_config, struct strct_U *pnt_U1, struct strct_U *pnt_U2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

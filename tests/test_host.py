@@ -33,6 +33,29 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, s)
 
 
+COMPAT_NAMES = ["CalcRungeKutta", "checkNAN", "WriteValuesFromU1ToU2", "compat_define", "compat_release"]
+
+
+def test_compat_header_declares_the_reference_names():
+    txt = open(os.path.join(ROOT, "shock_amd", "compat", "shock_compat.h")).read()
+    for name in COMPAT_NAMES:
+        assert re.search(r"\b" + name + r"\s*\(", txt), name
+    # the reference's argument order of CalcRungeKutta (src/Functions.h:206-214)
+    sig = re.search(r"void CalcRungeKutta\(([^;]*)\);", txt).group(1)
+    kinds = re.findall(r"struct (strct_[a-zA-Z]+)", sig)
+    assert kinds == ["strct_configuration", "strct_mesh", "strct_U", "strct_U", "strct_Flux", "strct_Flux",
+                     "strct_Flux", "strct_Flux"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "shock_amd", "lib", "libshock_compat.so")),
+                    reason="libshock_compat.so not built")
+def test_compat_library_exports_the_reference_names():
+    path = os.path.join(ROOT, "shock_amd", "lib", "libshock_compat.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (\w+)", out))
+    assert all(n in exported for n in COMPAT_NAMES), exported
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(shockx, "LIB_PATH", str(tmp_path / "nope.so"))
     monkeypatch.setattr(shockx, "_lib", None)
