@@ -215,6 +215,12 @@ int shx_film_download(shx_case *c, int sample, double *const out[8], double *tim
 /* probe histories of the last shx_simulate: time[iterations], then p, u, v, w each
  * [nprobes][iterations] (iterations = end - start); NULL entries skipped */
 int shx_history_download(shx_case *c, double *time, double *p, double *u, double *v, double *w);
+/* ErrorManufacturedSolution (src/ManufacturedSolution.c:745-790) of the current state against the
+ * exact state of shx_upload_boundary_state: out = L2(rho), Linf(rho), L2(p), Linf(p), as the
+ * reference forms them -- over the interior nodes of the last k plane only, each squared deviation
+ * divided by the full interior node count, p the stored pressure (as in shx_post).  FP64 sums where
+ * the reference accumulates in long double, against exact values rounded to FP64. */
+int shx_mms_error(shx_case *c, double out[4]);
 
 /* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
  * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux

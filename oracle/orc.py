@@ -216,6 +216,7 @@ class Oracle:
             for a, b in zip(un, urk):
                 a[...] = b
             p_last[...] = scratch[0]
+        self.last_p = p_last          # U_lastStep's stored p after the last step
         return urk
 
     def simulate(self, met, prim0, sim, src=None, bstate=None):
@@ -340,6 +341,12 @@ class Oracle:
         out = self.zeros(3)
         lib().orc_calc_post(ctypes.byref(self.c), _bundle(met), _bundle(prim), _ptr(p), _bundle(out))
         return out
+
+    def mms_error(self, prim, p, ex_rho, ex_p):
+        """ErrorManufacturedSolution's (L2 rho, Linf rho, L2 p, Linf p)."""
+        out = np.zeros(4)
+        lib().orc_mms_error(ctypes.byref(self.c), _bundle(prim), _ptr(p), _ptr(ex_rho), _ptr(ex_p), _ptr(out))
+        return tuple(out)
 
     def mirror_metric(self, met):
         """Wall and farfield faces: ghost metric copied from the mirror nodes (mirrorGhostCellsMetric), in place."""

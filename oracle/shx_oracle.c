@@ -1137,3 +1137,28 @@ void orc_calc_post(const orc_cfg *c, double *const met[10], double *const prim[5
                 }
             }
 }
+
+/* ErrorManufacturedSolution (src/ManufacturedSolution.c:745-790): long double sums of the squared
+ * deviations of rho and of the stored p from the exact values, each divided by the interior node
+ * count, over the interior nodes of the last k plane (the reference's k loop starts at kEndReal),
+ * and the largest deviations.  ex_rho, ex_p: exact values on the padded grid.
+ * out = L2(rho), Linf(rho), L2(p), Linf(p). */
+void orc_mms_error(const orc_cfg *c, double *const prim[5], const double *p, const double *ex_rho,
+                   const double *ex_p, double out[4])
+{
+    long double N = (long double)c->n[0] * c->n[1] * c->n[2], l2r = 0.0L, lir = 0.0L, l2p = 0.0L, lip = 0.0L;
+    int i, j, k = re(c, 2);
+    for (i = rs(c, 0); i <= re(c, 0); i++)
+        for (j = rs(c, 1); j <= re(c, 1); j++) {
+            long n = orc_idx(c, i, j, k);
+            long double dr = (long double)ex_rho[n] - prim[0][n], dp = (long double)ex_p[n] - p[n];
+            l2r += powl(dr, 2.0L) / N;
+            if (lir < fabsl(dr)) lir = fabsl(dr);
+            l2p += powl(dp, 2.0L) / N;
+            if (lip < fabsl(dp)) lip = fabsl(dp);
+        }
+    out[0] = (double)sqrtl(l2r);
+    out[1] = (double)lir;
+    out[2] = (double)sqrtl(l2p);
+    out[3] = (double)lip;
+}

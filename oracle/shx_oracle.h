@@ -33,6 +33,7 @@
  *   orc_calc_post         Mach closed form; gradRho exact on an anisotropic affine grid for a
  *                         linear density; lambda2 against numpy's eigenvalues of S^2 + Omega^2
  *                         for a linear velocity field on a sheared affine grid
+ *   orc_mms_error         brute force in numpy over the same plane and node count
  *   isothermal wall       no-slip ghost with T = T_wall and the Prandtl-0.72 factor; the energy
  *                         term's mean factor checked against a hand-set factor array
  *   subsonic inflow writers  normal: inflow state; Riemann: the far field's u + 2c/(gamma-1) and
@@ -153,6 +154,11 @@ void orc_mirror_metric(const orc_cfg *c, double *const met[10]);
  * p is the stored pressure of U_lastStep; out = Mach, gradRho, Lambda2 (3D only). */
 void orc_calc_post(const orc_cfg *c, double *const met[10], double *const prim[5], const double *p,
                    double *const out[3]);
+
+/* ErrorManufacturedSolution's norms (src/ManufacturedSolution.c:745-790): out = L2(rho),
+ * Linf(rho), L2(p), Linf(p) over the last k plane, p the stored pressure. */
+void orc_mms_error(const orc_cfg *c, double *const prim[5], const double *p, const double *ex_rho,
+                   const double *ex_p, double out[4]);
 
 void orc_zero(const orc_cfg *c, double *const f[], int nf);
 void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);

@@ -1800,6 +1800,58 @@ __global__ void k_probe(CFields5 pr, gcdp P, const long *idx, int np, gdp hist, 
     hist[base + 3 * fs] = pr.f[3][n];
 }
 
+// ErrorManufacturedSolution (src/ManufacturedSolution.c:745-790): squared deviations of rho and of
+// the stored p from the exact values, each divided by the interior node count N, and their largest
+// magnitudes, over the interior nodes of the last k plane (the reference's k loop starts at
+// kEndReal).  Pass 1: per-block partial sums and maxima; pass 2 (one block): the totals.
+__device__ __forceinline__ void block_sum_max(double &s0, double &m0, double &s1, double &m1, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = s0; sh[256 + t] = m0; sh[512 + t] = s1; sh[768 + t] = m1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sh[t] += sh[t + w];
+            sh[256 + t] = fmax(sh[256 + t], sh[256 + t + w]);
+            sh[512 + t] += sh[512 + t + w];
+            sh[768 + t] = fmax(sh[768 + t], sh[768 + t + w]);
+        }
+        __syncthreads();
+    }
+    s0 = sh[0]; m0 = sh[256]; s1 = sh[512]; m1 = sh[768];
+}
+__global__ __launch_bounds__(256) void k_mms_error(CFields5 pr, gcdp P, gcdp ex_rho, gcdp ex_p, Geom gm,
+                                                  double N, gdp part)
+{
+    __shared__ double sh[1024];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nplane = (long)gm.n[0] * gm.n[1];
+    double s0 = 0.0, m0 = 0.0, s1 = 0.0, m1 = 0.0;
+    if (t < nplane) {
+        const int jj = (int)(t % gm.n[1]), ii = (int)(t / gm.n[1]);
+        const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + (gm.lo[2] + gm.n[2] - 1);
+        const double dr = ex_rho[n] - pr.f[0][n], dp = ex_p[n] - P[n];
+        s0 = (dr * dr) / N; m0 = fabs(dr);
+        s1 = (dp * dp) / N; m1 = fabs(dp);
+    }
+    block_sum_max(s0, m0, s1, m1, sh);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x + 0] = s0; part[4 * blockIdx.x + 1] = m0;
+        part[4 * blockIdx.x + 2] = s1; part[4 * blockIdx.x + 3] = m1;
+    }
+}
+__global__ __launch_bounds__(256) void k_mms_error_total(gcdp part, int nb, gdp out)
+{
+    __shared__ double sh[1024];
+    double s0 = 0.0, m0 = 0.0, s1 = 0.0, m1 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        s0 += part[4 * b + 0]; m0 = fmax(m0, part[4 * b + 1]);
+        s1 += part[4 * b + 2]; m1 = fmax(m1, part[4 * b + 3]);
+    }
+    block_sum_max(s0, m0, s1, m1, sh);
+    if (threadIdx.x == 0) { out[0] = sqrt(s0); out[1] = m0; out[2] = sqrt(s1); out[3] = m1; }
+}
+
 // check_Metric (src/Functions.c:6198-6243): count the padded nodes with a negative Jacobian.
 __global__ void k_check_metric(gcdp J, Geom gm, int *count)
 {
@@ -3531,6 +3583,33 @@ extern "C" int shx_history_download(shx_case *c, double *time, double *p, double
                                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (time) memcpy(time, c->hist_time.data(), sizeof(double) * c->hist_iters);
+    return SHX_OK;
+}
+
+extern "C" int shx_mms_error(shx_case *c, double out[4])
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out) return set_err(SHX_ERR_ARG, "null output");
+    if (!c->has_bs) return set_err(SHX_ERR_STATE, "no exact state: call shx_upload_boundary_state");
+    if ((rc = keep_pressure(c))) return rc;
+    const Geom &gm = c->gm;
+    const long nplane = (long)gm.n[0] * gm.n[1];
+    const int nb = (int)nblk(nplane, 256);
+    double *part = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc((void **)&part, sizeof(double) * 4 * (size_t)nb));
+    HIPCHK(hipMalloc((void **)&dout, sizeof(double) * 4));
+    CFields5 cp;
+    for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+    const double N = (double)gm.n[0] * gm.n[1] * gm.n[2];
+    k_mms_error<<<nb, 256, 0, c->stream>>>(cp, c->P, c->bs[0], c->bs[5], gm, N, part);
+    LAUNCH_CHECK();
+    k_mms_error_total<<<1, 256, 0, c->stream>>>(part, nb, dout);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(part);
+    hipFree(dout);
     return SHX_OK;
 }
 

@@ -72,7 +72,7 @@ EXPORTS = [
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
     "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
-    "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download",
+    "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download", "shx_mms_error",
 ]
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
@@ -104,6 +104,7 @@ def lib():
         L.shx_film_count.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_film_download.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
         L.shx_history_download.argtypes = [P, P, P, P, P, P]
+        L.shx_mms_error.argtypes = [P, P]
         L.shx_check_nan.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_check_metric.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.shx_cfl_dt.argtypes = [P, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -263,6 +264,12 @@ class Case:
         f = [np.zeros((nprobes, iterations)) for _ in range(4)]
         _check(lib().shx_history_download(self.h, t.ctypes.data, *[a.ctypes.data for a in f]))
         return t, f
+
+    def mms_error(self):
+        """(L2 rho, Linf rho, L2 p, Linf p) as ErrorManufacturedSolution forms them."""
+        out = np.zeros(4)
+        _check(lib().shx_mms_error(self.h, out.ctypes.data))
+        return tuple(out)
 
     def film_open(self, samples):
         _check(lib().shx_film_open(self.h, samples))

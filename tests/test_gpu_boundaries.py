@@ -206,6 +206,7 @@ def test_manufactured_solution_path(dims, S, inviscid, quirks):
         assert max(aerr) <= 1e-12
         tau = 0.1 / (n[0] * 4.0)
         g.upload_state(ex[:5])
+        g.mms_error()                   # from here on the library keeps the stored pressure
         g.run(tau, 20)
         got = g.download_state()
         ref = o.run(tau, met, ex[:5], 20, src=src, bstate=ex)
@@ -214,6 +215,11 @@ def test_manufactured_solution_path(dims, S, inviscid, quirks):
             errs = errs[:3] + errs[4:]
         _report("20 steps " + tag, errs)
         assert max(errs) <= 1e-11
+        # ErrorManufacturedSolution's norms (stored p one stage behind, last k plane)
+        e_got = g.mms_error()
+        e_want = o.mms_error(ref, o.last_p, ex[0], ex[5])
+        print(f"[parity] MMS error {tag}: GPU {e_got} oracle {e_want}", flush=True)
+        assert np.allclose(e_got, e_want, rtol=1e-6, atol=0)
 
 
 def test_create_rejects_unpaired_periodic_faces():

@@ -11,6 +11,9 @@
     carries half the velocity derivative on its diagonal (s11 = 0.5 u_x, src/Functions.c:5952-5957),
     which the expected value reproduces; a rotation-free field (pure strain) gives 0 because S^2
     is positive semidefinite
+  * ErrorManufacturedSolution's norms (src/ManufacturedSolution.c:745-790) against a numpy brute
+    force over the same node set: the interior of the last k plane, squares divided by the full
+    interior node count
 """
 import numpy as np
 import pytest
@@ -80,3 +83,21 @@ def test_lambda2_of_a_linear_field(S):
     prim = [rho] + [np.ascontiguousarray(a) for a in vel] + [cases.energy(*vel, p, rho, GAMMA, MACH)]
     out = o.calc_post(met, prim, p)
     assert np.all(out[2][sl] == 0.0)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_mms_error_norms_brute_force(dims):
+    # ErrorManufacturedSolution: the last k plane only, squares divided by the full interior count
+    n = (7, 6, 5) if dims == 3 else (7, 6, 1)
+    o = orc.Oracle(dims=dims, S=5, n=n, mach=MACH)
+    rng = np.random.default_rng(4)
+    prim = [1.0 + rng.random(o.shape) for _ in range(5)]
+    p, ex_r, ex_p = (1.0 + rng.random(o.shape) for _ in range(3))
+    got = o.mms_error(prim, p, ex_r, ex_p)
+    g = o.g
+    k = g + n[2] - 1 if dims == 3 else 0
+    sl = (slice(g, g + n[0]), slice(g, g + n[1]), k)
+    N = n[0] * n[1] * n[2]
+    dr, dp = ex_r[sl] - prim[0][sl], ex_p[sl] - p[sl]
+    want = (np.sqrt(np.sum(dr * dr / N)), np.max(np.abs(dr)), np.sqrt(np.sum(dp * dp / N)), np.max(np.abs(dp)))
+    assert np.allclose(got, want, rtol=1e-14, atol=0)
