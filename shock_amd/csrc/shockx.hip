@@ -437,7 +437,8 @@ enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V
 // R . Phi are FP32.  Phi is a correction on top of Theta, so its rounding enters at the size of
 // the correction rather than of the flux.
 // LDS of one WENO sweep block, in doubles: the node window (NVK fields) and, in the mixed form,
-// the FP32 differences of F and U.  Diagonal metric: no normal components; the sign of the
+// room for two of the ten FP32 difference arrays (the other eight reuse four conserved-variable
+// fields of the window).  Diagonal metric: no normal components; the sign of the
 // metric entry rides on V_LAM (the spectral radius is positive) and J moves to V_NX, which keeps
 // the 8 x 32 tile's window under 40 KB (four blocks per CU).
 template <int S, int TA, int TDI, int AX, bool MIX>
@@ -445,7 +446,7 @@ constexpr int weno_lds()
 {
     constexpr int NW = (TDI - 1 + (S + 1)) * TA;
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
-    return NVK * NW + (MIX ? 5 * NW : 0);
+    return NVK * NW + (MIX ? NW : 0);
 }
 
 // KEEP: leave Q alone and return this sweep's contribution of the thread's node in keep[5] (a
@@ -462,7 +463,13 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
     constexpr int VJ = (AX < 0) ? V_J : V_NX;
     double *const sm = lds;                       // NVK * NW
-    float *const dfu = (float *)(lds + NVK * NW); // mixed form: FP32 differences of F (0..4) and U (5..9)
+    // mixed form: the FP32 differences of F (arrays 0..4) and U (5..9).  Once formed, the node
+    // values of rho u, rho v, rho w and rho e are not read again, so arrays 0..7 take their place
+    // (V_U1..V_U4) and only arrays 8 and 9 extend the window: the 8 x 32 tile keeps four blocks
+    // per CU.
+    auto dfa = [&](const int j) -> float * {
+        return (j < 8) ? (float *)(lds + V_U1 * NW) + j * NW : (float *)(lds + NVK * NW) + (j - 8) * NW;
+    };
 
     const int t = threadIdx.x;
     // first padded A index of the tile (aligned to TA when the A axis is contiguous in memory)
@@ -547,13 +554,28 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     else node_pass(std::false_type{});
     __syncthreads();
     if constexpr (MIX) {
-        // node-to-node differences of F and U, rounded once to FP32 (position o: node o+1 - node o)
-        for (int idx = t; idx < NW; idx += NT) {
-            if (idx % WD == WD - 1) continue;
+        // node-to-node differences of F and U, rounded once to FP32 (position o: node o+1 - node o),
+        // all read before any is written over the conserved values
+        constexpr int PER = (NW + NT - 1) / NT;
+        float dv[PER][10];
 #pragma unroll
-            for (int q = 0; q < 5; q++) {
-                dfu[q * NW + idx] = (float)(sm[(V_F0 + q) * NW + idx + 1] - sm[(V_F0 + q) * NW + idx]);
-                dfu[(5 + q) * NW + idx] = (float)(sm[(V_U0 + q) * NW + idx + 1] - sm[(V_U0 + q) * NW + idx]);
+        for (int r = 0; r < PER; r++) {
+            const int idx = t + r * NT;
+            if (idx < NW && idx % WD != WD - 1) {
+#pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    dv[r][q] = (float)(sm[(V_F0 + q) * NW + idx + 1] - sm[(V_F0 + q) * NW + idx]);
+                    dv[r][5 + q] = (float)(sm[(V_U0 + q) * NW + idx + 1] - sm[(V_U0 + q) * NW + idx]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int idx = t + r * NT;
+            if (idx < NW && idx % WD != WD - 1) {
+#pragma unroll
+                for (int j = 0; j < 10; j++) dfa(j)[idx] = dv[r][j];
             }
         }
         __syncthreads();
@@ -613,8 +635,8 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
                         if (AX < 0 || lnz(c, AX, q)) {
-                            x += Lf[q] * dfu[q * NW + ob + s];
-                            y += Lf[q] * dfu[(5 + q) * NW + ob + s];
+                            x += Lf[q] * dfa(q)[ob + s];
+                            y += Lf[q] * dfa(5 + q)[ob + s];
                         }
                     }
                     da[s] = x;
