@@ -759,11 +759,12 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         }
     }
 }
-// Waves per SIMD asked of the compiler: the diagonal W5 eta sweep otherwise takes 134 VGPRs
-// (three waves per SIMD); at 128 it runs four and is measured 9 % faster.  The other variants are
-// left to the compiler (the zeta sweep measured slower under the same bound).
+// Waves per SIMD asked of the compiler: the diagonal W5 xi and eta sweeps otherwise take 134
+// VGPRs (three waves per SIMD); at 128 (126 for xi, no spills) they run four and are measured 7-9 %
+// faster.  The other variants are left to the compiler (the zeta sweep measured slower under the
+// same bound).
 template <int S, int AX, bool MIX>
-constexpr int weno_min_waves() { return (S == 5 && AX == 1 && !MIX) ? 4 : 1; }
+constexpr int weno_min_waves() { return (S == 5 && (AX == 0 || AX == 1) && !MIX) ? 4 : 1; }
 
 template <int S, int TA, int TDI, int AX, bool MIX>
 __global__ __launch_bounds__(TA * TDI, (weno_min_waves<S, AX, MIX>())) void k_weno(WenoArgs a)
