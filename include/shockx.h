@@ -202,7 +202,8 @@ int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *report);
  * The Mach number uses U_lastStep's stored pressure, which the reference's RK update leaves one
  * stage behind the state; the library keeps that pressure from the first shx_post or
  * shx_film_open call on (from the start for cases with subsonic outflow or slip wall faces), so
- * call one of them before stepping for the reference's values.
+ * call one of them before stepping for the reference's values, and before shx_batch_create for a
+ * case that a batch will advance (a batch records its launches when it is created).
  * The film keeps samples on the device (WriteValuesFromUToFilm, src/Functions.c:2510-2553):
  * shx_film_open allocates room for `samples`, shx_film_sample appends one sample of the current
  * state with its time, shx_film_download copies sample s out as rho, u, v, w, p, gradRho, Mach,
