@@ -187,3 +187,82 @@ void WriteValuesFromU1ToU2(struct strct_configuration *pnt_config, struct strct_
     memcpy(pnt_U2->e, pnt_U1->e, bytes);
     if (pnt_U1->p && pnt_U2->p) memcpy(pnt_U2->p, pnt_U1->p, bytes);
 }
+
+/* ---- division file -------------------------------------------------------------------- */
+static void swap_bytes(void *mem, size_t size, int count)
+{
+    for (int k = 0; k < count; k++) {
+        unsigned char *b = (unsigned char *)mem + (size_t)k * size;
+        for (size_t i = 0; i < size / 2; i++) {
+            unsigned char t = b[i];
+            b[i] = b[size - 1 - i];
+            b[size - 1 - i] = t;
+        }
+    }
+}
+
+int compat_load_division(const char *path, int dims, int proccount, unsigned divzone, int swap,
+                         struct compat_division *out)
+{
+    const int D = dims, PHYS = 3;
+    const size_t cg = 8;   /* cgsize_t */
+    const size_t ifblock = sizeof(int) + D * sizeof(int) + 2 * D * cg * 2 + PHYS * sizeof(float) * 3;
+    const size_t datalen = 32 + cg * D * 2 + ifblock * 2 * D;
+    FILE *f = fopen(path, "rb");
+    if (!f) return -2;
+    fseek(f, 0, SEEK_END);
+    const long size = ftell(f);
+    if (size != (long)(proccount * datalen)) {
+        printf("(EE) Division file '%s' has size %ld but should be processors (%d) times data (%d)\n", path, size,
+               proccount, (int)datalen);
+        fclose(f);
+        return -1;
+    }
+    unsigned char *rec = (unsigned char *)malloc(datalen);
+    fseek(f, (long)((divzone - 1) * datalen), SEEK_SET);
+    const size_t got = fread(rec, 1, datalen, f);
+    fclose(f);
+    if (got != datalen) { free(rec); return -2; }
+    /* the reference swaps when asked to (2) or on a big-endian host; files are written little-endian */
+    const int one = 1;
+    const int do_swap = (swap == 2) || (swap == 0 && *(const char *)&one != 1);
+    memset(out, 0, sizeof *out);
+    memcpy(out->zonename, rec, 32);
+    out->zonename[32] = '\0';
+    long long c[6];
+    memcpy(c, rec + 32, 2 * D * cg);
+    if (do_swap) swap_bytes(c, cg, 2 * D);
+    for (int h = 0; h < 2; h++)
+        for (int d = 0; d < D; d++) out->corners[h][d] = c[h * D + d];
+    for (int fidx = 0; fidx < 2 * D; fidx++) {
+        const unsigned char *b = rec + 32 + 2 * D * cg + fidx * ifblock;
+        int nb;
+        memcpy(&nb, b, sizeof(int));
+        if (do_swap) swap_bytes(&nb, sizeof(int), 1);
+        if (!nb) continue;
+        struct compat_interface *it = &out->face[fidx];
+        out->ni++;
+        it->neighbour = nb;
+        int tr[3];
+        memcpy(tr, b + 4, D * sizeof(int));
+        long long rg[12];
+        memcpy(rg, b + 4 + 4 * D, 4 * D * cg);
+        float fl[9];
+        memcpy(fl, b + 4 + 4 * D + 4 * D * cg, 9 * sizeof(float));
+        if (do_swap) {
+            swap_bytes(tr, sizeof(int), D);
+            swap_bytes(rg, cg, 4 * D);
+            swap_bytes(fl, sizeof(float), 9);
+        }
+        for (int d = 0; d < D; d++) {
+            it->transform[d] = tr[d];
+            it->range[0][d] = rg[d];
+            it->range[1][d] = rg[D + d];
+            it->drange[0][d] = rg[2 * D + d];
+            it->drange[1][d] = rg[3 * D + d];
+        }
+        for (int q = 0; q < 3; q++) { it->rc[q] = fl[q]; it->ra[q] = fl[3 + q]; it->tr[q] = fl[6 + q]; }
+    }
+    free(rec);
+    return 0;
+}

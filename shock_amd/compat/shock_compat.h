@@ -65,6 +65,28 @@ struct strct_Flux {
     FLT *Mass, *xiMomentum, *etaMomentum, *zetaMomentum, *Energy;
 };
 
+/* loadDivision (src/Load.c:191-250): record `divzone` (1-based) of a division file written for
+ * `proccount` processes, DIM = dims.  Each record: zone name (32 bytes), corner indices
+ * (2 x DIM cgsize_t, 1-based, lower then upper), then per face (i-, i+, j-, j+, k-, k+) the
+ * neighbour zone (0: none), transform (DIM ints), range and donor range (2 x DIM cgsize_t each)
+ * and three PHYSDIM = 3 float vectors.  cgsize_t is 8 bytes; the bytes are the host's
+ * (little-endian) unless swap is 2 (always swap) -- the reference's flag_swapDivisionFile.
+ * Returns 0, or -1 (size mismatch: the file must hold proccount records) / -2 (unreadable). */
+struct compat_interface {
+    int neighbour;
+    int transform[3];
+    long long range[2][3], drange[2][3];
+    float rc[3], ra[3], tr[3];
+};
+struct compat_division {
+    char zonename[33];
+    long long corners[2][3];
+    struct compat_interface face[6];
+    int ni;                                   /* faces with a neighbour */
+};
+int compat_load_division(const char *path, int dims, int proccount, unsigned divzone, int swap,
+                         struct compat_division *out);
+
 void compat_define(struct strct_configuration *cfg);
 void compat_release(struct strct_configuration *cfg);
 
