@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 from shock_amd import cases, shockx  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+CLOCK_HZ = 2.4e9          # MI355X peak engine clock
 FP64 = 8
 
 
@@ -68,6 +69,7 @@ def cpu_baseline(n_sample, steps, seed, threads):
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # noqa: E402  (test infrastructure: CPU baseline leg only)
+    orc.lib().orc_set_threads(threads)
     nn = (n_sample,) * 3
     o = orc.Oracle(dims=3, S=5, n=nn, time_order=4, mach=0.5, reynolds=2000.0)
     xe, ye, ze = cases.periodic_ext_coords(3, 5, nn, cases.sine_bump_3d(0.05))
@@ -98,6 +100,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=128, help="CPU baseline sample size per direction")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--cpu1-n", type=int, default=64, help="one-core CPU sample size per direction (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
     ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
@@ -175,6 +178,7 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
 
     roofline = None
+    valu = None
     kernels = {}
     if kt:
         ng = n + S + 1
@@ -198,14 +202,33 @@ def main():
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algo_bytes": int(b), "avg_ms": round(avg, 4)}
+        # FP64 vector pipe use beside the HBM fraction: FP64 VALU instructions per launch (hardware
+        # counters of the same kernels, profiles/sq_counters_*.json) x 4 cycles per wave64 FP64
+        # instruction over 1024 SIMDs x 2.4 GHz x the live launch time.
+        sfile = os.path.join(ROOT, "profiles", f"sq_counters_turbulence{n}_{args.precision}.json")
+        if os.path.exists(sfile):
+            with open(sfile) as f:
+                sq = json.load(f)
+            for name, kd in kernels.items():
+                if name in sq:
+                    kd["fp64_pipe_busy"] = round(sq[name]["fp64_valu_instr_per_launch"] * 4.0
+                                                 / (1024 * kd["avg_ms"] * 1e-3 * CLOCK_HZ), 3)
+            if dom in sq:
+                valu = {"kernel": dom, "fp64_pipe_busy": kernels[dom]["fp64_pipe_busy"],
+                        "fp64_valu_instr_per_launch": sq[dom]["fp64_valu_instr_per_launch"],
+                        "peak_instr_per_s": 1024 * CLOCK_HZ / 4.0}
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps, args.seed, threads)
         cpu = {"value": round(rate, 1), "unit": "cell-updates/s/RK-stage", "cores": threads, "kind": "port",
                "sample": f"C oracle, {args.cpu_n}^3 block of the same workload (WENO5+viscous, FP64, RK4), "
                          f"{args.cpu_steps} steps = {4 * args.cpu_steps} stages, {secs:.1f}s"}
+        if args.cpu1_n:
+            rate1, secs1 = cpu_baseline(args.cpu1_n, 1, args.seed, 1)
+            cpu1 = {"value": round(rate1, 1), "unit": "cell-updates/s/RK-stage", "cores": 1, "kind": "port",
+                    "sample": f"C oracle, one thread, {args.cpu1_n}^3 block, 1 step = 4 stages, {secs1:.1f}s"}
 
     # stage-level fraction of the HBM roofline at the compulsory 260 B/node/stage (fused RK4 stage)
     stage_frac = value * 260.0 / (HBM_PEAK_GBS * 1e9)
@@ -220,10 +243,12 @@ def main():
                    "grid": [n, n, n], "scheme": "WENO5", "viscous": True, "time_order": time_order,
                    "precision": args.precision, "quirks": "reference", "tau": tau, "seed": args.seed},
         "roofline": roofline,
+        "valu": valu,
         "stage_hbm_frac_260B": round(stage_frac, 4),
         "weno_tile_variants": tiles,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
     }
     print(json.dumps(out), flush=True)
     case.close()

@@ -76,6 +76,7 @@ def lib():
         _lib.orc_visc_fac.argtypes = [ctypes.c_int, P, P]
         _lib.orc_step.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P]
         _lib.orc_step_b.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_double, P, P, P, P, P, P]
+        _lib.orc_set_threads.argtypes = [ctypes.c_int]
         _lib.orc_rk_update.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_int, ctypes.c_double, P, P, P, P, P]
     return _lib
 

@@ -10,6 +10,7 @@
  */
 #include "shx_oracle.h"
 #include <float.h>
+#include <omp.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1161,4 +1162,10 @@ void orc_mms_error(const orc_cfg *c, double *const prim[5], const double *p, con
     out[1] = (double)lir;
     out[2] = (double)sqrtl(l2p);
     out[3] = (double)lip;
+}
+
+void orc_set_threads(int n)
+{
+    if (n > 0)
+        omp_set_num_threads(n);
 }

@@ -160,6 +160,8 @@ void orc_calc_post(const orc_cfg *c, double *const met[10], double *const prim[5
 void orc_mms_error(const orc_cfg *c, double *const prim[5], const double *p, const double *ex_rho,
                    const double *ex_p, double out[4]);
 
+/* Thread count of the OpenMP loops (timing helper; results do not depend on it). */
+void orc_set_threads(int n);
 void orc_zero(const orc_cfg *c, double *const f[], int nf);
 void orc_ghost_periodic(const orc_cfg *c, double *const f[], int nf);
 /* Face boundary writers in the reference's order (SetAllBoundaryConditions, src/BC.c:13-74),
