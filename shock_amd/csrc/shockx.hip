@@ -1313,9 +1313,9 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
 }
 
 // 3D nodal derivatives, marching along i.  A block owns a GK x GJ column of (j, k) nodes over the
-// full padded extent and walks a chunk of i planes: the i derivative comes from a register window
-// of 2g+1 planes per thread (every plane is read from HBM once per chunk), the j and k derivatives
-// from the current plane staged in LDS with a g-wide halo.  Outputs as k_visc_grad; the T
+// full padded extent and walks a chunk of i planes: every plane is read from HBM once per chunk
+// into LDS with a g-wide halo; the j and k derivatives come from there, the i derivative from a
+// register window of 2g+1 plane centres per thread.  Outputs as k_visc_grad; the T
 // derivatives only when `tgrad` (the diagonal-metric viscous sweeps never read them).
 constexpr int GK = 64, GJ = 16;
 struct GradArgs {
@@ -1327,7 +1327,7 @@ struct GradArgs {
     int ci, tgrad;
     int tg[4];                    // tile grid (k columns, j columns, i chunks) of a single-case launch
 };
-template <int S>
+template <int S, bool TG>
 __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
 {
     const CFields5 &pr = a.pr;
@@ -1335,10 +1335,11 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     const Fields12 &gr = a.gr;
     const Geom &gm = a.gm;
     const Consts &k = a.k;
-    const int ci = a.ci, tgrad = a.tgrad;
+    const int ci = a.ci;
+    constexpr int NF = TG ? 4 : 3;             // u, v, w (+ T for its derivatives)
     constexpr int G = (S + 1) / 2, NL = 2 * G + 1;
     constexpr int LK = GK + 2 * G, LJ = GJ + 2 * G, LP = LK * LJ;
-    __shared__ double sm[4 * LP];
+    __shared__ double sm[NF * LP];
     const int tx = threadIdx.x % GK, ty = threadIdx.x / GK;
     const int k0 = bx * GK, j0 = by * GJ;
     const int kk = k0 + tx, jj = j0 + ty;
@@ -1346,75 +1347,110 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     const int i0 = bz * ci, i1 = min(i0 + ci, gm.ng[0]);
     const gcdp src[4] = {pr.f[1], pr.f[2], pr.f[3], T};
     const long col = (long)jj * gm.st[1] + kk;
-    double win[4][NL];
-    // prime the window with planes i0-G .. i0+G-1 (clamped; out-of-block planes are never used)
-    // T (field 3) is loaded only where its derivatives are written (tgrad)
+    constexpr int NT = GK * GJ, NST = (LP + NT - 1) / NT;   // staged plane elements per thread
+    const int c0 = (ty + G) * LK + (tx + G);                  // this thread's node in a staged plane
+    // Planes i0-G .. i1+G-1 pass through LDS once each (with their j/k halo): plane p gives its
+    // j and k derivatives when p lies in the chunk, and its centre value enters the register
+    // window of 2g+1 planes, from which the i derivative of plane p-G follows.  Planes outside
+    // the padded extent are clamped; they only reach windows of ghost planes, whose i derivatives
+    // are not written.
+    double nxt[NST][NF];
+    auto fetch = [&](const int p) {
+        const long pbase = (long)max(0, min(p, gm.ng[0] - 1)) * gm.st[0];
 #pragma unroll
-    for (int l = 0; l < NL - 1; l++) {
-        const int ip = max(0, min(i0 - G + l, gm.ng[0] - 1));
+        for (int s = 0; s < NST; s++) {
+            const int e = threadIdx.x + s * NT;
+            const int lj = e / LK, lk = e - lj * LK;
+            const int j = j0 - G + lj, kq = k0 - G + lk;
+            const bool in = (e < LP) && (j >= 0) && (j < gm.ng[1]) && (kq >= 0) && (kq < gm.ng[2]);
+            const long m = pbase + (long)j * gm.st[1] + kq;
 #pragma unroll
-        for (int f = 0; f < 4; f++) win[f][l + 1] = (mine && (f < 3 || tgrad)) ? src[f][(long)ip * gm.st[0] + col] : 0.0;
-    }
-    for (int i = i0; i < i1; i++) {
-        // shift the window and read plane i+G
+            for (int f = 0; f < NF; f++) nxt[s][f] = in ? src[f][m] : 0.0;
+        }
+    };
+    double win[NF][NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) win[f][l] = 0.0;
+    // the next plane is prefetched where the registers allow it (WENO5 order, no T derivatives)
+    constexpr bool PF = (S == 5) && !TG;
+    const int pend = i1 + G;
+    if constexpr (PF) fetch(i0 - G);
+    for (int p = i0 - G; p < pend; p++) {
+        if constexpr (PF) {
+#pragma unroll
+            for (int s = 0; s < NST; s++) {
+                const int e = threadIdx.x + s * NT;
+                if (e < LP) {
+#pragma unroll
+                    for (int f = 0; f < NF; f++) sm[f * LP + e] = nxt[s][f];
+                }
+            }
+        } else {
+            const long pbase = (long)max(0, min(p, gm.ng[0] - 1)) * gm.st[0];
+            for (int e = threadIdx.x; e < LP; e += NT) {
+                const int lj = e / LK, lk = e - lj * LK;
+                const int j = j0 - G + lj, kq = k0 - G + lk;
+                const bool in = (j >= 0) && (j < gm.ng[1]) && (kq >= 0) && (kq < gm.ng[2]);
+                const long m = pbase + (long)j * gm.st[1] + kq;
+#pragma unroll
+                for (int f = 0; f < NF; f++) sm[f * LP + e] = in ? src[f][m] : 0.0;
+            }
+        }
+        __syncthreads();
+        if constexpr (PF) {
+            if (p + 1 < pend) fetch(p + 1);
+        }
 #pragma unroll
         for (int l = 0; l < NL - 1; l++)
 #pragma unroll
-            for (int f = 0; f < 4; f++) win[f][l] = win[f][l + 1];
-        const int ip = min(i + G, gm.ng[0] - 1);
+            for (int f = 0; f < NF; f++) win[f][l] = win[f][l + 1];
 #pragma unroll
-        for (int f = 0; f < 4; f++) win[f][NL - 1] = (mine && (f < 3 || tgrad)) ? src[f][(long)ip * gm.st[0] + col] : 0.0;
-        // stage plane i with its halo
-        const long pbase = (long)i * gm.st[0];
-        for (int e = threadIdx.x; e < LP; e += GK * GJ) {
-            const int lj = e / LK, lk = e - lj * LK;
-            const int j = j0 - G + lj, kq = k0 - G + lk;
-            const bool in = (j >= 0) && (j < gm.ng[1]) && (kq >= 0) && (kq < gm.ng[2]);
-            const long m = pbase + (long)j * gm.st[1] + kq;
-#pragma unroll
-            for (int f = 0; f < 4; f++) sm[f * LP + e] = (in && (f < 3 || tgrad)) ? src[f][m] : 0.0;
-        }
-        __syncthreads();
+        for (int f = 0; f < NF; f++) win[f][NL - 1] = sm[f * LP + c0];
         if (mine) {
-            const long n = pbase + col;
-            const int c0 = (ty + G) * LK + (tx + G);
-            if (i >= gm.lo[0] && i < gm.lo[0] + gm.n[0]) {
-                double d[4] = {0.0, 0.0, 0.0, 0.0};
+            if (p >= i0 && p < i1) {
+                const long n = (long)p * gm.st[0] + col;
+                if (jj >= gm.lo[1] && jj < gm.lo[1] + gm.n[1]) {
+                    double d[NF] = {};
+#pragma unroll
+                    for (int l = 0; l < NL; l++)
+#pragma unroll
+                        for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G) * LK];
+                    gr.f[1][n] = d[0]; gr.f[4][n] = d[1]; gr.f[7][n] = d[2];
+                    if constexpr (TG) gr.f[10][n] = d[3];
+                }
+                if (kk >= gm.lo[2] && kk < gm.lo[2] + gm.n[2]) {
+                    double d[NF] = {};
+#pragma unroll
+                    for (int l = 0; l < NL; l++)
+#pragma unroll
+                        for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G)];
+                    gr.f[2][n] = d[0]; gr.f[5][n] = d[1]; gr.f[8][n] = d[2];
+                    if constexpr (TG) gr.f[11][n] = d[3];
+                }
+            }
+            const int i = p - G;
+            if (i >= i0 && i >= gm.lo[0] && i < gm.lo[0] + gm.n[0]) {
+                const long n = (long)i * gm.st[0] + col;
+                double d[NF] = {};
 #pragma unroll
                 for (int l = 0; l < NL; l++)
 #pragma unroll
-                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * win[f][l];
+                    for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * win[f][l];
                 gr.f[0][n] = d[0]; gr.f[3][n] = d[1]; gr.f[6][n] = d[2];
-                if (tgrad) gr.f[9][n] = d[3];
-            }
-            if (jj >= gm.lo[1] && jj < gm.lo[1] + gm.n[1]) {
-                double d[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int l = 0; l < NL; l++)
-#pragma unroll
-                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G) * LK];
-                gr.f[1][n] = d[0]; gr.f[4][n] = d[1]; gr.f[7][n] = d[2];
-                if (tgrad) gr.f[10][n] = d[3];
-            }
-            if (kk >= gm.lo[2] && kk < gm.lo[2] + gm.n[2]) {
-                double d[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int l = 0; l < NL; l++)
-#pragma unroll
-                    for (int f = 0; f < 4; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G)];
-                gr.f[2][n] = d[0]; gr.f[5][n] = d[1]; gr.f[8][n] = d[2];
-                if (tgrad) gr.f[11][n] = d[3];
+                if constexpr (TG) gr.f[9][n] = d[3];
             }
         }
         __syncthreads();
     }
 }
-template <int S>
+template <int S, bool TG>
 __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
 {
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
-    march_body<S>(a, bx, by, bz);
+    march_body<S, TG>(a, bx, by, bz);
 }
 
 // ---- batched launches ----------------------------------------------------------------------------
@@ -1524,13 +1560,29 @@ __global__ __launch_bounds__(256) void k_grad2d_batch(BATCH_PARAMS(GradArgs))
     });
 }
 
-template <int S, bool P>
+template <int S, bool P, bool TG>
 __global__ __launch_bounds__(GK * GJ) void k_march_batch(BATCH_PARAMS(GradArgs))
 {
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
-        march_body<S>(rec(A, w.x), w.y, w.z, w.w);
+        march_body<S, TG>(rec(A, w.x), w.y, w.z, w.w);
     });
+}
+
+// derivative march kernel for an order, with or without T derivatives (the general metric), as a
+// single-case launch or a batched one (persistent or not)
+static const void *march_kernel(int S, bool tgrad, bool batch, bool persistent = false)
+{
+    if (!batch) {
+        if (S == 5) return tgrad ? (const void *)&k_visc_grad_march<5, true> : (const void *)&k_visc_grad_march<5, false>;
+        return tgrad ? (const void *)&k_visc_grad_march<9, true> : (const void *)&k_visc_grad_march<9, false>;
+    }
+    if (S == 5) {
+        if (tgrad) return persistent ? (const void *)&k_march_batch<5, true, true> : (const void *)&k_march_batch<5, false, true>;
+        return persistent ? (const void *)&k_march_batch<5, true, false> : (const void *)&k_march_batch<5, false, false>;
+    }
+    if (tgrad) return persistent ? (const void *)&k_march_batch<9, true, true> : (const void *)&k_march_batch<9, false, true>;
+    return persistent ? (const void *)&k_march_batch<9, true, false> : (const void *)&k_march_batch<9, false, false>;
 }
 
 template <int S, int TA, int TDI, int AX, bool MIX, bool P>
@@ -2624,8 +2676,9 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
             const int tgrad = c->diag ? 0 : 1;
             GradArgs ga = {cp, c->T, gf, gm, c->k, CI, tgrad, {0, 0, 0, 0}};
             const dim3 grid = flat_grid(dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI)), ga.tg, 0);
-            if (c->d.space_order == 5) k_visc_grad_march<5><<<grid, GK * GJ, 0, c->stream>>>(ga);
-            else k_visc_grad_march<9><<<grid, GK * GJ, 0, c->stream>>>(ga);
+            const void *fn = march_kernel(c->d.space_order, tgrad, false);
+            void *args[] = {&ga};
+            HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * GJ), args, 0, c->stream));
         } else {
             k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
         }
@@ -3054,8 +3107,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             batch_add(b, (const void *)&k_ghost_values_batch<false>, (const void *)&k_ghost_values_batch<true>, 256, BK_VALUES, recs, sizeof(GhostValuesArgs), grids);
             add_faces();
             if (dims == 3)
-                batch_add(b, S == 5 ? (const void *)&k_march_batch<5, false> : (const void *)&k_march_batch<9, false>,
-                          S == 5 ? (const void *)&k_march_batch<5, true> : (const void *)&k_march_batch<9, true>, GK * GJ,
+                batch_add(b, march_kernel(S, !c0->diag, true, false), march_kernel(S, !c0->diag, true, true), GK * GJ,
                           BK_GRAD, grecs, sizeof(GradArgs), ggrids, ORDER_NODES);
             else
                 batch_add(b, (const void *)&k_grad2d_batch<false>, (const void *)&k_grad2d_batch<true>, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
