@@ -60,6 +60,9 @@ def test_batch_with_faces_equals_cases_alone():
                     orc.QUIRKS_REFERENCE, None),
         _faces_case(3, 5, (14, 12, 12), INOUT + (orc.BC_FARFIELD, orc.BC_WALL_ISOTHERMAL, 0, 0), False,
                     orc.QUIRKS_REFERENCE, "bump"),
+        # the full 3D metric (no diagonal quirk): derivative march with T derivatives, W5 and W9
+        _faces_case(3, 5, (14, 12, 12), INOUT + (0, 0, 0, 0), False, 0, "bump"),
+        _faces_case(3, 9, (14, 12, 12), INOUT + (0, 0, 0, 0), False, 0, "bump"),
     ]
     steps = 3
     taus = [0.2 / (m["n"][0] * 5.0) for m in made]

@@ -61,7 +61,7 @@ def test_rhs_2d(S, inviscid):
     assert max(errs[:3] + errs[4:]) <= 1e-12
 
 
-@pytest.mark.parametrize("S,quirks", [(5, orc.QUIRKS_REFERENCE), (5, 0), (9, orc.QUIRKS_REFERENCE)])
+@pytest.mark.parametrize("S,quirks", [(5, orc.QUIRKS_REFERENCE), (5, 0), (9, orc.QUIRKS_REFERENCE), (9, 0)])
 def test_rhs_3d(S, quirks):
     o, coords, met, prim, nn = setup_3d_turb(n=12, S=S, quirks=quirks)
     q_ref, _, _ = o.rhs(met, [a.copy() for a in prim])
