@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# SHX_ORACLE_LIB names another build of the same source (the mutation tests load deliberately
+# altered copies of the restatement this way, to show that the pins catch the alteration)
+_LIB = os.environ.get("SHX_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 QUIRK_ZETA_VISC_J = 1
 QUIRK_METRIC_DIAG3D = 2

@@ -20,9 +20,14 @@
  *                         (coefficients derived by exact rational solve); smoothness
  *                         indicators vs Jiang-Shu / Balsara-Shu integrals (sympy)
  *   orc_weno_dir          free-stream preservation, telescoping conservation,
- *                         mirror symmetry, order of accuracy on smooth data
+ *                         mirror symmetry, order of accuracy on smooth data; the LF weight
+ *                         J_{i+1/2} by free stream on separably stretched grids; the
+ *                         arithmetic-mean eigen-state by 1D data against numpy's eigenvectors
+ *                         of the Euler Jacobian (tests/test_oracle_pins.py, with altered copies
+ *                         of this file that the pins must reject)
  *   orc_visc_*            polynomial exactness vs analytic Navier-Stokes viscous
- *                         terms, order of accuracy
+ *                         terms, order of accuracy; the face coefficient (mu fac)_i/2 +
+ *                         (mu fac)_{i+1}/2 by the locality of the response to one node's mu
  *   orc_rk_*              closed form (linear ODE per node reproduces RK3/RK4 polynomials)
  *   orc_metric_*          exact on affine grids; 2nd-order central differences
  *   orc_ghost_periodic    permutation identity
