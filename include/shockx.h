@@ -266,8 +266,10 @@ int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[
  * RK order, WENO tiles) per stage over a tile work list, replayed from a captured HIP graph; the
  * face writers run as one launch per face index.  tau holds one
  * time step per case.  The cases keep their handles: their state can be read or replaced between
- * shx_batch_run calls (work on a case's stream is ordered after the batch).  Destroy the batch
- * before its cases. */
+ * shx_batch_run calls (work on a case's stream is ordered after the batch).  A case belongs to at
+ * most one batch; while it does, shx_calc_runge_kutta, shx_run and shx_simulate on it (which
+ * would move its state to another buffer than the batch's captured step uses) and shx_destroy
+ * return SHX_ERR_STATE.  Destroy the batch before its cases. */
 typedef struct shx_batch shx_batch;
 int shx_batch_create(shx_case *const *cases, int ncases, const double *tau, shx_batch **out);
 int shx_batch_set_tau(shx_batch *b, const double *tau);

@@ -311,8 +311,10 @@ class Oracle:
                 fields = list(un[:4]) + [p_last] + [post[1], post[0], post[2]]
                 film[sample] = (time, [f[sl].copy() for f in fields])
                 sample += 1
-            if probes:
-                row = it - (sim["start"] + 1)
+            row = it - (sim["start"] + 1)
+            if probes and 0 <= row < iters:
+                # a rollback below the start (the first backup is iteration 1) gives rows before
+                # the first, which the reference writes out of bounds; they are not recorded
                 hist_t[row] = time
                 for q, n in enumerate(probes):
                     idx = np.unravel_index(n, self.shape)

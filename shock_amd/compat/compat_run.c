@@ -2,7 +2,7 @@
  * compat_run.c -- the reference's step loop (src/SHOCK.c:557-666: CalcRungeKutta, checkNAN,
  * WriteValuesFromU1ToU2) written against shock_compat.h, on a case read from a file.
  *
- * usage: compat_run <case file> <output file>
+ * usage: compat_run <case file> <output file> [donor rank of the interface faces]
  * Case file (native endianness): int32 dims, SPACEORDER, n[3], TimeOrder, Inviscid, steps,
  * face[6] (0 neighbour, 1 Inflow + supersonic option, 2 Outflow + Riemann option,
  * 3 WallInviscid, 4 WallViscous, 5 Farfield); float64 gamma, Mach, Reynolds, Prandtl, T0, tau,
@@ -57,11 +57,21 @@ int main(int argc, char **argv)
     char *bc[6] = {cfg.BC_Left, cfg.BC_Right, cfg.BC_Bottom, cfg.BC_Top, cfg.BC_Behind, cfg.BC_InFront};
     int *nb[6] = {&cfg.InterfaceNeighbourLeft, &cfg.InterfaceNeighbourRight, &cfg.InterfaceNeighbourBottom,
                   &cfg.InterfaceNeighbourTop, &cfg.InterfaceNeighbourBehind, &cfg.InterfaceNeighbourInFront};
+    /* face f with code 0 is interface f, whose donor is this rank (rank 0) unless argv[3] names
+       another; transforms are the identity */
+    static int donor[6], tr_rows[6][3];
+    static int *tr[6];
+    cfg.MPI_rank = 0;
     for (int f = 0; f < 6; f++) {
         const int code = hdr[8 + f];
         strcpy(bc[f], code ? names[code] : "Interface");
-        *nb[f] = code ? NO_NEIGHBOUR : 0;
+        *nb[f] = code ? NO_NEIGHBOUR : f;
+        donor[f] = argc > 3 ? atoi(argv[3]) : cfg.MPI_rank;
+        for (int d = 0; d < 3; d++) tr_rows[f][d] = d + 1;
+        tr[f] = tr_rows[f];
     }
+    cfg.MPI_rankNeighbours = donor;
+    cfg.TransformMatrixOfInterface = tr;
     compat_define(&cfg);
     const long ntot = (long)cfg.int_iMeshPointsGhostCells * cfg.int_jMeshPointsGhostCells * cfg.int_kMeshPointsGhostCells;
     struct strct_mesh mesh;

@@ -50,13 +50,49 @@ void compat_define(struct strct_configuration *cfg)
     cfg->compat_quirks = SHX_QUIRKS_REFERENCE;
 }
 
+static const char *const face_name[6] = {"left", "right", "bottom", "top", "behind", "in front"};
+
+/* an interface face is one half of a periodic self-interface: its donor is this rank, its
+   transform the identity, and the opposite face of the direction is such a face too (the
+   single-rank self-transfer, src/Functions.c:3580-3684).  Anything else needs the halo exchange
+   of another block, which this path does not have. */
+static int self_interface(const struct strct_configuration *cfg, const int nbr[6], int f)
+{
+    const int i = nbr[f];
+    if (i == NO_NEIGHBOUR) return 0;
+    if (i < 0 || !cfg->MPI_rankNeighbours || !cfg->TransformMatrixOfInterface ||
+        !cfg->TransformMatrixOfInterface[i]) {
+        printf("SHOCK: GPU path: face %s: interface %d without donor rank and transform\n", face_name[f], i);
+        exit(4242);
+    }
+    if (cfg->MPI_rankNeighbours[i] != cfg->MPI_rank) {
+        printf("SHOCK: GPU path: face %s: interface to rank %d; only periodic self-interfaces of rank %d "
+               "are supported\n", face_name[f], cfg->MPI_rankNeighbours[i], cfg->MPI_rank);
+        exit(4242);
+    }
+    for (int d = 0; d < cfg->MESHDIMENSIONS; d++)
+        if (cfg->TransformMatrixOfInterface[i][d] != d + 1) {
+            printf("SHOCK: GPU path: face %s: interface transform is not the identity; only periodic "
+                   "self-interfaces are supported\n", face_name[f]);
+            exit(4242);
+        }
+    return 1;
+}
+
 /* face kind of one face (SetAllBoundaryConditions' selection, src/BC.c:75-637) */
-static int face_kind(const struct strct_configuration *cfg, const char *bc, int neighbour)
+static int face_kind(const struct strct_configuration *cfg, const char *bc, const int nbr[6], int f)
 {
     const int inflow = !strcmp(bc, cfg->BCInflow) || !strcmp(bc, cfg->BCInflowSubsonic) ||
                        !strcmp(bc, cfg->BCInflowSupersonic);
     const int outflow = !strcmp(bc, cfg->BCOutflow) || !strcmp(bc, cfg->BCOutflowSubsonic);
-    if (neighbour != NO_NEIGHBOUR) return SHX_BC_PERIODIC;
+    if (self_interface(cfg, nbr, f)) {
+        if (!self_interface(cfg, nbr, f ^ 1)) {
+            printf("SHOCK: GPU path: face %s is an interface but face %s is not; only periodic "
+                   "self-interfaces are supported\n", face_name[f], face_name[f ^ 1]);
+            exit(4242);
+        }
+        return SHX_BC_PERIODIC;
+    }
     if (!strcmp(bc, cfg->BCManufacturedSolution)) return SHX_BC_MANUFACTURED;
     if (!strcmp(bc, cfg->BCWallInviscid)) return SHX_BC_WALL_SLIP;
     if (!strcmp(bc, cfg->BCWallViscous)) return SHX_BC_WALL_NOSLIP;
@@ -92,7 +128,7 @@ static struct compat_ctx *ctx_of(struct strct_configuration *cfg, struct strct_m
     const int nbr[6] = {cfg->InterfaceNeighbourLeft, cfg->InterfaceNeighbourRight, cfg->InterfaceNeighbourBottom,
                         cfg->InterfaceNeighbourTop, cfg->InterfaceNeighbourBehind, cfg->InterfaceNeighbourInFront};
     for (int f = 0; f < 6; f++)
-        d.bc[f] = (f < 2 * cfg->MESHDIMENSIONS) ? face_kind(cfg, bcs[f], nbr[f]) : SHX_BC_PERIODIC;
+        d.bc[f] = (f < 2 * cfg->MESHDIMENSIONS) ? face_kind(cfg, bcs[f], nbr, f) : SHX_BC_PERIODIC;
     d.precision = SHX_FP64;
     d.rho_inflow = cfg->rho_inflow; d.u_inflow = cfg->u_inflow; d.v_inflow = cfg->v_inflow;
     d.w_inflow = cfg->w_inflow; d.p_inflow = cfg->p_inflow;

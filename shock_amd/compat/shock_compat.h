@@ -44,8 +44,15 @@ struct strct_configuration {
     int flag_BC_option_outflow_normal_sub, flag_BC_option_outflow_riemann_sub, flag_BC_option_outflow_rudy_sub;
     FLT rho_inflow, p_inflow, u_inflow, v_inflow, w_inflow;
     FLT p_out, AlphaNonRef, T_wall;
+    /* interface index of each face, or NO_NEIGHBOUR (getNeighbour, src/Functions.c:3795-3850);
+       the donor rank and the CGNS transform of interface i (src/SHOCK.h:257, 278, 285).  The
+       GPU path takes a face with an interface only as half of a periodic self-interface: donor
+       == MPI_rank, transform (1, 2, 3), and the opposite face of the direction the same. */
     int InterfaceNeighbourLeft, InterfaceNeighbourRight, InterfaceNeighbourBottom;
     int InterfaceNeighbourTop, InterfaceNeighbourBehind, InterfaceNeighbourInFront;
+    int MPI_rank;
+    int *MPI_rankNeighbours;
+    int **TransformMatrixOfInterface;
     unsigned compat_quirks;                   /* SHX_QUIRK_* bits; compat_define sets the reference's */
     void *compat_case;                        /* the device case, created by the first step */
 };

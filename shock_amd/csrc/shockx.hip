@@ -2036,6 +2036,10 @@ struct shx_case {
     size_t ev_used;
     double kt_ms[SHX_NUM_KERNELS];
     int kt_n[SHX_NUM_KERNELS];
+    // the batch that holds this case (its captured step graph uses this case's state buffers as
+    // they were at shx_batch_create); stepping the case alone while it is set would leave the
+    // batch advancing a stale buffer
+    shx_batch *batch = nullptr;
 };
 
 extern "C" const char *shx_last_error(void) { return g_err; }
@@ -2229,6 +2233,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
 extern "C" int shx_destroy(shx_case *c)
 {
     if (!c) return SHX_OK;
+    if (c->batch) return set_err(SHX_ERR_STATE, "the case is in a batch: destroy the batch first");
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int b = 0; b < 3; b++) for (int m = 0; m < 5; m++) hipFree(c->prim[b][m]);
     for (int m = 0; m < 5; m++) { hipFree(c->q[m]); hipFree(c->qsum[m]); hipFree(c->src[m]); }
@@ -2861,6 +2866,7 @@ extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
 {
     int rc = check_ready(c);
     if (rc) return rc;
+    if (c->batch) return set_err(SHX_ERR_STATE, "the case is in a batch: step it with shx_batch_run");
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
     const int un = c->cur;
@@ -3325,6 +3331,7 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
         if (!(tau[i] > 0.0)) return set_err(SHX_ERR_ARG, "case %d: tau must be positive", i);
         for (int j = 0; j < i; j++)
             if (cases[j] == c) return set_err(SHX_ERR_ARG, "case %d appears twice", i);
+        if (c->batch) return set_err(SHX_ERR_STATE, "case %d is in another batch", i);
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     shx_batch *b = new shx_batch();
@@ -3346,6 +3353,7 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
     if (!rc) rc = batch_grids(b);
     if (!rc) rc = batch_capture(b);
     if (rc) { shx_batch_destroy(b); return rc; }
+    for (shx_case *c : b->cases) c->batch = b;
     *out = b;
     return SHX_OK;
 }
@@ -3468,6 +3476,8 @@ extern "C" int shx_batch_destroy(shx_batch *b)
     for (hipEvent_t e : b->joins) hipEventDestroy(e);
     if (b->fork) hipEventDestroy(b->fork);
     if (b->stream) hipStreamDestroy(b->stream);
+    for (shx_case *c : b->cases)
+        if (c->batch == b) c->batch = nullptr;
     delete b;
     return SHX_OK;
 }
@@ -3821,8 +3831,12 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
             if ((rc = film_sample(c, time))) return rc;
             sample++;
         }
-        if (sim->nprobes > 0) {
-            const int row = it - (sim->start_iteration + 1);
+        // history row of this iteration (src/Functions.c:5381-5417 index the history by
+        // iteration - (start + 1)); a rollback to a backup older than the start (the first backup
+        // is iteration 1 whatever the start, src/SHOCK.c:531-540) gives iterations before the
+        // first row, which the reference writes out of bounds: they are not recorded here
+        const int row = it - (sim->start_iteration + 1);
+        if (sim->nprobes > 0 && row >= 0 && row < iters) {
             CFields5 cp;
             for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
             k_probe<<<nblk(sim->nprobes, 64), 64, 0, c->stream>>>(cp, c->P, c->d_probes, sim->nprobes, c->hist, row, iters);

@@ -142,11 +142,17 @@ class Server:
         used = 0
         batch = None
         finished = []
+        rejected = []
         cell_updates = 0.0
         while pending or waiting or active:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 r = pending.pop(0)
+                if self.footprint(r.n, r.S) > self.budget:
+                    # can never fit, even on an empty device: refused on arrival (queued, it would
+                    # hold the line for ever)
+                    rejected.append(r)
+                    continue
                 waiting.append(r)
                 if self.open_fn is open_case:
                     preps[r.rid] = pool.submit(prepare, r)
@@ -203,10 +209,10 @@ class Server:
         if batch is not None:
             batch.close()
         wall = time.perf_counter() - t0
-        tts = np.array([r.done - r.arrival for r in finished])
+        tts = np.array([r.done - r.arrival for r in finished] or [float("nan")])
         return {
-            "requests": len(finished), "wall_s": wall,
+            "requests": len(finished), "rejected": len(rejected), "wall_s": wall,
             "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
             "mean_s": float(np.mean(tts)), "cell_updates_per_s": cell_updates / wall,
-            "queue_wait_p50_s": float(np.percentile([r.admitted - r.arrival for r in finished], 50)),
+            "queue_wait_p50_s": float(np.percentile([r.admitted - r.arrival for r in finished] or [float("nan")], 50)),
         }, finished

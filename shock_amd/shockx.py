@@ -168,8 +168,9 @@ class Case:
         self.g = (space_order + 1) // 2
 
     def close(self):
+        """Free the device case (refused while a batch holds it: close the batch first)."""
         if self.h:
-            lib().shx_destroy(self.h)
+            _check(lib().shx_destroy(self.h))
             self.h = None
 
     def __del__(self):

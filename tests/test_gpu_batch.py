@@ -55,6 +55,9 @@ def test_batch_equals_cases_alone_and_oracle():
                     errs = errs[:3] + errs[4:]
                 print(f"[batch] case {k} vs oracle: " + " ".join(f"{e:.1e}" for e in errs), flush=True)
                 assert max(errs) <= 1e-12
+        # a batched case cannot be stepped alone (its state would leave the batch's buffers)
+        with pytest.raises(shockx.ShxError, match="in a batch"):
+            made[0][1].run(taus[0], 1)
         # a second run continues from the batch's own state
         b.set_tau([t * 0.5 for t in taus])
         b.run(1)

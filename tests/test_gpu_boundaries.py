@@ -10,11 +10,12 @@ values within an ulp), and the residual of a perturbed uniform stream is a small
 large fluxes.  Recomputing p and c in the oracle itself moves its residual by up to 4e-13 for
 these states.
 
-Every bar is also held against the oracle's own conditioning: the oracle is rerun on inputs
-perturbed by one ulp of random noise, and the GPU may differ by up to three times that spread
-where it exceeds the fixed bar.  WENO9 with the reference's epsilon (10 sqrt(DBL_MIN)) is the
-case where it does: one ulp of input noise moves the oracle's one-step state of a supersonic
-3D stream by 1.5e-12."""
+Every test also measures the oracle's own conditioning: the oracle is rerun on inputs perturbed
+by one ulp of random noise and that spread is printed beside the GPU's difference.  The stage
+residual may differ by up to three times that spread where it exceeds 1e-11; stepped states are
+held to the fixed bars with no allowance (cases whose spread reaches the bar would be listed in
+SPREAD_EXCEPTIONS with their own bar; WENO9 with the reference's epsilon, 10 sqrt(DBL_MIN), has a
+discontinuous weight map and is where that can happen)."""
 import numpy as np
 import pytest
 
@@ -53,9 +54,20 @@ def _noise(o, dims, n, met, prim, steps, tau=0.0, src=None, bstate=None, trials=
     return worst
 
 
-def _bar(errs, fixed, noise, dims):
+# Stepped states are held to the fixed bar (1e-12 after one step, 1e-11 after ten) with no
+# allowance, except for the cases listed here with their own bar: those where the oracle's own
+# one-ulp spread (printed by every test) comes within reach of the fixed bar.  Key: the test tag.
+SPREAD_EXCEPTIONS = {}
+
+
+def _bar(errs, fixed, noise, dims, steps=0, tag=""):
+    """steps == 0 (the stage residual, a small difference of large fluxes): the fixed bar or three
+    times the oracle's one-ulp spread; steps >= 1: the fixed bar (or the listed exception)."""
     keep = [0, 1, 2, 4] if dims == 2 else [0, 1, 2, 3, 4]
-    return all(errs[i] <= max(fixed, 3.0 * noise[i]) for i in keep)
+    if steps == 0:
+        return all(errs[i] <= max(fixed, 3.0 * noise[i]) for i in keep)
+    bar = SPREAD_EXCEPTIONS.get((tag, steps), fixed)
+    return all(errs[i] <= bar for i in keep)
 
 
 def _report(tag, errs):
@@ -167,7 +179,7 @@ def test_inflow_outflow_rhs_and_steps(dims, S, n, bc, inviscid, quirks, mp):
             what = "rhs" if steps == 0 else f"{steps} step(s)"
             _report(f"{what} {tag}", [e for i, e in enumerate(errs) if dims == 3 or i != 3])
             _report(f"{what} oracle 1-ulp spread {tag}", [e for i, e in enumerate(noise) if dims == 3 or i != 3])
-            assert _bar(errs, fixed, noise, dims), what
+            assert _bar(errs, fixed, noise, dims, steps, tag), what
 
 
 @pytest.mark.parametrize("dims,S,inviscid,quirks", [(2, 5, False, orc.QUIRKS_REFERENCE),
@@ -219,7 +231,10 @@ def test_manufactured_solution_path(dims, S, inviscid, quirks):
         e_got = g.mms_error()
         e_want = o.mms_error(ref, o.last_p, ex[0], ex[5])
         print(f"[parity] MMS error {tag}: GPU {e_got} oracle {e_want}", flush=True)
-        assert np.allclose(e_got, e_want, rtol=1e-6, atol=0)
+        # the norms are of differences from the exact field, so the states' 20-step bar (1e-11
+        # relative) carries over as an absolute bar on the scale of rho and p
+        scale = [float(np.max(np.abs(ex[0][sl])))] * 2 + [float(np.max(np.abs(ex[5][sl])))] * 2
+        assert all(abs(a - b) <= 1e-11 * s for a, b, s in zip(e_got, e_want, scale)), (e_got, e_want)
 
 
 def test_create_rejects_unpaired_periodic_faces():

@@ -218,3 +218,55 @@ def test_cfl_and_nan():
     ref_bad = [a.copy() for a in bad]
     assert cnt == o.check_nan(ref_bad) == 2
     assert fixed[0][o.g + 2, o.g + 3, o.g + 4] == 1e6 and fixed[1][o.g + 5, o.g + 1, o.g + 7] == 1e6
+
+
+def _smooth_periodic_state(o, n, seed, mach):
+    """A smooth periodic field on a non-cubic block: three integer-wave-number modes per variable
+    with seeded phases (SHOCK units, c = 1/mach)."""
+    rng = np.random.default_rng(seed)
+    g = o.g
+    I, J, K = np.meshgrid(*[np.arange(n[d]) / n[d] for d in range(3)], indexing="ij")
+    tp = 2.0 * np.pi
+
+    def field(amp):
+        f = np.zeros(I.shape)
+        for _ in range(3):
+            k = rng.integers(1, 4, size=3)
+            f += amp / 3.0 * np.sin(tp * (k[0] * I + k[1] * J + k[2] * K) + rng.uniform(0, tp))
+        return f
+    prim = o.zeros(5)
+    sl = interior(o, 3, n)
+    prim[0][...] = 1.0
+    prim[0][sl] += field(0.1)
+    for m in (1, 2, 3):
+        prim[m][sl] = field(0.4 / mach)
+    p = np.ones(o.shape)
+    p[sl] += field(0.1)
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], p, prim[0], o.c.gamma, mach)
+    o.ghost_periodic(prim)
+    return prim
+
+
+@pytest.mark.parametrize("S,quirks", [(5, orc.QUIRKS_REFERENCE), (5, 0), (9, orc.QUIRKS_REFERENCE), (9, 0)])
+def test_one_step_across_march_chunks_and_columns(S, quirks):
+    """40 x 12 x 64 block: the derivative march splits i into two chunks of planes and k into two
+    columns, so the chunk-seam window priming, the cross-column halo and (W5) the prefetch at a
+    seam all enter the step; with the reference's diagonal metric and with the full metric (T
+    derivatives marched too)."""
+    nn = (40, 12, 64)
+    o = orc.Oracle(dims=3, S=S, n=nn, quirks=quirks, mach=0.5, reynolds=2000.0)
+    xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    prim = _smooth_periodic_state(o, nn, 31 + S, 0.5)
+    tau = o.cfl_dt(0.5, met, prim)
+    ref = o.run(tau, met, prim, 1)
+    with _case(o, 3, S, 4, nn, quirks=quirks) as g:
+        g.build_metric(xe, ye, ze)
+        g.upload_state(prim)
+        g.calc_runge_kutta(tau)
+        out = g.download_state()
+    sl = interior(o, 3, nn)
+    errs = rel_linf(conserved(out), conserved(ref), sl)
+    _report(f"step3d 40x12x64 S={S} quirks={quirks}", errs)
+    assert max(errs) <= TOL_STEP

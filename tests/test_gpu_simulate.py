@@ -82,6 +82,34 @@ def test_unstable_tau_recovers_with_rudy_outflow():
     _compare(o, n, 2, want_state, want, got_state, got, "Rudy outflow 2D")
 
 
+def test_restart_with_nan_and_probes_records_no_row_before_the_start():
+    # start at iteration 5: every rollback returns to iteration 1, before the first history row;
+    # the loop must record no row for it (and touch nothing outside the history)
+    o, coords, met, prim, nn = setup_2d_mms(n=16, S=5)
+    probes = [int(np.ravel_multi_index((o.g + 3, o.g + 7, 0), o.shape))]
+    bad = [a.copy() for a in prim]
+    bad[0][o.g + 2, o.g + 2, 0] = np.nan
+    s = _sim(o.cfl_dt(0.5, met, prim), start=5, end=12, number_resets=3, probes=probes)
+    want_state, want = o.simulate(met, bad, s)
+    c = o.c
+    with shockx.Case(dims=2, space_order=5, time_order=4, n=nn, gamma=c.gamma, mach=c.mach,
+                     reynolds=c.reynolds, prandtl=c.prandtl, T0=c.T0) as g:
+        g.upload_metric(met)
+        g.upload_state(bad)
+        got = g.simulate(s["start"], s["end"], s["tau"], accelerator=True, accel_factor=s["accel_factor"],
+                         decel_factor=s["decel_factor"], number_resets=3, distance_nan=s["distance_nan"],
+                         distance_forward=s["distance_forward"], distance_backward=s["distance_backward"],
+                         probes=probes)
+        t_got, h_got = g.history(1, s["end"] - s["start"])
+    print(f"[simulate] restart with NaN: oracle {want} GPU {got}", flush=True)
+    for k in KEYS:
+        assert got[k] == want[k], k
+    t_want, h_want = want["history"]
+    assert np.array_equal(t_got, t_want) and np.all(t_got == 0.0)
+    for m in range(4):
+        assert np.array_equal(h_got[m], h_want[m][:, :])
+
+
 def test_nan_state_stops_after_the_resets():
     o, coords, met, prim, nn = setup_2d_mms(n=16, S=5)
     bad = [a.copy() for a in prim]

@@ -113,3 +113,32 @@ def test_c_driver_compiles_against_the_header(tmp_path):
     subprocess.run(["gcc", "-c", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                     build.DRIVER_SRC, "-o", str(out)], check=True)
     assert out.exists()
+
+
+def _compat_case(path, faces, n=(8, 8, 1), dims=2):
+    """A compat_run case file (shock_amd/compat/compat_run.c) with zero metric and state: only the
+    configuration is read before the checks under test end the run."""
+    ng = [n[0] + 6, n[1] + 6, 1 if dims == 2 else n[2] + 6]
+    hdr = np.array([dims, 5, n[0], n[1], n[2], 4, 0, 1] + list(faces), dtype=np.int32)
+    par = np.array([1.4, 0.5, 1000.0, 0.72, 300.0, 1e-4, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0])
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(par.tobytes())
+        f.write(np.zeros(15 * int(np.prod(ng))).tobytes())
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "shock_amd", "compat", "compat_run")),
+                    reason="compat_run not built")
+@pytest.mark.parametrize("faces,donor,msg", [
+    ((0, 0, 0, 0, 1, 1), "1", "interface to rank 1"),        # a neighbour that is another rank
+    ((0, 2, 0, 0, 1, 1), None, "face right is not"),         # one face of the pair is a boundary
+])
+def test_compat_rejects_interfaces_that_are_not_periodic_self_interfaces(tmp_path, faces, donor, msg):
+    # the configuration check runs before any device call, so this needs no GPU
+    from shock_amd import build
+    case = tmp_path / "case.bin"
+    _compat_case(case, faces)
+    cmd = [build.COMPAT_RUN, str(case), str(tmp_path / "out.bin")] + ([donor] if donor else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4242 % 256, (r.returncode, r.stdout, r.stderr)
+    assert msg in r.stdout

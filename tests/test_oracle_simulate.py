@@ -102,3 +102,25 @@ def test_probe_history_records_each_iteration():
             for m in range(3):
                 assert hist[1 + m, q, k] == ref[1 + m][idx]
     assert np.all(np.diff(t) > 0) and t[-1] == rep["time"]
+
+
+def test_restart_history_skips_iterations_before_the_start():
+    # a restart at iteration 5 with a NaN in the state: every rollback goes to the first backup,
+    # iteration 1, below the first history row (the reference writes such rows out of bounds);
+    # none is recorded, and a restart without NaN fills rows 0 .. end - start - 1
+    o, coords, met, prim, nn = setup_2d_mms(n=16, S=5)
+    tau = o.cfl_dt(0.5, met, prim)
+    probes = [int(np.ravel_multi_index((o.g + 3, o.g + 7, 0), o.shape))]
+    bad = [a.copy() for a in prim]
+    bad[0][o.g + 2, o.g + 2, 0] = np.nan
+    st, rep = o.simulate(met, bad, _sim(start=5, end=12, tau=tau, number_resets=3, probes=probes))
+    assert rep["aborted"] == 1 and rep["resets"] == 3 and rep["iteration"] == 2
+    t, hist = rep["history"]
+    assert t.shape == (7,) and np.all(t == 0.0) and np.all(hist == 0.0)
+    st, rep = o.simulate(met, prim, _sim(start=5, end=9, tau=tau, probes=probes))
+    t, hist = rep["history"]
+    idx = np.unravel_index(probes[0], o.shape)
+    for k in range(4):
+        ref = o.run(tau, met, prim, k + 1)
+        assert hist[1, 0, k] == ref[1][idx]
+    assert np.all(t > 0)

@@ -56,3 +56,21 @@ def test_every_request_runs_its_steps_within_budget():
     assert FakeBatch.resident_peak <= 4
     assert sorted(rid for _, rid in log) == list(range(12))
     assert stats["p99_s"] >= stats["p50_s"] >= 0.0
+
+
+def test_request_larger_than_the_budget_is_refused_not_queued():
+    # a request that cannot fit an empty device is refused on arrival; the others still run
+    reqs = serve.make_requests(5, rate=500.0, seed=4, n_lo=16, n_hi=24)
+    reqs[2].n = 400
+    log = []
+
+    def open_fn(r):
+        r.steps = 3
+        r.tau = r.t_end / r.steps
+        return FakeCase(r, log)
+
+    srv = serve.Server(budget_bytes=2 * 24 ** 3, max_slots=4, chunk=2, keep_results=True, open_fn=open_fn,
+                       batch_cls=FakeBatch, footprint_fn=lambda n, S: n ** 3)
+    stats, done = srv.run(reqs)
+    assert stats["requests"] == 4 and stats["rejected"] == 1
+    assert sorted(r.rid for r in done) == [0, 1, 3, 4]
