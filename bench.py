@@ -4,13 +4,17 @@
 Workload (the headline configuration): 3D single block 256^3 on a periodic curvilinear grid
 x = xi + 0.05 sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta), WENO5 + viscous fluxes, FP64,
 classic RK4, Re = 2000, Mach 0.5, decaying turbulence from a seeded random-phase velocity
-field (E(k) ~ k^4 exp(-2 (k/4)^2), turbulent Mach 0.5, uniform rho and p), fixed tau from
-CFL 0.5 on the initial state.  A "step" is one full RK4 time step (4 stages); the metric is
-interior cell-updates per second per RK stage with the state resident in HBM.
+field (E(k) ~ k^4 exp(-2 (k/4)^2), turbulent Mach 0.5, uniform rho and p), time step from
+CFL 0.5 on each step's starting state.  A "step" is one full RK4 time step (4 stages) followed
+by checkNAN; checkNAN and the CFL maximum of the new state are fused into the last stage's
+update.  The metric is interior cell-updates per second per RK stage with the state resident in
+HBM.
 
-Prints one JSON line.  Per-kernel durations come from HIP events recorded on the library's
-stream during the timed region; the CPU baseline is the C oracle (OpenMP) timed on a bounded
-sample of the same workload.
+Prints one JSON line: the FP64 form, a `mixed` object with the mixed FP32/FP64 form on the same
+workload, the CPU baseline (the C oracle with OpenMP on a bounded sample of the same workload,
+on the host's CPU share) and `config0`, the 2D 64^2 CPU-runnable configuration on the GPU and on
+one host core.  Per-kernel durations come from HIP events recorded on the library's stream
+during the timed region.
 """
 import argparse
 import json
@@ -99,10 +103,16 @@ def main():
     ap.add_argument("--seed", type=int, default=20251202)
     ap.add_argument("--cpu-n", type=int, default=128, help="CPU baseline sample size per direction")
     ap.add_argument("--cpu-steps", type=int, default=4)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, available cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the host's CPU share (affinity set, at most 16)")
     ap.add_argument("--cpu1-n", type=int, default=64, help="one-core CPU sample size per direction (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    ap.add_argument("--cfl", type=float, default=0.5, help="CFL number of the time step")
+    ap.add_argument("--fixed-tau", action="store_true",
+                    help="time the steps at the first step's tau (the reference's fixed tau) instead of a CFL "
+                         "step per step")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the default workload")
+    ap.add_argument("--no-config0", action="store_true", help="skip the 2D 64^2 CPU-runnable configuration")
     ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
                     default="turbulence",
                     help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
@@ -139,39 +149,137 @@ def main():
     if args.workload == "shock":
         return shock_workload(args)
 
+    fp64 = turbulence_run(args, "fp64" if args.precision == "fp64" else "mixed")
+    out = fp64["line"]
+    if args.precision == "fp64" and not args.no_mixed:
+        # the mixed FP32/FP64 form on the same workload, with its own roofline object
+        mixed = turbulence_run(args, "mixed")
+        m = mixed["line"]
+        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"], "dtype": m["dtype"],
+                        "speedup_vs_fp64": round(m["value"] / out["value"], 4),
+                        "roofline": m["roofline"], "kernels": m["kernels"],
+                        "time_advanced": m["config"]["time_advanced"]}
+    if not args.no_cpu_baseline:
+        out["host"] = host_info()
+        threads = args.cpu_threads or out["host"]["cpu_share"]
+        rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps, args.seed, threads)
+        out["cpu_baseline"] = {
+            "value": round(rate, 1), "unit": "cell-updates/s/RK-stage", "cores": threads, "kind": "port",
+            "sample": f"C oracle (OpenMP, {threads} threads: the host's CPU share), {args.cpu_n}^3 block of the "
+                      f"same workload (WENO5+viscous, FP64, RK4, fixed tau), {args.cpu_steps} steps = "
+                      f"{4 * args.cpu_steps} stages, {secs:.1f}s"}
+        if args.cpu1_n:
+            rate1, secs1 = cpu_baseline(args.cpu1_n, 1, args.seed, 1)
+            out["cpu_baseline_1core"] = {
+                "value": round(rate1, 1), "unit": "cell-updates/s/RK-stage", "cores": 1, "kind": "port",
+                "sample": f"C oracle, one thread, {args.cpu1_n}^3 block, 1 step = 4 stages, {secs1:.1f}s"}
+        if not args.no_config0:
+            out["config0"] = config0_run(args)
+    print(json.dumps(out), flush=True)
+
+
+def host_info():
+    """Host cores as the bench sees them: nproc, the threads this process may run on, the CPU
+    model, and the share the CPU baseline uses (the affinity set, at most 16 threads: the GPU
+    host grants each GPU a 16-thread share whatever nproc reports)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "affinity": aff, "model": model, "cpu_share": min(16, aff)}
+
+
+def config0_run(args):
+    """The CPU-runnable configuration: 2D 64x64 periodic unit square with the manufactured field,
+    WENO5 + viscous, RK4, CFL 0.5, 100 steps; the GPU path and the C oracle on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # noqa: E402  (test infrastructure: CPU baseline leg only)
+    n, S, steps = 64, 5, 100
+    nn = (n, n, 1)
+    xe, ye, ze = cases.periodic_ext_coords(2, S, nn, cases.cartesian_unit(2))
+    x, y, _ = cases.interior_coords(2, S, nn, xe, ye, ze)
+    prim = cases.mms_periodic_2d_state(np.ascontiguousarray(x), np.ascontiguousarray(y), mach=0.5)
+    with shockx.Case(dims=2, space_order=S, time_order=4, n=nn, mach=0.5, reynolds=1000.0) as g:
+        g.build_metric(xe, ye, ze)
+        g.upload_state(prim)
+        g.run_cfl(0.5, 5)
+        g.synchronize()
+        t0 = time.perf_counter()
+        t_gpu = g.run_cfl(0.5, steps)
+        g.synchronize()
+        gpu_s = time.perf_counter() - t0
+    orc.lib().orc_set_threads(1)
+    o = orc.Oracle(dims=2, S=S, n=nn, time_order=4, mach=0.5, reynolds=1000.0)
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    state = [a.copy() for a in prim]
+    o.ghost_periodic(state)
+    t0 = time.perf_counter()
+    t_cpu = 0.0
+    for _ in range(steps):
+        dt = o.cfl_dt(0.5, met, state)
+        state = o.run(dt, met, state, 1)
+        t_cpu += dt
+    cpu_s = time.perf_counter() - t0
+    cells = n * n * 4 * steps
+    return {"workload": "2D 64x64 periodic manufactured field, WENO5 + viscous, RK4, CFL 0.5, 100 steps",
+            "gpu": {"value": round(cells / gpu_s, 1), "ms_per_step": round(1e3 * gpu_s / steps, 4),
+                    "time_advanced": t_gpu},
+            "cpu_baseline": {"value": round(cells / cpu_s, 1), "unit": "cell-updates/s/RK-stage", "cores": 1,
+                             "kind": "port", "sample": f"C oracle, one thread, the whole run ({cpu_s:.2f}s; "
+                                                       f"CFL step per step, time {t_cpu:.6g})"}}
+
+
+def turbulence_run(args, precision):
+    """The 256^3 turbulence workload in one arithmetic form: setup, warm-up, timed CFL steps with
+    the fused end-of-step checks; returns the JSON line's fields."""
     n = args.n
     S, time_order = 5, 4
     nn = (n, n, n)
     t_setup = time.perf_counter()
     case = shockx.Case(dims=3, space_order=S, time_order=time_order, n=nn, gamma=1.4, mach=0.5,
                        reynolds=2000.0, prandtl=0.72, T0=300.0, quirks=shockx.QUIRKS_REFERENCE,
-                       precision=shockx.MIXED if args.precision == "mixed" else shockx.FP64)
+                       precision=shockx.MIXED if precision == "mixed" else shockx.FP64)
     xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
     case.build_metric(xe, ye, ze)
     del xe, ye, ze
     prim = cases.turbulence_3d_state(3, S, nn, args.seed, mach=0.5)
     case.upload_state(prim)
     del prim
-    tau = case.cfl_dt(0.5)
+    tau = case.cfl_dt(args.cfl)
     tiles = case.tune(3)
     t_setup = time.perf_counter() - t_setup
-    print(f"[bench] setup {t_setup:.1f}s tau={tau:.6g} n={n} weno tiles {tiles}", file=sys.stderr, flush=True)
+    print(f"[bench] {precision} setup {t_setup:.1f}s dt0={tau:.6g} n={n} weno tiles {tiles}", file=sys.stderr,
+          flush=True)
 
     if args.copy_probe:
         case.copy_probe(args.copy_probe)
-    case.run(tau, args.warmup)
+
+    def advance(k):
+        if args.fixed_tau:
+            case.run(tau, k)
+            return tau * k
+        return case.run_cfl(args.cfl, k)
+
+    advance(args.warmup)
     case.synchronize()
     if not args.no_timing:
         case.set_timing(True)
     case.synchronize()
     t0 = time.perf_counter()
-    case.run(tau, args.steps)
+    t_adv = advance(args.steps)
     case.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     kt = case.kernel_times() if not args.no_timing else {}
-    nan = case.check_nan()
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s; non-finite nodes {nan}", file=sys.stderr, flush=True)
+    print(f"[bench] {precision}: timed {args.steps} steps in {elapsed:.3f}s (checkNAN every step: no "
+          f"non-finite density; time advanced {t_adv:.6g})", file=sys.stderr, flush=True)
 
     stages = args.steps * time_order
     value = n ** 3 * stages / elapsed
@@ -195,7 +303,7 @@ def main():
         b = algo_bytes(dom, n, ng, time_order)
         ach = b / (avg * 1e-3) / 1e9
         traffic = None
-        tfile = os.path.join(ROOT, "profiles", f"traffic_turbulence{n}_{args.precision}.json")
+        tfile = os.path.join(ROOT, "profiles", f"traffic_turbulence{n}_{precision}.json")
         if os.path.exists(tfile):
             with open(tfile) as f:
                 traffic = json.load(f).get(dom)
@@ -205,7 +313,7 @@ def main():
         # FP64 vector pipe use beside the HBM fraction: FP64 VALU instructions per launch (hardware
         # counters of the same kernels, profiles/sq_counters_*.json) x 4 cycles per wave64 FP64
         # instruction over 1024 SIMDs x 2.4 GHz x the live launch time.
-        sfile = os.path.join(ROOT, "profiles", f"sq_counters_turbulence{n}_{args.precision}.json")
+        sfile = os.path.join(ROOT, "profiles", f"sq_counters_turbulence{n}_{precision}.json")
         if os.path.exists(sfile):
             with open(sfile) as f:
                 sq = json.load(f)
@@ -217,41 +325,31 @@ def main():
                 valu = {"kernel": dom, "fp64_pipe_busy": kernels[dom]["fp64_pipe_busy"],
                         "fp64_valu_instr_per_launch": sq[dom]["fp64_valu_instr_per_launch"],
                         "peak_instr_per_s": 1024 * CLOCK_HZ / 4.0}
-
-    cpu = cpu1 = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        rate, secs = cpu_baseline(args.cpu_n, args.cpu_steps, args.seed, threads)
-        cpu = {"value": round(rate, 1), "unit": "cell-updates/s/RK-stage", "cores": threads, "kind": "port",
-               "sample": f"C oracle, {args.cpu_n}^3 block of the same workload (WENO5+viscous, FP64, RK4), "
-                         f"{args.cpu_steps} steps = {4 * args.cpu_steps} stages, {secs:.1f}s"}
-        if args.cpu1_n:
-            rate1, secs1 = cpu_baseline(args.cpu1_n, 1, args.seed, 1)
-            cpu1 = {"value": round(rate1, 1), "unit": "cell-updates/s/RK-stage", "cores": 1, "kind": "port",
-                    "sample": f"C oracle, one thread, {args.cpu1_n}^3 block, 1 step = 4 stages, {secs1:.1f}s"}
+    case.close()
 
     # stage-level fraction of the HBM roofline at the compulsory 260 B/node/stage (fused RK4 stage)
     stage_frac = value * 260.0 / (HBM_PEAK_GBS * 1e9)
-    out = {
+    line = {
         "metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "dtype": "f64" if precision == "fp64" else "f64+f32",
         "data": "synthetic (seeded random-phase turbulence)",
         "config": {"workload": f"{n}^3 periodic curvilinear block, WENO5 + viscous, "
-                               f"{'FP64' if args.precision == 'fp64' else 'mixed FP32/FP64'}, RK4, Re=2000",
+                               f"{'FP64' if precision == 'fp64' else 'mixed FP32/FP64'}, RK4, Re=2000",
                    "grid": [n, n, n], "scheme": "WENO5", "viscous": True, "time_order": time_order,
-                   "precision": args.precision, "quirks": "reference", "tau": tau, "seed": args.seed},
+                   "precision": precision, "quirks": "reference",
+                   "time_step": (f"fixed tau {tau:.6g}" if args.fixed_tau
+                                 else f"CFL {args.cfl} per step (maximum fused into the last stage)"),
+                   "checks_per_step": "checkNAN and CFL maximum fused into the last stage's update",
+                   "dt_first": tau, "time_advanced": t_adv, "seed": args.seed},
         "roofline": roofline,
         "valu": valu,
         "stage_hbm_frac_260B": round(stage_frac, 4),
         "weno_tile_variants": tiles,
         "kernels": kernels,
-        "cpu_baseline": cpu,
-        "cpu_baseline_1core": cpu1,
     }
-    print(json.dumps(out), flush=True)
-    case.close()
+    return {"line": line}
 
 
 def batch_cases(count, seed, precision):

@@ -26,6 +26,8 @@ extern "C" {
 #define SHX_ERR_HIP         -2  /* HIP runtime error (message in shx_last_error) */
 #define SHX_ERR_UNSUPPORTED -3  /* configuration not implemented */
 #define SHX_ERR_STATE       -4  /* call out of order (e.g. stepping before a metric is set) */
+#define SHX_ERR_NAN         -5  /* shx_run / shx_run_cfl: checkNAN found a non-finite density after a
+                                   step; the case holds that step's state, marked as checkNAN marks it */
 
 /* Reference behaviours that can be switched off for verification runs. */
 #define SHX_QUIRK_ZETA_VISC_J   1u /* src/ZD.c:547-548: zeta viscous stencil k offset taken from j */
@@ -75,6 +77,11 @@ extern "C" {
                                        node below an upper isothermal face sees it
                                        (WriteWallNoSlipIsothermalBoundary, src/BC.c:2943-2978;
                                        src/ZD.c:193-194, 422-423, 617-618) */
+#define SHX_BC_INTERFACE         13 /* face joined to a face of another block of the same case on
+                                       this device (a zone interface of the reference's division,
+                                       src/Load.c:191-250); its ghost layers are gathered from that
+                                       block before every stage by shx_blocks_run
+                                       (TransferFlowParameterWithGhosts, src/Functions.c:3580-3684) */
 #define SHX_BC_FARFIELD          11 /* every value of the mirror node, interior tangential range;
                                        the ghost metric is mirrored as at walls
                                        (WriteFarfieldBoundary, src/BC.c:2817-2852, loops :638-803) */
@@ -152,8 +159,19 @@ int shx_upload_source(shx_case *c, const double *const src[5]);
 /* One time step with fixed tau: CalcRungeKutta (src/Functions.c:2034-2390) followed by the
  * step-end copy U_lastStep <- U_RK (WriteValuesFromU1ToU2, src/SHOCK.c:663-666). */
 int shx_calc_runge_kutta(shx_case *c, double tau);
-/* nsteps of the above (the startSimulation loop body, src/SHOCK.c:557-666, without I/O). */
+/* nsteps of the above with checkNAN after every step (the startSimulation loop body,
+ * src/SHOCK.c:557-666, without I/O; checkNAN src/Functions.c:5419-5467).  The check is fused into
+ * the last stage's update: non-finite interior densities are counted and those nodes get
+ * rho = u = v = w (and the stored p) = 1e6.  The first step that finds one ends the run with
+ * SHX_ERR_NAN (the reference's rollback is shx_simulate's).  The stream is synchronized after
+ * every step, as the reference's loop reads its flag every step. */
 int shx_run(shx_case *c, double tau, int nsteps);
+/* As shx_run with the time step of every step from the CFL condition on the state it starts from
+ * (shx_cfl_dt's formula, new: the reference uses a fixed tau): the first from the current state,
+ * later ones from the maximum the previous step's last stage reduced while writing the state, so
+ * no separate reduction pass runs between steps.  *time (may be NULL) receives the sum of the
+ * steps taken.  Not for cases with Rudy outflow faces (SHX_ERR_UNSUPPORTED). */
+int shx_run_cfl(shx_case *c, double cfl, int nsteps, double *time);
 
 /* startSimulation's iteration loop with its NaN rollback and time-step control
  * (src/SHOCK.c:531-691, without sampling and I/O).  Per iteration: time += tau L0/u0, one step,
@@ -255,10 +273,43 @@ int shx_copy_probe(shx_case *c, int reps);
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled, every launch is
  * bracketed by events; shx_kernel_times synchronizes and returns, per kernel class, the total
  * milliseconds and the launch count since the last reset.  names[k] points to static strings. */
-#define SHX_NUM_KERNELS 14
+#define SHX_NUM_KERNELS 15
 int shx_set_timing(shx_case *c, int enable);
 int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
                      int launches[SHX_NUM_KERNELS]);
+
+/* Multi-block cases on one device: the reference's division of a case into zones, one MPI rank
+ * per zone with a halo transfer per stage (src/Load.c:191-250, src/Functions.c:3580-3684), here with
+ * every zone a block (a shx_case) on the same GPU and the halo an on-device gather.  A connection
+ * joins face `face` of block `block` to face `donor_face` of block `donor` with the CGNS transform
+ * of the interface (TransformMatrixOfInterface, src/SHOCK.h:285; src/Functions.c:3902-4287): index
+ * direction d of the block runs along the donor's direction |transform[d]| - 1, reversed where it
+ * is negative; across the faces the layers count outwards on one side and inwards on the other
+ * (the reference derives that sign from the point ranges, src/Functions.c:3902-3990).  Faces join
+ * whole (tangential extents equal under the transform); both sides of a connection are listed; a
+ * block may join itself (a periodic pair with a transform).  Velocities are Cartesian, so the
+ * transfer moves values without rotating them (src/Functions.c:2809-2863); the ghost metric takes
+ * the donor's rows permuted and signed by the transform (src/Functions.c:2640-2690).  Ghost nodes
+ * beyond two or three faces (edges, corners) get the value the reference's repeated transfers
+ * settle to: the composition of the connections and periodic wraps of the faces they lie beyond.
+ * Limits: at most SHX_BLOCKS_MAX blocks; all blocks share dims, scheme, RK order and viscosity. */
+#define SHX_BLOCKS_MAX 8
+typedef struct shx_interface {
+    int block, face;        /* receiving block (index into the block array) and its face 0..5 */
+    int donor, donor_face;  /* the block across the face and its face */
+    int transform[3];       /* transform[2] ignored in 2D */
+} shx_interface;
+typedef struct shx_blocks shx_blocks;
+int shx_blocks_create(shx_case *const *blocks, int nblocks, const shx_interface *ifc, int nifc,
+                      shx_blocks **out);
+/* ghost metric of the interface faces from the donors (TransferMeshParameter,
+ * src/Functions.c:3481-3578): after every block's shx_build_metric or shx_upload_metric */
+int shx_blocks_transfer_metric(shx_blocks *g);
+/* nsteps steps of fixed tau: per stage the interface halo of every block, then each block's stage
+ * (blocks on their own streams, ordered by events); checkNAN over every block after each step
+ * (SHX_ERR_NAN as shx_run).  While a block belongs to a group it cannot be stepped alone. */
+int shx_blocks_run(shx_blocks *g, double tau, int nsteps);
+int shx_blocks_destroy(shx_blocks *g);
 
 /* Batches (no reference counterpart: SHOCK runs one case per job).  A batch advances many
  * independent cases, each created, uploaded and owned as above (any face kinds), with one launch

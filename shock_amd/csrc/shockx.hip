@@ -790,9 +790,78 @@ struct RkArgs {
     Fields5 out, qsum;
     double a, b, qf, ff, mf, tau;
     int first_stage, has_src, three;
+    // end of step (the last stage of shx_run, shx_run_cfl and shx_simulate): checkNAN on the new
+    // state and the CFL maximum over it, fused into the update.  nan_cnt: running count of
+    // non-finite interior densities (those nodes get rho = u = v = w (= stored p) = 1e6);
+    // cfl_red: the maximum of the node CFL sum, as the bits of a positive double
+    int *nan_cnt;
+    gdp nanP;
+    unsigned long long *cfl_red;
+    double gamma, upsilon;
 };
 
-__device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const double J, double qv[5])
+// Reductions over a wavefront (all 64 lanes active).
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Maxima of non-negative doubles reduce into RED_SLOTS slots (positive doubles order like their
+// bit patterns, so an unsigned 64-bit atomic max combines them); a block uses slot
+// blockIdx % RED_SLOTS, and the host takes the maximum of the slots.  One slot for a whole
+// launch would serialise every wave's atomic on one cache line.
+constexpr int RED_SLOTS = 256;
+__device__ __forceinline__ void wave_max_to(unsigned long long *out, double v)
+{
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v > 0.0) {
+        unsigned long long *slot = out + (blockIdx.x & (RED_SLOTS - 1));
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+        if (bits > *slot) atomicMax(slot, bits);     // a stale read only costs an extra atomic
+    }
+}
+
+// CFL sum of one node, sum_d |theta_d| + c |grad xi_d| (the per-node spectral radius of
+// src/WENO.c:1560-1615 with unit index spacing); m: the node's metric rows xi_x .. zeta_z.
+// Shared by k_cfl_max and the fused end-of-step check, which therefore give the same bits.
+// DIAG (the off-diagonal entries are zero): |grad xi_d| = |m_dd|, which is what the general form
+// gives, since u m_dd + v 0 + w 0 = u m_dd and sqrt(m_dd^2 + 0 + 0) = |m_dd| exactly (a correctly
+// rounded square root of a rounded square is the operand's magnitude).
+template <bool DIAG = false>
+__device__ __forceinline__ double node_cfl_sum(const double gamma, const double upsilon, const int dims,
+                                               const double rho, const double u, const double v, const double w,
+                                               const double e, const double m[9])
+{
+    const double p = fabs(rho * (gamma - 1) / upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
+    const double cs = sqrt(upsilon * gamma * p / rho);
+    double s = 0.0;
+    if constexpr (DIAG) {
+        const double uu[3] = {u, v, w};
+        for (int d = 0; d < dims; d++) {
+            const double a = m[4 * d];
+            s += fabs(uu[d] * a) + cs * fabs(a);
+        }
+    } else {
+        for (int d = 0; d < dims; d++) {
+            const double a = m[3 * d], b = m[3 * d + 1], z = m[3 * d + 2];
+            s += fabs(u * a + v * b + w * z) + cs * sqrt(a * a + b * b + z * z);
+        }
+    }
+    return s;
+}
+
+// m9, cfl: the node's metric rows and where its CFL sum goes (read only when A.cfl_red is set);
+// DIAG: the metric is diagonal (node_cfl_sum)
+template <bool DIAG = false>
+__device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const double J, double qv[5],
+                                         const double *m9 = nullptr, double *cfl = nullptr)
 {
     if (A.has_src) {
         qv[0] += J * A.src.f[0][n];
@@ -814,12 +883,23 @@ __device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const do
     const double rho_s = (A.b != 0.0) ? A.us.f[0][n] : 0.0;
     const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
     A.out.f[0][n] = rho;
+    double nv[5];
+    nv[0] = rho;
 #pragma unroll
     for (int m = 1; m < 5; m++) {
-        if (m == 3 && !A.three) { A.out.f[3][n] = 0.0; continue; }
+        if (m == 3 && !A.three) { A.out.f[3][n] = 0.0; nv[3] = 0.0; continue; }
         const double us = (A.b != 0.0) ? A.us.f[m][n] : 0.0;
-        A.out.f[m][n] = (A.a * A.un.f[m][n] * rho_n + A.b * us * rho_s
-                         + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
+        nv[m] = (A.a * A.un.f[m][n] * rho_n + A.b * us * rho_s
+                 + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
+        A.out.f[m][n] = nv[m];
+    }
+    if (A.nan_cnt && !isfinite(rho)) {
+        // checkNAN (src/Functions.c:5419-5467) on the step's U_RK
+        A.out.f[0][n] = 1000000; A.out.f[1][n] = 1000000; A.out.f[2][n] = 1000000; A.out.f[3][n] = 1000000;
+        if (A.nanP) A.nanP[n] = 1000000;
+        atomicAdd(A.nan_cnt, 1);
+    } else if (A.cfl_red && cfl) {
+        *cfl = node_cfl_sum<DIAG>(A.gamma, A.upsilon, A.three ? 3 : 2, nv[0], nv[1], nv[2], nv[3], nv[4], m9);
     }
 }
 
@@ -830,6 +910,7 @@ struct ViscArgs {
     Fields5 q;
     long sD, sA, sB;
     int nD, nA, nB, oD, oA, oB, ngD;
+    int ngA;                      // padded extent of the A axis
     int coalesce_d;
     int alignA;
     int three;
@@ -1049,7 +1130,20 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
             const int rem = idx - ln * NP * NLV;
             const int pos = rem / NLV, f = rem - pos * NLV;
             const int jpad = max(a.oA, min(pa0 + ln, a.oA + a.nA - 1));   // A axis is j for DIR = 2
-            const long n = bbase + (long)jpad * a.sA + (jpad - G + pos);
+            // the reference's index (i ng_j + j) ng_k + k' with k' = j - g + pos in its unpadded
+            // k-fastest layout: where k' passes the row end (ng_j > ng_k) it reads the next rows
+            // (or the next plane's first ones), which the padded layout finds at (j', k'') below
+            const int kq = jpad - G + pos;
+            long n;
+            if (kq < a.ngD) {
+                n = bbase + (long)jpad * a.sA + kq;
+            } else {
+                int jr = jpad + kq / a.ngD;
+                const int kr = kq % a.ngD;
+                const int ia = jr / a.ngA;
+                jr -= ia * a.ngA;
+                n = bbase + (long)ia * a.sB + (long)jr * a.sA + kr;
+            }
             double val;
             if (f == 0) val = a.u[n];
             else if (f == 1) val = a.v[n];
@@ -1140,6 +1234,7 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
         for (int m = 0; m < 4; m++) nxt[m] = (di < TDI - 1) ? fo[m * NT + t + 1] : 0.0;
     }
 
+    double cflv = 0.0;                           // CFL sum of this thread's node (end-of-step check)
     if (di < TDI - 1) {
         const int dn = d0 + di, pa = pa0 + ta;
         if (dn < a.nD && pa >= a.oA && pa < a.oA + a.nA) {
@@ -1193,7 +1288,21 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
                 const double gh = (a.iso_hi && dn == a.nD - 1) ? a.gam_iso : k.gam_heat;
                 qv[4] = q0[4] + (gh * dF[3] + k.psi * (work + prod));
                 const double J = DIAG ? sm[(VS_M + 3) * NW + p] : sm[(VS_M + 9) * NW + p];
-                rk_point(a.rk, n, J, qv);
+                if (a.rk.cfl_red) {
+                    // end of step: the node's metric rows for its CFL sum
+                    double m9[9];
+                    if constexpr (DIAG) {
+#pragma unroll
+                        for (int c = 0; c < 9; c++) m9[c] = 0.0;
+                        m9[0] = sm[(VS_M + 0) * NW + p]; m9[4] = sm[(VS_M + 1) * NW + p]; m9[8] = sm[(VS_M + 2) * NW + p];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 9; c++) m9[c] = sm[(VS_M + c) * NW + p];
+                    }
+                    rk_point<DIAG>(a.rk, n, J, qv, m9, &cflv);
+                } else {
+                    rk_point(a.rk, n, J, qv);
+                }
             } else {
                 if (ADD) a.q.f[0][n] = q0[0];
                 a.q.f[1][n] = q0[1] + k.psi * dF[0];
@@ -1205,6 +1314,7 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
             }
         }
     }
+    if (a.rk.cfl_red) wave_max_to(a.rk.cfl_red, cflv);
 }
 template <int S, int DIR, int TA, int TDI, bool DIAG>
 __global__ __launch_bounds__(TA * TDI) void k_visc_sweep(ViscArgs a)
@@ -1252,6 +1362,7 @@ __global__ __launch_bounds__(256, ((S == 5 && !MIX) ? 4 : 1)) void k_zeta(ZetaAr
 struct UpdArgs {
     Fields5 q;
     gcdp J;
+    CFields10 met;        // metric rows, read for the CFL sum of the end-of-step check
     Geom gm;
     RkArgs rk;
 };
@@ -1260,16 +1371,22 @@ __device__ __forceinline__ void rk_body(const UpdArgs &A, const long t)
 {
     const Geom &gm = A.gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    if (t >= nint) return;
-    const int kk = (int)(t % gm.n[2]);
-    const long r = t / gm.n[2];
-    const int jj = (int)(r % gm.n[1]);
-    const int ii = (int)(r / gm.n[1]);
-    const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
-    double qv[5];
+    double cfl = 0.0;
+    if (t < nint) {
+        const int kk = (int)(t % gm.n[2]);
+        const long r = t / gm.n[2];
+        const int jj = (int)(r % gm.n[1]);
+        const int ii = (int)(r / gm.n[1]);
+        const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
+        double qv[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) qv[m] = A.q.f[m][n];
-    rk_point(A.rk, n, A.J[n], qv);
+        for (int m = 0; m < 5; m++) qv[m] = A.q.f[m][n];
+        double m9[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) m9[c] = A.rk.cfl_red ? A.met.f[c][n] : 0.0;
+        rk_point(A.rk, n, A.J[n], qv, m9, &cfl);
+    }
+    if (A.rk.cfl_red) wave_max_to(A.rk.cfl_red, cfl);
 }
 __global__ __launch_bounds__(256) void k_rk_update(UpdArgs A)
 {
@@ -1317,6 +1434,14 @@ __global__ void k_visc_grad(CFields5 pr, const double *T, Fields12 gr, Geom gm, 
 // into LDS with a g-wide halo; the j and k derivatives come from there, the i derivative from a
 // register window of 2g+1 plane centres per thread.  Outputs as k_visc_grad; the T
 // derivatives only when `tgrad` (the diagonal-metric viscous sweeps never read them).
+//
+// VX (3D diagonal metric): the viscous xi direction rides along (CalcViscidFluxesInXiDirection-
+// Directly, src/ZD.c:47-260).  The march already holds, per column, the u, v, w window along i and
+// each plane's eta and zeta derivatives; with T, mu and the metric of the column read as the planes
+// pass, face i+1/2 follows at plane i+g from rings of the last 2g planes, and node i adds its xi
+// viscous terms to Q.  The face and node arithmetic is the xi sweep's (k_visc_sweep<..,0,..>) term
+// for term, so Q gets the same bits; u_eta and u_zeta, which only the xi direction reads, are not
+// stored.
 constexpr int GK = 64, GJ = 16;
 struct GradArgs {
     CFields5 pr;
@@ -1326,8 +1451,13 @@ struct GradArgs {
     Consts k;
     int ci, tgrad;
     int tg[4];                    // tile grid (k columns, j columns, i chunks) of a single-case launch
+    // VX: Q (after the WENO sweeps), mu, the diagonal metric and J; the isothermal upper xi face
+    Fields5 q;
+    gcdp mu, m0, m4, m8, J;
+    int iso_hi;
+    double gam_iso;
 };
-template <int S, bool TG>
+template <int S, bool TG, bool VX = false, int GJ = ::GJ>
 __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
 {
     const CFields5 &pr = a.pr;
@@ -1349,6 +1479,8 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     const long col = (long)jj * gm.st[1] + kk;
     constexpr int NT = GK * GJ, NST = (LP + NT - 1) / NT;   // staged plane elements per thread
     const int c0 = (ty + G) * LK + (tx + G);                  // this thread's node in a staged plane
+    const bool jin = mine && jj >= gm.lo[1] && jj < gm.lo[1] + gm.n[1];
+    const bool kin = mine && kk >= gm.lo[2] && kk < gm.lo[2] + gm.n[2];
     // Planes i0-G .. i1+G-1 pass through LDS once each (with their j/k halo): plane p gives its
     // j and k derivatives when p lies in the chunk, and its centre value enters the register
     // window of 2g+1 planes, from which the i derivative of plane p-G follows.  Planes outside
@@ -1373,6 +1505,23 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
     for (int l = 0; l < NL; l++)
 #pragma unroll
         for (int f = 0; f < NF; f++) win[f][l] = 0.0;
+    // VX rings over the last 2g planes (index 2G-1 = plane p): T and the cross derivatives
+    // v_eta, u_eta, w_zeta, u_zeta (the xi sweep's cross slots 0..3); mu and metric of planes
+    // i and i+1 of the face; the previous face's flux
+    constexpr int NR = VX ? 2 * G : 1;
+    double tw[NR], cw[4][NR];
+    double mv0[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, mv1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // mu, m0, m4, m8, J
+    double fprev[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (VX) {
+#pragma unroll
+        for (int l = 0; l < NR; l++) {
+            tw[l] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) cw[c][l] = 0.0;
+        }
+    }
+    const bool vcol = VX && jin && kin;
+    const int ilo = gm.lo[0], ihi = gm.lo[0] + gm.n[0];
     // the next plane is prefetched where the registers allow it (WENO5 order, no T derivatives)
     constexpr bool PF = (S == 5) && !TG;
     const int pend = i1 + G;
@@ -1409,37 +1558,117 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
 #pragma unroll
         for (int f = 0; f < NF; f++) win[f][NL - 1] = sm[f * LP + c0];
         if (mine) {
-            if (p >= i0 && p < i1) {
-                const long n = (long)p * gm.st[0] + col;
-                if (jj >= gm.lo[1] && jj < gm.lo[1] + gm.n[1]) {
-                    double d[NF] = {};
-#pragma unroll
-                    for (int l = 0; l < NL; l++)
-#pragma unroll
-                        for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G) * LK];
-                    gr.f[1][n] = d[0]; gr.f[4][n] = d[1]; gr.f[7][n] = d[2];
-                    if constexpr (TG) gr.f[10][n] = d[3];
-                }
-                if (kk >= gm.lo[2] && kk < gm.lo[2] + gm.n[2]) {
-                    double d[NF] = {};
-#pragma unroll
-                    for (int l = 0; l < NL; l++)
-#pragma unroll
-                        for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G)];
-                    gr.f[2][n] = d[0]; gr.f[5][n] = d[1]; gr.f[8][n] = d[2];
-                    if constexpr (TG) gr.f[11][n] = d[3];
-                }
-            }
-            const int i = p - G;
-            if (i >= i0 && i >= gm.lo[0] && i < gm.lo[0] + gm.n[0]) {
-                const long n = (long)i * gm.st[0] + col;
+            const bool chunk = (p >= i0 && p < i1);
+            const long n = (long)p * gm.st[0] + col;
+            if (jin && (chunk || vcol)) {
                 double d[NF] = {};
 #pragma unroll
                 for (int l = 0; l < NL; l++)
 #pragma unroll
-                    for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * win[f][l];
-                gr.f[0][n] = d[0]; gr.f[3][n] = d[1]; gr.f[6][n] = d[2];
-                if constexpr (TG) gr.f[9][n] = d[3];
+                    for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G) * LK];
+                if (chunk) {
+                    if (!VX) gr.f[1][n] = d[0];
+                    gr.f[4][n] = d[1]; gr.f[7][n] = d[2];
+                    if constexpr (TG) gr.f[10][n] = d[3];
+                }
+                if constexpr (VX) {
+#pragma unroll
+                    for (int l = 0; l < NR - 1; l++) { cw[0][l] = cw[0][l + 1]; cw[1][l] = cw[1][l + 1]; }
+                    cw[0][NR - 1] = d[1];   // v_eta
+                    cw[1][NR - 1] = d[0];   // u_eta
+                }
+            }
+            if (kin && (chunk || vcol)) {
+                double d[NF] = {};
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+#pragma unroll
+                    for (int f = 0; f < NF; f++) d[f] += k.zd_der[l] * sm[f * LP + c0 + (l - G)];
+                if (chunk) {
+                    if (!VX) gr.f[2][n] = d[0];
+                    gr.f[5][n] = d[1]; gr.f[8][n] = d[2];
+                    if constexpr (TG) gr.f[11][n] = d[3];
+                }
+                if constexpr (VX) {
+#pragma unroll
+                    for (int l = 0; l < NR - 1; l++) { cw[2][l] = cw[2][l + 1]; cw[3][l] = cw[3][l + 1]; }
+                    cw[2][NR - 1] = d[2];   // w_zeta
+                    cw[3][NR - 1] = d[0];   // u_zeta
+                }
+            }
+            const int i = p - G;
+            double gx[NF] = {};         // i derivatives of node i
+            const bool iout = (i >= i0 && i >= gm.lo[0] && i < gm.lo[0] + gm.n[0]);
+            if (iout) {
+                const long ni = (long)i * gm.st[0] + col;
+#pragma unroll
+                for (int l = 0; l < NL; l++)
+#pragma unroll
+                    for (int f = 0; f < NF; f++) gx[f] += k.zd_der[l] * win[f][l];
+                gr.f[0][ni] = gx[0]; gr.f[3][ni] = gx[1]; gr.f[6][ni] = gx[2];
+                if constexpr (TG) gr.f[9][ni] = gx[3];
+            }
+            if constexpr (VX) {
+                if (vcol) {
+                    // T of plane p; mu and metric of plane i+1 (the face's upper node)
+                    const long pc = (long)max(0, min(p, gm.ng[0] - 1)) * gm.st[0] + col;
+#pragma unroll
+                    for (int l = 0; l < NR - 1; l++) tw[l] = tw[l + 1];
+                    tw[NR - 1] = T[pc];
+                    const long pm = (long)max(0, min(i + 1, gm.ng[0] - 1)) * gm.st[0] + col;
+#pragma unroll
+                    for (int c = 0; c < 5; c++) mv0[c] = mv1[c];
+                    mv1[0] = a.mu[pm]; mv1[1] = a.m0[pm]; mv1[2] = a.m4[pm]; mv1[3] = a.m8[pm]; mv1[4] = a.J[pm];
+                    if (i >= max(i0, ilo) - 1 && i < min(i1, ihi)) {
+                        // face i+1/2 from planes i-G+1 .. i+G: window slots 1..2G, ring slots 0..2G-1
+                        double own4[4] = {0.0, 0.0, 0.0, 0.0}, cr[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int l = 0; l <= S; l++) {
+                            const double cm = k.zd_mid[l], cint = k.zd_int[l];
+                            own4[0] += cm * win[0][l + 1];
+                            own4[1] += cm * win[1][l + 1];
+                            own4[2] += cm * win[2][l + 1];
+                            own4[3] += cm * tw[l];
+#pragma unroll
+                            for (int c = 0; c < 4; c++) cr[c] += cint * cw[c][l];
+                        }
+                        double dd[12];
+#pragma unroll
+                        for (int s2 = 0; s2 < 12; s2++) dd[s2] = 0.0;
+                        dd[0 * 3 + 0] = own4[0]; dd[1 * 3 + 0] = own4[1]; dd[2 * 3 + 0] = own4[2]; dd[9 + 0] = own4[3];
+                        dd[1 * 3 + 1] = cr[0]; dd[0 * 3 + 1] = cr[1];
+                        dd[2 * 3 + 2] = cr[2]; dd[0 * 3 + 2] = cr[3];
+                        const double g0[3] = {mv0[1], mv0[2], mv0[3]}, g1[3] = {mv1[1], mv1[2], mv1[3]};
+                        double F[4];
+                        face_flux_diag<0>(dd, g0, mv0[4], g1, mv1[4], mv0[0], mv1[0], F);
+                        if (i >= max(i0, ilo)) {
+                            // node i: faces i-1/2 (fprev) and i+1/2
+                            const long ni = (long)i * gm.st[0] + col;
+                            double dF[4];
+#pragma unroll
+                            for (int m = 0; m < 4; m++) dF[m] = F[m] - fprev[m];
+                            double g[9];
+                            g[0] = gx[0]; g[3] = gx[1]; g[6] = gx[2];
+                            // cross derivatives of node i: ring slot G-1 (plane i)
+                            g[1 * 3 + 1] = cw[0][G - 1]; g[0 * 3 + 1] = cw[1][G - 1];
+                            g[2 * 3 + 2] = cw[2][G - 1]; g[0 * 3 + 2] = cw[3][G - 1];
+                            g[1 * 3 + 2] = 0.0; g[2 * 3 + 1] = 0.0;
+                            const double gmv[3] = {mv0[1], mv0[2], mv0[3]};
+                            const double prod = node_prod_diag<0>(g, gmv, mv0[4], mv0[0]);
+                            const double work = win[0][G] * dF[0] + win[1][G] * dF[1] + win[2][G] * dF[2];
+                            double q0[5];
+#pragma unroll
+                            for (int m = 1; m < 5; m++) q0[m] = a.q.f[m][ni];
+                            a.q.f[1][ni] = q0[1] + k.psi * dF[0];
+                            a.q.f[2][ni] = q0[2] + k.psi * dF[1];
+                            a.q.f[3][ni] = q0[3] + k.psi * dF[2];
+                            const double gh = (a.iso_hi && i == ihi - 1) ? a.gam_iso : k.gam_heat;
+                            a.q.f[4][ni] = q0[4] + (gh * dF[3] + k.psi * (work + prod));
+                        }
+#pragma unroll
+                        for (int m = 0; m < 4; m++) fprev[m] = F[m];
+                    }
+                }
             }
         }
         __syncthreads();
@@ -1451,6 +1680,16 @@ __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
     march_body<S, TG>(a, bx, by, bz);
+}
+// the march with the viscous xi direction (3D diagonal metric): 64 x 8 columns, whose 512 threads
+// may hold the rings in registers
+constexpr int GJX = 8;
+template <int S>
+__global__ __launch_bounds__(GK * GJX) void k_visc_grad_march_xi(GradArgs a)
+{
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, a.tg, bx, by, bz);
+    march_body<S, false, true, GJX>(a, bx, by, bz);
 }
 
 // ---- batched launches ----------------------------------------------------------------------------
@@ -1676,18 +1915,6 @@ __global__ void k_metric(const double *xe, const double *ye, const double *ze, F
 }
 
 // ---- reductions ----------------------------------------------------------------------------
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // CFL spectral-radius maximum over interior nodes; positive doubles order like their bit
 // patterns, so the block maxima combine with an unsigned 64-bit atomic max.
@@ -1702,13 +1929,10 @@ __global__ void k_cfl_max(CFields5 pr, CFields10 met, Geom gm, Consts k, unsigne
         const int jj = (int)(r % gm.n[1]);
         const int ii = (int)(r / gm.n[1]);
         const long n = (long)(ii + gm.lo[0]) * gm.st[0] + (long)(jj + gm.lo[1]) * gm.st[1] + kk + gm.lo[2];
-        const double rho = pr.f[0][n], u = pr.f[1][n], vv = pr.f[2][n], w = pr.f[3][n], e = pr.f[4][n];
-        const double p = pressure(k, rho, u, vv, w, e);
-        const double cs = sqrt(k.upsilon * k.gamma * p / rho);
-        for (int d = 0; d < gm.dims; d++) {
-            const double a = met.f[3 * d][n], b = met.f[3 * d + 1][n], z = met.f[3 * d + 2][n];
-            v += fabs(u * a + vv * b + w * z) + cs * sqrt(a * a + b * b + z * z);
-        }
+        double m9[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) m9[c] = met.f[c][n];
+        v = node_cfl_sum(k.gamma, k.upsilon, gm.dims, pr.f[0][n], pr.f[1][n], pr.f[2][n], pr.f[3][n], pr.f[4][n], m9);
     }
     v = wave_max(v);
     __shared__ double red[16];
@@ -1718,7 +1942,7 @@ __global__ void k_cfl_max(CFields5 pr, CFields10 met, Geom gm, Consts k, unsigne
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int s = 0; s < (int)(blockDim.x >> 6); s++) m = fmax(m, red[s]);
-        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+        atomicMax(out + (blockIdx.x & (RED_SLOTS - 1)), (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -1971,11 +2195,11 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                               \
     } while (0)
 
-enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA };
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA, KT_GRADXI };
 static const char *g_knames[SHX_NUM_KERNELS] = {
     "ghost_periodic", "ghost_values", "weno_xi", "weno_eta", "weno_zeta",
     "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces",
-    "weno_visc_zeta"};
+    "weno_visc_zeta", "visc_grad_xi"};
 
 // face kinds whose writers store a p and c that the equation of state does not give from the
 // ghost's rho,u,v,w,e (subsonic outflow; the slip wall, whose ghost velocity is not the mirror's)
@@ -2030,6 +2254,13 @@ struct shx_case {
     int wtile[3];          // WENO tile variant per direction (set by shx_tune)
     unsigned long long *d_red;
     int *d_cnt;
+    int *d_nan;            // running count of the fused checkNAN (shx_run, shx_run_cfl, shx_simulate)
+    int nan_seen;          // the count last read
+    struct HostIo { unsigned long long red[RED_SLOTS]; int nan; int pad; } *h_io;   // pinned landing slots
+    int dev;               // the device current at shx_create (the stream's)
+    int st_in, st_out, st_bx, st_by;   // buffers of the step in progress (step_begin .. step_end)
+    int iface;             // faces of kind SHX_BC_INTERFACE
+    shx_blocks *blocks = nullptr;      // the multi-block case that holds this block
     int timing;
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;
@@ -2106,7 +2337,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     if (d.precision != SHX_FP64 && d.precision != SHX_MIXED)
         return set_err(SHX_ERR_ARG, "precision must be SHX_FP64 or SHX_MIXED (got %d)", d.precision);
     for (int f = 0; f < 2 * d.dims; f++) {
-        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_WALL_ISOTHERMAL)
+        if (d.bc[f] < SHX_BC_PERIODIC || d.bc[f] > SHX_BC_INTERFACE)
             return set_err(SHX_ERR_ARG, "face %d: unknown boundary kind %d", f, d.bc[f]);
         if ((d.bc[f] == SHX_BC_PERIODIC) != (d.bc[f ^ 1] == SHX_BC_PERIODIC))
             return set_err(SHX_ERR_ARG, "faces %d and %d: a periodic face needs a periodic partner", f & ~1, f | 1);
@@ -2143,6 +2374,7 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
                 const int kind = d.bc[2 * a + u];
                 if (stores_pc(kind)) c->pc = 1;
                 if (kind == SHX_BC_OUTFLOW_RUDY) c->rudy = 1;
+                if (kind == SHX_BC_INTERFACE) c->iface++;
             }
         }
     }
@@ -2158,11 +2390,6 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
     }
     gm.ntot = (long)gm.ng[0] * gm.ng[1] * gm.ng[2];
     if (gm.nalloc > (1L << 31) - 1) { delete c; return set_err(SHX_ERR_ARG, "block too large for 32-bit tiles"); }
-    if ((d.quirks & SHX_QUIRK_ZETA_VISC_J) && d.dims == 3 && gm.ng[1] > gm.ng[2]) {
-        // the reference's zeta stencil indexes k with j: for ng_j > ng_k it leaves the row
-        delete c;
-        return set_err(SHX_ERR_UNSUPPORTED, "zeta-index quirk needs n_j <= n_k");
-    }
 
     Consts &k = c->k;
     memset(&k, 0, sizeof k);
@@ -2201,7 +2428,8 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
         memcpy(c->rk_f, z, sizeof z); memcpy(c->rk_m, z, sizeof z);
     }
 
-    hipError_t e = hipStreamCreate(&c->stream);
+    hipError_t e = hipGetDevice(&c->dev);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) { delete c; return set_err(SHX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     const size_t bytes = sizeof(double) * (size_t)gm.nalloc;
     std::vector<double **> all;
@@ -2221,8 +2449,11 @@ extern "C" int shx_create(const shx_desc *desc, shx_case **out)
             return set_err(SHX_ERR_HIP, "device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
         }
     }
-    e = hipMalloc((void **)&c->d_red, sizeof(unsigned long long));
+    e = hipMalloc((void **)&c->d_red, sizeof(unsigned long long) * RED_SLOTS);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_cnt, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->d_nan, sizeof(int));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_nan, 0, sizeof(int), c->stream);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_io, sizeof(shx_case::HostIo), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { shx_destroy(c); return set_err(SHX_ERR_HIP, "setup: %s", hipGetErrorString(e)); }
     c->cur = 0;
@@ -2234,13 +2465,15 @@ extern "C" int shx_destroy(shx_case *c)
 {
     if (!c) return SHX_OK;
     if (c->batch) return set_err(SHX_ERR_STATE, "the case is in a batch: destroy the batch first");
+    if (c->blocks) return set_err(SHX_ERR_STATE, "the case is a block of a multi-block case: destroy that first");
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int b = 0; b < 3; b++) for (int m = 0; m < 5; m++) hipFree(c->prim[b][m]);
     for (int m = 0; m < 5; m++) { hipFree(c->q[m]); hipFree(c->qsum[m]); hipFree(c->src[m]); }
     for (int m = 0; m < 10; m++) hipFree(c->met[m]);
     for (int m = 0; m < 12; m++) hipFree(c->grad[m]);
     for (int m = 0; m < 6; m++) hipFree(c->bs[m]);
-    hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt);
+    hipFree(c->T); hipFree(c->mu); hipFree(c->d_red); hipFree(c->d_cnt); hipFree(c->d_nan);
+    if (c->h_io) hipHostFree(c->h_io);
     hipFree(c->P); hipFree(c->C); hipFree(c->PL); hipFree(c->RL); hipFree(c->UL); hipFree(c->C0);
     for (int b = 0; b < 2; b++) {
         for (int m = 0; m < 5; m++) hipFree(c->bk[b][m]);
@@ -2336,7 +2569,7 @@ static void face_args(const shx_case *c, double *const *pr, int f, double tau, B
     a.p_out = c->d.p_out;
     a.alpha_tau = c->d.alpha_nonref * tau;
     a.k = c->k;
-    nodes = (kind == SHX_BC_PERIODIC) ? 0 : (long)a.n1 * a.n2 * a.g;
+    nodes = (kind == SHX_BC_PERIODIC || kind == SHX_BC_INTERFACE) ? 0 : (long)a.n1 * a.n2 * a.g;
 }
 static int face_fill(shx_case *c, double *const *pr, double tau)
 {
@@ -2651,6 +2884,49 @@ static bool fuse_zeta(const shx_case *c)
 {
     return c->gm.dims == 3 && c->diag && !c->d.inviscid && c->wtile[2] == 0;
 }
+// the viscous xi direction rides on the derivative march (k_visc_grad_march_xi; WENO5 order, whose
+// rings fit the march's registers)
+static bool fuse_xi(const shx_case *c)
+{
+    return c->gm.dims == 3 && c->diag && !c->d.inviscid && c->d.space_order == 5;
+}
+
+static double iso_factor(const shx_case *c);
+// Nodal derivatives of the viscous terms (3D: the march; 2D: pointwise).  xi: the march also adds
+// the viscous xi terms to Q (fuse_xi cases, after the WENO sweeps of xi and eta).
+static int grad_march(shx_case *c, double *const *pr, const bool xi)
+{
+    const Geom &gm = c->gm;
+    CFields5 cp;
+    Fields12 gf;
+    for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+    for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
+    int rc = tbeg(c, xi ? KT_GRADXI : KT_GRAD);
+    if (rc) return rc;
+    if (gm.dims == 3) {
+        constexpr int CI = 32;
+        const int tgrad = c->diag ? 0 : 1;
+        GradArgs ga;
+        memset(&ga, 0, sizeof ga);
+        ga.pr = cp; ga.T = c->T; ga.gr = gf; ga.gm = gm; ga.k = c->k; ga.ci = CI; ga.tgrad = tgrad;
+        const void *fn = march_kernel(c->d.space_order, tgrad, false);
+        if (xi) {
+            for (int m = 0; m < 5; m++) ga.q.f[m] = c->q[m];
+            ga.mu = c->mu; ga.m0 = c->met[0]; ga.m4 = c->met[4]; ga.m8 = c->met[8]; ga.J = c->met[9];
+            ga.iso_hi = (c->d.bc[1] == SHX_BC_WALL_ISOTHERMAL) ? 1 : 0;
+            ga.gam_iso = iso_factor(c);
+            fn = (const void *)&k_visc_grad_march_xi<5>;
+        }
+        const int gj = xi ? GJX : GJ;
+        const dim3 grid = flat_grid(dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], gj), nblk(gm.ng[0], CI)), ga.tg, 0);
+        void *args[] = {&ga};
+        HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * gj), args, 0, c->stream));
+    } else {
+        k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
+    }
+    LAUNCH_CHECK();
+    return tend(c);
+}
 // rk: when given, the last viscous sweep applies this stage update instead of storing Q
 static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double tau)
 {
@@ -2669,37 +2945,24 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
         LAUNCH_CHECK();
         if ((rc = tend(c))) return rc;
     }
+    const bool fx = fuse_xi(c);
     if (!c->d.inviscid) {
-        CFields5 cp;
-        Fields12 gf;
-        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
-        for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         if ((rc = face_fill(c, pr, tau))) return rc;
-        if ((rc = tbeg(c, KT_GRAD))) return rc;
-        if (gm.dims == 3) {
-            constexpr int CI = 32;
-            const int tgrad = c->diag ? 0 : 1;
-            GradArgs ga = {cp, c->T, gf, gm, c->k, CI, tgrad, {0, 0, 0, 0}};
-            const dim3 grid = flat_grid(dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], GJ), nblk(gm.ng[0], CI)), ga.tg, 0);
-            const void *fn = march_kernel(c->d.space_order, tgrad, false);
-            void *args[] = {&ga};
-            HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * GJ), args, 0, c->stream));
-        } else {
-            k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
-        }
-        LAUNCH_CHECK();
-        if ((rc = tend(c))) return rc;
+        if (!fx && (rc = grad_march(c, pr, false))) return rc;
     } else if ((rc = face_fill(c, pr, tau))) {
         return rc;
     }
     // Q accumulates the WENO divergences, then the viscous terms; with the fused zeta direction
-    // the order is WENO xi, eta, viscous xi, eta, then WENO zeta and viscous zeta together
+    // the order is WENO xi, eta, viscous xi, eta, then WENO zeta and viscous zeta together.  With
+    // the viscous xi direction in the derivative march, the march runs after the WENO xi and eta
+    // sweeps, in the place of the xi viscous sweep.
     const bool fz = fuse_zeta(c);
     const int nsep = fz ? gm.dims - 1 : gm.dims;
     for (int dir = 0; dir < nsep; dir++)
         if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
+    if (fx && (rc = grad_march(c, pr, true))) return rc;
     if (!c->d.inviscid)
-        for (int dir = 0; dir < nsep; dir++)
+        for (int dir = fx ? 1 : 0; dir < nsep; dir++)
             if ((rc = visc_sweep(c, pr, dir, dir == gm.dims - 1 ? rk : nullptr))) return rc;
     if (fz && (rc = zeta_sweep(c, pr, rk))) return rc;
     return SHX_OK;
@@ -2737,6 +3000,15 @@ static Pick visc_pick(const shx_case *c, int dir)
     return (c->d.space_order == 5) ? visc_pick_s<5>(dir, coalesce, diag) : visc_pick_s<9>(dir, coalesce, diag);
 }
 
+// heat-conduction factor the isothermal writer gives its ghosts (Prandtl 0.72,
+// src/BC.c:2974-2976), averaged with the interior one as src/ZD.c:193-194
+static double iso_factor(const shx_case *c)
+{
+    const shx_desc &d = c->d;
+    const double gw = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * 0.72);
+    return 0.5 * (gw + c->k.gam_heat);
+}
+
 static ViscArgs visc_args(const shx_case *c, double *const *pr, int dir, const RkArgs *rk)
 {
     const Geom &gm = c->gm;
@@ -2760,19 +3032,14 @@ static ViscArgs visc_args(const shx_case *c, double *const *pr, int dir, const R
     a.nD = gm.n[dir]; a.nA = gm.n[A]; a.nB = gm.n[B];
     a.oD = gm.lo[dir]; a.oA = gm.lo[A]; a.oB = gm.lo[B];
     a.ngD = gm.ng[dir];
+    a.ngA = gm.ng[A];
     a.coalesce_d = (a.sD == 1);
     a.alignA = (a.sA == 1);
     a.three = (gm.dims == 3);
     a.bug = (dir == 2 && (c->d.quirks & SHX_QUIRK_ZETA_VISC_J)) ? 1 : 0;
     a.k = c->k;
     a.iso_hi = (c->d.bc[2 * dir + 1] == SHX_BC_WALL_ISOTHERMAL) ? 1 : 0;
-    {
-        // heat-conduction factor the isothermal writer gives its ghosts (Prandtl 0.72,
-        // src/BC.c:2974-2976), averaged with the interior one as src/ZD.c:193-194
-        const shx_desc &d = c->d;
-        const double gw = 1.0 / ((d.gamma - 1.0) * pow(d.mach, 2.0) * d.reynolds * 0.72);
-        a.gam_iso = 0.5 * (gw + c->k.gam_heat);
-    }
+    a.gam_iso = iso_factor(c);
     a.fuse_rk = rk ? 1 : 0;
     if (rk) a.rk = *rk;
     return a;
@@ -2832,6 +3099,8 @@ static RkArgs rk_args(shx_case *c, double *const *us, double *const *out, int s,
     A.first_stage = (s == 0);
     A.has_src = c->has_src;
     A.three = (c->gm.dims == 3);
+    A.gamma = c->k.gamma;
+    A.upsilon = c->k.upsilon;
     return A;
 }
 
@@ -2842,6 +3111,7 @@ static int rk_update(shx_case *c, const RkArgs &rk)
     UpdArgs A;
     memset(&A, 0, sizeof A);
     for (int m = 0; m < 5; m++) A.q.f[m] = c->q[m];
+    for (int m = 0; m < 10; m++) A.met.f[m] = c->met[m];
     A.J = c->met[9];
     A.gm = gm;
     A.rk = rk;
@@ -2862,31 +3132,365 @@ static int check_ready(shx_case *c)
     return SHX_OK;
 }
 
-extern "C" int shx_calc_runge_kutta(shx_case *c, double tau)
+// end-of-step checks fused into the last stage's update (step_rk flags)
+enum { STEP_NAN = 1, STEP_CFL = 2 };
+
+// One time step (CalcRungeKutta and the step-end copy).  flags: STEP_NAN adds checkNAN on the new
+// state to the running count c->d_nan; STEP_CFL puts the maximum node CFL sum of the new state
+// into c->d_red (which the caller clears).
+// A step in three parts, so that the blocks of a multi-block case advance stage by stage together:
+// step_begin (the Rudy faces' step-start state; buffers), step_stage (one stage), step_end (the
+// step-end copy).  Stage s reads buffer st_in and writes st_out; stage s+1 reads what stage s wrote
+// and writes the other scratch buffer.
+static int step_begin(shx_case *c)
+{
+    const int un = c->cur;
+    c->st_bx = (un + 1) % 3;
+    c->st_by = (un + 2) % 3;
+    c->st_in = un;
+    c->st_out = c->st_bx;
+    return rudy_save(c, c->prim[un]);
+}
+static int step_stage(shx_case *c, int s, double tau, int flags)
+{
+    int rc;
+    RkArgs rk = rk_args(c, c->prim[c->st_in], c->prim[c->st_out], s, tau);
+    if (s == c->d.time_order - 1) {
+        if (flags & STEP_NAN) { rk.nan_cnt = c->d_nan; rk.nanP = c->P; }
+        if (flags & STEP_CFL) rk.cfl_red = c->d_red;
+    }
+    // viscous stages apply the update in their last sweep; inviscid ones in k_rk_update
+    if ((rc = stage_fluxes(c, c->prim[c->st_in], c->d.inviscid ? nullptr : &rk, tau))) return rc;
+    if ((rc = carry_ghosts(c, c->prim[c->st_in], c->prim[c->st_out]))) return rc;
+    if (c->d.inviscid && (rc = rk_update(c, rk))) return rc;
+    c->st_in = c->st_out;
+    c->st_out = (c->st_out == c->st_bx) ? c->st_by : c->st_bx;
+    return SHX_OK;
+}
+static void step_end(shx_case *c)
+{
+    // the last stage's input holds the ghost values the reference's U_RK (and so U_lastStep) keeps
+    // after the step; post-processing copies them over when it needs them
+    c->last_in = (c->st_in == c->st_bx) ? c->st_by : c->st_bx;
+    c->cur = c->st_in;   // U_lastStep <- U_RK (pointer swap instead of WriteValuesFromU1ToU2)
+}
+static int step_rk(shx_case *c, double tau, int flags)
 {
     int rc = check_ready(c);
     if (rc) return rc;
     if (c->batch) return set_err(SHX_ERR_STATE, "the case is in a batch: step it with shx_batch_run");
-    const Geom &gm = c->gm;
-    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
-    const int un = c->cur;
-    const int bx = (un + 1) % 3, by = (un + 2) % 3;
-    int in = un, outb = bx;
-    if ((rc = rudy_save(c, c->prim[un]))) return rc;
-    for (int s = 0; s < c->d.time_order; s++) {
-        const RkArgs rk = rk_args(c, c->prim[in], c->prim[outb], s, tau);
-        // viscous stages apply the update in their last sweep; inviscid ones in k_rk_update
-        if ((rc = stage_fluxes(c, c->prim[in], c->d.inviscid ? nullptr : &rk, tau))) return rc;
-        if ((rc = carry_ghosts(c, c->prim[in], c->prim[outb]))) return rc;
-        if (c->d.inviscid && (rc = rk_update(c, rk))) return rc;
-        // stage s+1 reads what stage s wrote and writes the other scratch buffer
-        in = outb;
-        outb = (outb == bx) ? by : bx;
+    if (c->iface) return set_err(SHX_ERR_STATE, "the case has interface faces: step it with shx_blocks_run");
+    if ((rc = step_begin(c))) return rc;
+    for (int s = 0; s < c->d.time_order; s++)
+        if ((rc = step_stage(c, s, tau, flags))) return rc;
+    step_end(c);
+    return SHX_OK;
+}
+
+extern "C" int shx_calc_runge_kutta(shx_case *c, double tau) { return step_rk(c, tau, 0); }
+
+// Non-finite densities the fused checkNAN found since the last call (synchronizes the stream).
+static int nan_found(shx_case *c, int &fresh)
+{
+    HIPCHK(hipMemcpyAsync(&c->h_io->nan, c->d_nan, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    fresh = c->h_io->nan - c->nan_seen;
+    c->nan_seen = c->h_io->nan;
+    return SHX_OK;
+}
+
+// ---- multi-block cases on one device -----------------------------------------------------------
+// The blocks of a case share the GPU; before every stage one gather fills the ghost layers of every
+// interface face of every block from the donor blocks' stage input (the reference's
+// TransferFlowParameterWithGhosts, src/Functions.c:3580-3684, whose repeated ghost-inclusive passes
+// settle edges and corners).  A ghost node is followed through the faces it lies beyond, one at a
+// time in axis order: a periodic face wraps the index, an interface face maps it into the donor
+// block (normal layer l outside the face = layer l inside the donor's face; tangential indices
+// through the transform); the value is read once the index is interior.  A path that meets a face
+// with a boundary writer ends without a write (such corners belong to the writers).  Every read is
+// an interior node and every write a ghost node, so one launch serves all blocks.
+enum { XK_WRITER = 0, XK_PERIODIC = 1, XK_INTERFACE = 2 };
+struct XRec {
+    Geom gm;
+    gdp f[10];            // fields moved: the stage input rho,u,v,w,e, or the metric rows and J
+    int kind[6], donor[6], dface[6], tr[6][3];
+    long first;           // first work item of this block
+};
+struct XTab {
+    XRec r[SHX_BLOCKS_MAX];
+    int nb, metric;
+    long total;
+};
+__global__ __launch_bounds__(256) void k_interface_gather(XTab t)
+{
+    const long w = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= t.total) return;
+    int b0 = 0;
+    while (b0 + 1 < t.nb && w >= t.r[b0 + 1].first) b0++;
+    const XRec &R0 = t.r[b0];
+    const int dims = R0.gm.dims;
+    int x[3];
+    const long n0 = padded_node(R0.gm, w - R0.first, x);
+    bool across = false;                 // a ghost in the direction of an interface face
+    for (int d = 0; d < dims; d++) {
+        const int lo = R0.gm.lo[d], hi = lo + R0.gm.n[d];
+        if (x[d] < lo && R0.kind[2 * d] == XK_INTERFACE) across = true;
+        if (x[d] >= hi && R0.kind[2 * d + 1] == XK_INTERFACE) across = true;
     }
-    // the last stage's input holds the ghost values the reference's U_RK (and so U_lastStep) keeps
-    // after the step; post-processing copies them over when it needs them
-    c->last_in = (in == bx) ? by : bx;
-    c->cur = in;   // U_lastStep <- U_RK (pointer swap instead of WriteValuesFromU1ToU2)
+    if (!across) return;
+    int b = b0;
+    int perm[3] = {0, 1, 2};             // axis of block b that each axis of block b0 runs along
+    double sgn[3] = {1.0, 1.0, 1.0};
+    for (int it = 0; it < 2 * dims; it++) {
+        const XRec &R = t.r[b];
+        int a = -1, up = 0;
+        for (int d = 0; d < dims && a < 0; d++) {
+            if (x[d] < R.gm.lo[d]) { a = d; up = 0; }
+            else if (x[d] >= R.gm.lo[d] + R.gm.n[d]) { a = d; up = 1; }
+        }
+        if (a < 0) break;
+        const int f = 2 * a + up;
+        if (R.kind[f] == XK_PERIODIC) { x[a] += up ? -R.gm.n[a] : R.gm.n[a]; continue; }
+        if (R.kind[f] != XK_INTERFACE) return;
+        const XRec &D = t.r[R.donor[f]];
+        const int fd = R.dface[f], ad = fd >> 1, dup = fd & 1;
+        const int l = up ? x[a] - (R.gm.lo[a] + R.gm.n[a]) : R.gm.lo[a] - 1 - x[a];
+        int y[3] = {0, 0, 0}, m[3] = {0, 1, 2};
+        double s[3] = {1.0, 1.0, 1.0};
+        y[ad] = dup ? D.gm.lo[ad] + D.gm.n[ad] - 1 - l : D.gm.lo[ad] + l;
+        m[a] = ad;
+        s[a] = (up != dup) ? 1.0 : -1.0;
+        for (int q = 0; q < dims; q++) {
+            if (q == a) continue;
+            const int tq = R.tr[f][q], aq = (tq > 0 ? tq : -tq) - 1, r = x[q] - R.gm.lo[q];
+            y[aq] = (tq > 0) ? D.gm.lo[aq] + r : D.gm.lo[aq] + D.gm.n[aq] - 1 - r;
+            m[q] = aq;
+            s[q] = (tq > 0) ? 1.0 : -1.0;
+        }
+        for (int q = 0; q < dims; q++) { sgn[q] *= s[perm[q]]; perm[q] = m[perm[q]]; }
+        for (int q = 0; q < 3; q++) x[q] = y[q];
+        b = R.donor[f];
+    }
+    const XRec &S = t.r[b];
+    for (int d = 0; d < dims; d++)
+        if (x[d] < S.gm.lo[d] || x[d] >= S.gm.lo[d] + S.gm.n[d]) return;
+    const long ns = (long)x[0] * S.gm.st[0] + (long)x[1] * S.gm.st[1] + (long)x[2] * S.gm.st[2];
+    if (!t.metric) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) R0.f[m][n0] = S.f[m][ns];
+    } else {
+        // row a of the block's metric is the donor's row perm[a], signed (src/Functions.c:2664-2686)
+        for (int a = 0; a < 3; a++) {
+            const int pa = (a < dims) ? perm[a] : a;
+            const double sa = (a < dims) ? sgn[a] : 1.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) R0.f[3 * a + c][n0] = sa * S.f[3 * pa + c][ns];
+        }
+        R0.f[9][n0] = S.f[9][ns];
+    }
+}
+
+struct shx_blocks {
+    std::vector<shx_case *> b;
+    int kind[SHX_BLOCKS_MAX][6], donor[SHX_BLOCKS_MAX][6], dface[SHX_BLOCKS_MAX][6], tr[SHX_BLOCKS_MAX][6][3];
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_x = nullptr;
+    std::vector<hipEvent_t> ev_b;
+};
+
+// the gather's table: the stage input buffers of every block (metric: its metric arrays)
+static XTab xtab(const shx_blocks *g, bool metric)
+{
+    XTab t;
+    memset(&t, 0, sizeof t);
+    t.nb = (int)g->b.size();
+    t.metric = metric ? 1 : 0;
+    long first = 0;
+    for (int i = 0; i < t.nb; i++) {
+        const shx_case *c = g->b[i];
+        XRec &r = t.r[i];
+        r.gm = c->gm;
+        for (int m = 0; m < 10; m++) r.f[m] = metric ? c->met[m] : (m < 5 ? c->prim[c->st_in][m] : nullptr);
+        for (int f = 0; f < 6; f++) {
+            r.kind[f] = g->kind[i][f]; r.donor[f] = g->donor[i][f]; r.dface[f] = g->dface[i][f];
+            for (int q = 0; q < 3; q++) r.tr[f][q] = g->tr[i][f][q];
+        }
+        r.first = first;
+        first += c->gm.ntot;
+    }
+    t.total = first;
+    return t;
+}
+static int gather(shx_blocks *g, bool metric)
+{
+    XTab t = xtab(g, metric);
+    k_interface_gather<<<nblk(t.total, 256), 256, 0, g->stream>>>(t);
+    LAUNCH_CHECK();
+    return SHX_OK;
+}
+
+extern "C" int shx_blocks_create(shx_case *const *blocks, int nblocks, const shx_interface *ifc, int nifc,
+                                 shx_blocks **out)
+{
+    if (!blocks || !out || nblocks <= 0 || (nifc > 0 && !ifc) || nifc < 0) return set_err(SHX_ERR_ARG, "null argument or no blocks");
+    *out = nullptr;
+    if (nblocks > SHX_BLOCKS_MAX) return set_err(SHX_ERR_UNSUPPORTED, "at most %d blocks", SHX_BLOCKS_MAX);
+    const shx_case *c0 = blocks[0];
+    for (int i = 0; i < nblocks; i++) {
+        const shx_case *c = blocks[i];
+        if (!c) return set_err(SHX_ERR_ARG, "block %d is null", i);
+        for (int j = 0; j < i; j++)
+            if (blocks[j] == c) return set_err(SHX_ERR_ARG, "block %d appears twice", i);
+        if (c->blocks || c->batch) return set_err(SHX_ERR_STATE, "block %d belongs to another group or a batch", i);
+        if (c->gm.dims != c0->gm.dims || c->d.space_order != c0->d.space_order || c->d.time_order != c0->d.time_order ||
+            c->d.inviscid != c0->d.inviscid)
+            return set_err(SHX_ERR_ARG, "block %d: dims, scheme, RK order and viscosity must match block 0", i);
+        if (c->dev != c0->dev) return set_err(SHX_ERR_ARG, "block %d is on another device", i);
+    }
+    const int dims = c0->gm.dims;
+    shx_blocks *g = new shx_blocks();
+    g->b.assign(blocks, blocks + nblocks);
+    for (int i = 0; i < nblocks; i++)
+        for (int f = 0; f < 6; f++) {
+            const int kind = (f < 2 * dims) ? blocks[i]->d.bc[f] : SHX_BC_PERIODIC;
+            g->kind[i][f] = (kind == SHX_BC_PERIODIC) ? XK_PERIODIC : (kind == SHX_BC_INTERFACE ? XK_INTERFACE : XK_WRITER);
+            g->donor[i][f] = -1; g->dface[i][f] = -1;
+            for (int q = 0; q < 3; q++) g->tr[i][f][q] = q + 1;
+        }
+    auto fail = [&](int code, const char *msg, int k) {
+        delete g;
+        return set_err(code, "connection %d: %s", k, msg);
+    };
+    for (int k = 0; k < nifc; k++) {
+        const shx_interface &e = ifc[k];
+        if (e.block < 0 || e.block >= nblocks || e.donor < 0 || e.donor >= nblocks) return fail(SHX_ERR_ARG, "block index out of range", k);
+        if (e.face < 0 || e.face >= 2 * dims || e.donor_face < 0 || e.donor_face >= 2 * dims) return fail(SHX_ERR_ARG, "face out of range", k);
+        if (g->kind[e.block][e.face] != XK_INTERFACE || g->kind[e.donor][e.donor_face] != XK_INTERFACE)
+            return fail(SHX_ERR_ARG, "both faces must be of kind SHX_BC_INTERFACE", k);
+        if (g->donor[e.block][e.face] >= 0) return fail(SHX_ERR_ARG, "the face is listed twice", k);
+        // a signed permutation whose normal entry is the donor face's axis; whole faces
+        int seen = 0;
+        for (int q = 0; q < dims; q++) {
+            const int t = e.transform[q], at = (t > 0 ? t : -t) - 1;
+            if (at < 0 || at >= dims || (seen & (1 << at))) return fail(SHX_ERR_ARG, "transform is not a signed permutation", k);
+            seen |= 1 << at;
+        }
+        const int a = e.face / 2, ad = e.donor_face / 2;
+        if ((e.transform[a] > 0 ? e.transform[a] : -e.transform[a]) - 1 != ad)
+            return fail(SHX_ERR_ARG, "the transform must take the face's normal to the donor face's", k);
+        const shx_case *B = blocks[e.block], *D = blocks[e.donor];
+        for (int q = 0; q < dims; q++) {
+            if (q == a) continue;
+            const int aq = (e.transform[q] > 0 ? e.transform[q] : -e.transform[q]) - 1;
+            if (B->gm.n[q] != D->gm.n[aq]) return fail(SHX_ERR_UNSUPPORTED, "faces join only whole (tangential extents differ)", k);
+        }
+        if (B->gm.n[a] < B->gm.g || D->gm.n[ad] < B->gm.g) return fail(SHX_ERR_ARG, "block thinner than its ghost layers", k);
+        g->donor[e.block][e.face] = e.donor;
+        g->dface[e.block][e.face] = e.donor_face;
+        for (int q = 0; q < 3; q++) g->tr[e.block][e.face][q] = (q < dims) ? e.transform[q] : q + 1;
+    }
+    for (int i = 0; i < nblocks; i++)
+        for (int f = 0; f < 2 * dims; f++) {
+            if (g->kind[i][f] != XK_INTERFACE) continue;
+            const int d = g->donor[i][f], fd = g->dface[i][f];
+            if (d < 0) { delete g; return set_err(SHX_ERR_ARG, "block %d face %d: interface without a connection", i, f); }
+            if (g->donor[d][fd] != i || g->dface[d][fd] != f) {
+                delete g;
+                return set_err(SHX_ERR_ARG, "block %d face %d: the donor face does not connect back", i, f);
+            }
+            // the two sides' transforms are inverse permutations of each other
+            for (int q = 0; q < dims; q++) {
+                const int t = g->tr[i][f][q], aq = (t > 0 ? t : -t) - 1;
+                const int back = g->tr[d][fd][aq], bq = (back > 0 ? back : -back) - 1;
+                if (bq != q || (q != f / 2 && (t > 0) != (back > 0))) {
+                    delete g;
+                    return set_err(SHX_ERR_ARG, "block %d face %d: transforms of the two sides do not invert", i, f);
+                }
+            }
+        }
+    hipError_t e = hipStreamCreate(&g->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->ev_x, hipEventDisableTiming);
+    g->ev_b.assign(nblocks, nullptr);
+    for (int i = 0; i < nblocks && e == hipSuccess; i++) e = hipEventCreateWithFlags(&g->ev_b[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        shx_blocks_destroy(g);
+        return set_err(SHX_ERR_HIP, "group setup: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < nblocks; i++) blocks[i]->blocks = g;
+    *out = g;
+    return SHX_OK;
+}
+
+extern "C" int shx_blocks_destroy(shx_blocks *g)
+{
+    if (!g) return SHX_OK;
+    if (g->stream) hipStreamSynchronize(g->stream);
+    for (shx_case *c : g->b) {
+        if (c->blocks == g) c->blocks = nullptr;
+    }
+    for (hipEvent_t ev : g->ev_b)
+        if (ev) hipEventDestroy(ev);
+    if (g->ev_x) hipEventDestroy(g->ev_x);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return SHX_OK;
+}
+
+// the group's stream after every block's stream, and every block's after the group's
+static int join_blocks(shx_blocks *g)
+{
+    for (size_t i = 0; i < g->b.size(); i++) {
+        HIPCHK(hipEventRecord(g->ev_b[i], g->b[i]->stream));
+        HIPCHK(hipStreamWaitEvent(g->stream, g->ev_b[i], 0));
+    }
+    return SHX_OK;
+}
+static int fork_blocks(shx_blocks *g)
+{
+    HIPCHK(hipEventRecord(g->ev_x, g->stream));
+    for (shx_case *c : g->b) HIPCHK(hipStreamWaitEvent(c->stream, g->ev_x, 0));
+    return SHX_OK;
+}
+
+extern "C" int shx_blocks_transfer_metric(shx_blocks *g)
+{
+    if (!g) return set_err(SHX_ERR_ARG, "null group");
+    for (shx_case *c : g->b)
+        if (!c->has_metric) return set_err(SHX_ERR_STATE, "a block has no metric yet");
+    int rc;
+    if ((rc = join_blocks(g)) || (rc = gather(g, true)) || (rc = fork_blocks(g))) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_blocks_run(shx_blocks *g, double tau, int nsteps)
+{
+    if (!g) return set_err(SHX_ERR_ARG, "null group");
+    if (nsteps < 0) return set_err(SHX_ERR_ARG, "negative step count");
+    int rc;
+    for (shx_case *c : g->b)
+        if ((rc = check_ready(c))) return rc;
+    const int order = g->b[0]->d.time_order;
+    for (int step = 0; step < nsteps; step++) {
+        for (shx_case *c : g->b)
+            if ((rc = step_begin(c))) return rc;
+        for (int s = 0; s < order; s++) {
+            if ((rc = join_blocks(g)) || (rc = gather(g, false)) || (rc = fork_blocks(g))) return rc;
+            for (shx_case *c : g->b)
+                if ((rc = step_stage(c, s, tau, STEP_NAN))) return rc;
+        }
+        for (shx_case *c : g->b) step_end(c);
+        int bad = 0;
+        for (shx_case *c : g->b) {
+            int fresh = 0;
+            if ((rc = nan_found(c, fresh))) return rc;
+            bad += fresh;
+        }
+        if (bad > 0)
+            return set_err(SHX_ERR_NAN, "checkNAN: %d interior nodes with a non-finite density after step %d of %d "
+                           "(set to 1e6)", bad, step + 1, nsteps);
+    }
+    if ((rc = join_blocks(g))) return rc;
     return SHX_OK;
 }
 
@@ -3332,6 +3936,7 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
         for (int j = 0; j < i; j++)
             if (cases[j] == c) return set_err(SHX_ERR_ARG, "case %d appears twice", i);
         if (c->batch) return set_err(SHX_ERR_STATE, "case %d is in another batch", i);
+        if (c->iface) return set_err(SHX_ERR_UNSUPPORTED, "case %d has interface faces (a block of a multi-block case)", i);
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     shx_batch *b = new shx_batch();
@@ -3526,10 +4131,78 @@ extern "C" int shx_run(shx_case *c, double tau, int nsteps)
 {
     if (nsteps < 0) return set_err(SHX_ERR_ARG, "negative step count");
     for (int s = 0; s < nsteps; s++) {
-        int rc = shx_calc_runge_kutta(c, tau);
+        int rc = step_rk(c, tau, STEP_NAN);
         if (rc) return rc;
+        int bad = 0;
+        if ((rc = nan_found(c, bad))) return rc;
+        if (bad > 0)
+            return set_err(SHX_ERR_NAN, "checkNAN: %d interior nodes with a non-finite density after step %d of %d "
+                           "(set to 1e6)", bad, s + 1, nsteps);
         if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
     }
+    return SHX_OK;
+}
+
+// the maximum node CFL sum the last reduction into c->d_red left (its slots copied to h_io->red,
+// stream synchronized)
+static double red_max(shx_case *c)
+{
+    unsigned long long mb = 0;
+    for (int s = 0; s < RED_SLOTS; s++) mb = std::max(mb, c->h_io->red[s]);
+    double mx;
+    memcpy(&mx, &mb, sizeof mx);
+    return mx;
+}
+static int red_clear(shx_case *c)
+{
+    HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long) * RED_SLOTS, c->stream));
+    return SHX_OK;
+}
+static int red_fetch(shx_case *c)
+{
+    HIPCHK(hipMemcpyAsync(c->h_io->red, c->d_red, sizeof(unsigned long long) * RED_SLOTS, hipMemcpyDeviceToHost,
+                          c->stream));
+    return SHX_OK;
+}
+
+extern "C" int shx_run_cfl(shx_case *c, double cfl, int nsteps, double *time)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nsteps < 0 || !(cfl > 0.0)) return set_err(SHX_ERR_ARG, "negative step count or bad cfl");
+    if (c->rudy) return set_err(SHX_ERR_UNSUPPORTED, "CFL stepping with Rudy outflow faces (their relaxation uses tau)");
+    const Geom &gm = c->gm;
+    const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
+    // the first step's maximum from the current state; later ones from the fused reduction
+    CFields5 pf;
+    CFields10 mf;
+    for (int m = 0; m < 5; m++) pf.f[m] = c->prim[c->cur][m];
+    for (int m = 0; m < 10; m++) mf.f[m] = c->met[m];
+    if ((rc = red_clear(c))) return rc;
+    if ((rc = tbeg(c, KT_REDUCE))) return rc;
+    k_cfl_max<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, mf, gm, c->k, c->d_red);
+    LAUNCH_CHECK();
+    if ((rc = tend(c))) return rc;
+    if ((rc = red_fetch(c))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        const double mx = red_max(c);
+        if (!(mx > 0.0) || !std::isfinite(mx)) return set_err(SHX_ERR_NAN, "CFL maximum %g before step %d", mx, s + 1);
+        const double dt = cfl / mx;
+        if ((rc = red_clear(c))) return rc;
+        if ((rc = step_rk(c, dt, STEP_NAN | STEP_CFL))) return rc;
+        if ((rc = red_fetch(c))) return rc;
+        int bad = 0;
+        if ((rc = nan_found(c, bad))) return rc;
+        t += dt;
+        if (time) *time = t;
+        if (bad > 0)
+            return set_err(SHX_ERR_NAN, "checkNAN: %d interior nodes with a non-finite density after step %d of %d "
+                           "(set to 1e6)", bad, s + 1, nsteps);
+        if (c->timing && c->ev_used > 4096) { rc = drain_events(c); if (rc) return rc; }
+    }
+    if (time) *time = t;
     return SHX_OK;
 }
 
@@ -3788,9 +4461,9 @@ extern "C" int shx_simulate(shx_case *c, const shx_sim *sim, shx_sim_report *rep
     for (int it = sim->start_iteration + 1; it <= end; it++) {
         last_run = it;
         time += tau * L0 / u0;
-        if ((rc = shx_calc_runge_kutta(c, tau))) return rc;
+        if ((rc = step_rk(c, tau, STEP_NAN))) return rc;
         int bad = 0;
-        if ((rc = shx_check_nan(c, &bad))) return rc;
+        if ((rc = nan_found(c, bad))) return rc;
         if (bad > 0) {
             rep->nan_steps++;
             if (rep->resets >= sim->number_resets) { rep->aborted = 1; break; }
@@ -3859,6 +4532,7 @@ extern "C" int shx_rhs(shx_case *c, double *const q[5])
     int rc = check_ready(c);
     if (rc) return rc;
     if (!q) return set_err(SHX_ERR_ARG, "null output");
+    if (c->iface) return set_err(SHX_ERR_UNSUPPORTED, "the case has interface faces (its ghosts need its multi-block case)");
     const Geom &gm = c->gm;
     const long nint = (long)gm.n[0] * gm.n[1] * gm.n[2];
     // a Rudy face takes the current state as its step-start state, with tau = 0
@@ -3918,16 +4592,14 @@ extern "C" int shx_cfl_dt(shx_case *c, double cfl, double *dt)
     CFields10 mf;
     for (int m = 0; m < 5; m++) pf.f[m] = c->prim[c->cur][m];
     for (int m = 0; m < 10; m++) mf.f[m] = c->met[m];
-    HIPCHK(hipMemsetAsync(c->d_red, 0, sizeof(unsigned long long), c->stream));
+    if ((rc = red_clear(c))) return rc;
     if ((rc = tbeg(c, KT_REDUCE))) return rc;
     k_cfl_max<<<nblk(nint, 256), 256, 0, c->stream>>>(pf, mf, gm, c->k, c->d_red);
     LAUNCH_CHECK();
     if ((rc = tend(c))) return rc;
-    unsigned long long bits = 0;
-    HIPCHK(hipMemcpyAsync(&bits, c->d_red, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = red_fetch(c))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
-    double mx;
-    memcpy(&mx, &bits, sizeof mx);
+    const double mx = red_max(c);
     *dt = mx > 0.0 ? cfl / mx : 0.0;
     return SHX_OK;
 }

@@ -22,12 +22,17 @@ BC_OUTFLOW_NORMAL, BC_OUTFLOW_RUDY = 4, 5
 BC_WALL_SLIP, BC_WALL_NOSLIP = 6, 7
 BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
 BC_FARFIELD, BC_WALL_ISOTHERMAL = 11, 12
+BC_INTERFACE = 13
 FP64, MIXED = 0, 1
-NUM_KERNELS = 14
+NUM_KERNELS = 15
 
 
 class ShxError(RuntimeError):
-    pass
+    """A negative status of libshockx; `code` is the SHX_ERR_* value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class Desc(ctypes.Structure):
@@ -73,7 +78,10 @@ EXPORTS = [
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
     "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
     "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download", "shx_mms_error",
+    "shx_run_cfl",
+    "shx_blocks_create", "shx_blocks_transfer_metric", "shx_blocks_run", "shx_blocks_destroy",
 ]
+ERR_NAN = -5
 BATCH_CLASSES = ("ghost", "values", "grad", "weno", "visc", "rk")
 
 _lib = None
@@ -97,6 +105,7 @@ def lib():
             getattr(L, f).argtypes = [P, P]
         L.shx_calc_runge_kutta.argtypes = [P, ctypes.c_double]
         L.shx_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
+        L.shx_run_cfl.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.shx_simulate.argtypes = [P, ctypes.POINTER(Sim), ctypes.POINTER(SimReport)]
         L.shx_post.argtypes = [P, P]
         L.shx_film_open.argtypes = [P, ctypes.c_int]
@@ -120,6 +129,10 @@ def lib():
         L.shx_batch_synchronize.argtypes = [P]
         L.shx_batch_destroy.argtypes = [P]
         L.shx_tune.argtypes = [P, ctypes.c_int, P]
+        L.shx_blocks_create.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(P)]
+        L.shx_blocks_transfer_metric.argtypes = [P]
+        L.shx_blocks_run.argtypes = [P, ctypes.c_double, ctypes.c_int]
+        L.shx_blocks_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -127,7 +140,7 @@ def lib():
 def _check(rc):
     if rc != 0:
         msg = lib().shx_last_error()
-        raise ShxError(f"shockx error {rc}: {msg.decode() if msg else ''}")
+        raise ShxError(f"shockx error {rc}: {msg.decode() if msg else ''}", rc)
 
 
 def _f64(a, shape):
@@ -237,7 +250,15 @@ class Case:
         _check(lib().shx_calc_runge_kutta(self.h, tau))
 
     def run(self, tau, nsteps):
+        """nsteps fixed-tau steps, checkNAN after each (ShxError with code ERR_NAN on a NaN)."""
         _check(lib().shx_run(self.h, tau, nsteps))
+
+    def run_cfl(self, cfl, nsteps):
+        """nsteps steps with the CFL time step of each step's starting state; returns the time
+        advanced (ShxError with code ERR_NAN on a NaN)."""
+        t = ctypes.c_double(0.0)
+        _check(lib().shx_run_cfl(self.h, cfl, nsteps, ctypes.byref(t)))
+        return t.value
 
     def simulate(self, start, end, tau, time=0.0, time_last_action=0.0, L0=1.0, u0=1.0, accelerator=False,
                  accel_factor=1.1, decel_factor=0.7, number_resets=10, distance_nan=5000, distance_forward=1000,
@@ -335,6 +356,53 @@ class Case:
         cnt = (ctypes.c_int * NUM_KERNELS)()
         _check(lib().shx_kernel_times(self.h, names, ms, cnt))
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(NUM_KERNELS) if names[i] is not None}
+
+
+class Interface(ctypes.Structure):
+    """One face-to-face connection of a multi-block case (shx_interface)."""
+    _fields_ = [("block", ctypes.c_int), ("face", ctypes.c_int), ("donor", ctypes.c_int),
+                ("donor_face", ctypes.c_int), ("transform", ctypes.c_int * 3)]
+
+
+class Blocks:
+    """The blocks of one case on the device, joined at their SHX_BC_INTERFACE faces (shx_blocks).
+    connections: (block, face, donor, donor_face, (t0, t1, t2)) per interface face, both sides
+    listed.  Close the group before its blocks."""
+
+    def __init__(self, blocks, connections):
+        self.blocks = list(blocks)
+        hs = (ctypes.c_void_p * len(self.blocks))(*[c.h.value for c in self.blocks])
+        arr = (Interface * max(1, len(connections)))()
+        for k, (b, f, d, fd, t) in enumerate(connections):
+            arr[k].block, arr[k].face, arr[k].donor, arr[k].donor_face = b, f, d, fd
+            for q in range(3):
+                arr[k].transform[q] = t[q]
+        h = ctypes.c_void_p()
+        _check(lib().shx_blocks_create(hs, len(self.blocks), arr, len(connections), ctypes.byref(h)))
+        self.h = h
+
+    def transfer_metric(self):
+        _check(lib().shx_blocks_transfer_metric(self.h))
+
+    def run(self, tau, nsteps):
+        _check(lib().shx_blocks_run(self.h, tau, nsteps))
+
+    def close(self):
+        if self.h:
+            lib().shx_blocks_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Batch:

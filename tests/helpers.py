@@ -55,3 +55,30 @@ def setup_3d_turb(n=16, S=5, time_order=4, inviscid=False, quirks=orc.QUIRKS_REF
     prim = cases.turbulence_3d_state(3, S, nn, seed, mach=mach)
     o.ghost_periodic(prim)
     return o, (xe, ye, ze), met, prim, nn
+
+
+def smooth_periodic_state(o, n, seed, mach):
+    """A smooth periodic field on a non-cubic block: three integer-wave-number modes per variable
+    with seeded phases (SHOCK units, c = 1/mach)."""
+    rng = np.random.default_rng(seed)
+    g = o.g
+    I, J, K = np.meshgrid(*[np.arange(n[d]) / n[d] for d in range(3)], indexing="ij")
+    tp = 2.0 * np.pi
+
+    def field(amp):
+        f = np.zeros(I.shape)
+        for _ in range(3):
+            k = rng.integers(1, 4, size=3)
+            f += amp / 3.0 * np.sin(tp * (k[0] * I + k[1] * J + k[2] * K) + rng.uniform(0, tp))
+        return f
+    prim = o.zeros(5)
+    sl = interior(o, 3, n)
+    prim[0][...] = 1.0
+    prim[0][sl] += field(0.1)
+    for m in (1, 2, 3):
+        prim[m][sl] = field(0.4 / mach)
+    p = np.ones(o.shape)
+    p[sl] += field(0.1)
+    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], p, prim[0], o.c.gamma, mach)
+    o.ghost_periodic(prim)
+    return prim

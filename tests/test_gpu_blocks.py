@@ -1,0 +1,202 @@
+"""Multi-block cases on one device (shx_blocks): the reference's zone interfaces with their halo
+transfer (src/Functions.c:3580-3684) and CGNS transforms (src/Functions.c:3902-4287), as an
+on-device gather before every stage.
+
+A periodic box cut into two blocks must evolve as the uncut box: every interior node sees the same
+stencil values, so the blocks' states equal the single block's.
+  * identity cut (3D along i; 2D along j): bit for bit against the single block on the GPU, and
+    against the oracle's single block after one step (1e-12);
+  * rotated cut: block B stores its axes turned a quarter turn about k (its i runs along -j of the
+    box, its j along i), so the interface transform is (2, -1, 3) / (-2, 1, 3) and the ghost metric
+    takes the donor's rows permuted and signed; the general metric (no diagonal-metric quirk).  The
+    sweeps of B run along other box directions than the single block's, so the sums round
+    differently: 1e-12 after three steps.
+  * the ghost metric of the interface faces, zeroed and transferred again, equals the metric the
+    blocks had (bit for bit for the identity cut; within 1e-14 for the rotated one).
+"""
+import numpy as np
+import pytest
+
+import orc
+from helpers import conserved, interior, rel_linf, smooth_periodic_state
+from shock_amd import cases, shockx
+
+pytestmark = pytest.mark.gpu
+
+P = shockx.BC_PERIODIC
+I = shockx.BC_INTERFACE
+
+
+def _case(n, dims, quirks, bc, S=5, inviscid=False):
+    return shockx.Case(dims=dims, space_order=S, time_order=4, n=n, mach=0.5, reynolds=2000.0,
+                       quirks=quirks, bc=bc, inviscid=inviscid)
+
+
+def _box(dims, n, S, seed, quirks=orc.QUIRKS_REFERENCE, inviscid=False):
+    """Whole periodic box: the oracle's configuration, extended coordinates (sine bump in 3D) and a
+    smooth periodic state with its ghosts filled."""
+    o = orc.Oracle(dims=dims, S=S, n=n, quirks=quirks, mach=0.5, reynolds=2000.0, inviscid=inviscid)
+    if dims == 3:
+        coords = cases.periodic_ext_coords(3, S, n, cases.sine_bump_3d(0.05))
+        prim = smooth_periodic_state(o, n, seed, 0.5)
+    else:
+        coords = cases.periodic_ext_coords(2, S, n, cases.cartesian_unit(2))
+        x, y, _ = cases.interior_coords(2, S, n, *coords)
+        prim = cases.mms_periodic_2d_state(np.ascontiguousarray(x), np.ascontiguousarray(y), mach=0.5)
+        o.ghost_periodic(prim)
+    return o, coords, prim
+
+
+def _slab(a, axis, start, length):
+    sl = [slice(None)] * a.ndim
+    sl[axis] = slice(start, start + length)
+    return np.ascontiguousarray(a[tuple(sl)])
+
+
+@pytest.mark.parametrize("dims,n,axis,quirks,S,inviscid", [
+    (3, (24, 16, 16), 0, orc.QUIRKS_REFERENCE, 5, False),
+    (2, (24, 24, 1), 1, orc.QUIRKS_REFERENCE, 9, False),
+])
+def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid):
+    g = (S + 1) // 2
+    o, coords, prim = _box(dims, n, S, 11, quirks, inviscid)
+    half = n[axis] // 2
+    nb = list(n)
+    nb[axis] = half
+    nb = tuple(nb)
+    steps = 3
+    met = o.metric_from_coords(*coords)
+    o.ghost_periodic(met)
+    tau = o.cfl_dt(0.5, met, prim)
+    with _case(n, dims, quirks, (P,) * 6, S, inviscid) as whole:
+        whole.build_metric(*coords)
+        whole.upload_state(prim)
+        whole.run(tau, 1)
+        one = whole.download_state()
+        whole.run(tau, steps - 1)
+        ref = whole.download_state()
+    bc = [P] * 6
+    bc[2 * axis] = bc[2 * axis + 1] = I
+    blocks = []
+    try:
+        for b in range(2):
+            c = _case(nb, dims, quirks, tuple(bc), S, inviscid)
+            blocks.append(c)
+            c.build_metric(*[_slab(x, axis, b * half, nb[axis] + 2 * g + 2) for x in coords])
+            c.upload_state([_slab(f, axis, b * half, nb[axis] + 2 * g) for f in prim])
+        lo, hi = 2 * axis, 2 * axis + 1
+        ident = (1, 2, 3)
+        conn = [(0, hi, 1, lo, ident), (1, lo, 0, hi, ident), (0, lo, 1, hi, ident), (1, hi, 0, lo, ident)]
+        with shockx.Blocks(blocks, conn) as grp:
+            grp.transfer_metric()
+            grp.run(tau, 1)
+            got1 = [np.concatenate([_slab(blocks[b].download_state()[m], axis, g, half) for b in range(2)], axis=axis)
+                    for m in range(5)]
+            grp.run(tau, steps - 1)
+            got = [np.concatenate([_slab(blocks[b].download_state()[m], axis, g, half) for b in range(2)], axis=axis)
+                   for m in range(5)]
+    finally:
+        for c in blocks:
+            c.close()
+    sl = interior(o, dims, n)
+    cut = list(sl)
+    cut[axis] = slice(None)
+    cut = tuple(cut)
+    same = all(np.array_equal(a[cut], _slab(b, axis, g, n[axis])[cut]) for a, b in zip(got, ref))
+    diff = rel_linf(conserved(got), conserved([_slab(b, axis, g, n[axis]) for b in ref]), cut)
+    print(f"[blocks] identity cut dims={dims} S={S}: bit-equal to the whole box: {same}; rel Linf "
+          + " ".join(f"{e:.2e}" for e in diff), flush=True)
+    assert same
+    want1 = o.run(tau, met, prim, 1)
+    errs = rel_linf(conserved(got1), conserved([_slab(b, axis, g, n[axis]) for b in want1]), cut)
+    print(f"[blocks] identity cut one step vs oracle: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= 1e-12
+    assert all(np.array_equal(a[cut], b[cut]) for a, b in
+               zip(got1, [_slab(x, axis, g, n[axis]) for x in one]))
+
+
+def _rot(a):
+    """Box array (padded or extended; block B holds box I in [12, 24) of the interior) -> block B
+    storage: B's i runs along -J of the box, its j along I: B[i', j', k] = box[12 + j', last - i', k]."""
+    return np.ascontiguousarray(a[12:, ::-1, :].transpose(1, 0, 2))
+
+
+def test_rotated_cut_equals_whole_box():
+    S, g = 5, 3
+    n = (24, 16, 16)
+    quirks = 0
+    _, coords, prim = _box(3, n, S, 13, quirks)
+    steps = 3
+    with _case(n, 3, quirks, (P,) * 6, S) as whole:
+        whole.build_metric(*coords)
+        whole.upload_state(prim)
+        tau = whole.cfl_dt(0.5)
+        whole.run(tau, steps)
+        ref = whole.download_state()
+    # block A: box I in [0, 12), unrotated; block B: box I in [12, 24) turned (n = 16 x 12 x 16)
+    A = _case((12, 16, 16), 3, quirks, (I, I, P, P, P, P), S)
+    B = _case((16, 12, 16), 3, quirks, (P, P, I, I, P, P), S)
+    try:
+        A.build_metric(*[np.ascontiguousarray(x[:12 + 2 * g + 2]) for x in coords])
+        A.upload_state([np.ascontiguousarray(f[:12 + 2 * g]) for f in prim])
+        B.build_metric(*[_rot(x) for x in coords])
+        B.upload_state([_rot(f) for f in prim])
+        conn = [(0, 1, 1, 2, (2, -1, 3)), (1, 2, 0, 1, (-2, 1, 3)),
+                (0, 0, 1, 3, (2, -1, 3)), (1, 3, 0, 0, (-2, 1, 3))]
+        with shockx.Blocks([A, B], conn) as grp:
+            ma = A.download_metric()
+            mb = B.download_metric()
+            # zero the interface ghost layers of the metric; the transfer restores them
+            for blk, m0, ax in ((A, ma, 0), (B, mb, 1)):
+                z = [x.copy() for x in m0]
+                for x in z:
+                    s1 = [slice(None)] * 3
+                    s1[ax] = slice(0, g)
+                    x[tuple(s1)] = 0.0
+                    s1[ax] = slice(x.shape[ax] - g, None)
+                    x[tuple(s1)] = 0.0
+                blk.upload_metric(z)
+            grp.transfer_metric()
+            for blk, m0 in ((A, ma), (B, mb)):
+                m1 = blk.download_metric()
+                err = max(float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300) for a, b in zip(m1, m0))
+                print(f"[blocks] rotated cut: transferred ghost metric vs built {err:.2e}", flush=True)
+                assert err <= 1e-14
+            grp.run(tau, steps)
+            sa, sb = A.download_state(), B.download_state()
+    finally:
+        A.close()
+        B.close()
+    # B back to box order: box[12 + j', 21 - i', k] = B[i', j', k]
+    got = []
+    for m in range(5):
+        a = sa[m][g:g + 12, g:g + 16, g:g + 16]
+        b = sb[m][g:g + 16, g:g + 12, g:g + 16].transpose(1, 0, 2)[:, ::-1, :]
+        got.append(np.concatenate([a, b], axis=0))
+    want = [x[g:g + 24, g:g + 16, g:g + 16] for x in ref]
+    full = (slice(None),) * 3
+    errs = rel_linf(conserved(got), conserved(want), full)
+    print(f"[blocks] rotated cut {steps} steps vs whole box: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-12
+
+
+def test_group_refusals():
+    n = (12, 12, 12)
+    _, coords, prim = _box(3, (24, 12, 12), 5, 3, 0)
+    A = _case(n, 3, 0, (I, I, P, P, P, P))
+    try:
+        A.build_metric(*[np.ascontiguousarray(x[:12 + 8]) for x in coords])
+        A.upload_state([np.ascontiguousarray(f[:12 + 6]) for f in prim])
+        with pytest.raises(shockx.ShxError):
+            A.run(1e-4, 1)                       # interface faces need the group
+        with pytest.raises(shockx.ShxError):
+            shockx.Blocks([A], [(0, 1, 0, 0, (1, 2, 3))])          # face 0 not listed
+        with pytest.raises(shockx.ShxError):
+            shockx.Blocks([A], [(0, 1, 0, 0, (2, 1, 3)), (0, 0, 0, 1, (2, 1, 3))])   # normal not mapped to normal
+        with shockx.Blocks([A], [(0, 1, 0, 0, (1, 2, 3)), (0, 0, 0, 1, (1, 2, 3))]) as grp:
+            with pytest.raises(shockx.ShxError):
+                A.close()                        # a block of a live group
+            grp.transfer_metric()
+            grp.run(1e-4, 1)                     # a block joined to itself: a periodic pair
+    finally:
+        A.close()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import orc
-from helpers import conserved, interior, rel_linf, setup_2d_mms, setup_3d_turb
+from helpers import conserved, interior, rel_linf, setup_2d_mms, setup_3d_turb, smooth_periodic_state
 from shock_amd import cases, shockx
 
 pytestmark = pytest.mark.gpu
@@ -220,33 +220,6 @@ def test_cfl_and_nan():
     assert fixed[0][o.g + 2, o.g + 3, o.g + 4] == 1e6 and fixed[1][o.g + 5, o.g + 1, o.g + 7] == 1e6
 
 
-def _smooth_periodic_state(o, n, seed, mach):
-    """A smooth periodic field on a non-cubic block: three integer-wave-number modes per variable
-    with seeded phases (SHOCK units, c = 1/mach)."""
-    rng = np.random.default_rng(seed)
-    g = o.g
-    I, J, K = np.meshgrid(*[np.arange(n[d]) / n[d] for d in range(3)], indexing="ij")
-    tp = 2.0 * np.pi
-
-    def field(amp):
-        f = np.zeros(I.shape)
-        for _ in range(3):
-            k = rng.integers(1, 4, size=3)
-            f += amp / 3.0 * np.sin(tp * (k[0] * I + k[1] * J + k[2] * K) + rng.uniform(0, tp))
-        return f
-    prim = o.zeros(5)
-    sl = interior(o, 3, n)
-    prim[0][...] = 1.0
-    prim[0][sl] += field(0.1)
-    for m in (1, 2, 3):
-        prim[m][sl] = field(0.4 / mach)
-    p = np.ones(o.shape)
-    p[sl] += field(0.1)
-    prim[4][...] = cases.energy(prim[1], prim[2], prim[3], p, prim[0], o.c.gamma, mach)
-    o.ghost_periodic(prim)
-    return prim
-
-
 @pytest.mark.parametrize("S,quirks", [(5, orc.QUIRKS_REFERENCE), (5, 0), (9, orc.QUIRKS_REFERENCE), (9, 0)])
 def test_one_step_across_march_chunks_and_columns(S, quirks):
     """40 x 12 x 64 block: the derivative march splits i into two chunks of planes and k into two
@@ -258,7 +231,7 @@ def test_one_step_across_march_chunks_and_columns(S, quirks):
     xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
     met = o.metric_from_coords(xe, ye, ze)
     o.ghost_periodic(met)
-    prim = _smooth_periodic_state(o, nn, 31 + S, 0.5)
+    prim = smooth_periodic_state(o, nn, 31 + S, 0.5)
     tau = o.cfl_dt(0.5, met, prim)
     ref = o.run(tau, met, prim, 1)
     with _case(o, 3, S, 4, nn, quirks=quirks) as g:
@@ -269,4 +242,37 @@ def test_one_step_across_march_chunks_and_columns(S, quirks):
     sl = interior(o, 3, nn)
     errs = rel_linf(conserved(out), conserved(ref), sl)
     _report(f"step3d 40x12x64 S={S} quirks={quirks}", errs)
+    assert max(errs) <= TOL_STEP
+
+
+@pytest.mark.parametrize("S", [5, 9])
+def test_zeta_quirk_with_eta_longer_than_zeta(S):
+    """14 x 20 x 12 block with the reference's quirks: the zeta viscous stencil takes its k offset
+    from j (src/ZD.c:547-548), and with n_eta > n_zeta the offsets run past the end of the k row, so
+    the reference reads the next row's first nodes (its unpadded k-fastest index); the oracle indexes
+    the same way.  Stage residual and one step against the oracle."""
+    nn = (14, 20, 12)
+    o = orc.Oracle(dims=3, S=S, n=nn, quirks=orc.QUIRKS_REFERENCE, mach=0.5, reynolds=2000.0)
+    assert o.shape[1] > o.shape[2]
+    xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
+    met = o.metric_from_coords(xe, ye, ze)
+    o.ghost_periodic(met)
+    prim = smooth_periodic_state(o, nn, 17 + S, 0.5)
+    q_ref, _, _ = o.rhs(met, [a.copy() for a in prim])
+    tau = o.cfl_dt(0.5, met, prim)
+    ref = o.run(tau, met, prim, 1)
+    sl = interior(o, 3, nn)
+    with _case(o, 3, S, 4, nn) as g:
+        g.build_metric(xe, ye, ze)
+        g.upload_state(prim)
+        q = g.rhs()
+        g.calc_runge_kutta(tau)
+        out = g.download_state()
+    errs = rel_linf(q, q_ref, sl)
+    _report(f"rhs3d 14x20x12 zeta quirk S={S}", errs)
+    # the residual is a difference of fluxes: WENO9 with the reference's epsilon moves it by up
+    # to ~5e-12 under rounding; a wrong row mapping would move it by O(1)
+    assert max(errs) <= (1e-12 if S == 5 else 1e-10)
+    errs = rel_linf(conserved(out), conserved(ref), sl)
+    _report(f"step3d 14x20x12 zeta quirk S={S}", errs)
     assert max(errs) <= TOL_STEP
