@@ -240,6 +240,16 @@ int shx_history_download(shx_case *c, double *time, double *p, double *u, double
  * divided by the full interior node count, p the stored pressure (as in shx_post).  FP64 sums where
  * the reference accumulates in long double, against exact values rounded to FP64. */
 int shx_mms_error(shx_case *c, double out[4]);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): shx_mms_error with its
+ * partial sums in a static device array of the library (256 KiB per device, held until the
+ * process ends) instead of per-call allocations, so the call allocates nothing.  Limit: at most
+ * 4096 * 256 = 1048576 interior nodes per k plane (n0 * n1), else SHX_ERR_ARG.  The case must
+ * already keep its stored pressure (after one shx_mms_error, shx_post or shx_film_open call, or
+ * from creation for cases with subsonic outflow or slip wall faces), else SHX_ERR_STATE.  Every
+ * call on a device shares that device's array: calls on one device must not overlap (one host
+ * thread, or streams ordered against each other). */
+int shx_mms_error_static(shx_case *c, double out[4]);
 
 /* Stage right-hand side of the current U_lastStep (ghost fill, CalcValues, all WENO and
  * viscous fluxes and the source) returned in q[5] (strct_Flux Q after one stage's flux

@@ -2256,7 +2256,7 @@ struct shx_case {
     int *d_cnt;
     int *d_nan;            // running count of the fused checkNAN (shx_run, shx_run_cfl, shx_simulate)
     int nan_seen;          // the count last read
-    struct HostIo { unsigned long long red[RED_SLOTS]; int nan; int pad; } *h_io;   // pinned landing slots
+    struct HostIo { unsigned long long red[RED_SLOTS]; int nan; int pad; double mms[4]; } *h_io;   // pinned landing slots
     int dev;               // the device current at shx_create (the stream's)
     int st_in, st_out, st_bx, st_by;   // buffers of the step in progress (step_begin .. step_end)
     int iface;             // faces of kind SHX_BC_INTERFACE
@@ -4370,6 +4370,67 @@ extern "C" int shx_mms_error(shx_case *c, double out[4])
     HIPCHK(hipStreamSynchronize(c->stream));
     hipFree(part);
     hipFree(dout);
+    return SHX_OK;
+}
+
+// ---- the entry point that allocates nothing -------------------------------------------------
+// shx_mms_error_static: shx_mms_error with its partial sums and result in a file-scope device
+// array instead of per-call allocations, for callers that cannot allocate (code under HIP graph
+// capture, serving loops).  Every call on a device shares that device's array: calls that overlap
+// on one device race.  No element needs clearing: k_mms_error writes every partial it later sums
+// and k_mms_error_total writes every result.
+constexpr int SCRATCH_BLOCKS = 4096;                   // n0 * n1 <= SCRATCH_BLOCKS * 256 interior nodes
+static __device__ __align__(256) double scratch[4 + 4 * SCRATCH_BLOCKS];
+constexpr int MAX_DEVICES = 64;
+static double *g_scratch[MAX_DEVICES];                 // scratch's address on each device
+
+// scratch's device address on the case's device, looked up once per device with that device
+// current (in host code the array's name is only a stand-in for the symbol)
+static int scratch_ptr(const shx_case *c, double **ptr)
+{
+    if (c->dev < 0 || c->dev >= MAX_DEVICES) return set_err(SHX_ERR_UNSUPPORTED, "device %d", c->dev);
+    if (!g_scratch[c->dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != c->dev) HIPCHK(hipSetDevice(c->dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != c->dev) HIPCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p) return set_err(SHX_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch[c->dev] = (double *)p;
+    }
+    *ptr = g_scratch[c->dev];
+    return SHX_OK;
+}
+
+extern "C" int shx_mms_error_static(shx_case *c, double out[4])
+{
+    if (!c || !out) return set_err(SHX_ERR_ARG, "null argument");
+    const Geom &gm = c->gm;
+    const long nplane = (long)gm.n[0] * gm.n[1];
+    const long nb = (long)nblk(nplane, 256);
+    if (nb > SCRATCH_BLOCKS)
+        return set_err(SHX_ERR_ARG, "shx_mms_error_static: n0 * n1 = %ld interior nodes per k plane exceeds the "
+                       "static scratch (at most %d)", nplane, SCRATCH_BLOCKS * 256);
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->has_bs) return set_err(SHX_ERR_STATE, "no exact state: call shx_upload_boundary_state");
+    if (!c->pc)
+        return set_err(SHX_ERR_STATE, "shx_mms_error_static reads the stored pressure, which this case does not "
+                       "keep yet (keeping it allocates): call shx_mms_error, shx_post or shx_film_open once first");
+    double *scr = nullptr;
+    if ((rc = scratch_ptr(c, &scr))) return rc;
+    double *const dout = scr, *const part = scr + 4;
+    CFields5 cp;
+    for (int m = 0; m < 5; m++) cp.f[m] = c->prim[c->cur][m];
+    const double N = (double)gm.n[0] * gm.n[1] * gm.n[2];
+    k_mms_error<<<(unsigned)nb, 256, 0, c->stream>>>(cp, c->P, c->bs[0], c->bs[5], gm, N, part);
+    LAUNCH_CHECK();
+    k_mms_error_total<<<1, 256, 0, c->stream>>>(part, (int)nb, dout);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(c->h_io->mms, dout, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int m = 0; m < 4; m++) out[m] = c->h_io->mms[m];
     return SHX_OK;
 }
 

@@ -78,7 +78,7 @@ EXPORTS = [
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
     "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
     "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download", "shx_mms_error",
-    "shx_run_cfl",
+    "shx_run_cfl", "shx_mms_error_static",
     "shx_blocks_create", "shx_blocks_transfer_metric", "shx_blocks_run", "shx_blocks_destroy",
 ]
 ERR_NAN = -5
