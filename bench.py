@@ -113,6 +113,7 @@ def main():
                          "step per step")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the default workload")
     ap.add_argument("--no-config0", action="store_true", help="skip the 2D 64^2 CPU-runnable configuration")
+    ap.add_argument("--no-serve", action="store_true", help="skip the short serving run of the default line")
     ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
                     default="turbulence",
                     help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
@@ -175,7 +176,23 @@ def main():
                 "sample": f"C oracle, one thread, {args.cpu1_n}^3 block, 1 step = 4 stages, {secs1:.1f}s"}
         if not args.no_config0:
             out["config0"] = config0_run(args)
+    if not args.no_serve:
+        out["serve"] = serve_short(args)
     print(json.dumps(out), flush=True)
+
+
+def serve_short(args):
+    """A short serving run beside the throughput line: 10 Poisson requests at 2 per second, 3D
+    periodic turbulence or shock cases of 48^3-96^3, FP64; time-to-solution p50/p99 and the
+    aggregate rate."""
+    from shock_amd import serve
+    reqs = serve.make_requests(10, 2.0, args.seed, n_lo=48, n_hi=96, t_lo=0.05, t_hi=0.2, shock_frac=0.5)
+    stats, done = serve.Server().run(reqs)
+    return {"workload": "10 Poisson requests at 2/s, 3D periodic turbulence or Mach 1.5-3.5 shock cases, "
+                        "48^3-96^3 (log-uniform), t_end U(0.05, 0.2), W5/W9, FP64",
+            "requests": stats["requests"], "p50_s": round(stats["p50_s"], 3), "p99_s": round(stats["p99_s"], 3),
+            "queue_wait_p50_s": round(stats["queue_wait_p50_s"], 3),
+            "cell_updates_per_s": round(stats["cell_updates_per_s"], 1), "wall_s": round(stats["wall_s"], 2)}
 
 
 def host_info():
@@ -352,16 +369,16 @@ def turbulence_run(args, precision):
     return {"line": line}
 
 
-def batch_cases(count, seed, precision):
+def batch_cases(count, seed, precision, lo=32 ** 2, hi=160 ** 3):
     """Batch workload: sizes log-uniform in node count from 32^2 (2D) to 160^3 (3D) -- counts up to
     256^2 become 2D squares, larger ones 3D cubes; Mach U(0.2, 3), Reynolds logU(1e2, 1e4), WENO5 or
     WENO9; 2D cases start from the manufactured field of the 2D parity case, 3D cases from the
-    random-phase turbulence on the curvilinear box."""
+    random-phase turbulence on the curvilinear box.  lo, hi: the node-count range."""
     import math
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        N = math.exp(rng.uniform(math.log(32 ** 2), math.log(160 ** 3)))
+        N = math.exp(rng.uniform(math.log(lo), math.log(hi)))
         S = int(rng.choice([5, 9]))
         mach = float(rng.uniform(0.2, 3.0))
         re = float(math.exp(rng.uniform(math.log(1e2), math.log(1e4))))
@@ -387,10 +404,19 @@ def batch_cases(count, seed, precision):
 
 def batch_workload(args):
     """All cases advanced together from one graph replay per step, against the same cases stepped
-    one after another through the single-case path."""
+    one after another through the single-case path: the configured mix (32^2 .. 160^3), and a mix
+    of small cases only (16^2 .. 40^3), where launches rather than the largest case set the time."""
+    out = batch_mix(args, 32 ** 2, 160 ** 3)
+    small = batch_mix(args, 16 ** 2, 40 ** 3)
+    out["small_mix"] = {k: small[k] for k in ("value", "ms_per_step", "sequential_value", "batch_speedup")}
+    out["small_mix"]["config"] = small["config"]
+    print(json.dumps(out), flush=True)
+
+
+def batch_mix(args, lo, hi):
     prec = shockx.MIXED if args.precision == "mixed" else shockx.FP64
     t0 = time.perf_counter()
-    cs = batch_cases(args.cases, args.seed, prec)
+    cs = batch_cases(args.cases, args.seed, prec, lo, hi)
     cells = sum(x[1] for x in cs)
     print(f"[batch] {len(cs)} cases, {cells:.3e} nodes, setup {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
@@ -425,7 +451,7 @@ def batch_workload(args):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if prec == shockx.FP64 else "f64+f32", "data": "synthetic (seeded cases)",
-           "config": {"workload": f"batch of {len(cs)} independent cases (32^2 2D to 160^3 3D, W5/W9, "
+           "config": {"workload": f"batch of {len(cs)} independent cases ({lo} to {hi} nodes, 2D up to 256^2, W5/W9, "
                                   f"Mach 0.2-3, Re 1e2-1e4), one graph replay per RK step",
                       "nodes": cells, "mix": mix, "seed": args.seed},
            "sequential_value": round(cells * 4 * args.steps / seq, 1),
@@ -433,7 +459,8 @@ def batch_workload(args):
            "tune_s": round(tune_s, 2)}
     for c, _, _, _ in cs:
         c.close()
-    print(json.dumps(out), flush=True)
+    print(f"[batch] {lo}..{hi} nodes: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
 
 
 def serve_workload(args):

@@ -283,7 +283,7 @@ int shx_copy_probe(shx_case *c, int reps);
 /* Per-kernel timing with HIP events on the handle's stream.  When enabled, every launch is
  * bracketed by events; shx_kernel_times synchronizes and returns, per kernel class, the total
  * milliseconds and the launch count since the last reset.  names[k] points to static strings. */
-#define SHX_NUM_KERNELS 15
+#define SHX_NUM_KERNELS 16
 int shx_set_timing(shx_case *c, int enable);
 int shx_kernel_times(shx_case *c, const char *names[SHX_NUM_KERNELS], double ms[SHX_NUM_KERNELS],
                      int launches[SHX_NUM_KERNELS]);

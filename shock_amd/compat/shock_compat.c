@@ -237,6 +237,28 @@ static void swap_bytes(void *mem, size_t size, int count)
     }
 }
 
+int compat_division_connections(const struct compat_division *zones, int nzones, int dims,
+                                struct shx_interface *out)
+{
+    int n = 0;
+    for (int z = 0; z < nzones; z++)
+        for (int f = 0; f < 2 * dims; f++) {
+            const struct compat_interface *it = &zones[z].face[f];
+            if (!it->neighbour) continue;
+            int df = -1;
+            for (int k = 0; k < dims && df < 0; k++)
+                if (it->drange[0][k] == it->drange[1][k]) df = 2 * k + (it->drange[0][k] == 1 ? 0 : 1);
+            if (df < 0) return -1;
+            struct shx_interface *e = &out[n++];
+            e->block = z;
+            e->face = f;
+            e->donor = it->neighbour - 1;
+            e->donor_face = df;
+            for (int k = 0; k < 3; k++) e->transform[k] = (k < dims) ? it->transform[k] : k + 1;
+        }
+    return n;
+}
+
 int compat_load_division(const char *path, int dims, int proccount, unsigned divzone, int swap,
                          struct compat_division *out)
 {

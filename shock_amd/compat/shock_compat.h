@@ -94,6 +94,16 @@ struct compat_division {
 int compat_load_division(const char *path, int dims, int proccount, unsigned divzone, int swap,
                          struct compat_division *out);
 
+/* The face-to-face connections of a whole division (zones[z] = record z + 1), for
+ * shx_blocks_create (shockx.h): every face with a neighbour is joined to the neighbour's face on
+ * which the donor range lies (its constant index: 1 is the lower face, as
+ * check_TransformationMatrix reads it, src/Functions.c:3902-3990), with the record's transform;
+ * zone z is block z.  Writes at most 2 * dims * nzones entries and returns their number, or -1
+ * when a donor range lies on no face. */
+struct shx_interface;
+int compat_division_connections(const struct compat_division *zones, int nzones, int dims,
+                                struct shx_interface *out);
+
 void compat_define(struct strct_configuration *cfg);
 void compat_release(struct strct_configuration *cfg);
 

@@ -1358,6 +1358,38 @@ __global__ __launch_bounds__(256, ((S == 5 && !MIX) ? 4 : 1)) void k_zeta(ZetaAr
     zeta_body<S, MIX>(z, bx, by, bz, lds);
 }
 
+// ---- fused eta direction: WENO and viscous eta sweeps in one kernel ----------------------------
+// One block sweeps an 8 x 32 tile along eta twice, as the fused zeta kernel does along zeta: the
+// WENO phase keeps each node's inviscid divergence in registers, the viscous phase (after a barrier,
+// in the same LDS) adds it and its own terms to Q, which this direction reads and writes once.
+// 3D diagonal-metric viscous WENO5 cases, where the viscous xi terms ride on the derivative march.
+struct EtaArgs { WenoArgs w; ViscArgs v; };
+template <bool MIX>
+constexpr int eta_lds()
+{
+    return weno_lds<5, 8, 32, 1, MIX>() > visc_lds<5, 1, 8, 32, true>() ? weno_lds<5, 8, 32, 1, MIX>()
+                                                                       : visc_lds<5, 1, 8, 32, true>();
+}
+template <bool MIX>
+__device__ __forceinline__ void eta_body(const EtaArgs &z, const int bx, const int by, const int bz, double *lds)
+{
+    double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    weno_body<5, 8, 32, 1, MIX, true>(z.w, bx, by, bz, lds, inv);
+    __syncthreads();
+    visc_body<5, 1, 8, 32, true, true>(z.v, bx, by, bz, lds, inv);
+}
+// FP64: three waves per SIMD (140 VGPRs, no spills) measured 1.53 ms at 256^3 against 1.82 ms when
+// held to 128 VGPRs for four (which spills)
+constexpr int ETA_WAVES = 3;
+template <bool MIX>
+__global__ __launch_bounds__(256, (MIX ? 1 : ETA_WAVES)) void k_eta(EtaArgs z)
+{
+    __shared__ double lds[eta_lds<MIX>()];
+    int bx, by, bz;
+    sweep_tile(blockIdx.x, z.w.tg, bx, by, bz);
+    eta_body<MIX>(z, bx, by, bz, lds);
+}
+
 // Stage update (src/Functions.c:2300-2365) + manufactured source: pointwise.
 struct UpdArgs {
     Fields5 q;
@@ -1853,6 +1885,24 @@ __global__ __launch_bounds__(256, ((S == 5 && !MIX) ? 4 : 1)) void k_zeta_batch(
     });
 }
 
+template <bool MIX, bool P>
+__global__ __launch_bounds__(256, (MIX ? 1 : ETA_WAVES)) void k_eta_batch(BATCH_PARAMS(EtaArgs))
+{
+    __shared__ double lds[eta_lds<MIX>()];
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        eta_body<MIX>(rec(A, w.x), w.y, w.z, w.w, lds);
+    });
+}
+template <bool P>
+__global__ __launch_bounds__(GK * GJX) void k_march_xi_batch(BATCH_PARAMS(GradArgs))
+{
+    batch_walk<P>(total, [&](const int b) {
+        const int4 w = rec(work, b);
+        march_body<5, false, true, GJX>(rec(A, w.x), w.y, w.z, w.w);
+    });
+}
+
 // Q += J * src (manufactured-solution source) for the stage residual returned by shx_rhs.
 __global__ void k_add_source(Fields5 q, CFields5 src, const double *J, Geom gm)
 {
@@ -2195,11 +2245,11 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                               \
     } while (0)
 
-enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA, KT_GRADXI };
+enum { KT_GHOST, KT_VALUES, KT_WENO0, KT_WENO1, KT_WENO2, KT_GRAD, KT_VISC0, KT_VISC1, KT_VISC2, KT_RK, KT_REDUCE, KT_METRIC, KT_FACES, KT_ZETA, KT_GRADXI, KT_ETA };
 static const char *g_knames[SHX_NUM_KERNELS] = {
     "ghost_periodic", "ghost_values", "weno_xi", "weno_eta", "weno_zeta",
     "visc_grad", "visc_xi", "visc_eta", "visc_zeta", "rk_update", "reduce", "metric", "ghost_faces",
-    "weno_visc_zeta", "visc_grad_xi"};
+    "weno_visc_zeta", "visc_grad_xi", "weno_visc_eta"};
 
 // face kinds whose writers store a p and c that the equation of state does not give from the
 // ghost's rho,u,v,w,e (subsonic outflow; the slip wall, whose ghost velocity is not the mirror's)
@@ -2891,42 +2941,55 @@ static bool fuse_xi(const shx_case *c)
     return c->gm.dims == 3 && c->diag && !c->d.inviscid && c->d.space_order == 5;
 }
 
+// the eta direction runs as one fused kernel (k_eta) where the viscous xi terms ride on the march
+// and the zeta direction is fused too
+static bool fuse_eta(const shx_case *c) { return fuse_xi(c) && fuse_zeta(c); }
+
 static double iso_factor(const shx_case *c);
+// argument record of the 3D derivative march on stage input pr; xi: with the viscous xi terms
+static GradArgs grad_args(const shx_case *c, double *const *pr, const bool xi)
+{
+    GradArgs ga;
+    memset(&ga, 0, sizeof ga);
+    for (int m = 0; m < 5; m++) ga.pr.f[m] = pr[m];
+    for (int m = 0; m < 12; m++) ga.gr.f[m] = c->grad[m];
+    ga.T = c->T; ga.gm = c->gm; ga.k = c->k; ga.ci = 32; ga.tgrad = c->diag ? 0 : 1;
+    if (xi) {
+        for (int m = 0; m < 5; m++) ga.q.f[m] = c->q[m];
+        ga.mu = c->mu; ga.m0 = c->met[0]; ga.m4 = c->met[4]; ga.m8 = c->met[8]; ga.J = c->met[9];
+        ga.iso_hi = (c->d.bc[1] == SHX_BC_WALL_ISOTHERMAL) ? 1 : 0;
+        ga.gam_iso = iso_factor(c);
+    }
+    return ga;
+}
+static dim3 march_grid(const Geom &gm, const bool xi)
+{
+    return dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], xi ? GJX : GJ), nblk(gm.ng[0], 32));
+}
 // Nodal derivatives of the viscous terms (3D: the march; 2D: pointwise).  xi: the march also adds
-// the viscous xi terms to Q (fuse_xi cases, after the WENO sweeps of xi and eta).
+// the viscous xi terms to Q (fuse_xi cases, after the WENO xi sweep).
 static int grad_march(shx_case *c, double *const *pr, const bool xi)
 {
     const Geom &gm = c->gm;
-    CFields5 cp;
-    Fields12 gf;
-    for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
-    for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
     int rc = tbeg(c, xi ? KT_GRADXI : KT_GRAD);
     if (rc) return rc;
     if (gm.dims == 3) {
-        constexpr int CI = 32;
-        const int tgrad = c->diag ? 0 : 1;
-        GradArgs ga;
-        memset(&ga, 0, sizeof ga);
-        ga.pr = cp; ga.T = c->T; ga.gr = gf; ga.gm = gm; ga.k = c->k; ga.ci = CI; ga.tgrad = tgrad;
-        const void *fn = march_kernel(c->d.space_order, tgrad, false);
-        if (xi) {
-            for (int m = 0; m < 5; m++) ga.q.f[m] = c->q[m];
-            ga.mu = c->mu; ga.m0 = c->met[0]; ga.m4 = c->met[4]; ga.m8 = c->met[8]; ga.J = c->met[9];
-            ga.iso_hi = (c->d.bc[1] == SHX_BC_WALL_ISOTHERMAL) ? 1 : 0;
-            ga.gam_iso = iso_factor(c);
-            fn = (const void *)&k_visc_grad_march_xi<5>;
-        }
-        const int gj = xi ? GJX : GJ;
-        const dim3 grid = flat_grid(dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], gj), nblk(gm.ng[0], CI)), ga.tg, 0);
+        GradArgs ga = grad_args(c, pr, xi);
+        const void *fn = xi ? (const void *)&k_visc_grad_march_xi<5> : march_kernel(c->d.space_order, ga.tgrad, false);
+        const dim3 grid = flat_grid(march_grid(gm, xi), ga.tg, 0);
         void *args[] = {&ga};
-        HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * gj), args, 0, c->stream));
+        HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * (xi ? GJX : GJ)), args, 0, c->stream));
     } else {
+        CFields5 cp;
+        Fields12 gf;
+        for (int m = 0; m < 5; m++) cp.f[m] = pr[m];
+        for (int m = 0; m < 12; m++) gf.f[m] = c->grad[m];
         k_visc_grad<<<nblk(gm.ntot, 256), 256, 0, c->stream>>>(cp, c->T, gf, gm, c->k);
     }
     LAUNCH_CHECK();
     return tend(c);
 }
+static int eta_sweep(shx_case *c, double *const *pr);
 // rk: when given, the last viscous sweep applies this stage update instead of storing Q
 static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double tau)
 {
@@ -2956,14 +3019,21 @@ static int stage_fluxes(shx_case *c, double *const *pr, const RkArgs *rk, double
     // the order is WENO xi, eta, viscous xi, eta, then WENO zeta and viscous zeta together.  With
     // the viscous xi direction in the derivative march, the march runs after the WENO xi and eta
     // sweeps, in the place of the xi viscous sweep.
-    const bool fz = fuse_zeta(c);
+    // With the eta direction fused as well, the order is WENO xi, the march with viscous xi, then
+    // the fused eta and zeta kernels (each WENO and viscous together).
+    const bool fz = fuse_zeta(c), fe = fuse_eta(c);
     const int nsep = fz ? gm.dims - 1 : gm.dims;
-    for (int dir = 0; dir < nsep; dir++)
+    for (int dir = 0; dir < nsep; dir++) {
+        if (fe && dir == 1) continue;
         if ((rc = weno_sweep(c, pr, dir, dir == 0))) return rc;
+    }
     if (fx && (rc = grad_march(c, pr, true))) return rc;
     if (!c->d.inviscid)
-        for (int dir = fx ? 1 : 0; dir < nsep; dir++)
-            if ((rc = visc_sweep(c, pr, dir, dir == gm.dims - 1 ? rk : nullptr))) return rc;
+        for (int dir = fx ? 1 : 0; dir < nsep; dir++) {
+            if (fe && dir == 1) rc = eta_sweep(c, pr);
+            else rc = visc_sweep(c, pr, dir, dir == gm.dims - 1 ? rk : nullptr);
+            if (rc) return rc;
+        }
     if (fz && (rc = zeta_sweep(c, pr, rk))) return rc;
     return SHX_OK;
 }
@@ -3080,6 +3150,33 @@ static int zeta_sweep(shx_case *c, double *const *pr, const RkArgs *rk)
     ZetaArgs z = zeta_args(c, pr, rk);
     const Pick p = zeta_pick(c);
     int rc = tbeg(c, KT_ZETA);
+    if (rc) return rc;
+    const dim3 grid = flat_grid(sweep_grid(z.w.nA, z.w.oA, z.w.alignA, z.w.nB, z.w.nD, p), z.w.tg, 1);
+    if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &z, c->stream))) return rc;
+    return tend(c);
+}
+
+template <bool MIX>
+static Pick eta_pick_t()
+{
+    return {(const void *)&k_eta<MIX>, (const void *)&k_eta_batch<MIX, false>, (const void *)&k_eta_batch<MIX, true>, 8, 32};
+}
+static Pick eta_pick(const shx_case *c)
+{
+    return (c->d.precision == SHX_MIXED) ? eta_pick_t<true>() : eta_pick_t<false>();
+}
+static EtaArgs eta_args(const shx_case *c, double *const *pr)
+{
+    EtaArgs z;
+    z.w = weno_args(c, pr, 1, 0);
+    z.v = visc_args(c, pr, 1, nullptr);
+    return z;
+}
+static int eta_sweep(shx_case *c, double *const *pr)
+{
+    EtaArgs z = eta_args(c, pr);
+    const Pick p = eta_pick(c);
+    int rc = tbeg(c, KT_ETA);
     if (rc) return rc;
     const dim3 grid = flat_grid(sweep_grid(z.w.nA, z.w.oA, z.w.alignA, z.w.nB, z.w.nD, p), z.w.tg, 1);
     if ((rc = launch_pick(p.single, grid, p.TA * p.TDI, &z, c->stream))) return rc;
@@ -3204,11 +3301,14 @@ static int nan_found(shx_case *c, int &fresh)
 // interface face of every block from the donor blocks' stage input (the reference's
 // TransferFlowParameterWithGhosts, src/Functions.c:3580-3684, whose repeated ghost-inclusive passes
 // settle edges and corners).  A ghost node is followed through the faces it lies beyond, one at a
-// time in axis order: a periodic face wraps the index, an interface face maps it into the donor
-// block (normal layer l outside the face = layer l inside the donor's face; tangential indices
-// through the transform); the value is read once the index is interior.  A path that meets a face
-// with a boundary writer ends without a write (such corners belong to the writers).  Every read is
-// an interior node and every write a ghost node, so one launch serves all blocks.
+// time in axis order, periodic and interface faces first: a periodic face wraps the index, an
+// interface face maps it into the donor block (normal layer l outside the face = layer l inside the
+// donor's face; tangential indices through the transform).  The value is read where no such face
+// is left: an interior node, or a ghost node beyond boundary-writer faces only, whose value is the
+// donor's writer output of the previous stage (carried into the stage buffer), as in the reference,
+// whose transfers copy the donors' ghost-inclusive planes before its writers run.  A node that is
+// read is never one that is written (written nodes lie beyond an interface face), so one launch
+// serves all blocks.
 enum { XK_WRITER = 0, XK_PERIODIC = 1, XK_INTERFACE = 2 };
 struct XRec {
     Geom gm;
@@ -3241,17 +3341,20 @@ __global__ __launch_bounds__(256) void k_interface_gather(XTab t)
     int b = b0;
     int perm[3] = {0, 1, 2};             // axis of block b that each axis of block b0 runs along
     double sgn[3] = {1.0, 1.0, 1.0};
+    bool moved = false;
     for (int it = 0; it < 2 * dims; it++) {
         const XRec &R = t.r[b];
         int a = -1, up = 0;
         for (int d = 0; d < dims && a < 0; d++) {
-            if (x[d] < R.gm.lo[d]) { a = d; up = 0; }
-            else if (x[d] >= R.gm.lo[d] + R.gm.n[d]) { a = d; up = 1; }
+            int u = -1;
+            if (x[d] < R.gm.lo[d]) u = 0;
+            else if (x[d] >= R.gm.lo[d] + R.gm.n[d]) u = 1;
+            if (u >= 0 && R.kind[2 * d + u] != XK_WRITER) { a = d; up = u; }
         }
         if (a < 0) break;
+        moved = true;
         const int f = 2 * a + up;
         if (R.kind[f] == XK_PERIODIC) { x[a] += up ? -R.gm.n[a] : R.gm.n[a]; continue; }
-        if (R.kind[f] != XK_INTERFACE) return;
         const XRec &D = t.r[R.donor[f]];
         const int fd = R.dface[f], ad = fd >> 1, dup = fd & 1;
         const int l = up ? x[a] - (R.gm.lo[a] + R.gm.n[a]) : R.gm.lo[a] - 1 - x[a];
@@ -3271,9 +3374,10 @@ __global__ __launch_bounds__(256) void k_interface_gather(XTab t)
         for (int q = 0; q < 3; q++) x[q] = y[q];
         b = R.donor[f];
     }
+    if (!moved) return;
     const XRec &S = t.r[b];
     for (int d = 0; d < dims; d++)
-        if (x[d] < S.gm.lo[d] || x[d] >= S.gm.lo[d] + S.gm.n[d]) return;
+        if (x[d] < 0 || x[d] >= S.gm.ng[d]) return;
     const long ns = (long)x[0] * S.gm.st[0] + (long)x[1] * S.gm.st[1] + (long)x[2] * S.gm.st[2];
     if (!t.metric) {
 #pragma unroll
@@ -3716,16 +3820,20 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             }
             batch_add(b, (const void *)&k_ghost_values_batch<false>, (const void *)&k_ghost_values_batch<true>, 256, BK_VALUES, recs, sizeof(GhostValuesArgs), grids);
             add_faces();
-            if (dims == 3)
-                batch_add(b, march_kernel(S, !c0->diag, true, false), march_kernel(S, !c0->diag, true, true), GK * GJ,
-                          BK_GRAD, grecs, sizeof(GradArgs), ggrids, ORDER_NODES);
-            else
+            if (dims == 3) {
+                // the march with the viscous xi terms follows the WENO xi sweep (below)
+                if (!fuse_xi(c0))
+                    batch_add(b, march_kernel(S, !c0->diag, true, false), march_kernel(S, !c0->diag, true, true), GK * GJ,
+                              BK_GRAD, grecs, sizeof(GradArgs), ggrids, ORDER_NODES);
+            } else {
                 batch_add(b, (const void *)&k_grad2d_batch<false>, (const void *)&k_grad2d_batch<true>, 256, BK_GRAD, grecs, sizeof(GradArgs), ggrids);
+            }
         }
-        // the same sweep sequence as stage_fluxes (fused zeta direction where it applies)
-        const bool fz = fuse_zeta(c0);
+        // the same sweep sequence as stage_fluxes (fused eta and zeta directions where they apply)
+        const bool fz = fuse_zeta(c0), fx = fuse_xi(c0), fe = fuse_eta(c0);
         const int nsep = fz ? dims - 1 : dims;
         for (int dir = 0; dir < nsep; dir++) {
+            if (fe && dir == 1) continue;
             const Pick p = weno_pick(c0, dir);
             std::vector<char> recs;
             std::vector<dim3> grids;
@@ -3736,12 +3844,35 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             }
             batch_add(b, p.batch, p.batchp, p.TA * p.TDI, BK_WENO, recs, sizeof(WenoArgs), grids, ORDER_SWEEP);
         }
+        if (fx) {
+            std::vector<char> recs;
+            std::vector<dim3> grids;
+            for (int i = 0; i < nc; i++) {
+                put(recs, grad_args(g[i], in[i], true));
+                grids.push_back(march_grid(g[i]->gm, true));
+            }
+            batch_add(b, (const void *)&k_march_xi_batch<false>, (const void *)&k_march_xi_batch<true>, GK * GJX,
+                      BK_GRAD, recs, sizeof(GradArgs), grids, ORDER_NODES);
+        }
         if (fz) {
             const Pick p = zeta_pick(c0);
             std::vector<char> zrecs;
             std::vector<dim3> grids;
-            // viscous xi and eta first, then the fused zeta kernel
-            for (int dir = 0; dir < nsep; dir++) {
+            // viscous xi (unless the march took it) and eta (or the fused eta kernel), then the
+            // fused zeta kernel
+            for (int dir = fx ? 1 : 0; dir < nsep; dir++) {
+                if (fe && dir == 1) {
+                    const Pick pe = eta_pick(c0);
+                    std::vector<char> recs;
+                    std::vector<dim3> eg;
+                    for (int i = 0; i < nc; i++) {
+                        const EtaArgs z = eta_args(g[i], in[i]);
+                        put(recs, z);
+                        eg.push_back(sweep_grid(z.w.nA, z.w.oA, z.w.alignA, z.w.nB, z.w.nD, pe));
+                    }
+                    batch_add(b, pe.batch, pe.batchp, pe.TA * pe.TDI, BK_VISC, recs, sizeof(EtaArgs), eg, ORDER_SWEEP);
+                    continue;
+                }
                 const Pick pv = visc_pick(c0, dir);
                 std::vector<char> recs;
                 std::vector<dim3> vg;
@@ -4103,6 +4234,7 @@ extern "C" int shx_tune(shx_case *c, int reps, int choice[3])
     for (int dir = 0; dir < c->gm.dims && !rc; dir++) {
         // the zeta sweep of a fusable case stays on the 4 x 64 tile of the fused kernel (k_zeta)
         if (dir == 2 && c->gm.dims == 3 && c->diag && !c->d.inviscid) { c->wtile[2] = 0; continue; }
+        if (dir == 1 && fuse_eta(c)) { c->wtile[1] = 1; continue; }   // the fused eta kernel's 8 x 32 tile
         float best = 1e30f;
         int bestv = 0;
         for (int v = 0; v < NWTILE && !rc; v++) {

@@ -5,7 +5,8 @@ Requests arrive as a Poisson stream.  Each asks for a 3D solve of size n^3 up to
 it is admitted when its HBM footprint fits the budget next to the cases already resident (and
 a slot is free), its state stays on the device, and every resident periodic case is advanced
 by the shared launches of one batch (shx_batch).  A request retires when it has taken its
-number of steps; the batch is rebuilt whenever the resident set changes.  A request is a
+number of steps; the batch is rebuilt whenever the resident set changes.  New cases are created
+and uploaded while the resident batch's chunk runs on the device.  A request is a
 periodic block of decaying turbulence or a normal shock meeting a vorticity field with
 inflow/outflow faces in x.
 
@@ -134,13 +135,19 @@ class Server:
             return self._run(requests, pool)
 
     def _run(self, requests, pool):
+        """One loop turn: take arrivals; start the next chunk of the resident batch (its graph
+        replays are queued on the device and run while the host goes on); admit waiting requests
+        that fit, creating and uploading their cases while that chunk runs; wait for the chunk;
+        retire the requests it finished; rebuild the batch when the resident set changed."""
         t0 = time.perf_counter()
         preps = {}
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting = []                        # arrived, not yet admitted (FIFO)
-        active = {}                         # rid -> (request, case, steps left)
+        active = {}                         # rid -> [request, case, steps left], in the batch
+        admitted = []                       # opened this turn, join the batch after the chunk
         used = 0
         batch = None
+        batch_ids = []
         finished = []
         rejected = []
         cell_updates = 0.0
@@ -156,46 +163,36 @@ class Server:
                 waiting.append(r)
                 if self.open_fn is open_case:
                     preps[r.rid] = pool.submit(prepare, r)
-            changed = False
+            k = 0
+            if active:
+                if batch is None:
+                    batch_ids = sorted(active)
+                    batch = self.batch_cls([active[i][1] for i in batch_ids], [active[i][0].tau for i in batch_ids])
+                k = min(self.chunk, min(active[i][2] for i in batch_ids))
+                batch.run(k)                # queued: the device works while admissions proceed
             # admit in arrival order while memory and slots allow
-            while waiting and len(active) < self.max_slots:
+            while waiting and len(active) + len(admitted) < self.max_slots:
                 r = waiting[0]
                 need = self.footprint(r.n, r.S)
                 if used + need > self.budget:
                     break
                 waiting.pop(0)
-                if batch is not None:
-                    batch.close()
-                    batch = None
                 if r.rid in preps:
                     c = self.open_fn(r, preps.pop(r.rid).result())
                 else:
                     c = self.open_fn(r)
                 r.admitted = time.perf_counter() - t0
-                active[r.rid] = [r, c, r.steps]
+                admitted.append([r, c, r.steps])
                 used += need
-                changed = True
-            if not active:
-                if pending:
-                    time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
-                continue
-            if batch is None or changed:
-                if batch is not None:
+            if k:
+                batch.synchronize()
+                for i in batch_ids:
+                    active[i][2] -= k
+                    cell_updates += active[i][0].n ** 3 * 4 * k
+                retired = [i for i in batch_ids if active[i][2] == 0]
+                if retired or admitted:
                     batch.close()
-                ids = sorted(active)
-                batch = self.batch_cls([active[i][1] for i in ids], [active[i][0].tau for i in ids])
-                batch_ids = ids
-            k = min(self.chunk, min(active[i][2] for i in batch_ids))
-            batch.run(k)
-            batch.synchronize()
-            for i in batch_ids:
-                active[i][2] -= k
-                r = active[i][0]
-                cell_updates += r.n ** 3 * 4 * k
-            retired = [i for i in batch_ids if active[i][2] == 0]
-            if retired:
-                batch.close()
-                batch = None
+                    batch = None
                 for i in retired:
                     r, c, _ = active.pop(i)
                     if self.keep:
@@ -206,6 +203,14 @@ class Server:
                     c.close()
                     used -= self.footprint(r.n, r.S)
                     finished.append(r)
+            elif batch is not None and admitted:
+                batch.close()
+                batch = None
+            for e in admitted:
+                active[e[0].rid] = e
+            admitted = []
+            if not active and pending and not waiting:
+                time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
         if batch is not None:
             batch.close()
         wall = time.perf_counter() - t0

@@ -24,7 +24,7 @@ BC_INFLOW_SUB_NORMAL, BC_INFLOW_SUB_RIEMANN, BC_INFLOW_SUB_ISENTROPIC = 8, 9, 10
 BC_FARFIELD, BC_WALL_ISOTHERMAL = 11, 12
 BC_INTERFACE = 13
 FP64, MIXED = 0, 1
-NUM_KERNELS = 15
+NUM_KERNELS = 16
 
 
 class ShxError(RuntimeError):

@@ -126,3 +126,41 @@ def test_wrong_process_count_is_rejected():
     L = _lib()
     d = Division()
     assert L.compat_load_division(FILES[0].encode(), 2, 3, 1, 0, ctypes.byref(d)) == -1
+
+
+class Connection(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("face", ctypes.c_int), ("donor", ctypes.c_int),
+                ("donor_face", ctypes.c_int), ("transform", ctypes.c_int * 3)]
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_division_connections_for_blocks_on_one_device(path):
+    """compat_division_connections turns every zone's records into the shx_blocks connections: one
+    per face with a neighbour, joined to the neighbour's face that holds the donor range, with a
+    transform that takes the face's normal to the donor face's, is the inverse of the donor side's,
+    and maps the tangential extents onto equal ones (whole faces)."""
+    D, nproc, zones = _read_all(path)
+    L = _lib()
+    L.compat_division_connections.argtypes = [ctypes.POINTER(Division), ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(Connection)]
+    arr = (Division * nproc)(*zones)
+    out = (Connection * (2 * D * nproc))()
+    n = L.compat_division_connections(arr, nproc, D, out)
+    assert n == sum(d.ni for d in zones)
+    ext = [[zones[z].corners[1][k] - zones[z].corners[0][k] + 1 for k in range(D)] for z in range(nproc)]
+    conns = {(c.block, c.face): c for c in out[:n]}
+    assert len(conns) == n
+    for (b, f), c in conns.items():
+        a, ad = f // 2, c.donor_face // 2
+        t = list(c.transform[:D])
+        assert sorted(abs(x) for x in t) == list(range(1, D + 1))
+        assert abs(t[a]) - 1 == ad
+        back = conns[(c.donor, c.donor_face)]
+        assert back.donor == b and back.donor_face == f
+        tb = list(back.transform[:D])
+        for q in range(D):
+            aq = abs(t[q]) - 1
+            assert abs(tb[aq]) - 1 == q
+            if q != a:
+                assert (t[q] > 0) == (tb[aq] > 0)
+                assert ext[b][q] == ext[c.donor][aq]

@@ -200,3 +200,88 @@ def test_group_refusals():
             grp.run(1e-4, 1)                     # a block joined to itself: a periodic pair
     finally:
         A.close()
+
+
+DIVISIONS = ["2d_cube_16x16.sddc", "3d_cube_16x16x16.sddc"]
+
+
+@pytest.mark.parametrize("fname", DIVISIONS)
+def test_reference_division_equals_single_block(fname):
+    """The reference's testcase divisions (tests/golden/division, 4 zones in 2D, 8 in 3D) run as
+    blocks on one device, joined by compat_division_connections, on a stretched Cartesian box with
+    supersonic-inflow writers on the outer faces: three steps equal the undivided block bit for bit.
+    The zeta-index quirk is off, since with it the reference's operator depends on the division (the
+    k offset is taken from j)."""
+    import ctypes
+    import test_division as TD
+    path = [f for f in TD.FILES if f.endswith(fname)][0]
+    D, nz, zones = TD._read_all(path)
+    L = TD._lib()
+    L.compat_division_connections.argtypes = [ctypes.POINTER(TD.Division), ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(TD.Connection)]
+    arr = (TD.Division * nz)(*zones)
+    out = (TD.Connection * (2 * D * nz))()
+    nc = L.compat_division_connections(arr, nz, D, out)
+    conn = [(c.block, c.face, c.donor, c.donor_face, tuple(c.transform)) for c in out[:nc]]
+    lo = [[zones[z].corners[0][k] - 1 for k in range(D)] for z in range(nz)]
+    ext = [[zones[z].corners[1][k] - zones[z].corners[0][k] + 1 for k in range(D)] for z in range(nz)]
+    n = tuple(max(lo[z][k] + ext[z][k] for z in range(nz)) for k in range(D)) + ((1,) if D == 2 else ())
+    S, g = 5, 3
+    quirks = orc.QUIRK_METRIC_DIAG3D
+    inflow = (1.0, 2.5, 0.0, 0.0, 1.0)
+    amp = 0.05
+    tp = 2.0 * np.pi
+    if D == 3:
+        mapping = lambda xi, eta, zeta: (xi + amp / tp * np.sin(tp * xi), eta + amp / tp * np.sin(tp * eta),
+                                         zeta + amp / tp * np.sin(tp * zeta))
+    else:
+        mapping = lambda xi, eta, zeta: (xi + amp / tp * np.sin(tp * xi), eta + amp / tp * np.sin(tp * eta), zeta)
+    coords = cases.ext_coords(D, S, n, mapping, periodic=(False, False, False))
+    o = orc.Oracle(dims=D, S=S, n=n, quirks=quirks, mach=0.5, reynolds=2000.0)
+    if D == 3:
+        prim = smooth_periodic_state(o, n, 5, 0.5)
+    else:
+        x, y, _ = cases.interior_coords(2, S, n, *coords)
+        prim = cases.mms_periodic_2d_state(np.ascontiguousarray(x), np.ascontiguousarray(y), mach=0.5)
+    W = shockx.BC_INFLOW_SUPERSONIC
+
+    def case(nn, bc):
+        return shockx.Case(dims=D, space_order=S, time_order=4, n=nn, mach=0.5, reynolds=2000.0,
+                           quirks=quirks, bc=bc, inflow=inflow)
+
+    steps = 3
+    with case(n, (W,) * 6) as whole:
+        whole.build_metric(*coords)
+        whole.upload_state(prim)
+        tau = 0.5 * whole.cfl_dt(0.5)
+        whole.run(tau, steps)
+        ref = whole.download_state()
+    blocks = []
+    try:
+        for z in range(nz):
+            nn = tuple(ext[z]) + ((1,) if D == 2 else ())
+            bc = [W] * 6
+            for f in range(2 * D):
+                if zones[z].face[f].neighbour:
+                    bc[f] = shockx.BC_INTERFACE
+            c = case(nn, tuple(bc))
+            blocks.append(c)
+            es = tuple(slice(lo[z][k], lo[z][k] + ext[z][k] + 2 * g + 2) for k in range(D)) + ((slice(None),) if D == 2 else ())
+            ps = tuple(slice(lo[z][k], lo[z][k] + ext[z][k] + 2 * g) for k in range(D)) + ((slice(None),) if D == 2 else ())
+            c.build_metric(*[np.ascontiguousarray(x[es]) for x in coords])
+            c.upload_state([np.ascontiguousarray(f[ps]) for f in prim])
+        with shockx.Blocks(blocks, conn) as grp:
+            grp.transfer_metric()
+            grp.run(tau, steps)
+            got = [c.download_state() for c in blocks]
+    finally:
+        for c in blocks:
+            c.close()
+    same = True
+    for z in range(nz):
+        zi = tuple(slice(g, g + ext[z][k]) for k in range(D)) + ((slice(None),) if D == 2 else ())
+        wi = tuple(slice(g + lo[z][k], g + lo[z][k] + ext[z][k]) for k in range(D)) + ((slice(None),) if D == 2 else ())
+        for m in range(5):
+            same = same and np.array_equal(got[z][m][zi], ref[m][wi])
+    print(f"[blocks] {fname}: {nz} zones, {nc} connections, bit-equal to the single block: {same}", flush=True)
+    assert same
