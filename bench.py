@@ -61,6 +61,11 @@ def algo_bytes(name, n, ng, time_order=4, diag=True):
         return nint * (vin + 5 + 5 + qsum + 5) * FP64
     if name == "weno_visc_zeta":                 # fused zeta sweeps: the viscous inputs + rho, e
         return nint * (vin + 2 + 5 + 5 + qsum + 5) * FP64
+    if name == "weno_visc_eta":                  # fused eta sweeps: the viscous inputs + rho, e, Q (5) r/w
+        return nint * (vin + 2 + 5 + 5) * FP64
+    if name == "visc_grad_xi":                   # the march with viscous xi: u,v,w (padded); T, mu,
+        # the diagonal metric and J, Q (momentum, energy) r/w per interior node; 7 derivatives out
+        return (npad * 3 + nint * (2 + 4 + 8)) * FP64 + 7 * n * ng * ng * FP64
     if name == "rk_update":          # U_n, Q, J in (Qsum r/w on RK4 stages), U_s+1 out
         return nint * (5 + 5 + 1 + 5 + qsum) * FP64
     if name == "ghost_periodic":     # 5 fields read + written on ghost nodes

@@ -1489,7 +1489,7 @@ struct GradArgs {
     int iso_hi;
     double gam_iso;
 };
-template <int S, bool TG, bool VX = false, int GJ = ::GJ>
+template <int S, bool TG, bool VX = false, int GJ = ::GJ, int GK = ::GK>
 __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, const int by, const int bz)
 {
     const CFields5 &pr = a.pr;
@@ -1714,14 +1714,16 @@ __global__ __launch_bounds__(GK * GJ) void k_visc_grad_march(GradArgs a)
     march_body<S, TG>(a, bx, by, bz);
 }
 // the march with the viscous xi direction (3D diagonal metric): 64 x 8 columns, whose 512 threads
-// may hold the rings in registers
-constexpr int GJX = 8;
+// hold the rings in registers (201 VGPRs, two waves per SIMD).  Measured against it at 256^3: 64 x
+// 16 columns spill 160 VGPRs, and 64 x 4 held to three waves per SIMD spills 60 and is slower than
+// the fused zeta kernel.
+constexpr int GKX = 64, GJX = 8;
 template <int S>
-__global__ __launch_bounds__(GK * GJX) void k_visc_grad_march_xi(GradArgs a)
+__global__ __launch_bounds__(GKX * GJX) void k_visc_grad_march_xi(GradArgs a)
 {
     int bx, by, bz;
     sweep_tile(blockIdx.x, a.tg, bx, by, bz);
-    march_body<S, false, true, GJX>(a, bx, by, bz);
+    march_body<S, false, true, GJX, GKX>(a, bx, by, bz);
 }
 
 // ---- batched launches ----------------------------------------------------------------------------
@@ -1895,11 +1897,11 @@ __global__ __launch_bounds__(256, (MIX ? 1 : ETA_WAVES)) void k_eta_batch(BATCH_
     });
 }
 template <bool P>
-__global__ __launch_bounds__(GK * GJX) void k_march_xi_batch(BATCH_PARAMS(GradArgs))
+__global__ __launch_bounds__(GKX * GJX) void k_march_xi_batch(BATCH_PARAMS(GradArgs))
 {
     batch_walk<P>(total, [&](const int b) {
         const int4 w = rec(work, b);
-        march_body<5, false, true, GJX>(rec(A, w.x), w.y, w.z, w.w);
+        march_body<5, false, true, GJX, GKX>(rec(A, w.x), w.y, w.z, w.w);
     });
 }
 
@@ -2964,7 +2966,7 @@ static GradArgs grad_args(const shx_case *c, double *const *pr, const bool xi)
 }
 static dim3 march_grid(const Geom &gm, const bool xi)
 {
-    return dim3(nblk(gm.ng[2], GK), nblk(gm.ng[1], xi ? GJX : GJ), nblk(gm.ng[0], 32));
+    return dim3(nblk(gm.ng[2], xi ? GKX : GK), nblk(gm.ng[1], xi ? GJX : GJ), nblk(gm.ng[0], 32));
 }
 // Nodal derivatives of the viscous terms (3D: the march; 2D: pointwise).  xi: the march also adds
 // the viscous xi terms to Q (fuse_xi cases, after the WENO xi sweep).
@@ -2978,7 +2980,7 @@ static int grad_march(shx_case *c, double *const *pr, const bool xi)
         const void *fn = xi ? (const void *)&k_visc_grad_march_xi<5> : march_kernel(c->d.space_order, ga.tgrad, false);
         const dim3 grid = flat_grid(march_grid(gm, xi), ga.tg, 0);
         void *args[] = {&ga};
-        HIPCHK(hipLaunchKernel(fn, grid, dim3(GK * (xi ? GJX : GJ)), args, 0, c->stream));
+        HIPCHK(hipLaunchKernel(fn, grid, dim3(xi ? GKX * GJX : GK * GJ), args, 0, c->stream));
     } else {
         CFields5 cp;
         Fields12 gf;
@@ -3851,7 +3853,7 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
                 put(recs, grad_args(g[i], in[i], true));
                 grids.push_back(march_grid(g[i]->gm, true));
             }
-            batch_add(b, (const void *)&k_march_xi_batch<false>, (const void *)&k_march_xi_batch<true>, GK * GJX,
+            batch_add(b, (const void *)&k_march_xi_batch<false>, (const void *)&k_march_xi_batch<true>, GKX * GJX,
                       BK_GRAD, recs, sizeof(GradArgs), grids, ORDER_NODES);
         }
         if (fz) {
