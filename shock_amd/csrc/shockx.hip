@@ -842,26 +842,46 @@ __device__ __forceinline__ double node_cfl_sum(const double gamma, const double 
     const double p = fabs(rho * (gamma - 1) / upsilon * (e - 0.5 * ((u * u) + (v * v) + (w * w))));
     const double cs = sqrt(upsilon * gamma * p / rho);
     double s = 0.0;
+    // unrolled over three directions with a guard, so that m and uu stay in registers
     if constexpr (DIAG) {
         const double uu[3] = {u, v, w};
-        for (int d = 0; d < dims; d++) {
-            const double a = m[4 * d];
-            s += fabs(uu[d] * a) + cs * fabs(a);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (d < dims) {
+                const double a = m[4 * d];
+                s += fabs(uu[d] * a) + cs * fabs(a);
+            }
         }
     } else {
-        for (int d = 0; d < dims; d++) {
-            const double a = m[3 * d], b = m[3 * d + 1], z = m[3 * d + 2];
-            s += fabs(u * a + v * b + w * z) + cs * sqrt(a * a + b * b + z * z);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (d < dims) {
+                const double a = m[3 * d], b = m[3 * d + 1], z = m[3 * d + 2];
+                s += fabs(u * a + v * b + w * z) + cs * sqrt(a * a + b * b + z * z);
+            }
         }
     }
     return s;
 }
 
+// The node's U_n and Q_sum loaded ahead of the update (rk_preload), so that a fused sweep can
+// issue those loads before its window and flux work and their latency overlaps that work.
+struct RkPre { double un[5], qs[5]; };
+__device__ __forceinline__ void rk_preload(const RkArgs &A, const long n, RkPre &p)
+{
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        p.un[m] = (m == 3 && !A.three) ? 0.0 : A.un.f[m][n];
+        p.qs[m] = A.first_stage ? 0.0 : A.qsum.f[m][n];
+    }
+}
+
 // m9, cfl: the node's metric rows and where its CFL sum goes (read only when A.cfl_red is set);
-// DIAG: the metric is diagonal (node_cfl_sum)
+// DIAG: the metric is diagonal (node_cfl_sum); pre: U_n and Q_sum loaded by rk_preload
 template <bool DIAG = false>
 __device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const double J, double qv[5],
-                                         const double *m9 = nullptr, double *cfl = nullptr)
+                                         const double *m9 = nullptr, double *cfl = nullptr,
+                                         const RkPre *pre = nullptr)
 {
     if (A.has_src) {
         qv[0] += J * A.src.f[0][n];
@@ -873,12 +893,12 @@ __device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const do
     double qs[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-        const double old = A.first_stage ? 0.0 : A.qsum.f[m][n];
+        const double old = A.first_stage ? 0.0 : (pre ? pre->qs[m] : A.qsum.f[m][n]);
         qs[m] = old + A.mf * qv[m];
         if (A.mf != 0.0) A.qsum.f[m][n] = qs[m];
     }
     const double tj = A.tau / J;
-    const double rho_n = A.un.f[0][n];
+    const double rho_n = pre ? pre->un[0] : A.un.f[0][n];
     // b = 0 on every RK4 stage and the first SSP-RK3 stage: U_s is not read there
     const double rho_s = (A.b != 0.0) ? A.us.f[0][n] : 0.0;
     const double rho = A.a * rho_n + A.b * rho_s + tj * (A.qf * qv[0] + A.ff * qs[0]);
@@ -889,7 +909,7 @@ __device__ __forceinline__ void rk_point(const RkArgs &A, const long n, const do
     for (int m = 1; m < 5; m++) {
         if (m == 3 && !A.three) { A.out.f[3][n] = 0.0; nv[3] = 0.0; continue; }
         const double us = (A.b != 0.0) ? A.us.f[m][n] : 0.0;
-        nv[m] = (A.a * A.un.f[m][n] * rho_n + A.b * us * rho_s
+        nv[m] = (A.a * (pre ? pre->un[m] : A.un.f[m][n]) * rho_n + A.b * us * rho_s
                  + tj * (A.qf * qv[m] + A.ff * qs[m])) / rho;
         A.out.f[m][n] = nv[m];
     }
@@ -1065,8 +1085,8 @@ constexpr int visc_lds()
 }
 
 // ADD: add add[5] (a fused WENO sweep's contribution of the thread's node) to Q before the
-// viscous terms.
-template <int S, int DIR, int TA, int TDI, bool DIAG, bool ADD = false>
+// viscous terms.  PRE: load the node's Q (and U_n, Q_sum for the stage update) before the window.
+template <int S, int DIR, int TA, int TDI, bool DIAG, bool ADD = false, bool PRE = false>
 __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const int by, const int bz,
                                           double *lds, const double *add = nullptr)
 {
@@ -1094,6 +1114,19 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
     const int d0 = bz * (TDI - 1);
     const long bbase = (long)(b + a.oB) * a.sB;
     const Consts &k = a.k;
+    const int ta = t / TDI, di = t - ta * TDI;
+    const int dn = d0 + di, pa = pa0 + ta;
+    const bool own = di < TDI - 1 && dn < a.nD && pa >= a.oA && pa < a.oA + a.nA;
+    const long n = bbase + (long)pa * a.sA + (long)(dn + a.oD) * a.sD;
+    // PRE: the node's Q, and U_n and Q_sum where the sweep applies the stage update, are loaded
+    // before the window, so that their latency overlaps the staging and the face fluxes
+    double q0[5];
+    RkPre pre;
+    if (PRE && own) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) q0[m] = a.q.f[m][n];
+        if (a.fuse_rk) rk_preload(a.rk, n, pre);
+    }
 
     for (int idx = t; idx < NW; idx += NT) {
         int aa, dd;
@@ -1154,7 +1187,6 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
         }
     }
     __syncthreads();
-    const int ta = t / TDI, di = t - ta * TDI;
     if (bug && di < 2) {
         // di = 0: minus-side values (positions 0..S), di = 1: plus side (positions 1..S+1)
         const double *Lp = &line[(ta * NP + di) * NLV];
@@ -1236,9 +1268,7 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
 
     double cflv = 0.0;                           // CFL sum of this thread's node (end-of-step check)
     if (di < TDI - 1) {
-        const int dn = d0 + di, pa = pa0 + ta;
-        if (dn < a.nD && pa >= a.oA && pa < a.oA + a.nA) {
-            const long n = bbase + (long)pa * a.sA + (long)(dn + a.oD) * a.sD;
+        if (own) {
             const int p = ob + G;                    // window position of this node
             // faces: minus = this thread's face (A values), plus = next thread's face (B values)
             double dF[4];
@@ -1276,9 +1306,11 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
                 prod = node_prod_generic<DIR>(g, m, mu);
             }
             const double work = sm[VS_U * NW + p] * dF[0] + sm[VS_V * NW + p] * dF[1] + sm[VS_W * NW + p] * dF[2];
-            double q0[5];
 #pragma unroll
-            for (int m = 0; m < 5; m++) q0[m] = ADD ? a.q.f[m][n] + add[m] : a.q.f[m][n];
+            for (int m = 0; m < 5; m++) {
+                const double qn = PRE ? q0[m] : a.q.f[m][n];
+                q0[m] = ADD ? qn + add[m] : qn;
+            }
             if (a.fuse_rk) {
                 double qv[5];
                 qv[0] = q0[0];
@@ -1299,9 +1331,9 @@ __device__ __forceinline__ void visc_body(const ViscArgs &a, const int bx, const
 #pragma unroll
                         for (int c = 0; c < 9; c++) m9[c] = sm[(VS_M + c) * NW + p];
                     }
-                    rk_point<DIAG>(a.rk, n, J, qv, m9, &cflv);
+                    rk_point<DIAG>(a.rk, n, J, qv, m9, &cflv, PRE ? &pre : nullptr);
                 } else {
-                    rk_point(a.rk, n, J, qv);
+                    rk_point(a.rk, n, J, qv, nullptr, nullptr, PRE ? &pre : nullptr);
                 }
             } else {
                 if (ADD) a.q.f[0][n] = q0[0];
@@ -1345,7 +1377,9 @@ __device__ __forceinline__ void zeta_body(const ZetaArgs &z, const int bx, const
     double inv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     weno_body<S, 4, 64, 2, MIX, true>(z.w, bx, by, bz, lds, inv);
     __syncthreads();
-    visc_body<S, 2, 4, 64, true, true>(z.v, bx, by, bz, lds, inv);
+    // Q, U_n and Q_sum loaded ahead (measured at 256^3: 1.95 -> 1.75 ms; the same in the fused eta
+    // kernel, which stores Q instead of updating the stage, made it slower)
+    visc_body<S, 2, 4, 64, true, true, true>(z.v, bx, by, bz, lds, inv);
 }
 // FP64 W5: held to 128 VGPRs (four waves per SIMD, as the LDS allows); measured 11 % faster than
 // the compiler's own 136
@@ -1642,7 +1676,8 @@ __device__ __forceinline__ void march_body(const GradArgs &a, const int bx, cons
             }
             if constexpr (VX) {
                 if (vcol) {
-                    // T of plane p; mu and metric of plane i+1 (the face's upper node)
+                    // T of plane p; mu and metric of plane i+1 (the face's upper node).  Loading
+                    // these and Q one plane ahead was measured slower at 256^3 (0.95 -> 1.03 ms)
                     const long pc = (long)max(0, min(p, gm.ng[0] - 1)) * gm.st[0] + col;
 #pragma unroll
                     for (int l = 0; l < NR - 1; l++) tw[l] = tw[l + 1];
