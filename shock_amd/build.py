@@ -42,7 +42,8 @@ def build_compat(verbose=False):
     driver (plain gcc)."""
     cmds = [
         ["gcc", "-O2", "-Wall", "-fPIC", "-shared", "-I", INCLUDE, "-I", COMPAT_DIR,
-         os.path.join(COMPAT_DIR, "shock_compat.c"), "-L", OUT_DIR, "-lshockx", "-Wl,-rpath,$ORIGIN",
+         os.path.join(COMPAT_DIR, "shock_compat.c"), os.path.join(COMPAT_DIR, "cgns_mesh.c"),
+         "-L", OUT_DIR, "-lshockx", "-Wl,-rpath,$ORIGIN",
          "-o", COMPAT_LIB],
         ["gcc", "-O2", "-Wall", "-I", COMPAT_DIR, os.path.join(COMPAT_DIR, "compat_run.c"), "-L", OUT_DIR,
          "-lshock_compat", "-lshockx", "-Wl,-rpath,$ORIGIN/../lib", "-o", COMPAT_RUN],

@@ -70,6 +70,21 @@ def ext_coords(dims, S, n, mapping, periodic=(True, True, True), period=(1.0, 1.
     return xe, ye, ze
 
 
+def ext_from_nodes(dims, S, xyz, periodic=(False, False, False), period=(1.0, 1.0, 1.0)):
+    """Extended coordinate arrays of a block from its node coordinates (x, y, z of shape
+    (ni, nj, nk), as a mesh file gives them): g+1 layers added along each direction, i, j, k in
+    turn, as ext_coords does."""
+    g = (S + 1) // 2
+    arrs = [np.asarray(a, dtype=np.float64) for a in xyz]
+    for d in range(dims):
+        arrs = [_extend_axis(a, d, g + 1, periodic[d], period[d] if comp == d else 0.0)
+                for comp, a in enumerate(arrs)]
+    xe, ye, ze = [np.ascontiguousarray(a) for a in arrs]
+    if dims == 2:
+        ze = np.zeros(xe.shape)
+    return xe, ye, ze
+
+
 def periodic_ext_coords(dims, S, n, mapping, period=(1.0, 1.0, 1.0)):
     """Extended coordinates of a block periodic in every direction (see ext_coords)."""
     return ext_coords(dims, S, n, mapping, (True, True, True), period)

@@ -104,6 +104,25 @@ struct shx_interface;
 int compat_division_connections(const struct compat_division *zones, int nzones, int dims,
                                 struct shx_interface *out);
 
+/* The mesh part of loadFile (src/Load.c:252-744) for CGNS files stored in HDF5, read without the
+ * CGNS and HDF5 libraries (the HDF5 subset is described in cgns_mesh.c).  The block is division
+ * record `div`'s corner range of the zone it names; x, y, z receive its nodes, i fastest
+ * (i2 + ni (j2 + nj k2)), as doubles (z is zeroed in 2D; any of them may be NULL).  For every face
+ * without a neighbour, info->bc holds the string assignBC writes for the zone's boundary
+ * condition on that face (FamilySpecified resolved through the family); faces with a neighbour
+ * keep an empty string.  Returns 0, or -1 with a message in err (a wrong cell dimension, a zone
+ * or a face boundary condition that cannot be found, an unsupported BC type, or an HDF5 feature
+ * outside the subset). */
+struct compat_mesh_info {
+    int celldim, physdim;
+    long long zone_size[3];                   /* vertex counts of the whole zone */
+    long start_iteration;                     /* the base's "Iterations" descriptor, else 0 */
+    int coord_real8;                          /* the coordinates are stored as RealDouble */
+    char bc[6][30];                           /* i-, i+, j-, j+, k-, k+ */
+};
+int compat_load_mesh(const char *path, int dims, const struct compat_division *div, double *x, double *y,
+                     double *z, struct compat_mesh_info *info, char *err, int errlen);
+
 void compat_define(struct strct_configuration *cfg);
 void compat_release(struct strct_configuration *cfg);
 

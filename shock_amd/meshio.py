@@ -1,0 +1,132 @@
+"""The reference's mesh inputs, read through libshock_compat.so (plain C, no GPU):
+
+* the division record of one zone (loadDivision, src/Load.c:191-250): `load_division`;
+* the block of a CGNS mesh that a record names (loadFile's mesh part, src/Load.c:252-744), read
+  from the HDF5 file without the CGNS and HDF5 libraries: `load_mesh` gives the block's node
+  coordinates in the library's layout and the boundary condition of every face without a
+  neighbour, as the strings assignBC writes (src/Load.c:46-97);
+* `face_kinds` turns those strings into SHX_BC_* face kinds the way the compat layer does for the
+  reference's configuration flags (subsonic/supersonic inflow and outflow options).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import shockx
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+COMPAT_LIB = os.path.join(HERE, "lib", "libshock_compat.so")
+
+
+class Interface(ctypes.Structure):
+    _fields_ = [("neighbour", ctypes.c_int), ("transform", ctypes.c_int * 3),
+                ("range", (ctypes.c_longlong * 3) * 2), ("drange", (ctypes.c_longlong * 3) * 2),
+                ("rc", ctypes.c_float * 3), ("ra", ctypes.c_float * 3), ("tr", ctypes.c_float * 3)]
+
+
+class Division(ctypes.Structure):
+    _fields_ = [("zonename", ctypes.c_char * 33), ("corners", (ctypes.c_longlong * 3) * 2),
+                ("face", Interface * 6), ("ni", ctypes.c_int)]
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [("celldim", ctypes.c_int), ("physdim", ctypes.c_int), ("zone_size", ctypes.c_longlong * 3),
+                ("start_iteration", ctypes.c_long), ("coord_real8", ctypes.c_int),
+                ("bc", (ctypes.c_char * 30) * 6)]
+
+
+class MeshError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(COMPAT_LIB):
+            raise MeshError(f"{COMPAT_LIB} is not built (python -m shock_amd.build)")
+        L = ctypes.CDLL(COMPAT_LIB)
+        L.compat_load_division.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint, ctypes.c_int,
+                                           ctypes.POINTER(Division)]
+        L.compat_load_mesh.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Division), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshInfo), ctypes.c_char_p,
+                                       ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def load_division(path, dims, proccount, zone, swap=0):
+    """Record `zone` (1-based) of a division file written for `proccount` processes."""
+    d = Division()
+    rc = lib().compat_load_division(os.fsencode(path), dims, proccount, zone, swap, ctypes.byref(d))
+    if rc:
+        raise MeshError(f"division file {path}: record {zone} of {proccount} not read (code {rc})")
+    return d
+
+
+def whole_zone(name, size):
+    """A division record covering a whole zone of `size` nodes, with no neighbour (one process)."""
+    d = Division()
+    d.zonename = name.encode() if isinstance(name, str) else name
+    for k, s in enumerate(size):
+        d.corners[0][k] = 1
+        d.corners[1][k] = s
+    return d
+
+
+def block_size(dims, div):
+    return tuple(div.corners[1][k] - div.corners[0][k] + 1 for k in range(dims)) + ((1,) if dims == 2 else ())
+
+
+def load_mesh(path, dims, div):
+    """The block of the CGNS mesh at `path` that division record `div` names: node coordinates
+    x, y, z of shape (ni, nj, nk) (k fastest, the library's layout; z = 0 in 2D) and the MeshInfo
+    (zone size, start iteration, boundary condition strings per face)."""
+    n = block_size(dims, div)
+    flat = [np.zeros(n[0] * n[1] * n[2]) for _ in range(3)]
+    info = MeshInfo()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().compat_load_mesh(os.fsencode(path), dims, ctypes.byref(div), flat[0].ctypes.data,
+                                flat[1].ctypes.data, flat[2].ctypes.data, ctypes.byref(info), err, len(err))
+    if rc:
+        raise MeshError(err.value.decode(errors="replace"))
+    # the reader gives i fastest
+    xyz = [np.ascontiguousarray(a.reshape(n[2], n[1], n[0]).transpose(2, 1, 0)) for a in flat]
+    return xyz, info
+
+
+def bc_strings(info, dims):
+    return [info.bc[f].value.decode() for f in range(2 * dims)]
+
+
+def face_kinds(info, dims, div=None, inflow="supersonic", outflow="riemann"):
+    """SHX_BC_* per face (6 entries) from the boundary-condition strings of `info`, as the compat
+    layer maps the reference's BC strings and its inflow/outflow option flags; faces with a
+    neighbour in `div` are SHX_BC_INTERFACE; the k faces of a 2D block are periodic."""
+    inflows = {"supersonic": shockx.BC_INFLOW_SUPERSONIC, "normal": shockx.BC_INFLOW_SUB_NORMAL,
+               "riemann": shockx.BC_INFLOW_SUB_RIEMANN, "isentropic": shockx.BC_INFLOW_SUB_ISENTROPIC}
+    outflows = {"riemann": shockx.BC_OUTFLOW_RIEMANN, "normal": shockx.BC_OUTFLOW_NORMAL,
+                "rudy": shockx.BC_OUTFLOW_RUDY}
+    table = {"BCWallInviscid": shockx.BC_WALL_SLIP, "BCWallViscous": shockx.BC_WALL_NOSLIP,
+             "BCWallViscousIsothermal": shockx.BC_WALL_ISOTHERMAL, "BCFarfield": shockx.BC_FARFIELD}
+    out = []
+    for f in range(6):
+        if f >= 2 * dims:
+            out.append(shockx.BC_PERIODIC)
+            continue
+        if div is not None and div.face[f].neighbour:
+            out.append(shockx.BC_INTERFACE)
+            continue
+        s = info.bc[f].value.decode()
+        if s in ("BCInflow", "BCInflowSubsonic", "BCInflowSupersonic"):
+            out.append(inflows[inflow])
+        elif s in ("BCOutflow", "BCOutflowSubsonic"):
+            out.append(outflows[outflow])
+        elif s in table:
+            out.append(table[s])
+        else:
+            raise MeshError(f"face {f}: boundary condition '{s}' has no face kind here")
+    return tuple(out)
