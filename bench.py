@@ -403,7 +403,12 @@ def batch_cases(count, seed, precision, lo=32 ** 2, hi=160 ** 3):
             c.build_metric(*cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05)))
             c.upload_state(cases.turbulence_3d_state(3, S, nn, int(rng.integers(1, 2 ** 31 - 1)), mach=mach))
             cells = n ** 3
-        out.append((c, cells, c.cfl_dt(0.5), (2 if N <= 256 ** 2 else 3, n, S)))
+        # the convective CFL step (the reference's), capped by the explicit viscous limit: a
+        # fine, low-Reynolds, high-Mach draw is otherwise unstable and fails the per-step checkNAN
+        h = 1.0 / n
+        dims = 2 if N <= 256 ** 2 else 3
+        tau = min(c.cfl_dt(0.5), 0.2 * re * h * h / dims)
+        out.append((c, cells, tau, (dims, n, S, mach, re)))
     return out
 
 
@@ -426,14 +431,17 @@ def batch_mix(args, lo, hi):
     print(f"[batch] {len(cs)} cases, {cells:.3e} nodes, setup {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     taus = [x[2] for x in cs]
-    # sequential single-case path
+    # the same cases through the single-case path, one after another, each step queued on the
+    # case's stream without a check or a synchronization (the batch's steps below do more: they
+    # count non-finite densities, read once per run)
     for c, _, tau, _ in cs:
-        c.run(tau, 1)
+        c.calc_runge_kutta(tau)
     for c, _, _, _ in cs:
         c.synchronize()
     t0 = time.perf_counter()
     for c, _, tau, _ in cs:
-        c.run(tau, args.steps)
+        for _ in range(args.steps):
+            c.calc_runge_kutta(tau)
     for c, _, _, _ in cs:
         c.synchronize()
     seq = time.perf_counter() - t0
@@ -445,12 +453,16 @@ def batch_mix(args, lo, hi):
     b.synchronize()
     t0 = time.perf_counter()
     b.run(args.steps)
-    b.synchronize()
+    try:
+        b.synchronize()                  # checkNAN of every case over the timed steps
+    except shockx.ShxError as e:
+        bad = [k for k, cnt in enumerate(b.nan_counts()) if cnt > 0]
+        raise SystemExit(f"batch cases {[(k, cs[k][3]) for k in bad]} (dims, n, S, Mach, Re): {e}")
     wall = time.perf_counter() - t0
     b.close()
     value = cells * 4 * args.steps / wall
     mix = {"2d_w5": 0, "2d_w9": 0, "3d_w5": 0, "3d_w9": 0}
-    for _, _, _, (d, n, S) in cs:
+    for _, _, _, (d, n, S, _, _) in cs:
         mix[f"{d}d_w{S}"] += 1
     out = {"metric": "cell-updates/s/RK-stage", "value": round(value, 1), "unit": "cell-updates/s/RK-stage",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),

@@ -338,7 +338,14 @@ int shx_batch_run(shx_batch *b, int nsteps);
 /* Picks the resident blocks per CU of each kernel class (ghost, values, grad, weno, visc, rk) by
  * timing replayed steps; the case states are restored.  factors_out may be NULL. */
 int shx_batch_tune(shx_batch *b, int steps, int factors_out[6]);
+/* Waits for the batch's steps.  checkNAN (src/Functions.c:5419-5467) is fused into every case's
+ * last stage as in shx_run, but read here rather than after every step, so that shx_batch_run
+ * returns at once: SHX_ERR_NAN when any case's new state held a non-finite density in the steps
+ * since the last call (those nodes hold 1e6; the message names the cases). */
 int shx_batch_synchronize(shx_batch *b);
+/* Waits for the batch and writes, per case (in creation order), the running count of non-finite
+ * densities its fused checkNAN has found. */
+int shx_batch_nan_counts(shx_batch *b, int *counts);
 int shx_batch_destroy(shx_batch *b);
 
 #ifdef __cplusplus

@@ -3678,6 +3678,10 @@ struct shx_batch {
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> joins;
     hipEvent_t fork = nullptr;
+    // checkNAN fused into every case's last stage: running counts per case on the device, read by
+    // shx_batch_synchronize into a pinned host copy; nan_seen: the counts already reported
+    int *d_nan = nullptr, *h_nan = nullptr;
+    std::vector<int> nan_seen;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -3766,6 +3770,12 @@ static int batch_group(shx_batch *b, const std::vector<shx_case *> &g, const std
             in[i] = g[i]->prim[ib];
             out[i] = g[i]->prim[ob];
             rk[i] = rk_args(g[i], g[i]->prim[ib], g[i]->prim[ob], s, tau[i]);
+            if (s == c0->d.time_order - 1 && b->d_nan) {
+                // checkNAN on the step's new state (as shx_run), counted per case
+                const long k = std::find(b->cases.begin(), b->cases.end(), g[i]) - b->cases.begin();
+                rk[i].nan_cnt = b->d_nan + k;
+                rk[i].nanP = g[i]->P;
+            }
         }
         if (s == 0) {   // start of the step: step-start ghost state of every Rudy face
             for (int f = 0; f < 2 * dims; f++) {
@@ -4120,7 +4130,12 @@ extern "C" int shx_batch_create(shx_case *const *cases, int ncases, const double
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) e = hipStreamCreate(&b->stream);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_nan, sizeof(int) * (size_t)ncases);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_nan, sizeof(int) * (size_t)ncases, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_nan, 0, sizeof(int) * (size_t)ncases, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) { shx_batch_destroy(b); return set_err(SHX_ERR_HIP, "batch setup: %s", hipGetErrorString(e)); }
+    b->nan_seen.assign((size_t)ncases, 0);
     int rc = batch_build(b);
     if (!rc) rc = batch_upload(b, true);
     if (!rc) rc = batch_grids(b);
@@ -4166,10 +4181,42 @@ extern "C" int shx_batch_run(shx_batch *b, int nsteps)
     return SHX_OK;
 }
 
+// Waits for the batch, then reads the fused checkNAN counts: SHX_ERR_NAN names the cases whose
+// new states held a non-finite density in the steps since the last call (those nodes hold 1e6).
+static int batch_nan_read(shx_batch *b)
+{
+    const size_t n = b->cases.size();
+    HIPCHK(hipMemcpyAsync(b->h_nan, b->d_nan, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return SHX_OK;
+}
+
 extern "C" int shx_batch_synchronize(shx_batch *b)
 {
     if (!b) return set_err(SHX_ERR_ARG, "null batch");
-    HIPCHK(hipStreamSynchronize(b->stream));
+    int rc = batch_nan_read(b);
+    if (rc) return rc;
+    char msg[400];
+    int len = 0, bad = 0;
+    for (size_t i = 0; i < b->cases.size(); i++) {
+        const int fresh = b->h_nan[i] - b->nan_seen[i];
+        b->nan_seen[i] = b->h_nan[i];
+        if (fresh <= 0) continue;
+        if (bad++ < 8 && len < (int)sizeof msg - 64)
+            len += snprintf(msg + len, sizeof msg - len, "%scase %zu: %d", bad > 1 ? ", " : "", i, fresh);
+    }
+    if (bad)
+        return set_err(SHX_ERR_NAN, "checkNAN: interior nodes with a non-finite density (set to 1e6) in %d case%s "
+                       "since the last synchronize (%s%s)", bad, bad > 1 ? "s" : "", msg, bad > 8 ? ", ..." : "");
+    return SHX_OK;
+}
+
+extern "C" int shx_batch_nan_counts(shx_batch *b, int *counts)
+{
+    if (!b || !counts) return set_err(SHX_ERR_ARG, "null argument");
+    int rc = batch_nan_read(b);
+    if (rc) return rc;
+    for (size_t i = 0; i < b->cases.size(); i++) counts[i] = b->h_nan[i];
     return SHX_OK;
 }
 
@@ -4231,6 +4278,9 @@ extern "C" int shx_batch_tune(shx_batch *b, int steps, int factors_out[6])
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
+    // the tuning steps are not part of the solution: their checkNAN counts are not reported
+    if ((rc = batch_nan_read(b))) return rc;
+    for (size_t i = 0; i < b->cases.size(); i++) b->nan_seen[i] = b->h_nan[i];
     if ((rc = batch_grids(b)) || (rc = batch_capture(b))) return rc;
     if (factors_out)
         for (int k = 0; k < BK_NUM; k++) factors_out[k] = b->factor[k];
@@ -4248,6 +4298,8 @@ extern "C" int shx_batch_destroy(shx_batch *b)
     for (hipStream_t l : b->lanes) { hipStreamSynchronize(l); hipStreamDestroy(l); }
     for (hipEvent_t e : b->joins) hipEventDestroy(e);
     if (b->fork) hipEventDestroy(b->fork);
+    if (b->d_nan) hipFree(b->d_nan);
+    if (b->h_nan) hipHostFree(b->h_nan);
     if (b->stream) hipStreamDestroy(b->stream);
     for (shx_case *c : b->cases)
         if (c->batch == b) c->batch = nullptr;

@@ -149,6 +149,7 @@ class Server:
         batch = None
         batch_ids = []
         finished = []
+        failed = []
         rejected = []
         cell_updates = 0.0
         while pending or waiting or active:
@@ -185,11 +186,19 @@ class Server:
                 admitted.append([r, c, r.steps])
                 used += need
             if k:
-                batch.synchronize()
+                blown = set()
+                try:
+                    batch.synchronize()
+                except shockx.ShxError as e:
+                    # the fused checkNAN found non-finite densities: those requests end as failed
+                    # (their states hold 1e6 where checkNAN wrote it); the others go on
+                    if e.code != shockx.ERR_NAN:
+                        raise
+                    blown = {i for i, cnt in zip(batch_ids, batch.nan_counts()) if cnt > 0}
                 for i in batch_ids:
                     active[i][2] -= k
                     cell_updates += active[i][0].n ** 3 * 4 * k
-                retired = [i for i in batch_ids if active[i][2] == 0]
+                retired = [i for i in batch_ids if active[i][2] == 0 or i in blown]
                 if retired or admitted:
                     batch.close()
                     batch = None
@@ -202,7 +211,11 @@ class Server:
                     r.done = time.perf_counter() - t0
                     c.close()
                     used -= self.footprint(r.n, r.S)
-                    finished.append(r)
+                    if i in blown:
+                        r.failed = "checkNAN: non-finite density"
+                        failed.append(r)
+                    else:
+                        finished.append(r)
             elif batch is not None and admitted:
                 batch.close()
                 batch = None
@@ -216,7 +229,7 @@ class Server:
         wall = time.perf_counter() - t0
         tts = np.array([r.done - r.arrival for r in finished] or [float("nan")])
         return {
-            "requests": len(finished), "rejected": len(rejected), "wall_s": wall,
+            "requests": len(finished), "rejected": len(rejected), "failed": len(failed), "wall_s": wall,
             "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
             "mean_s": float(np.mean(tts)), "cell_updates_per_s": cell_updates / wall,
             "queue_wait_p50_s": float(np.percentile([r.admitted - r.arrival for r in finished] or [float("nan")], 50)),

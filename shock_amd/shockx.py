@@ -76,7 +76,7 @@ EXPORTS = [
     "shx_check_nan", "shx_cfl_dt", "shx_synchronize", "shx_set_timing", "shx_kernel_times",
     "shx_copy_probe", "shx_upload_boundary_state",
     "shx_batch_create", "shx_batch_set_tau", "shx_batch_run", "shx_batch_tune", "shx_batch_synchronize",
-    "shx_batch_destroy", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
+    "shx_batch_destroy", "shx_batch_nan_counts", "shx_tune", "shx_check_metric", "shx_simulate", "shx_post", "shx_film_open",
     "shx_film_sample", "shx_film_count", "shx_film_download", "shx_history_download", "shx_mms_error",
     "shx_run_cfl", "shx_mms_error_static",
     "shx_blocks_create", "shx_blocks_transfer_metric", "shx_blocks_run", "shx_blocks_destroy",
@@ -127,6 +127,7 @@ def lib():
         L.shx_batch_run.argtypes = [P, ctypes.c_int]
         L.shx_batch_tune.argtypes = [P, ctypes.c_int, P]
         L.shx_batch_synchronize.argtypes = [P]
+        L.shx_batch_nan_counts.argtypes = [P, P]
         L.shx_batch_destroy.argtypes = [P]
         L.shx_tune.argtypes = [P, ctypes.c_int, P]
         L.shx_blocks_create.argtypes = [P, ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(P)]
@@ -431,7 +432,15 @@ class Batch:
         return dict(zip(BATCH_CLASSES, list(f)))
 
     def synchronize(self):
+        """Waits for the batch; ShxError with code ERR_NAN when the fused checkNAN of a case found
+        a non-finite density since the last call (nan_counts says which)."""
         _check(lib().shx_batch_synchronize(self.h))
+
+    def nan_counts(self):
+        """Per case, the running count of non-finite densities found by the fused checkNAN."""
+        out = np.zeros(len(self.cases), dtype=np.int32)
+        _check(lib().shx_batch_nan_counts(self.h, out.ctypes.data))
+        return out
 
     def close(self):
         if self.h:

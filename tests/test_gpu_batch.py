@@ -98,3 +98,50 @@ def test_batch_of_many_uneven_cases_equals_cases_alone():
             assert all(np.array_equal(a[sl], r[sl]) for a, r in zip(got, ref)), (k, specs[k])
     for _, c, _, _, _ in made:
         c.close()
+
+
+def test_batch_reports_the_case_whose_state_goes_non_finite():
+    """checkNAN fused into every case's last stage: a NaN planted in one case's state is reported by
+    synchronize (ERR_NAN naming the case; its count in nan_counts, the others zero), that case ends
+    where shx_run leaves its twin (the step with the non-finite nodes set to 1e6), the other cases
+    end where they end alone, and a second synchronize reports nothing new."""
+    specs = [(2, 5, 24, shockx.FP64, False, 1), (3, 5, 12, shockx.FP64, False, 2), (2, 9, 20, shockx.FP64, False, 3)]
+    made = [_make(sp) for sp in specs]
+    taus = [c.cfl_dt(0.4) for _, c, _, _, _ in made]
+    o1, c1, _, prim1, nn1 = made[1]
+    bad = [a.copy() for a in prim1]
+    g = o1.g
+    bad[0][g + 4, g + 5, g + 6] = np.nan
+    o1.ghost_periodic(bad)
+    c1.upload_state(bad)
+    alone = []
+    for k, ((o, c, met, prim, nn), tau) in enumerate(zip(made, taus)):
+        if k == 1:
+            twin = shockx.Case(dims=3, space_order=5, time_order=4, n=nn1, gamma=1.4, mach=o1.c.mach,
+                               reynolds=o1.c.reynolds)
+            twin.upload_metric(c1.download_metric())
+            twin.upload_state(bad)
+            with pytest.raises(shockx.ShxError) as ei:
+                twin.run(tau, 1)
+            assert ei.value.code == shockx.ERR_NAN
+            alone.append(twin.download_state())
+            twin.close()
+        else:
+            c.run(tau, 1)
+            alone.append(c.download_state())
+            c.upload_state(prim)
+    with shockx.Batch([m[1] for m in made], taus) as b:
+        b.run(1)
+        with pytest.raises(shockx.ShxError) as ei:
+            b.synchronize()
+        assert ei.value.code == shockx.ERR_NAN and "case 1:" in str(ei.value)
+        print(f"[batch] {ei.value}", flush=True)
+        counts = b.nan_counts()
+        assert counts[1] > 1 and counts[0] == 0 and counts[2] == 0
+        b.synchronize()                      # nothing new since the last call
+        for k, ((o, c, met, prim, nn), ref) in enumerate(zip(made, alone)):
+            got = c.download_state()
+            sl = interior(o, o.c.dims, nn)
+            assert all(np.array_equal(a[sl], r[sl], equal_nan=True) for a, r in zip(got, ref)), k
+    for _, c, _, _, _ in made:
+        c.close()

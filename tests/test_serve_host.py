@@ -1,6 +1,9 @@
 """Scheduler of the solve server, on the host with stand-in cases: admission by footprint and
-slots, every request takes exactly its own number of steps, retirement frees memory."""
-from shock_amd import serve
+slots, every request takes exactly its own number of steps, retirement frees memory, a request
+whose state goes non-finite ends as failed while the others finish."""
+import numpy as np
+
+from shock_amd import serve, shockx
 
 
 class FakeCase:
@@ -29,7 +32,11 @@ class FakeBatch:
             c.steps += k
 
     def synchronize(self):
-        pass
+        if any(getattr(c, "blow_at", None) is not None and c.steps >= c.blow_at for c in self.cs):
+            raise shockx.ShxError("checkNAN: stand-in", shockx.ERR_NAN)
+
+    def nan_counts(self):
+        return np.array([int(getattr(c, "blow_at", None) is not None and c.steps >= c.blow_at) for c in self.cs])
 
     def close(self):
         pass
@@ -74,3 +81,25 @@ def test_request_larger_than_the_budget_is_refused_not_queued():
     stats, done = srv.run(reqs)
     assert stats["requests"] == 4 and stats["rejected"] == 1
     assert sorted(r.rid for r in done) == [0, 1, 3, 4]
+
+
+def test_request_that_goes_non_finite_fails_alone():
+    reqs = serve.make_requests(6, rate=300.0, seed=5, n_lo=16, n_hi=24)
+    log = []
+
+    def open_fn(r):
+        r.steps = 12
+        r.tau = r.t_end / r.steps
+        c = FakeCase(r, log)
+        if r.rid == 2:
+            c.blow_at = 5                   # its state goes non-finite during the second chunk
+        return c
+
+    srv = serve.Server(budget_bytes=10 * 24 ** 3, max_slots=8, chunk=4, keep_results=True, open_fn=open_fn,
+                       batch_cls=FakeBatch, footprint_fn=lambda n, S: n ** 3)
+    stats, done = srv.run(reqs)
+    assert stats["failed"] == 1 and stats["requests"] == 5
+    assert sorted(r.rid for r in done) == [0, 1, 3, 4, 5]
+    for r in done:
+        assert r.result == [12]
+    assert sorted(rid for _, rid in log) == list(range(6))  # the failed case was closed too
