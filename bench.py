@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the default workload")
     ap.add_argument("--no-config0", action="store_true", help="skip the 2D 64^2 CPU-runnable configuration")
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving run of the default line")
+    ap.add_argument("--no-power", action="store_true",
+                    help="skip the board power / clock sample (amd-smi beside ~3 s of further steps)")
     ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
                     default="turbulence",
                     help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
@@ -258,6 +260,52 @@ def config0_run(args):
                                                        f"CFL step per step, time {t_cpu:.6g})"}}
 
 
+def power_sample(run, seconds=3.0):
+    """Board power and graphics clock sampled with amd-smi (a child process every ~0.3 s) while
+    run(k) keeps the GPU busy with more steps of the same workload, after the timed region;
+    None where amd-smi is absent or prints nothing usable."""
+    import re
+    import shutil
+    import subprocess
+    import threading
+    exe = shutil.which("amd-smi") or ("/opt/rocm/bin/amd-smi" if os.path.exists("/opt/rocm/bin/amd-smi") else None)
+    if not exe:
+        return None
+    samples = []
+    stop = threading.Event()
+
+    def poll():
+        while not stop.is_set():
+            try:
+                out = subprocess.run([exe, "metric", "-g", "0", "-p", "-c"], capture_output=True, text=True,
+                                     timeout=10).stdout
+            except (OSError, subprocess.SubprocessError):
+                return
+            pw = re.search(r"SOCKET_POWER:\s*([\d.]+)", out)
+            clks = [float(x) for x in re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*([\d.]+)", out)]
+            if pw and clks:
+                samples.append((float(pw.group(1)), sum(clks) / len(clks)))
+            stop.wait(0.3)
+
+    th = threading.Thread(target=poll, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        run(10)
+        steps += 10
+    stop.set()
+    th.join(timeout=15)
+    if not samples:
+        return None
+    pw = [a for a, _ in samples]
+    ck = [b for _, b in samples]
+    return {"tool": "amd-smi metric -p -c", "samples": len(samples), "steps": steps,
+            "board_w_mean": round(sum(pw) / len(pw), 1), "board_w_max": max(pw),
+            "gfx_mhz_mean": round(sum(ck) / len(ck), 1), "gfx_mhz_min": round(min(ck), 1),
+            "board_w_rated": 1400, "gfx_mhz_peak": 2400}
+
+
 def turbulence_run(args, precision):
     """The 256^3 turbulence workload in one arithmetic form: setup, warm-up, timed CFL steps with
     the fused end-of-step checks; returns the JSON line's fields."""
@@ -302,6 +350,12 @@ def turbulence_run(args, precision):
     kt = case.kernel_times() if not args.no_timing else {}
     print(f"[bench] {precision}: timed {args.steps} steps in {elapsed:.3f}s (checkNAN every step: no "
           f"non-finite density; time advanced {t_adv:.6g})", file=sys.stderr, flush=True)
+    power = None
+    if precision == "fp64" and not args.no_power:
+        if not args.no_timing:
+            case.set_timing(False)
+        power = power_sample(lambda k: (advance(k), case.synchronize()))
+        print(f"[bench] power beside {power['steps'] if power else 0} more steps: {power}", file=sys.stderr, flush=True)
 
     stages = args.steps * time_order
     value = n ** 3 * stages / elapsed
@@ -371,6 +425,8 @@ def turbulence_run(args, precision):
         "weno_tile_variants": tiles,
         "kernels": kernels,
     }
+    if power is not None:
+        line["power"] = power
     return {"line": line}
 
 
