@@ -197,7 +197,8 @@ def serve_short(args):
     stats, done = serve.Server().run(reqs)
     return {"workload": "10 Poisson requests at 2/s, 3D periodic turbulence or Mach 1.5-3.5 shock cases, "
                         "48^3-96^3 (log-uniform), t_end U(0.05, 0.2), W5/W9, FP64",
-            "requests": stats["requests"], "p50_s": round(stats["p50_s"], 3), "p99_s": round(stats["p99_s"], 3),
+            "requests": stats["requests"], "failed": stats["failed"],
+            "p50_s": round(stats["p50_s"], 3), "p99_s": round(stats["p99_s"], 3),
             "queue_wait_p50_s": round(stats["queue_wait_p50_s"], 3),
             "cell_updates_per_s": round(stats["cell_updates_per_s"], 1), "wall_s": round(stats["wall_s"], 2)}
 
@@ -545,11 +546,16 @@ def serve_workload(args):
     for rate in rates:
         reqs = serve.make_requests(args.cases, rate, args.seed, n_lo=args.n_lo, n_hi=args.n_hi, precision=prec,
                                    shock_frac=args.shock_frac)
-        stats, done = serve.Server().run(reqs)
+        srv = serve.Server()
+        stats, done = srv.run(reqs)
+        for r in srv.failed:
+            print(f"[serve] request {r.rid} failed ({r.failed}): {r.kind}, n {r.n}, S {r.S}, Mach {r.mach:.2f}, "
+                  f"Re {r.reynolds:.0f}, tau {r.tau:.3g}, {r.steps} steps", file=sys.stderr, flush=True)
         row = {"rate": rate, "cell_updates_per_s": round(stats["cell_updates_per_s"], 1),
                "p50_s": round(stats["p50_s"], 3), "p99_s": round(stats["p99_s"], 3), "mean_s": round(stats["mean_s"], 3),
                "queue_wait_p50_s": round(stats["queue_wait_p50_s"], 3), "wall_s": round(stats["wall_s"], 2),
-               "requests": stats["requests"], "shock_requests": sum(r.kind == "shock" for r in done)}
+               "requests": stats["requests"], "failed": stats["failed"], "rejected": stats["rejected"],
+               "shock_requests": sum(r.kind == "shock" for r in done)}
         print(f"[serve] {json.dumps(row)}", file=sys.stderr, flush=True)
         rows.append(row)
     top = rows[-1]

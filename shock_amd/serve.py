@@ -51,6 +51,7 @@ class Request:
     steps: int = 0
     tau: float = 0.0
     result: list = field(default_factory=list)
+    failed: str = ""               # why the request ended without a result (checkNAN)
 
 
 def make_requests(count, rate, seed, n_lo=64, n_hi=256, t_lo=0.5, t_hi=2.0, precision=shockx.FP64,
@@ -100,7 +101,10 @@ def open_case(r, prep=None, cfl=0.5):
                         precision=r.precision)
     c.build_metric(*coords)
     c.upload_state(state)
-    tau = c.cfl_dt(cfl)
+    # the convective CFL step (the reference's), capped by the explicit viscous limit on the unit
+    # box (spacing 1/n): low-Reynolds requests on fine grids are otherwise unstable
+    h = 1.0 / r.n
+    tau = min(c.cfl_dt(cfl), 0.2 * r.reynolds * h * h / 3.0)
     r.steps = max(1, int(math.ceil(r.t_end / tau)))
     r.tau = r.t_end / r.steps          # land exactly on t_end
     return c
@@ -228,6 +232,7 @@ class Server:
             batch.close()
         wall = time.perf_counter() - t0
         tts = np.array([r.done - r.arrival for r in finished] or [float("nan")])
+        self.failed = failed
         return {
             "requests": len(finished), "rejected": len(rejected), "failed": len(failed), "wall_s": wall,
             "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
