@@ -441,12 +441,16 @@ enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V
 // fields of the window).  Diagonal metric: no normal components; the sign of the
 // metric entry rides on V_LAM (the spectral radius is positive) and J moves to V_NX, which keeps
 // the 8 x 32 tile's window under 40 KB (four blocks per CU).
+// DIFF (FP64 WENO5, diagonal metric): the F and U arrays hold node-to-node differences once the
+// node pass is done, and one more array keeps rho for the interface state (below).
+template <int S, int AX, bool MIX>
+constexpr bool weno_diff() { return S == 5 && AX >= 0 && !MIX; }
 template <int S, int TA, int TDI, int AX, bool MIX>
 constexpr int weno_lds()
 {
     constexpr int NW = (TDI - 1 + (S + 1)) * TA;
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
-    return NVK * NW + (MIX ? NW : 0);
+    return NVK * NW + ((MIX || weno_diff<S, AX, MIX>()) ? NW : 0);
 }
 
 // KEEP: leave Q alone and return this sweep's contribution of the thread's node in keep[5] (a
@@ -462,7 +466,15 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     constexpr int NT = TA * TDI;
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
     constexpr int VJ = (AX < 0) ? V_J : V_NX;
-    double *const sm = lds;                       // NVK * NW
+    // DIFF: after the node pass, position o of the F and U arrays holds node o+1 minus node o.
+    // The projection then runs on the five differences the indicators use (instead of six node
+    // values, whose differences were taken after projecting), and the central part Theta is
+    // written in differences: Theta = F[oc] + (D[oc-1] + 6 D[oc] - D[oc+1]) / 12, whose F[oc]
+    // terms telescope in the divergence (node oc+1's divergence = D[oc] + the difference of the
+    // two interface values without F).  rho, for the interface state, keeps its own array VRHO.
+    constexpr bool DIFF = weno_diff<S, AX, MIX>();
+    constexpr int VRHO = NVK;
+    double *const sm = lds;                       // NVK * NW (+ NW)
     // mixed form: the FP32 differences of F (arrays 0..4) and U (5..9).  Once formed, the node
     // values of rho u, rho v, rho w and rho e are not read again, so arrays 0..7 take their place
     // (V_U1..V_U4) and only arrays 8 and 9 extend the window: the 8 x 32 tile keeps four blocks
@@ -539,6 +551,7 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             sm[V_LAM * NW + o] = (md > 0.0) ? lam : -lam;
         }
         sm[V_U0 * NW + o] = rho;
+        if constexpr (DIFF) sm[VRHO * NW + o] = rho;
         sm[V_U1 * NW + o] = rho * u;
         sm[V_U2 * NW + o] = rho * v;
         sm[V_U3 * NW + o] = rho * w;
@@ -580,18 +593,43 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
         }
         __syncthreads();
     }
+    if constexpr (DIFF) {
+        // node-to-node differences of F and U, all read before any is written over the node values
+        constexpr int PER = (NW + NT - 1) / NT;
+        double dv[PER][10];
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int idx = t + r * NT;
+            if (idx < NW && idx % WD != WD - 1) {
+#pragma unroll
+                for (int q = 0; q < 10; q++) dv[r][q] = sm[(V_F0 + q) * NW + idx + 1] - sm[(V_F0 + q) * NW + idx];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int idx = t + r * NT;
+            if (idx < NW && idx % WD != WD - 1) {
+#pragma unroll
+                for (int q = 0; q < 10; q++) sm[(V_F0 + q) * NW + idx] = dv[r][q];
+            }
+        }
+        __syncthreads();
+    }
 
     // phase 2: interface flux between nodes (d0-1+di) and (d0+di)
     const int ta = t / TDI, di = t - ta * TDI;
     const int ob = ta * WD + di;                 // window position of the stencil's first node
     const int oc = ob + R;                       // left node of the interface
     double out[5];
+    double dcur[5];                              // DIFF: D[oc] of every field, for the divergence
     {
         const double Jh = (sm[VJ * NW + oc] + sm[VJ * NW + oc + 1]) / 2.0;
         const double u = 0.5 * (sm[V_U * NW + oc] + sm[V_U * NW + oc + 1]);
         const double v = 0.5 * (sm[V_V * NW + oc] + sm[V_V * NW + oc + 1]);
         const double w = 0.5 * (sm[V_W * NW + oc] + sm[V_W * NW + oc + 1]);
-        const double rho = 0.5 * (sm[V_U0 * NW + oc] + sm[V_U0 * NW + oc + 1]);
+        constexpr int VR = DIFF ? VRHO : V_U0;
+        const double rho = 0.5 * (sm[VR * NW + oc] + sm[VR * NW + oc + 1]);
         const double cs = 0.5 * (sm[V_C * NW + oc] + sm[V_C * NW + oc + 1]);
         double nx, ny, nz;
         if constexpr (AX < 0) {
@@ -661,6 +699,29 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             double Lc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             if constexpr (AX < 0) left_row<c>(e, Lc);
             else left_row_ax<c, (AX < 0 ? 0 : AX)>(e, nsig, Lc);
+            if constexpr (DIFF) {
+                // projected differences da_s = L_c . (F_{s+1} - F_s), db_s likewise for U, on the
+                // five node pairs of the stencil; f+ differences on pairs 0..3, f- on 4..1
+                double da[5], db[5];
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    double aa = 0.0, bb = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) {
+                        if (lnz(c, AX, q)) {
+                            aa += Lc[q] * sm[(V_F0 + q) * NW + ob + s];
+                            bb += Lc[q] * sm[(V_U0 + q) * NW + ob + s];
+                        }
+                    }
+                    da[s] = aa;
+                    db[s] = bb;
+                }
+                phis[c] = 0.5 * (-phi_w5_fast(k.eps, da[0] + lj * db[0], da[1] + lj * db[1], da[2] + lj * db[2],
+                                              da[3] + lj * db[3])
+                                 + phi_w5_fast(k.eps, da[4] - lj * db[4], da[3] - lj * db[3], da[2] - lj * db[2],
+                                               da[1] - lj * db[1]));
+                return;
+            }
             double av[2 * G], bv[2 * G];
 #pragma unroll
             for (int s = 0; s < 2 * G; s++) {
@@ -706,7 +767,11 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             constexpr int m = decltype(mm)::value;
             const double *F = &sm[(V_F0 + m) * NW];
             double th;
-            if constexpr (S == 5) {
+            if constexpr (DIFF) {
+                // Theta without F[oc] (added back in the divergence as D[oc], the F difference)
+                th = 1. / 12. * ((F[oc - 1] - F[oc + 1]) + 6. * F[oc]);
+                dcur[m] = F[oc];
+            } else if constexpr (S == 5) {
                 th = 1. / 12. * (-1. * (F[oc - 1] + F[oc + 2]) + 7. * (F[oc] + F[oc + 1]));
             } else {
                 th = 1. / 840. * (533. * (F[oc + 1] + F[oc]) - 139. * (F[oc + 2] + F[oc - 1])
@@ -752,7 +817,8 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
             const long n = bbase + (long)pa * a.sA + (long)(dn + a.oD) * a.sD;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const double dv = sm[m * NT + t + 1] - sm[m * NT + t];
+                double dv = sm[m * NT + t + 1] - sm[m * NT + t];
+                if constexpr (DIFF) dv = dcur[m] + dv;
                 if constexpr (KEEP) keep[m] = -dv;
                 else a.q.f[m][n] = (a.first ? 0.0 : a.q.f[m][n]) - dv;
             }
@@ -1412,8 +1478,9 @@ __device__ __forceinline__ void eta_body(const EtaArgs &z, const int bx, const i
     __syncthreads();
     visc_body<5, 1, 8, 32, true, true>(z.v, bx, by, bz, lds, inv);
 }
-// FP64: three waves per SIMD (140 VGPRs, no spills) measured 1.53 ms at 256^3 against 1.82 ms when
-// held to 128 VGPRs for four (which spills)
+// FP64: asked for three waves per SIMD.  With the WENO phase in difference form it takes 124
+// VGPRs, so four fit (1.44 ms at 256^3; 1.53-1.57 ms at 140 VGPRs and three waves); forcing four
+// when it needed 140 spilled and measured 1.82 ms.
 constexpr int ETA_WAVES = 3;
 template <bool MIX>
 __global__ __launch_bounds__(256, (MIX ? 1 : ETA_WAVES)) void k_eta(EtaArgs z)
