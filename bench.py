@@ -12,8 +12,8 @@ HBM.
 
 Prints one JSON line: the FP64 form, a `mixed` object with the mixed FP32/FP64 form on the same
 workload, the CPU baseline (the C oracle with OpenMP on a bounded sample of the same workload,
-on the host's CPU share) and `config0`, the 2D 64^2 CPU-runnable configuration on the GPU and on
-one host core.  Per-kernel durations come from HIP events recorded on the library's stream
+on the host's CPU share) and `mms2d_64`, the 2D 64^2 periodic manufactured-field case on the GPU and
+on one host core.  Per-kernel durations come from HIP events recorded on the library's stream
 during the timed region.
 """
 import argparse
@@ -117,7 +117,7 @@ def main():
                     help="time the steps at the first step's tau (the reference's fixed tau) instead of a CFL "
                          "step per step")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-form run of the default workload")
-    ap.add_argument("--no-config0", action="store_true", help="skip the 2D 64^2 CPU-runnable configuration")
+    ap.add_argument("--no-mms2d", action="store_true", help="skip the 2D 64^2 periodic manufactured-field case")
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving run of the default line")
     ap.add_argument("--no-power", action="store_true",
                     help="skip the board power / clock sample (amd-smi beside ~3 s of further steps)")
@@ -181,8 +181,8 @@ def main():
             out["cpu_baseline_1core"] = {
                 "value": round(rate1, 1), "unit": "cell-updates/s/RK-stage", "cores": 1, "kind": "port",
                 "sample": f"C oracle, one thread, {args.cpu1_n}^3 block, 1 step = 4 stages, {secs1:.1f}s"}
-        if not args.no_config0:
-            out["config0"] = config0_run(args)
+        if not args.no_mms2d:
+            out["mms2d_64"] = mms2d_64_run(args)
     if not args.no_serve:
         out["serve"] = serve_short(args)
     print(json.dumps(out), flush=True)
@@ -220,8 +220,8 @@ def host_info():
     return {"nproc": os.cpu_count(), "affinity": aff, "model": model, "cpu_share": min(16, aff)}
 
 
-def config0_run(args):
-    """The CPU-runnable configuration: 2D 64x64 periodic unit square with the manufactured field,
+def mms2d_64_run(args):
+    """The CPU-runnable case: 2D 64x64 periodic unit square with the manufactured field,
     WENO5 + viscous, RK4, CFL 0.5, 100 steps; the GPU path and the C oracle on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # noqa: E402  (test infrastructure: CPU baseline leg only)
