@@ -441,10 +441,10 @@ enum { V_F0, V_F1, V_F2, V_F3, V_F4, V_U0, V_U1, V_U2, V_U3, V_U4, V_LAM, V_U, V
 // fields of the window).  Diagonal metric: no normal components; the sign of the
 // metric entry rides on V_LAM (the spectral radius is positive) and J moves to V_NX, which keeps
 // the 8 x 32 tile's window under 40 KB (four blocks per CU).
-// DIFF (FP64 WENO5, diagonal metric): the F and U arrays hold node-to-node differences once the
-// node pass is done, and one more array keeps rho for the interface state (below).
+// DIFF (FP64 WENO5): the F and U arrays hold node-to-node differences once the node pass is
+// done, and one more array keeps rho for the interface state (below).
 template <int S, int AX, bool MIX>
-constexpr bool weno_diff() { return S == 5 && AX >= 0 && !MIX; }
+constexpr bool weno_diff() { return S == 5 && !MIX; }
 template <int S, int TA, int TDI, int AX, bool MIX>
 constexpr int weno_lds()
 {
@@ -466,7 +466,8 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
     constexpr int NT = TA * TDI;
     constexpr int NVK = (AX < 0) ? NV : NV - 3;
     constexpr int VJ = (AX < 0) ? V_J : V_NX;
-    // DIFF: after the node pass, position o of the F and U arrays holds node o+1 minus node o.
+    // DIFF (FP64 WENO5, any metric): after the node pass, position o of the F and U arrays holds
+    // node o+1 minus node o.
     // The projection then runs on the five differences the indicators use (instead of six node
     // values, whose differences were taken after projecting), and the central part Theta is
     // written in differences: Theta = F[oc] + (D[oc-1] + 6 D[oc] - D[oc+1]) / 12, whose F[oc]
@@ -708,7 +709,7 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
                     double aa = 0.0, bb = 0.0;
 #pragma unroll
                     for (int q = 0; q < 5; q++) {
-                        if (lnz(c, AX, q)) {
+                        if (AX < 0 || lnz(c, AX, q)) {
                             aa += Lc[q] * sm[(V_F0 + q) * NW + ob + s];
                             bb += Lc[q] * sm[(V_U0 + q) * NW + ob + s];
                         }
@@ -746,15 +747,12 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
                 fp[s] = av[s] + lj * bv[s];
                 fm[s] = av[2 * G - 1 - s] - lj * bv[2 * G - 1 - s];
             }
-            if constexpr (S == 5) {
-                phis[c] = 0.5 * (-phi_w5_fast(k.eps, fp[1] - fp[0], fp[2] - fp[1], fp[3] - fp[2], fp[4] - fp[3])
-                                 + phi_w5_fast(k.eps, fm[0] - fm[1], fm[1] - fm[2], fm[2] - fm[3], fm[3] - fm[4]));
-            } else {
-                double dp_[8], dm_[8];
+            // WENO9 (WENO5 takes the difference form above): the indicators need the node values
+            static_assert(S == 9, "FP64 WENO5 runs in difference form");
+            double dp_[8], dm_[8];
 #pragma unroll
-                for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
-                phis[c] = 0.5 * (-phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_));
-            }
+            for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
+            phis[c] = 0.5 * (-phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_));
             }
         };
         project(std::integral_constant<int, 0>{});
