@@ -748,11 +748,12 @@ __device__ __forceinline__ void weno_body(const WenoArgs &a, const int bx, const
                 fm[s] = av[2 * G - 1 - s] - lj * bv[2 * G - 1 - s];
             }
             // WENO9 (WENO5 takes the difference form above): the indicators need the node values
-            static_assert(S == 9, "FP64 WENO5 runs in difference form");
-            double dp_[8], dm_[8];
+            if constexpr (S == 9) {
+                double dp_[8], dm_[8];
 #pragma unroll
-            for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
-            phis[c] = 0.5 * (-phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_));
+                for (int s = 0; s < 8; s++) { dp_[s] = fp[s + 1] - fp[s]; dm_[s] = fm[s] - fm[s + 1]; }
+                phis[c] = 0.5 * (-phi_w9(k.eps, fp, dp_) + phi_w9(k.eps, fm, dm_));
+            }
             }
         };
         project(std::integral_constant<int, 0>{});
