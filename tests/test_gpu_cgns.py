@@ -7,7 +7,8 @@ divisions from their .sddc files (tests/golden/cgns, tests/golden/division).
     as supersonic-inflow, Riemann-outflow and slip-wall writers: one step against the oracle on
     the same inputs, <= 1e-12 per conserved variable.  The 3D testcase is 64 x 64 x 16, so with
     the reference's quirks the zeta-index quirk reads across rows (n_eta > n_zeta);
-  * divided: the testcase's division (4 zones in 2D, 8 in 3D) as blocks on one device, each block
+  * divided: the testcase's division (4 zones in 2D, 8 in 3D: the 64 x 64 x 16 zone cut along i
+    and j, the 32^3 zone along all three axes) as blocks on one device, each block
     read with its own record (its coordinates and its faces' BCs), joined by
     compat_division_connections: three inviscid steps equal the one-zone run bit for bit.  The
     zeta-index quirk is off here, since with it the reference's operator depends on the division.
@@ -67,7 +68,8 @@ def _case(dims, S, n, bc, quirks, inviscid=False):
 # WENO5: on these states (inflow ghosts against a perturbed interior) WENO9 with the reference's
 # epsilon, 10 sqrt(DBL_MIN), is ill-conditioned -- the oracle's own one-ulp spread after one step
 # reaches 8e-13 (2D) and 3e-10 (3D) -- while WENO5's stays near 2e-15
-@pytest.mark.parametrize("name,dims,nproc,S", [("2d_cube_64x64", 2, 4, 5), ("3d_cube_16x16x16", 3, 8, 5)])
+@pytest.mark.parametrize("name,dims,nproc,S", [("2d_cube_64x64", 2, 4, 5), ("3d_cube_16x16x16", 3, 8, 5),
+                                                ("3d_cube_32x32x32", 3, 8, 5)])
 def test_testcase_zone_one_step_against_oracle(name, dims, nproc, S):
     mesh, recs, size = _zone(name, dims, nproc)
     whole = meshio.whole_zone(recs[0].zonename, size)
@@ -105,7 +107,8 @@ def test_testcase_zone_one_step_against_oracle(name, dims, nproc, S):
     assert max(errs) <= 1e-12
 
 
-@pytest.mark.parametrize("name,dims,nproc", [("2d_cube_64x64", 2, 4), ("3d_cube_16x16x16", 3, 8)])
+@pytest.mark.parametrize("name,dims,nproc", [("2d_cube_64x64", 2, 4), ("3d_cube_16x16x16", 3, 8),
+                                             ("3d_cube_32x32x32", 3, 8)])
 def test_testcase_division_blocks_equal_one_zone(name, dims, nproc):
     S, g = 5, 3
     quirks = orc.QUIRK_METRIC_DIAG3D
