@@ -55,6 +55,21 @@ def build_compat(verbose=False):
     return COMPAT_LIB
 
 
+TESTCASE_SRC = os.path.join(HERE, "host", "testcase_run.c")
+TESTCASE = os.path.join(HERE, "host", "testcase_run")
+
+
+def build_testcase(verbose=False):
+    """The C program that runs a reference testcase from its mesh and division files (plain gcc
+    against both libraries)."""
+    cmd = ["gcc", "-O2", "-Wall", "-I", INCLUDE, "-I", COMPAT_DIR, TESTCASE_SRC, "-L", OUT_DIR, "-lshock_compat",
+           "-lshockx", "-Wl,-rpath,$ORIGIN/../lib", "-lm", "-o", TESTCASE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return TESTCASE
+
+
 def build(verbose=False, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -68,6 +83,7 @@ def build(verbose=False, extra=()):
     os.replace(OUT + ".tmp", OUT)
     build_driver(verbose)
     build_compat(verbose)
+    build_testcase(verbose)
     return OUT
 
 

@@ -639,3 +639,72 @@ int compat_load_mesh(const char *path, int dims, const struct compat_division *d
     munmap(m, (size_t)st.st_size);
     return rc;
 }
+
+/* ExtrapolateGhostCells (src/Functions.c:6526-6670) for a block of node coordinates: along each
+ * direction in turn (i, then j over the extended i, then k), layer l beyond a face mirrors the
+ * spacing of the interior node at the same distance from it, x[-1-l] = (x[-l] + x[l]) - x[l+1]. */
+static void extend_axis(const double *src, double *dst, const long long sh[3], int axis, int gx)
+{
+    long long dsh[3] = {sh[0], sh[1], sh[2]};
+    dsh[axis] += 2 * gx;
+    const long long n = sh[axis];
+    for (long long a = 0; a < dsh[0]; a++)
+        for (long long b = 0; b < dsh[1]; b++)
+            for (long long c = 0; c < dsh[2]; c++) {
+                const long long idx[3] = {a, b, c};
+                const long long p = idx[axis] - gx;          /* position along the axis, source frame */
+                if (p < 0 || p >= n) continue;
+                long long s[3] = {a, b, c};
+                s[axis] = p;
+                dst[(a * dsh[1] + b) * dsh[2] + c] = src[(s[0] * sh[1] + s[1]) * sh[2] + s[2]];
+            }
+    const long long st = (axis == 0) ? dsh[1] * dsh[2] : (axis == 1) ? dsh[2] : 1;
+    for (long long a = 0; a < dsh[0]; a++)
+        for (long long b = 0; b < dsh[1]; b++)
+            for (long long c = 0; c < dsh[2]; c++) {
+                const long long idx[3] = {a, b, c};
+                if (idx[axis] != 0) continue;                 /* one pass per line along the axis */
+                double *L = dst + (a * dsh[1] + b) * dsh[2] + c;
+                for (int l = 0; l < gx; l++) {
+                    const long long lo = gx - 1 - l, hi = gx + n + l;
+                    L[lo * st] = (L[(lo + 1) * st] + L[(gx + l) * st]) - L[(gx + l + 1) * st];
+                    L[hi * st] = (L[(hi - 1) * st] + L[(gx + n - 1 - l) * st]) - L[(gx + n - 2 - l) * st];
+                }
+            }
+}
+
+int compat_extend_coords(int dims, int space_order, const long long n[3], const double *x, const double *y,
+                         const double *z, double *xe, double *ye, double *ze)
+{
+    if ((dims != 2 && dims != 3) || (space_order != 5 && space_order != 9)) return -1;
+    const int gx = (space_order + 1) / 2 + 1;
+    const long long ni = n[0], nj = n[1], nk = (dims == 3) ? n[2] : 1;
+    if (ni < 2 || nj < 2 || nk < 1 || (dims == 3 && nk < 2)) return -1;
+    const long long E[3] = {ni + 2 * gx, nj + 2 * gx, (dims == 3) ? nk + 2 * gx : 1};
+    const size_t tot = (size_t)(E[0] * E[1] * E[2]);
+    const double *in[3] = {x, y, z};
+    double *out[3] = {xe, ye, ze};
+    double *t0 = (double *)malloc(tot * sizeof(double)), *t1 = (double *)malloc(tot * sizeof(double));
+    if (!t0 || !t1) { free(t0); free(t1); return -1; }
+    for (int a = 0; a < 3; a++) {
+        if (a == 2 && dims == 2) {
+            for (size_t q = 0; q < tot; q++) out[a][q] = 0.0;
+            continue;
+        }
+        /* i-fastest input (as compat_load_mesh gives it) into k-fastest (i, j, k) order */
+        long long sh[3] = {ni, nj, nk};
+        for (long long i = 0; i < ni; i++)
+            for (long long j = 0; j < nj; j++)
+                for (long long k = 0; k < nk; k++) t0[(i * nj + j) * nk + k] = in[a][i + ni * (j + nj * k)];
+        double *cur = t0, *nxt = t1;
+        for (int d = 0; d < dims; d++) {
+            extend_axis(cur, nxt, sh, d, gx);
+            sh[d] += 2 * gx;
+            double *sw = cur; cur = nxt; nxt = sw;
+        }
+        memcpy(out[a], cur, tot * sizeof(double));
+    }
+    free(t0);
+    free(t1);
+    return 0;
+}

@@ -123,6 +123,16 @@ struct compat_mesh_info {
 int compat_load_mesh(const char *path, int dims, const struct compat_division *div, double *x, double *y,
                      double *z, struct compat_mesh_info *info, char *err, int errlen);
 
+/* ExtrapolateGhostCells (src/Functions.c:6526-6670) for a block read by compat_load_mesh: from the
+ * node coordinates x, y, z (n[0] x n[1] x n[2] nodes, i fastest) the extended arrays
+ * shx_build_metric takes (extent n + space_order + 3 per direction, k fastest; 1 in 2D z, where
+ * ze is zeroed): layer l beyond a face is x[-1-l] = (x[-l] + x[l]) - x[l+1], direction by
+ * direction (i, then j over the extended i, then k).  At a face with a neighbour the reference
+ * extrapolates as well, and the metric ghosts then come from the neighbour
+ * (shx_blocks_transfer_metric).  Returns 0, or -1 (bad arguments or no memory). */
+int compat_extend_coords(int dims, int space_order, const long long n[3], const double *x, const double *y,
+                         const double *z, double *xe, double *ye, double *ze);
+
 void compat_define(struct strct_configuration *cfg);
 void compat_release(struct strct_configuration *cfg);
 

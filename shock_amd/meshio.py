@@ -54,6 +54,9 @@ def lib():
         L.compat_load_mesh.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Division), ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshInfo), ctypes.c_char_p,
                                        ctypes.c_int]
+        L.compat_extend_coords.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -96,6 +99,22 @@ def load_mesh(path, dims, div):
     # the reader gives i fastest
     xyz = [np.ascontiguousarray(a.reshape(n[2], n[1], n[0]).transpose(2, 1, 0)) for a in flat]
     return xyz, info
+
+
+def extend_coords(dims, space_order, xyz):
+    """The C library's ExtrapolateGhostCells for node coordinates of shape (ni, nj, nk): the
+    extended arrays shx_build_metric takes (what cases.ext_from_nodes computes in numpy)."""
+    n = xyz[0].shape
+    flat = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(2, 1, 0)).ravel() for a in xyz]
+    g = (space_order + 1) // 2 + 1
+    ext = (n[0] + 2 * g, n[1] + 2 * g, n[2] + 2 * g if dims == 3 else 1)
+    out = [np.zeros(ext) for _ in range(3)]
+    nn = (ctypes.c_longlong * 3)(*n)
+    rc = lib().compat_extend_coords(dims, space_order, nn, *[f.ctypes.data for f in flat],
+                                    *[o.ctypes.data for o in out])
+    if rc:
+        raise MeshError("compat_extend_coords refused its arguments")
+    return out
 
 
 def bc_strings(info, dims):

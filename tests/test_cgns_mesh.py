@@ -92,6 +92,27 @@ def test_boundary_conditions_of_the_faces_without_neighbour(name, dims, nproc):
         assert shockx.BC_WALL_SLIP in kinds
 
 
+@pytest.mark.parametrize("dims,S,n", [(3, 5, (7, 6, 5)), (3, 9, (12, 11, 10)), (2, 5, (9, 8, 1)), (2, 9, (14, 13, 1))])
+def test_c_ghost_extrapolation_equals_the_tests_extension(dims, S, n):
+    """compat_extend_coords (the C host's ExtrapolateGhostCells) gives, bit for bit, the extended
+    coordinates of cases.ext_from_nodes, which every oracle parity test builds its metric from; on
+    an affine grid the extrapolated layers are the grid's own continuation."""
+    _need_lib()
+    from shock_amd import cases
+    rng = np.random.default_rng(sum(n) + S)
+    xyz = [np.cumsum(rng.uniform(0.5, 1.5, size=n), axis=a) + 0.1 * rng.standard_normal(n) for a in range(3)]
+    if dims == 2:
+        xyz[2] = np.zeros(n)
+    got = meshio.extend_coords(dims, S, xyz)
+    want = cases.ext_from_nodes(dims, S, xyz)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    g = (S + 1) // 2 + 1
+    ii = np.arange(-g, n[0] + g, dtype=np.float64)
+    aff = [np.broadcast_to((0.25 * np.arange(n[0]))[:, None, None], n).copy()] + [np.zeros(n), np.zeros(n)]
+    ext = meshio.extend_coords(dims, S, aff)
+    assert np.array_equal(ext[0][:, 0, 0], 0.25 * ii)
+
+
 def test_refusals(tmp_path):
     _need_lib()
     mesh, div = _paths("3d_cube_16x16x16")

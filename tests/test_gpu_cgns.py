@@ -17,9 +17,14 @@ divisions from their .sddc files (tests/golden/cgns, tests/golden/division).
     transfers copy ghost-inclusive planes before its writers run), and the viscous cross
     derivatives read them; the one-zone run has this stage's writer values there (measured: 6e-7
     relative after three steps of the 2D testcase).
+  * the C driver: shock_amd/host/testcase_run (plain C over both libraries: division, mesh, BCs,
+    ghost extrapolation, metric, blocks, shx_blocks_run) on the 3D testcase's 8 zones, viscous,
+    three steps, equals bit for bit the same run assembled here from the Python readers and
+    shockx.Blocks from the driver's own initial states (written to its output file).
 """
 import ctypes
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -175,3 +180,68 @@ def test_testcase_division_blocks_equal_one_zone(name, dims, nproc):
     print(f"[cgns] {name}: {nproc} blocks, {nc} connections, {steps} inviscid steps: bit-equal to the one-zone run: "
           f"{same} (largest relative difference {worst:.2e})", flush=True)
     assert same
+
+
+def _connections(recs, dims):
+    L = meshio.lib()
+    L.compat_division_connections.argtypes = [ctypes.POINTER(meshio.Division), ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(shockx.Interface)]
+    arr = (meshio.Division * len(recs))(*recs)
+    out = (shockx.Interface * (2 * dims * len(recs)))()
+    nc = L.compat_division_connections(arr, len(recs), dims, out)
+    assert nc > 0
+    return [(c.block, c.face, c.donor, c.donor_face, tuple(c.transform)) for c in out[:nc]]
+
+
+def test_c_driver_equals_the_python_assembly(tmp_path):
+    exe = os.path.join(ROOT, "shock_amd", "host", "testcase_run")
+    if not os.path.exists(exe):
+        pytest.skip("testcase_run not built")
+    name, dims, nproc, S, steps = "3d_cube_16x16x16", 3, 8, 5, 3
+    mesh, recs, size = _zone(name, dims, nproc)
+    div = os.path.join(GOLD, "division", name + ".sddc")
+    out = str(tmp_path / "states.bin")
+    print(f"[cgns] C driver on {name}: {nproc} zones, {steps} steps", flush=True)
+    r = subprocess.run([exe, mesh, div, str(dims), str(nproc), str(steps), out], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout + r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = np.fromfile(out, dtype=np.float64)
+    blocks, shapes = [], []
+    try:
+        for z, rec in enumerate(recs):
+            xyz, info = meshio.load_mesh(mesh, dims, rec)
+            n = meshio.block_size(dims, rec)
+            c = shockx.Case(dims=dims, space_order=S, time_order=4, n=n, gamma=GAMMA, mach=MACH, reynolds=RE,
+                            quirks=orc.QUIRK_METRIC_DIAG3D, bc=meshio.face_kinds(info, dims, rec), inflow=INFLOW,
+                            outflow=OUTFLOW, T_wall=0.0)
+            blocks.append(c)
+            c.build_metric(*meshio.extend_coords(dims, S, xyz))
+            shapes.append(cases.padded_shape(dims, S, n)[0])
+        sizes = [int(np.prod(sh)) for sh in shapes]
+        assert raw.size == 2 * 5 * sum(sizes)
+        halves, off = [], 0
+        for _ in range(2):
+            states = []
+            for sh, sz in zip(shapes, sizes):
+                states.append([raw[off + m * sz: off + (m + 1) * sz].reshape(sh) for m in range(5)])
+                off += 5 * sz
+            halves.append(states)
+        initial, final = halves
+        for c, st in zip(blocks, initial):
+            c.upload_state(st)
+        with shockx.Blocks(blocks, _connections(recs, dims)) as grp:
+            grp.transfer_metric()
+            tau = 0.5 * min(c.cfl_dt(0.5) for c in blocks)
+            grp.run(tau, steps)
+            got = [c.download_state() for c in blocks]
+    finally:
+        for c in blocks:
+            c.close()
+    same = all(np.array_equal(g, f) for gz, fz in zip(got, final) for g, f in zip(gz, fz))
+    g = (S + 1) // 2
+    inner = (slice(g, -g),) * dims
+    moved = max(float(np.max(np.abs(f[inner] - i[inner]))) for fz, iz in zip(final, initial) for f, i in zip(fz, iz))
+    print(f"[cgns] C driver final states bit-equal to the Python assembly: {same} (largest interior change "
+          f"over the run {moved:.3e})", flush=True)
+    assert same and np.isfinite(moved) and moved > 1e-6
