@@ -11,13 +11,14 @@ from shock_amd import cases, shockx
 pytestmark = pytest.mark.gpu
 
 
-def _make(spec):
+def _make(spec, time_order=4):
     dims, S, n, prec, inviscid, seed = spec
     if dims == 2:
-        o, (xe, ye, ze), met, prim, nn = setup_2d_mms(n=n, S=S, inviscid=inviscid)
+        o, (xe, ye, ze), met, prim, nn = setup_2d_mms(n=n, S=S, inviscid=inviscid, time_order=time_order)
     else:
-        o, (xe, ye, ze), met, prim, nn = setup_3d_turb(n=n, S=S, inviscid=inviscid, seed=seed)
-    c = shockx.Case(dims=dims, space_order=S, time_order=4, n=nn, gamma=1.4, mach=o.c.mach,
+        o, (xe, ye, ze), met, prim, nn = setup_3d_turb(n=n, S=S, inviscid=inviscid, seed=seed,
+                                                       time_order=time_order)
+    c = shockx.Case(dims=dims, space_order=S, time_order=time_order, n=nn, gamma=1.4, mach=o.c.mach,
                     reynolds=o.c.reynolds, inviscid=inviscid, precision=prec)
     c.build_metric(xe, ye, ze)
     c.upload_state(prim)
@@ -62,6 +63,40 @@ def test_batch_equals_cases_alone_and_oracle():
         b.set_tau([t * 0.5 for t in taus])
         b.run(1)
         b.synchronize()
+    for _, c, _, _, _ in made:
+        c.close()
+
+
+def test_batch_with_ssp_rk3_cases():
+    """SSP-RK3 cases (the stage update reads the stage input U_s as well as U_n) batched beside RK4
+    cases: every case ends bit for bit where it ends alone, the FP64 RK3 cases within 1e-12 of the
+    oracle after three steps."""
+    specs = [((2, 5, 24, shockx.FP64, False, 11), 3), ((3, 5, 12, shockx.FP64, False, 12), 3),
+             ((3, 9, 12, shockx.FP64, False, 13), 3), ((3, 5, 14, shockx.MIXED, False, 14), 3),
+             ((2, 5, 20, shockx.FP64, True, 15), 3), ((3, 5, 12, shockx.FP64, False, 16), 4)]
+    made = [_make(sp, to) for sp, to in specs]
+    taus = [c.cfl_dt(0.4) for _, c, _, _, _ in made]
+    steps = 3
+    alone = []
+    for (o, c, met, prim, nn), tau in zip(made, taus):
+        c.run(tau, steps)
+        alone.append(c.download_state())
+        c.upload_state(prim)
+    with shockx.Batch([m[1] for m in made], taus) as b:
+        b.run(steps)
+        b.synchronize()
+        for k, ((o, c, met, prim, nn), tau, ref) in enumerate(zip(made, taus, alone)):
+            got = c.download_state()
+            sl = interior(o, o.c.dims, nn)
+            assert all(np.array_equal(a[sl], r[sl]) for a, r in zip(got, ref)), (k, specs[k])
+            if specs[k][0][3] == shockx.FP64:
+                want = o.run(tau, met, prim, steps)
+                errs = rel_linf(conserved(got), conserved(want), sl)
+                if o.c.dims == 2:
+                    errs = errs[:3] + errs[4:]
+                print(f"[batch] RK{specs[k][1]} case {k} {specs[k][0]} vs oracle: "
+                      + " ".join(f"{e:.1e}" for e in errs), flush=True)
+                assert max(errs) <= 1e-12
     for _, c, _, _, _ in made:
         c.close()
 
