@@ -4,8 +4,10 @@ on-device gather before every stage.
 
 A periodic box cut into two blocks must evolve as the uncut box: every interior node sees the same
 stencil values, so the blocks' states equal the single block's.
-  * identity cut (3D along i; 2D along j): bit for bit against the single block on the GPU, and
-    against the oracle's single block after one step (1e-12);
+  * identity cut (3D along i and, for SSP-RK3, along k; 2D along j): bit for bit against the
+    single block on the GPU, and against the oracle's single block after one step (1e-12); the
+    mixed FP32/FP64 form likewise (along i and j, WENO5 and WENO9), bit for bit against its own
+    single block and within its one-step bar (1e-7) of the FP64 oracle;
   * rotated cut: block B stores its axes turned a quarter turn about k (its i runs along -j of the
     box, its j along i), so the interface transform is (2, -1, 3) / (-2, 1, 3) and the ghost metric
     takes the donor's rows permuted and signed; the general metric (no diagonal-metric quirk).  The
@@ -27,9 +29,9 @@ P = shockx.BC_PERIODIC
 I = shockx.BC_INTERFACE
 
 
-def _case(n, dims, quirks, bc, S=5, inviscid=False, time_order=4):
+def _case(n, dims, quirks, bc, S=5, inviscid=False, time_order=4, precision=shockx.FP64):
     return shockx.Case(dims=dims, space_order=S, time_order=time_order, n=n, mach=0.5, reynolds=2000.0,
-                       quirks=quirks, bc=bc, inviscid=inviscid)
+                       quirks=quirks, bc=bc, inviscid=inviscid, precision=precision)
 
 
 def _box(dims, n, S, seed, quirks=orc.QUIRKS_REFERENCE, inviscid=False, time_order=4):
@@ -54,13 +56,18 @@ def _slab(a, axis, start, length):
     return np.ascontiguousarray(a[tuple(sl)])
 
 
-@pytest.mark.parametrize("dims,n,axis,quirks,S,inviscid,rk", [
-    (3, (24, 16, 16), 0, orc.QUIRKS_REFERENCE, 5, False, 4),
-    (2, (24, 24, 1), 1, orc.QUIRKS_REFERENCE, 9, False, 4),
+@pytest.mark.parametrize("dims,n,axis,quirks,S,inviscid,rk,prec", [
+    (3, (24, 16, 16), 0, orc.QUIRKS_REFERENCE, 5, False, 4, shockx.FP64),
+    (2, (24, 24, 1), 1, orc.QUIRKS_REFERENCE, 9, False, 4, shockx.FP64),
     # SSP-RK3, cut along k; without the zeta-index quirk, whose reads depend on the block's extents
-    (3, (16, 16, 24), 2, orc.QUIRK_METRIC_DIAG3D, 5, False, 3),
+    (3, (16, 16, 24), 2, orc.QUIRK_METRIC_DIAG3D, 5, False, 3, shockx.FP64),
+    # the mixed FP32/FP64 form: bit-equal to its own whole box, the mixed bar against the oracle
+    (3, (24, 16, 16), 0, orc.QUIRKS_REFERENCE, 5, False, 4, shockx.MIXED),
+    # (a cut along j without the zeta-index quirk: the quirk's zeta stencil sits at k = j + l - g,
+    # so with it the operator depends on where a block starts along j)
+    (3, (16, 24, 16), 1, orc.QUIRK_METRIC_DIAG3D, 9, False, 4, shockx.MIXED),
 ])
-def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid, rk):
+def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid, rk, prec):
     g = (S + 1) // 2
     o, coords, prim = _box(dims, n, S, 11, quirks, inviscid, rk)
     half = n[axis] // 2
@@ -71,7 +78,7 @@ def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid, rk):
     met = o.metric_from_coords(*coords)
     o.ghost_periodic(met)
     tau = o.cfl_dt(0.5, met, prim)
-    with _case(n, dims, quirks, (P,) * 6, S, inviscid, rk) as whole:
+    with _case(n, dims, quirks, (P,) * 6, S, inviscid, rk, prec) as whole:
         whole.build_metric(*coords)
         whole.upload_state(prim)
         whole.run(tau, 1)
@@ -83,7 +90,7 @@ def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid, rk):
     blocks = []
     try:
         for b in range(2):
-            c = _case(nb, dims, quirks, tuple(bc), S, inviscid, rk)
+            c = _case(nb, dims, quirks, tuple(bc), S, inviscid, rk, prec)
             blocks.append(c)
             c.build_metric(*[_slab(x, axis, b * half, nb[axis] + 2 * g + 2) for x in coords])
             c.upload_state([_slab(f, axis, b * half, nb[axis] + 2 * g) for f in prim])
@@ -107,13 +114,14 @@ def test_identity_cut_equals_whole_box(dims, n, axis, quirks, S, inviscid, rk):
     cut = tuple(cut)
     same = all(np.array_equal(a[cut], _slab(b, axis, g, n[axis])[cut]) for a, b in zip(got, ref))
     diff = rel_linf(conserved(got), conserved([_slab(b, axis, g, n[axis]) for b in ref]), cut)
-    print(f"[blocks] identity cut dims={dims} S={S} RK{rk}: bit-equal to the whole box: {same}; rel Linf "
+    print(f"[blocks] identity cut dims={dims} S={S} RK{rk} {'mixed' if prec == shockx.MIXED else 'fp64'}: bit-equal to the whole box: {same}; rel Linf "
           + " ".join(f"{e:.2e}" for e in diff), flush=True)
     assert same
     want1 = o.run(tau, met, prim, 1)
     errs = rel_linf(conserved(got1), conserved([_slab(b, axis, g, n[axis]) for b in want1]), cut)
     print(f"[blocks] identity cut one step vs oracle: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
-    assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= 1e-12
+    # the mixed form's one-step bar against the FP64 oracle is its FP32 rounding of the deviation
+    assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= (1e-7 if prec == shockx.MIXED else 1e-12)
     assert all(np.array_equal(a[cut], b[cut]) for a, b in
                zip(got1, [_slab(x, axis, g, n[axis]) for x in one]))
 

@@ -7,8 +7,8 @@
 * CFL stepping (shx_run_cfl): each step's time step is CFL / max node CFL sum of the state it
   starts from.  After the first step that maximum comes from the fused reduction of the previous
   step's last stage; it must equal the separate reduction (shx_cfl_dt) bit for bit, for each kind
-  of last-stage kernel (fused zeta kernel, last viscous sweep with a general metric, 2D viscous,
-  inviscid update).  Several steps match the oracle stepping with its own CFL step each step.
+  of last-stage kernel (fused zeta kernel for WENO5, WENO9 and the mixed form, last viscous sweep
+  with a general metric, 2D viscous in FP64 and mixed, inviscid update).  Several steps match the oracle stepping with its own CFL step each step.
 """
 import numpy as np
 import pytest
@@ -20,32 +20,36 @@ from shock_amd import shockx
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_case(o, coords, prim, nn, dims, quirks):
+def _gpu_case(o, coords, prim, nn, dims, quirks, precision=shockx.FP64):
     g = shockx.Case(dims=dims, space_order=o.c.S, time_order=4, n=nn, mach=o.c.mach, reynolds=o.c.reynolds,
-                    inviscid=bool(o.c.inviscid), quirks=quirks)
+                    inviscid=bool(o.c.inviscid), quirks=quirks, precision=precision)
     g.build_metric(*coords)
     g.upload_state(prim)
     return g
 
 
 CASES = [
-    ("3d_fused_zeta", 3, False, orc.QUIRKS_REFERENCE),
-    ("3d_general_metric", 3, False, 0),
-    ("3d_inviscid", 3, True, orc.QUIRKS_REFERENCE),
-    ("2d_viscous", 2, False, 0),
+    ("3d_fused_zeta", 3, False, orc.QUIRKS_REFERENCE, 5, shockx.FP64),
+    ("3d_fused_zeta_w9", 3, False, orc.QUIRKS_REFERENCE, 9, shockx.FP64),
+    ("3d_fused_zeta_mixed", 3, False, orc.QUIRKS_REFERENCE, 5, shockx.MIXED),
+    ("3d_general_metric", 3, False, 0, 5, shockx.FP64),
+    ("3d_inviscid", 3, True, orc.QUIRKS_REFERENCE, 5, shockx.FP64),
+    ("2d_viscous", 2, False, 0, 5, shockx.FP64),
+    ("2d_viscous_mixed", 2, False, 0, 5, shockx.MIXED),
 ]
 
 
-def _setup(dims, inviscid, quirks):
+def _setup(dims, inviscid, quirks, S=5):
     if dims == 3:
-        return setup_3d_turb(n=16, S=5, inviscid=inviscid, quirks=quirks)
-    return setup_2d_mms(n=24, S=5, inviscid=inviscid)
+        return setup_3d_turb(n=16, S=S, inviscid=inviscid, quirks=quirks)
+    return setup_2d_mms(n=24, S=S, inviscid=inviscid)
 
 
-@pytest.mark.parametrize("name,dims,inviscid,quirks", CASES, ids=[c[0] for c in CASES])
-def test_fused_cfl_max_equals_separate_reduction(name, dims, inviscid, quirks):
-    o, coords, met, prim, nn = _setup(dims, inviscid, quirks)
-    with _gpu_case(o, coords, prim, nn, dims, quirks) as a, _gpu_case(o, coords, prim, nn, dims, quirks) as b:
+@pytest.mark.parametrize("name,dims,inviscid,quirks,S,prec", CASES, ids=[c[0] for c in CASES])
+def test_fused_cfl_max_equals_separate_reduction(name, dims, inviscid, quirks, S, prec):
+    o, coords, met, prim, nn = _setup(dims, inviscid, quirks, S)
+    with _gpu_case(o, coords, prim, nn, dims, quirks, prec) as a, \
+            _gpu_case(o, coords, prim, nn, dims, quirks, prec) as b:
         t2 = a.run_cfl(0.5, 2)                 # second step from the fused maximum
         t1 = b.run_cfl(0.5, 1)
         dt1 = b.cfl_dt(0.5)                    # separate reduction on the same state
