@@ -39,13 +39,15 @@ def _run(tmp_path, o, met, prim, tau, steps, bc, inflow=(1.0, 1.0, 0.0, 0.0, 1.0
     return [data[m] for m in range(5)]
 
 
-def test_compat_step_loop_periodic_2d(tmp_path):
-    o, coords, met, prim, nn = setup_2d_mms(n=32, S=5)
+@pytest.mark.parametrize("S,time_order", [(5, 4), (9, 4), (5, 3)])
+def test_compat_step_loop_periodic_2d(tmp_path, S, time_order):
+    o, coords, met, prim, nn = setup_2d_mms(n=32, S=S, time_order=time_order)
     tau = o.cfl_dt(0.5, met, prim)
     got = _run(tmp_path, o, met, prim, tau, 10, (orc.BC_PERIODIC,) * 6)
     want = o.run(tau, met, prim, 10)
     errs = rel_linf(conserved(got), conserved(want), interior(o, 2, nn))
-    print("[compat] periodic 2D, 10 steps: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    print(f"[compat] periodic 2D W{S} RK{time_order}, 10 steps: " + " ".join(f"{e:.2e}" for e in errs),
+          flush=True)
     assert max(errs[:3] + errs[4:]) <= 1e-11
 
 
