@@ -19,6 +19,7 @@ during the timed region.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -121,12 +122,14 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the short serving run of the default line")
     ap.add_argument("--no-power", action="store_true",
                     help="skip the board power / clock sample (amd-smi beside ~3 s of further steps)")
-    ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock"),
+    ap.add_argument("--workload", choices=("turbulence", "mms-ladder", "batch", "serve", "shock", "compat"),
                     default="turbulence",
                     help="turbulence: the 256^3 throughput workload (default); mms-ladder: the "
                          "manufactured-solution convergence ladder in both arithmetic forms; batch: 96 "
                          "uneven cases per graph-replayed step; serve: Poisson solve requests; shock: "
-                         "the 384^3 WENO9 Mach-3 shock / vorticity block with inflow and outflow")
+                         "the 384^3 WENO9 Mach-3 shock / vorticity block with inflow and outflow; compat: "
+                         "the default workload through the reference-named host loop (state in host "
+                         "memory between steps)")
     ap.add_argument("--cases", type=int, default=96, help="batch: number of cases; serve: requests")
     ap.add_argument("--rate", default="0.5", help="serve: arrival rates (requests per second, comma-separated)")
     ap.add_argument("--shock-frac", type=float, default=0.5, help="serve: fraction of shock requests")
@@ -156,6 +159,8 @@ def main():
         return serve_workload(args)
     if args.workload == "shock":
         return shock_workload(args)
+    if args.workload == "compat":
+        return compat_workload(args)
 
     fp64 = turbulence_run(args, "fp64" if args.precision == "fp64" else "mixed")
     out = fp64["line"]
@@ -282,7 +287,6 @@ def power_sample(run, seconds=3.0):
     """Board power and graphics clock sampled with amd-smi (a child process every ~0.3 s) while
     run(k) keeps the GPU busy with more steps of the same workload, after the timed region;
     None where amd-smi is absent or prints nothing usable."""
-    import re
     import shutil
     import subprocess
     import threading
@@ -629,6 +633,70 @@ def shock_workload(args):
                       "precision": args.precision, "seed": args.seed, "tau": tau},
            "non_finite_nodes": nan, "kernels": kernels}
     print(json.dumps(out), flush=True)
+
+
+def compat_workload(args):
+    """The default workload (n^3, WENO5 + viscous, FP64, RK4, the reference's quirks, periodic) at a
+    fixed tau through the reference-named entry points and the reference's step loop
+    (shock_amd/compat/compat_run: CalcRungeKutta, checkNAN, WriteValuesFromU1ToU2 on host arrays):
+    every step hands the state over in host memory, so its rate includes the PCIe transfers and the
+    loop's own host copies.  Printed beside the resident rate of the same steps (shx_run)."""
+    import subprocess
+    import tempfile
+    n, S, time_order = args.n, 5, 4
+    nn = (n, n, n)
+    steps = max(2, min(args.steps, 10))
+    case = shockx.Case(dims=3, space_order=S, time_order=time_order, n=nn, gamma=1.4, mach=0.5,
+                       reynolds=2000.0, prandtl=0.72, T0=300.0, quirks=shockx.QUIRKS_REFERENCE)
+    xe, ye, ze = cases.periodic_ext_coords(3, S, nn, cases.sine_bump_3d(0.05))
+    case.build_metric(xe, ye, ze)
+    del xe, ye, ze
+    prim = cases.turbulence_3d_state(3, S, nn, args.seed, mach=0.5)
+    case.upload_state(prim)
+    tau = case.cfl_dt(args.cfl)
+    case.run(tau, 1)
+    case.synchronize()
+    t0 = time.perf_counter()
+    case.run(tau, steps)
+    case.synchronize()
+    resident_ms = 1e3 * (time.perf_counter() - t0) / steps
+    met = case.download_metric()
+    case.close()
+    hdr = np.array([3, S, n, n, n, time_order, 0, steps + 1] + [0] * 6, dtype=np.int32)
+    par = np.array([1.4, 0.5, 2000.0, 0.72, 300.0, tau, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0], dtype=np.float64)
+    tmp = tempfile.mkdtemp(prefix="shockx_compat_")
+    path, out = os.path.join(tmp, "case.bin"), os.path.join(tmp, "out.bin")
+    try:
+        with open(path, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            for a in list(met) + list(prim):
+                f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        del met, prim
+        exe = os.path.join(ROOT, "shock_amd", "compat", "compat_run")
+        print(f"[bench] compat loop: {steps + 1} steps of {n}^3 at tau {tau:.6g}", file=sys.stderr, flush=True)
+        r = subprocess.run([exe, path, out], capture_output=True, text=True, timeout=900)
+        print(r.stdout.strip(), file=sys.stderr, flush=True)
+        if r.returncode:
+            print(r.stderr, file=sys.stderr)
+            sys.exit(r.returncode)
+    finally:
+        for fn in (path, out):
+            if os.path.exists(fn):
+                os.remove(fn)
+        os.rmdir(tmp)
+    m = re.search(r"first step ([0-9.]+) ms, then ([0-9.]+) ms per step", r.stdout)
+    first_ms, ms = float(m.group(1)), float(m.group(2))
+    line = {"metric": "cell-updates/s/RK-stage", "value": round(n ** 3 * time_order / (ms * 1e-3), 1),
+            "unit": "cell-updates/s/RK-stage", "n_gpus": 1, "steps": steps, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded random-phase turbulence)",
+            "config": {"workload": f"{n}^3 periodic curvilinear block, WENO5 + viscous, FP64, RK4, Re=2000, "
+                                   "fixed tau, through the reference-named host loop (compat_run)",
+                       "state": "host memory between steps (PCIe-inclusive)"},
+            "first_step_ms": round(first_ms, 3),
+            "resident_ms_per_step": round(resident_ms, 3),
+            "resident_value": round(n ** 3 * time_order / (resident_ms * 1e-3), 1)}
+    print(json.dumps(line), flush=True)
 
 
 def mms_ladder(args):

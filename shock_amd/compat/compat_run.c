@@ -8,12 +8,15 @@
  * 3 WallInviscid, 4 WallViscous, 5 Farfield); float64 gamma, Mach, Reynolds, Prandtl, T0, tau,
  * rho, u, v, w, p of the inflow, p_out, AlphaNonRef, T_wall; then the metric (xi_x .. zeta_z, J)
  * and rho, u, v, w, e, each on the padded grid.  Output: U_lastStep's rho, u, v, w, e.
+ * Prints the time of the first step (it uploads the metric and the state) and the mean time of
+ * the others, host-to-host: the loop's own state copies and compares in host memory included.
  */
 #include "shock_compat.h"
 
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 static void rd(void *p, size_t sz, size_t n, FILE *f)
 {
@@ -86,11 +89,23 @@ int main(int argc, char **argv)
     fclose(in);
     WriteValuesFromU1ToU2(&cfg, &U_lastStep, &U_RK);
     struct strct_Flux Q = {0}, Q_sum = {0}, Flux = {0}, Flux_PlusHalf = {0};
+    struct timespec t0, t1, t2;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    t1 = t0;
+    int done = 0;
     for (int it = 1; it <= steps; it++) {
         CalcRungeKutta(&cfg, &mesh, &U_lastStep, &U_RK, &Q, &Q_sum, &Flux, &Flux_PlusHalf);
         if (checkNAN(&cfg, &U_RK, &mesh) == 1) { printf("SHOCK: NAN detected at iteration %d\n", it); break; }
         WriteValuesFromU1ToU2(&cfg, &U_RK, &U_lastStep);
+        done = it;
+        if (it == 1) clock_gettime(CLOCK_MONOTONIC, &t1);
     }
+    clock_gettime(CLOCK_MONOTONIC, &t2);
+    const double first = 1e3 * ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+    const double rest = 1e3 * ((double)(t2.tv_sec - t1.tv_sec) + 1e-9 * (double)(t2.tv_nsec - t1.tv_nsec));
+    if (done > 1)
+        printf("compat_run: first step %.3f ms, then %.3f ms per step over %d steps\n", first, rest / (done - 1),
+               done - 1);
     FILE *out = fopen(argv[2], "wb");
     if (!out) { perror(argv[2]); return 2; }
     for (int m = 0; m < 5; m++) fwrite(sf[m], sizeof(FLT), ntot, out);
