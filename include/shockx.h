@@ -143,7 +143,10 @@ int shx_upload_metric(shx_case *c, const double *const met[10]);
 int shx_download_metric(shx_case *c, double *const met[10]);
 
 /* State: primitive rho, u, v, w, e (strct_U, src/SHOCK.h:590-620).  Upload sets U_lastStep
- * (and therefore U_RK at the next step start); download returns U_lastStep. */
+ * (and therefore U_RK at the next step start); download returns U_lastStep.  Field transfers
+ * (state, metric, source, boundary state, post-processing fields, residual) go through two
+ * pinned staging buffers of up to 8 MB the handle allocates on its first transfer; an upload
+ * returns once the caller's arrays have been read. */
 int shx_upload_state(shx_case *c, const double *const prim[5]);
 int shx_download_state(shx_case *c, double *const prim[5]);
 
