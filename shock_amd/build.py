@@ -43,7 +43,7 @@ def build_compat(verbose=False):
     cmds = [
         ["gcc", "-O2", "-Wall", "-fPIC", "-shared", "-I", INCLUDE, "-I", COMPAT_DIR,
          os.path.join(COMPAT_DIR, "shock_compat.c"), os.path.join(COMPAT_DIR, "cgns_mesh.c"),
-         "-L", OUT_DIR, "-lshockx", "-Wl,-rpath,$ORIGIN",
+         "-L", OUT_DIR, "-lshockx", "-Wl,-rpath,$ORIGIN", "-pthread",
          "-o", COMPAT_LIB],
         ["gcc", "-O2", "-Wall", "-I", COMPAT_DIR, os.path.join(COMPAT_DIR, "compat_run.c"), "-L", OUT_DIR,
          "-lshock_compat", "-lshockx", "-Wl,-rpath,$ORIGIN/../lib", "-o", COMPAT_RUN],

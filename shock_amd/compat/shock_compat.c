@@ -10,9 +10,65 @@
 #include "shock_compat.h"
 #include "shockx.h"
 
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Host-memory passes over whole fields (the shadow compare and copy, WriteValuesFromU1ToU2) on up
+ * to PAR_THREADS threads: one thread reaches a fraction of the host's memory bandwidth.  Fields
+ * under PAR_MIN bytes, or a thread that cannot be started, are done on the calling thread. */
+#define PAR_THREADS 8
+#define PAR_MIN ((size_t)4 << 20)
+struct par_job {
+    const char *a, *b;   /* compare a with b, or copy a to dst */
+    char *dst;
+    size_t n;
+    int diff;
+};
+static void *par_cmp(void *p)
+{
+    struct par_job *j = (struct par_job *)p;
+    j->diff = memcmp(j->a, j->b, j->n) != 0;
+    return NULL;
+}
+static void *par_cpy(void *p)
+{
+    struct par_job *j = (struct par_job *)p;
+    memcpy(j->dst, j->a, j->n);
+    return NULL;
+}
+/* returns 1 when any compared part differs (par_cmp), 0 otherwise */
+static int par_run(void *(*fn)(void *), const void *a, const void *b, void *dst, size_t n)
+{
+    struct par_job job[PAR_THREADS];
+    pthread_t th[PAR_THREADS];
+    int started[PAR_THREADS] = {0};
+    const int nt = n < PAR_MIN ? 1 : PAR_THREADS;
+    const size_t part = ((n + nt - 1) / nt + 63) & ~(size_t)63;
+    int parts = 0;
+    for (int t = 0; t < nt; t++) {
+        const size_t off = (size_t)t * part;
+        if (off >= n) break;
+        job[t].a = (const char *)a + off;
+        job[t].b = b ? (const char *)b + off : NULL;
+        job[t].dst = dst ? (char *)dst + off : NULL;
+        job[t].n = (n - off < part) ? n - off : part;
+        job[t].diff = 0;
+        parts = t + 1;
+        if (t > 0) started[t] = pthread_create(&th[t], NULL, fn, &job[t]) == 0;
+    }
+    fn(&job[0]);
+    int diff = job[0].diff;
+    for (int t = 1; t < parts; t++) {
+        if (started[t]) pthread_join(th[t], NULL);
+        else fn(&job[t]);
+        diff |= job[t].diff;
+    }
+    return diff;
+}
+static int fields_differ(const double *a, const double *b, size_t bytes) { return par_run(par_cmp, a, b, NULL, bytes); }
+static void copy_field(double *dst, const double *src, size_t bytes) { par_run(par_cpy, src, NULL, dst, bytes); }
 
 struct compat_ctx {
     shx_case *c;
@@ -165,11 +221,11 @@ static void ensure_on_device(struct compat_ctx *x, struct strct_U *U)
     double *f[5];
     fields(U, f);
     int same = x->have_shadow;
-    for (int m = 0; m < 5 && same; m++) same = !memcmp(f[m], x->shadow[m], sizeof(double) * x->ntot);
+    for (int m = 0; m < 5 && same; m++) same = !fields_differ(f[m], x->shadow[m], sizeof(double) * x->ntot);
     if (same) return;
     const double *cf[5] = {f[0], f[1], f[2], f[3], f[4]};
     CALL(shx_upload_state(x->c, cf));
-    for (int m = 0; m < 5; m++) memcpy(x->shadow[m], f[m], sizeof(double) * x->ntot);
+    for (int m = 0; m < 5; m++) copy_field(x->shadow[m], f[m], sizeof(double) * x->ntot);
     x->have_shadow = 1;
 }
 
@@ -178,7 +234,7 @@ static void write_back(struct compat_ctx *x, struct strct_U *U)
     double *f[5];
     fields(U, f);
     CALL(shx_download_state(x->c, f));
-    for (int m = 0; m < 5; m++) memcpy(x->shadow[m], f[m], sizeof(double) * x->ntot);
+    for (int m = 0; m < 5; m++) copy_field(x->shadow[m], f[m], sizeof(double) * x->ntot);
     x->have_shadow = 1;
 }
 
@@ -216,12 +272,12 @@ void WriteValuesFromU1ToU2(struct strct_configuration *pnt_config, struct strct_
     const long ntot = (long)pnt_config->int_iMeshPointsGhostCells * pnt_config->int_jMeshPointsGhostCells *
                       pnt_config->int_kMeshPointsGhostCells;
     const size_t bytes = sizeof(double) * (size_t)ntot;
-    memcpy(pnt_U2->rho, pnt_U1->rho, bytes);
-    memcpy(pnt_U2->u, pnt_U1->u, bytes);
-    memcpy(pnt_U2->v, pnt_U1->v, bytes);
-    memcpy(pnt_U2->w, pnt_U1->w, bytes);
-    memcpy(pnt_U2->e, pnt_U1->e, bytes);
-    if (pnt_U1->p && pnt_U2->p) memcpy(pnt_U2->p, pnt_U1->p, bytes);
+    copy_field(pnt_U2->rho, pnt_U1->rho, bytes);
+    copy_field(pnt_U2->u, pnt_U1->u, bytes);
+    copy_field(pnt_U2->v, pnt_U1->v, bytes);
+    copy_field(pnt_U2->w, pnt_U1->w, bytes);
+    copy_field(pnt_U2->e, pnt_U1->e, bytes);
+    if (pnt_U1->p && pnt_U2->p) copy_field(pnt_U2->p, pnt_U1->p, bytes);
 }
 
 /* ---- division file -------------------------------------------------------------------- */

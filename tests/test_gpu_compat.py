@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import orc
-from helpers import conserved, interior, rel_linf, setup_2d_mms
+from helpers import conserved, interior, rel_linf, setup_2d_mms, setup_3d_turb
 from shock_amd import build, cases
 
 pytestmark = pytest.mark.gpu
@@ -70,3 +70,16 @@ def test_compat_step_loop_faces_3d(tmp_path):
     errs = rel_linf(conserved(got), conserved(want), interior(o, 3, n))
     print("[compat] inflow/outflow + slip walls 3D, 5 steps: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
     assert max(errs) <= 1e-11
+
+
+def test_compat_step_loop_periodic_3d_large_fields(tmp_path):
+    # fields of 5 MB (86^3 padded nodes): the host passes of the loop (shadow compare and copy,
+    # WriteValuesFromU1ToU2) and the staged transfers run on several host threads
+    o, coords, met, prim, nn = setup_3d_turb(n=80, S=5)
+    orc.lib().orc_set_threads(min(16, len(os.sched_getaffinity(0))))
+    tau = o.cfl_dt(0.5, met, prim)
+    got = _run(tmp_path, o, met, prim, tau, 3, (orc.BC_PERIODIC,) * 6)
+    want = o.run(tau, met, prim, 3)
+    errs = rel_linf(conserved(got), conserved(want), interior(o, 3, nn))
+    print("[compat] periodic 3D 80^3, 3 steps: " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert max(errs) <= 1e-12
