@@ -145,7 +145,8 @@ int shx_download_metric(shx_case *c, double *const met[10]);
 /* State: primitive rho, u, v, w, e (strct_U, src/SHOCK.h:590-620).  Upload sets U_lastStep
  * (and therefore U_RK at the next step start); download returns U_lastStep.  Field transfers
  * (state, metric, source, boundary state, post-processing fields, residual) go through two
- * pinned staging buffers of up to 8 MB the handle allocates on its first transfer; an upload
+ * pinned staging buffers of up to 8 MB the handle allocates on its first transfer (the copies
+ * between them and the caller's arrays run on up to four short-lived host threads); an upload
  * returns once the caller's arrays have been read. */
 int shx_upload_state(shx_case *c, const double *const prim[5]);
 int shx_download_state(shx_case *c, double *const prim[5]);
