@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 KEYS = ("resets", "nan_steps", "accelerations", "tau_resets", "aborted", "iteration")
 
 
-def _compare(o, nn, dims, want_state, want, got_state, got, tag):
+def _compare(o, nn, dims, want_state, want, got_state, got, tag, bar=1e-10):
     print(f"[simulate] {tag}: oracle {want}", flush=True)
     print(f"[simulate] {tag}: GPU    {got}", flush=True)
     for k in KEYS:
@@ -26,7 +26,7 @@ def _compare(o, nn, dims, want_state, want, got_state, got, tag):
         sl = interior(o, dims, nn)
         errs = rel_linf(conserved(got_state), conserved(want_state), sl)
         print(f"[simulate] {tag}: state " + " ".join(f"{e:.2e}" for e in errs), flush=True)
-        assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= 1e-10
+        assert max(errs[:3] + errs[4:] if dims == 2 else errs) <= bar
 
 
 def _sim(tau, **kw):
@@ -43,21 +43,43 @@ def _gpu(g, s):
                       s["distance_nan"], s["distance_forward"], s["distance_backward"])
 
 
-def test_unstable_tau_recovers_periodic_2d():
+def _spread(o, met, prim, s, want_state, dims, nn):
+    """the oracle's own one-ulp spread of the run's final state (two perturbed starts)"""
+    rng = np.random.default_rng(3)
+    out = 0.0
+    for _ in range(2):
+        pert = [a * (1.0 + 2.2e-16 * rng.standard_normal(a.shape)) for a in prim]
+        ps, _ = o.simulate(met, pert, s)
+        e = rel_linf(conserved(ps), conserved(want_state), interior(o, dims, nn))
+        out = max(out, max(e[:3] + e[4:] if dims == 2 else e))
+    return out
+
+
+@pytest.mark.parametrize("S,time_order", [(5, 4), (5, 3), (9, 4)])
+def test_unstable_tau_recovers_periodic_2d(S, time_order):
     # tau at six times the CFL-0.5 step: the state blows up, rollbacks halve tau, the resetter
-    # tries the starting tau again
-    o, coords, met, prim, nn = setup_2d_mms(n=32, S=5)
+    # tries the starting tau again (WENO5 and WENO9, RK4 and SSP-RK3).  The decisions, time and tau
+    # must match exactly.  The WENO5 RK4 run is well conditioned (the oracle's own one-ulp spread
+    # of the final state 6e-13) and held to the many-step bar; in the SSP-RK3 and WENO9 runs the
+    # steps after the rollbacks amplify round-off (spread 2.8e-9 and 4.6e-6), so their state is
+    # held to twice the spread the test measures
+    o, coords, met, prim, nn = setup_2d_mms(n=32, S=S, time_order=time_order)
     s = _sim(6.0 * o.cfl_dt(0.5, met, prim))
     want_state, want = o.simulate(met, prim, s)
     assert want["resets"] >= 1 and want["tau_resets"] >= 1
+    bar = 1e-10
+    if (S, time_order) != (5, 4):
+        spread = _spread(o, met, prim, s, want_state, 2, nn)
+        print(f"[simulate] W{S} RK{time_order}: the oracle's one-ulp spread {spread:.2e}", flush=True)
+        bar = max(bar, 2.0 * spread)
     c = o.c
-    with shockx.Case(dims=2, space_order=5, time_order=4, n=nn, gamma=c.gamma, mach=c.mach,
+    with shockx.Case(dims=2, space_order=S, time_order=time_order, n=nn, gamma=c.gamma, mach=c.mach,
                      reynolds=c.reynolds, prandtl=c.prandtl, T0=c.T0) as g:
         g.upload_metric(met)
         g.upload_state(prim)
         got = _gpu(g, s)
         got_state = g.download_state()
-    _compare(o, nn, 2, want_state, want, got_state, got, "periodic 2D")
+    _compare(o, nn, 2, want_state, want, got_state, got, f"periodic 2D W{S} RK{time_order}", bar)
 
 
 def test_unstable_tau_recovers_with_rudy_outflow():
