@@ -20,7 +20,7 @@ divisions from their .sddc files (tests/golden/cgns, tests/golden/division).
   * the C driver: shock_amd/host/testcase_run (plain C over both libraries: division, mesh, BCs,
     ghost extrapolation, metric, blocks, shx_blocks_run) on every zone of the 2D and 3D testcases'
     divisions, viscous, three steps, equals bit for bit the same run assembled here from the
-    Python readers and shockx.Blocks from the driver's own initial states (written to its output
+    Python readers and shockx.Blocks from the program's own initial states (written to its output
     file).
 """
 import ctypes
