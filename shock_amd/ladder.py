@@ -17,6 +17,7 @@ coordinates, src/Functions.c:6388-6509) limits the observed order to two; the un
 the order of the flux discretisation.
 """
 import math
+import sys
 import time
 
 import numpy as np
@@ -40,8 +41,20 @@ def setup(n, S=5, precision=shockx.FP64, reynolds=1000.0, quirks=shockx.QUIRK_ME
     xe, ye, ze = cases.ext_coords(3, S, nn, stretched(amp), periodic=per)
     x, y, z = cases.interior_coords(3, S, nn, xe, ye, ze)
     P = mms.reference_params(3, False, False, gamma=GAMMA, mach=MACH)
-    exact = mms.exact_state(P, x, y, z, GAMMA, MACH)
-    src = mms.source(P, x, y, z, GAMMA, MACH, reynolds, PRANDTL, False)
+    # the exact fields and the source are pointwise: evaluated in i slabs (long double arithmetic
+    # takes minutes at 256^3), with a progress line per slab for large rungs
+    nslab = max(1, x.shape[0] // 32)
+    cuts = np.linspace(0, x.shape[0], nslab + 1).astype(int)
+    exact = [np.empty_like(x) for _ in range(6)]
+    src = [np.empty_like(x) for _ in range(5)]
+    for q in range(nslab):
+        sl = slice(cuts[q], cuts[q + 1])
+        for a, b in zip(exact, mms.exact_state(P, x[sl], y[sl], z[sl], GAMMA, MACH)):
+            a[sl] = b
+        for a, b in zip(src, mms.source(P, x[sl], y[sl], z[sl], GAMMA, MACH, reynolds, PRANDTL, False)):
+            a[sl] = b
+        if nslab > 2:
+            print(f"[ladder] n={n}: exact state and source, slab {q + 1} of {nslab}", file=sys.stderr, flush=True)
     c = shockx.Case(dims=3, space_order=S, time_order=4, n=nn, gamma=GAMMA, mach=MACH, reynolds=reynolds,
                     prandtl=PRANDTL, manufactured=True, quirks=quirks, bc=(shockx.BC_MANUFACTURED,) * 6,
                     precision=precision)
@@ -63,12 +76,18 @@ def rung(n, S=5, precision=shockx.FP64, t_end=2.0, cfl=0.5, reynolds=1000.0, chu
     done = 0
     prev = None
     change = float("nan")
+    last_note = t0
     while done < steps:
         k = min(chunk, steps - done)
         c.run(tau, k)
         done += k
         if steps - done < chunk and prev is None:
             prev = c.download_state()[0][g:-g, g:-g, g:-g].copy()
+        if time.perf_counter() - last_note > 30.0:           # progress of a long rung
+            c.synchronize()
+            last_note = time.perf_counter()
+            print(f"[ladder] n={n} S={S} {'fp64' if precision == shockx.FP64 else 'mixed'}: step {done} of "
+                  f"{steps}", file=sys.stderr, flush=True)
     c.synchronize()
     wall = time.perf_counter() - t0
     st = c.download_state()
