@@ -569,7 +569,7 @@ def serve_workload(args):
     for rate in rates:
         reqs = serve.make_requests(args.cases, rate, args.seed, n_lo=args.n_lo, n_hi=args.n_hi, precision=prec,
                                    shock_frac=args.shock_frac)
-        srv = serve.Server()
+        srv = serve.Server(progress_s=30.0)
         stats, done = srv.run(reqs)
         for r in srv.failed:
             print(f"[serve] request {r.rid} failed ({r.failed}): {r.kind}, n {r.n}, S {r.S}, Mach {r.mach:.2f}, "

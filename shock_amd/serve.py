@@ -15,6 +15,7 @@ result does not depend on what else is resident: served alone or in a mix, it ru
 kernels on the same data.
 """
 import math
+import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -125,7 +126,8 @@ class Server:
     factory, the batch type and the footprint model (replaceable for host-only tests)."""
 
     def __init__(self, budget_bytes=270 * 2 ** 30, max_slots=64, chunk=20, keep_results=False,
-                 open_fn=open_case, batch_cls=None, footprint_fn=footprint_bytes):
+                 open_fn=open_case, batch_cls=None, footprint_fn=footprint_bytes, progress_s=None):
+        self.progress_s = progress_s        # seconds between progress lines on stderr (None: quiet)
         self.budget = budget_bytes
         self.max_slots = max_slots
         self.chunk = chunk
@@ -156,8 +158,13 @@ class Server:
         failed = []
         rejected = []
         cell_updates = 0.0
+        last_note = 0.0
         while pending or waiting or active:
             now = time.perf_counter() - t0
+            if self.progress_s and now - last_note >= self.progress_s:
+                last_note = now
+                print(f"[serve] t={now:.0f}s: {len(active)} active, {len(waiting)} waiting, {len(pending)} not yet "
+                      f"arrived, {len(finished)} finished, {len(failed)} failed", file=sys.stderr, flush=True)
             while pending and pending[0].arrival <= now:
                 r = pending.pop(0)
                 if self.footprint(r.n, r.S) > self.budget:
